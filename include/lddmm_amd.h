@@ -1,0 +1,222 @@
+/*
+ * lddmm_amd.h -- C ABI of the MI355X (gfx950) point-set hot path of py-lddmm.
+ *
+ * Every entry point takes DEVICE pointers (row-major float64 arrays, int32
+ * face / adjacency arrays), plain sizes and a hipStream_t passed as void*.
+ * Work is enqueued on that stream and the call returns without waiting,
+ * except where a scalar result is returned through a host pointer (then the
+ * call synchronises the stream).  Outputs are written, never accumulated.
+ *
+ * Return value: 0 on success, a negative LDDMM_E* code otherwise; the text of
+ * the last error on the calling thread is given by lddmm_last_error().
+ *
+ * Precision (lddmm_kernel.prec):
+ *   LDDMM_PREC_F64  -- float64 arithmetic throughout (the reference's numba
+ *                      path is float64).
+ *   LDDMM_PREC_F32C -- float32 pair arithmetic on centred coordinates, float32
+ *                      accumulation compensated (two-sum) across source tiles
+ *                      and chunks; results are returned as float64.
+ *
+ * Scratch memory: the library keeps one grow-only device workspace per GPU.
+ * Calls that share a GPU must be issued on one stream (or be serialised by
+ * the caller); calls on different GPUs are independent.
+ *
+ * Each declaration names the reference function it replaces
+ * (paths relative to py-lddmm/).
+ */
+#ifndef LDDMM_AMD_H
+#define LDDMM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LDDMM_MAX_SCALES 8
+
+#define LDDMM_KERNEL_GAUSS 0
+
+#define LDDMM_PREC_F64  0
+#define LDDMM_PREC_F32C 1
+
+#define LDDMM_OK           0
+#define LDDMM_EINVAL      -1   /* bad argument (sizes, modes, null pointers) */
+#define LDDMM_EUNSUPPORTED -2  /* kernel family / option not built */
+#define LDDMM_EHIP        -3   /* HIP runtime error (launch, allocation) */
+
+/* diffKT modes: base/kernelFunctions_util.py:785-791 */
+#define LDDMM_DIFFKT_PLAIN 0
+#define LDDMM_DIFFKT_SYM   1
+#define LDDMM_DIFFKT_LDDMM 2
+
+/* applyDiffK options: base/kernelFunctions_util.py:662-677 */
+#define LDDMM_DIFFK_1      0
+#define LDDMM_DIFFK_2      1
+#define LDDMM_DIFFK_1AND2  2
+
+/* applyDDiffK options: base/kernelFunctions_util.py:1100-1127 */
+#define LDDMM_DDIFFK_11     0
+#define LDDMM_DDIFFK_12     1
+#define LDDMM_DDIFFK_11AND12 2
+
+/* Kernel description: mirrors KernelSpec (base/kernelFunctions.py:339-362):
+ * name -> kind, sigma -> sigma[0..nscale), ms_exponent -> kp, pk_dtype -> prec. */
+typedef struct lddmm_kernel {
+    int32_t kind;                      /* LDDMM_KERNEL_GAUSS */
+    int32_t nscale;                    /* 1 .. LDDMM_MAX_SCALES */
+    double  sigma[LDDMM_MAX_SCALES];
+    double  kp;                        /* multi-scale exponent, reference default -1 */
+    int32_t prec;                      /* LDDMM_PREC_F64 / LDDMM_PREC_F32C */
+    int32_t reserved;
+} lddmm_kernel;
+
+const char* lddmm_last_error(void);
+const char* lddmm_version(void);
+/* Number of visible GPUs (0 when no HIP device is present). */
+int lddmm_device_count(void);
+/* Releases the calling device's workspace. */
+int lddmm_release_workspace(void);
+
+/* out[n,k] = sum_s w_s sum_j K_s(y_i, x_j) a[j,:]
+ * Replaces ku.applyK / applyK_numba / applyK_pykeops
+ * (base/kernelFunctions_util.py:443-466, :499-535), reached from
+ * Kernel.applyK (base/kernelFunctions.py:452-475). dim in {2,3}, k >= 1. */
+int lddmm_applyK(const lddmm_kernel* kd,
+                 const double* y, int64_t n, const double* x, int64_t m, int32_t dim,
+                 const double* a, int32_t k, double* out, void* stream);
+
+/* out[n,dim] = sum_s s^(KP-1)/W sum_j -(y_i-x_j)/s K_s(y_i,x_j) c_ij with
+ * c_ij = p_i.a_j (+ a_i.p_j for SYM, + a_i.p_j - 2 regweight a_i.a_j for LDDMM).
+ * p is [n,k], a is [m,k]; SYM and LDDMM require n == m.
+ * Replaces ku.applyDiffKT / applyDiffKT_numba / _pykeops
+ * (base/kernelFunctions_util.py:765-845), reached from Kernel.applyDiffKT
+ * (base/kernelFunctions.py:581-600). */
+int lddmm_applyDiffKT(const lddmm_kernel* kd,
+                      const double* y, int64_t n, const double* x, int64_t m, int32_t dim,
+                      const double* p, const double* a, int32_t k,
+                      int32_t mode, double regweight, double* out, void* stream);
+
+/* out[n] = sum_s w_s sum_j K_s(y_i,x_j) (ay_i ax_j + (betay_i . betax_j)^2)
+ * Replaces ku.applyktensor (base/kernelFunctions_util.py:1424-1506),
+ * reached from Kernel.applyKTensor (base/kernelFunctions.py:509-517). */
+int lddmm_applyktensor(const lddmm_kernel* kd,
+                       const double* y, int64_t n, const double* x, int64_t m, int32_t dim,
+                       const double* ay, const double* ax,
+                       const double* betay, const double* betax, int32_t kb,
+                       double* out, void* stream);
+
+/* out[n,kb] = sum_s w_s sum_j K_s(y_i,x_j) beta[i,j,:]   (dense beta [n,m,kb])
+ * Replaces ku.applykmat (base/kernelFunctions_util.py:1509-1540), reached from
+ * Kernel.applyK(..., matrixWeights=True) (base/kernelFunctions.py:456,467). */
+int lddmm_applykmat(const lddmm_kernel* kd,
+                    const double* y, int64_t n, const double* x, int64_t m, int32_t dim,
+                    const double* beta, int32_t kb, double* out, void* stream);
+
+/* out[n,dim] = -sum_s s^(KP-1)/W sum_j K_s (y_i-x_j)/s beta[i,j]  (dense beta [n,m])
+ * Replaces ku.applykdiffmat (base/kernelFunctions_util.py:1631-1669), reached
+ * from Kernel.applyDiffKmat (base/kernelFunctions.py:565-570). */
+int lddmm_applykdiffmat(const lddmm_kernel* kd,
+                        const double* y, int64_t n, const double* x, int64_t m, int32_t dim,
+                        const double* beta, double* out, void* stream);
+
+/* First-derivative sums on one point set x[n,dim], a1 [n,dim], a2 [n,k]:
+ * option 1AND2: out[i] = sum_s s^(KP-1)/W sum_l -K ((x_i-x_l)/s.(a1_i-a1_l)) a2_l
+ * (options 1 and 2 as in the reference).  Replaces ku.applyDiffK
+ * (base/kernelFunctions_util.py:646-680), reached from Kernel.applyDiffK1and2
+ * (base/kernelFunctions.py:549-562) through applykdiff1and2 (:954-955). */
+int lddmm_applyDiffK(const lddmm_kernel* kd,
+                     const double* x, int64_t n, int32_t dim,
+                     const double* a1, const double* a2, int32_t k,
+                     int32_t option, double* out, void* stream);
+
+/* Second-derivative sums on one point set: a1 [n,k], a2 [n,k], p [n,dim]:
+ * out[i] = sum_s s^(KP-2)/W sum_l ((d.q) d - q) K_s (a1_i.a2_l), d=(x_i-x_l)/s,
+ * q = p_i (11), -p_l (12), p_i - p_l (11AND12).  Replaces ku.applyDDiffK
+ * (base/kernelFunctions_util.py:1084-1129), reached from
+ * Kernel.applyDDiffK11and12 (base/kernelFunctions.py:665-674). */
+int lddmm_applyDDiffK(const lddmm_kernel* kd,
+                      const double* x, int64_t n, int32_t dim,
+                      const double* a1, const double* a2, int32_t k, const double* p,
+                      int32_t option, double* out, void* stream);
+
+/* Triangulated surface on the device: vertices [nv,3] f64, faces [nf,3] int32,
+ * vertex weights [nv] f64 (NULL = all ones, the reference default,
+ * base/surfaces.py:238-240). */
+typedef struct lddmm_surface {
+    const double*  vertices;
+    int64_t        nv;
+    const int32_t* faces;
+    int64_t        nf;
+    const double*  vweights;
+    /* vertex -> incident (face*3 + corner) adjacency, CSR, built by
+     * lddmm_surface_adjacency; required only for gradients. */
+    const int32_t* adj_ptr;            /* [nv+1] */
+    const int32_t* adj_idx;            /* [3*nf] */
+} lddmm_surface;
+
+/* Host helper: builds the CSR vertex->(face,corner) adjacency of int32 faces
+ * (host arrays).  adj_ptr has nv+1 entries, adj_idx 3*nf entries. */
+int lddmm_surface_adjacency(const int32_t* faces_host, int64_t nf, int64_t nv,
+                            int32_t* adj_ptr_host, int32_t* adj_idx_host);
+
+/* Varifold data term between surfaces (orientation-free, f0(u) = 1 + u^2):
+ *   tgt == NULL : *energy = |S1|^2                       (varifoldNorm0)
+ *   tgt != NULL : *energy = |S1|^2 - 2 <S1, tgt>           (varifoldNormDef)
+ * energy is a DEVICE pointer to one float64 (FP64 total reduction).
+ * grad (may be NULL) receives d/dx of |S1 - tgt|^2 wrt S1's vertices,
+ * [nv,3] f64 (varifoldNormGradient); requires S1's adjacency.
+ * Replaces base/surfaceDistances.py:286-468 (varifoldNorm0/Def/Gradient). */
+int lddmm_surface_varifold(const lddmm_kernel* kd,
+                           const lddmm_surface* s1, const lddmm_surface* tgt,
+                           double* energy, double* grad, void* stream);
+
+/* Face features of a surface: centers [nf,3], surfel*face_weight [nf,3],
+ * face weights [nf] (base/surfaces.py:326-335).  Any output may be NULL. */
+int lddmm_surface_features(const lddmm_surface* s, double* centers, double* wsurfel,
+                           double* face_weights, void* stream);
+
+/* Scatter of per-face (dz1, z1) to vertices with the cross-product chain rule
+ * px[f_v] += dz1_f - (x_{v+2} - x_{v+1}) x z1_f
+ * (base/surfaceDistances.py:440-453 / :117-131).  Requires the adjacency. */
+int lddmm_surface_face_gather(const lddmm_surface* s, const double* dz1, const double* z1,
+                              double* px, void* stream);
+
+/* Forward Euler landmark flow x_{t+1} = x_t + (1/T) K(x_t,x_t) a_t, t < T.
+ * at [T,n,dim], xt [T+1,n,dim] (xt[0] is read, xt[1..T] written).
+ * If energy != NULL (device f64[1]) it receives sum_t r dt a_t.K(x_t)a_t.
+ * Replaces base/pointEvolution.py:88-164 and the deformation energy of
+ * base/surfaceMatching.py:533-548. */
+int lddmm_shoot_forward(const lddmm_kernel* kd, const double* at, int32_t T, int64_t n,
+                        int32_t dim, double* xt, double regweight, double* energy,
+                        void* stream);
+
+/* Discrete adjoint: p_T = px1; p_t = p_{t+1} + dt applyDiffKT(x_t,p_{t+1},a_t,lddmm,r);
+ * grad_t = K(x_t)(2 r a_t - p_{t+1}) / T when euclidean != 0, else
+ * (2 r a_t - p_{t+1}) / T.  pxt [T+1,n,dim] (may be NULL), grad [T,n,dim].
+ * Replaces base/surfaceMatching.py:728-908 (hamiltonianCovector/Gradient)
+ * and base/pointEvolution.py:167-236, with the /T of getGradient (:1086). */
+int lddmm_shoot_adjoint(const lddmm_kernel* kd, const double* at, const double* xt,
+                        int32_t T, int64_t n, int32_t dim, const double* px1,
+                        double regweight, int32_t euclidean, double* pxt, double* grad,
+                        void* stream);
+
+/* Geodesic (EPDiff) shooting: x_{k+1} = x_k + dt K a_k, a_{k+1} = a_k - dt applyDiffKT(x_k,a_k,a_k)
+ * for k < T; xt, at [T+1,n,dim] with xt[0], at[0] read.
+ * Replaces base/pointEvolution.py:385-472 (secondOrderEvolution). */
+int lddmm_epdiff_forward(const lddmm_kernel* kd, int32_t T, int64_t n, int32_t dim,
+                         double* xt, double* at, void* stream);
+
+/* Adjoint of lddmm_epdiff_forward given (xt, at), px1, pa1; writes pxt, pat
+ * [T+1,n,dim] and da0 = K(x0)a0 - pa_0 [n,dim].
+ * Replaces base/pointEvolution.py:489-618 (secondOrderCovector/Gradient). */
+int lddmm_epdiff_adjoint(const lddmm_kernel* kd, int32_t T, int64_t n, int32_t dim,
+                         const double* xt, const double* at, const double* px1,
+                         const double* pa1, double* pxt, double* pat, double* da0,
+                         void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LDDMM_AMD_H */

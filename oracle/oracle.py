@@ -1,0 +1,673 @@
+"""FP64 numpy oracle for the py-lddmm point-set hot path.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product package (`lddmm_amd`)
+imports this module; only `tests/`, `__graft_entry__.smoke()` and the
+`cpu_baseline` leg of `bench.py` use it, always as the checker.
+
+Each function restates one reference function (paths relative to
+`py-lddmm/`), in float64, evaluated over blocks of target rows so that the
+O(N*M) pair sums never materialise more than `block * M` pairs at once.
+Index conventions follow the reference: `y` are targets (rows, N), `x` are
+sources (columns, M), weights `a` are [M, k], scales are an array of sigmas
+and the multi-scale weight is s**KP / sum(s**KP) with KP = -1.
+
+How each function is pinned (tests/test_oracle.py):
+  applyK          closed forms (one point, two points), sigma->inf limit,
+                  duplicated-sigma identity, adjoint symmetry
+  applyDiffKT     central finite differences of <p, K(y,x) a> (plain),
+                  of p^T K(x) a (sym) and p^T K a - r a^T K a (lddmm)
+  applyktensor    single pair closed form, adjoint symmetry, rigid invariance
+  applykmat       reduces to applyK when beta[i,j,:] = a[j,:]
+  applykdiffmat   reduces to applyDiffKT(plain) when beta[i,j] = p_i.a_j
+  applyDiffK      finite differences of applyK along the point set
+  applyDDiffK     finite differences of applyDiffKT(plain) along the point set
+  surface / curve varifold and current norms: ||S - S|| = 0, rigid-motion
+                  invariance, face-flip invariance (varifold), single pair
+                  closed forms, finite-difference gradients
+  shooting        zero control -> identity, finite differences of the full
+                  objective through the discrete adjoint
+  EPDiff          trajectory identity with landmarkEPDiff, Hamiltonian drift
+                  O(dt), finite differences of the second-order gradient
+"""
+import numpy as np
+
+__all__ = [
+    "applyK", "applyDiffKT", "applyktensor", "applykmat", "applykdiffmat",
+    "applyDiffK", "applyDDiffK",
+    "surface_geometry", "surface_volume",
+    "varifoldNorm0", "varifoldNormDef", "varifoldNorm", "varifoldNormGradient",
+    "currentNorm0", "currentNormDef", "currentNorm", "currentNormGradient",
+    "currentMagnitude", "currentMagnitudeGradient",
+    "curve_geometry", "curveVarifoldNorm0", "curveVarifoldNormDef",
+    "curveVarifoldNormGradient", "curveCurrentNorm0", "curveCurrentNormDef",
+    "curveCurrentNormGradient",
+    "landmarkDirectEvolutionEuler", "landmarkHamiltonianCovector",
+    "landmarkHamiltonianGradient", "matching_objective", "matching_gradient",
+    "landmarkEPDiff", "secondOrderEvolution", "secondOrderHamiltonian",
+    "secondOrderCovector", "secondOrderGradient",
+]
+
+_BLOCK_PAIRS = 1 << 22   # pairs held in memory per block (~100 MB of f64 temporaries)
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _scales(scale, KP):
+    s = np.atleast_1d(np.asarray(scale, dtype=np.float64))
+    sKP = s ** KP
+    return s, sKP, sKP.sum()
+
+
+def _row_blocks(n, m):
+    bs = max(1, min(n, _BLOCK_PAIRS // max(m, 1)))
+    for i0 in range(0, n, bs):
+        yield i0, min(n, i0 + bs)
+
+
+def _as2d(a):
+    a = _f64(a)
+    return a[:, None] if a.ndim == 1 else a
+
+
+# ---------------------------------------------------------------------------
+# L1 kernel sums (Gaussian)
+# ---------------------------------------------------------------------------
+
+def applyK(y, x, a, scale, KP=-1):
+    """out[i] = sum_s w_s sum_j exp(-|y_i/s - x_j/s|^2 / 2) a[j].
+
+    Follows base/kernelFunctions_util.py:451-466 (applyK_numba) with the
+    pair function gauss_fun at :86-89; w_s = s^KP / sum(s^KP) (:454-456, :465).
+    """
+    y, x, a = _f64(y), _f64(x), _as2d(a)
+    s, sKP, W = _scales(scale, KP)
+    res = np.zeros((y.shape[0], a.shape[1]))
+    for si, sc in enumerate(s):
+        ys, xs = y / sc, x / sc
+        for i0, i1 in _row_blocks(y.shape[0], x.shape[0]):
+            d = ys[i0:i1, None, :] - xs[None, :, :]
+            K = np.exp(-(d ** 2).sum(-1) / 2)
+            res[i0:i1] += (K @ a) * sKP[si]
+    return res / W
+
+
+def applyDiffKT(y, x, p, a, scale, regweight=1., sym=False, lddmm=False, KP=-1):
+    """out[i] = sum_s s^(KP-1)/W sum_j -(y_i - x_j)/s * K_s(y_i, x_j) * c_ij.
+
+    c_ij = p_i.a_j (plain), + a_i.p_j (sym), + a_i.p_j - 2 r a_i.a_j (lddmm).
+    Follows base/kernelFunctions_util.py:773-793 (applyDiffKT_numba) with
+    gauss_fun_diffT at :232-236.  sym / lddmm need y and x of the same size.
+    """
+    y, x, p, a = _f64(y), _f64(x), _as2d(p), _as2d(a)
+    s, sKP, W = _scales(scale, KP)
+    sKP1 = s ** (KP - 1)
+    res = np.zeros(y.shape)
+    for si, sc in enumerate(s):
+        ys, xs = y / sc, x / sc
+        for i0, i1 in _row_blocks(y.shape[0], x.shape[0]):
+            d = ys[i0:i1, None, :] - xs[None, :, :]
+            K = np.exp(-(d ** 2).sum(-1) / 2)
+            c = p[i0:i1] @ a.T
+            if lddmm:
+                c = c + a[i0:i1] @ p.T - 2 * regweight * (a[i0:i1] @ a.T)
+            elif sym:
+                c = c + a[i0:i1] @ p.T
+            res[i0:i1] += -(d * (K * c)[..., None]).sum(1) * sKP1[si]
+    return res / W
+
+
+def applyktensor(y, x, ay, ax, betay, betax, scale, KP=-1):
+    """out[i] = sum_s w_s sum_j K_s(y_i, x_j) (ay_i ax_j + (betay_i . betax_j)^2).
+
+    Follows base/kernelFunctions_util.py:1475-1506 (applyktensor_numba, gauss
+    branch :1490-1496).
+    """
+    y, x, ay, ax = _f64(y), _f64(x), _f64(ay).ravel(), _f64(ax).ravel()
+    by, bx = _as2d(betay), _as2d(betax)
+    s, sKP, W = _scales(scale, KP)
+    res = np.zeros(y.shape[0])
+    for si, sc in enumerate(s):
+        ys, xs = y / sc, x / sc
+        for i0, i1 in _row_blocks(y.shape[0], x.shape[0]):
+            d = ys[i0:i1, None, :] - xs[None, :, :]
+            K = np.exp(-(d ** 2).sum(-1) / 2)
+            g = by[i0:i1] @ bx.T
+            res[i0:i1] += (K * (ay[i0:i1, None] * ax[None, :] + g * g)).sum(1) * sKP[si]
+    return res / W
+
+
+def applykmat(y, x, beta, scale, KP=-1):
+    """out[i] = sum_s w_s sum_j K_s(y_i, x_j) beta[i, j, :] (dense matrix weights).
+
+    Follows base/kernelFunctions_util.py:1509-1540 (applykmat, gauss branch).
+    """
+    y, x, beta = _f64(y), _f64(x), _f64(beta)
+    if beta.ndim == 2:
+        beta = beta[..., None]
+    s, sKP, W = _scales(scale, KP)
+    res = np.zeros((y.shape[0], beta.shape[2]))
+    for si, sc in enumerate(s):
+        ys, xs = y / sc, x / sc
+        for i0, i1 in _row_blocks(y.shape[0], x.shape[0]):
+            d = ys[i0:i1, None, :] - xs[None, :, :]
+            K = np.exp(-(d ** 2).sum(-1) / 2)
+            res[i0:i1] += (K[..., None] * beta[i0:i1]).sum(1) * sKP[si]
+    return res / W
+
+
+def applykdiffmat(y, x, beta, scale, KP=-1):
+    """out[i] = -sum_s s^(KP-1)/W sum_j K_s(y_i, x_j) (y_i - x_j)/s beta[i, j].
+
+    Follows base/kernelFunctions_util.py:1638-1669 (applykdiffmat_numba).
+    """
+    y, x, beta = _f64(y), _f64(x), _f64(beta)
+    s, sKP, W = _scales(scale, KP)
+    sKP1 = s ** (KP - 1)
+    res = np.zeros(y.shape)
+    for si, sc in enumerate(s):
+        ys, xs = y / sc, x / sc
+        for i0, i1 in _row_blocks(y.shape[0], x.shape[0]):
+            d = ys[i0:i1, None, :] - xs[None, :, :]
+            K = np.exp(-(d ** 2).sum(-1) / 2)
+            res[i0:i1] -= (d * (K * beta[i0:i1])[..., None]).sum(1) * sKP1[si]
+    return res / W
+
+
+def applyDiffK(x, a1, a2, scale, KP=-1, option='1and2'):
+    """First derivative sums on one point set (y = x).
+
+    option '1':     out[k] = sum_l -K (d_kl . a1_k) a2_l
+    option '2':     out[k] = -sum_l -K (d_kl . a1_l) a2_l
+    option '1and2': out[k] = sum_l -K (d_kl . (a1_k - a1_l)) a2_l
+    with d_kl = (x_k - x_l)/s and the scale weight s^(KP-1)/W.
+    Follows base/kernelFunctions_util.py:655-680 (applyDiffK_numba) with
+    gauss_fun_diff at :190-194.  Kernel.applyDiffK1and2 reaches it through
+    applykdiff1and2 (:954-955), which always passes KP = -1.
+    """
+    x, a1, a2 = _f64(x), _as2d(a1), _as2d(a2)
+    s, sKP, W = _scales(scale, KP)
+    sKP1 = s ** (KP - 1)
+    res = np.zeros(x.shape)
+    for si, sc in enumerate(s):
+        xs = x / sc
+        for i0, i1 in _row_blocks(x.shape[0], x.shape[0]):
+            d = xs[i0:i1, None, :] - xs[None, :, :]
+            K = np.exp(-(d ** 2).sum(-1) / 2)
+            if option == '1':
+                da = (d * a1[i0:i1, None, :]).sum(-1)
+                res[i0:i1] += ((-K * da) @ a2) * sKP1[si]
+            elif option == '2':
+                da = (d * a1[None, :, :]).sum(-1)
+                res[i0:i1] -= ((-K * da) @ a2) * sKP1[si]
+            elif option == '1and2':
+                da = (d * (a1[i0:i1, None, :] - a1[None, :, :])).sum(-1)
+                res[i0:i1] += ((-K * da) @ a2) * sKP1[si]
+            else:
+                raise ValueError(option)
+    return res / W
+
+
+def applyDDiffK(x, a1, a2, p, scale, KP=-1, option='11and12'):
+    """Second derivative sums on one point set.
+
+    out[k] = sum_s s^(KP-2)/W sum_l ((d.q) d - q) K_s(x_k, x_l) (a1_k . a2_l)
+    with d = (x_k - x_l)/s and q = p_k (option '11'), -p_l ('12'),
+    p_k - p_l ('11and12').  Follows base/kernelFunctions_util.py:1092-1129
+    (applyDDiffK_numba) with gauss_fun_ddiff at :298-306.
+    """
+    x, a1, a2, p = _f64(x), _as2d(a1), _as2d(a2), _as2d(p)
+    s, sKP, W = _scales(scale, KP)
+    res = np.zeros(x.shape)
+    for sc in s:
+        xs = x / sc
+        KS = sc ** (KP - 2)
+        for i0, i1 in _row_blocks(x.shape[0], x.shape[0]):
+            d = xs[i0:i1, None, :] - xs[None, :, :]
+            K = np.exp(-(d ** 2).sum(-1) / 2)
+            aa = a1[i0:i1] @ a2.T
+            if option == '11':
+                q = np.broadcast_to(p[i0:i1, None, :], d.shape)
+                sign = 1.
+            elif option == '12':
+                q = np.broadcast_to(p[None, :, :], d.shape)
+                sign = -1.
+            elif option == '11and12':
+                q = p[i0:i1, None, :] - p[None, :, :]
+                sign = 1.
+            else:
+                raise ValueError(option)
+            dq = (d * q).sum(-1)
+            t = (dq[..., None] * d - q) * (K * aa)[..., None]
+            res[i0:i1] += sign * t.sum(1) * KS
+    return res / W
+
+
+# ---------------------------------------------------------------------------
+# Shapes
+# ---------------------------------------------------------------------------
+
+def surface_geometry(vertices, faces, weights=None):
+    """(centers, surfel, face_weights) of a triangulated surface.
+
+    Follows base/surfaces.py:326-335 (computeCentersAreas); unit vertex
+    weights by default (base/surfaces.py:238-240).
+    """
+    v = _f64(vertices)
+    f = np.asarray(faces, dtype=np.int64)
+    x1, x2, x3 = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+    centers = (x1 + x2 + x3) / 3
+    surfel = np.cross(x2 - x1, x3 - x1) / 2
+    if weights is None:
+        fw = np.ones(f.shape[0])
+    else:
+        w = _f64(weights)
+        fw = (w[f[:, 0]] + w[f[:, 1]] + w[f[:, 2]]) / 3
+    return centers, surfel, fw
+
+
+def surface_volume(vertices, faces):
+    """Signed volume sum_f det(v[f]) / 6 (base/surfaces.py:1645-1651)."""
+    v = _f64(vertices)
+    f = np.asarray(faces, dtype=np.int64)
+    return np.linalg.det(v[f]).sum() / 6
+
+
+def _surf(fv):
+    """Accept (vertices, faces) or (vertices, faces, weights)."""
+    v, f = fv[0], fv[1]
+    w = fv[2] if len(fv) > 2 else None
+    c, S, fw = surface_geometry(v, f, w)
+    return _f64(v), np.asarray(f, dtype=np.int64), c, S, fw
+
+
+def _varifold_features(S, fw):
+    a = np.sqrt((S ** 2).sum(axis=1) + 1e-10)
+    n = S / a[:, None]
+    return a, n, a * fw
+
+
+def varifoldNorm0(fv1, sigma, KP=-1):
+    """base/surfaceDistances.py:293-302 (varifoldNorm0_numpy)."""
+    _, _, c2, S2, fw2 = _surf(fv1)
+    _, n2, a2 = _varifold_features(S2, fw2)
+    b2 = n2 * np.sqrt(a2[:, None])
+    return applyktensor(c2, c2, a2, a2, b2, b2, sigma, KP).sum()
+
+
+def varifoldNormDef(fvDef, fv1, sigma, KP=-1):
+    """base/surfaceDistances.py:335-353 (varifoldNormDef_numpy)."""
+    _, _, c1, S1, fw1 = _surf(fvDef)
+    _, _, c2, S2, fw2 = _surf(fv1)
+    _, n1, a1 = _varifold_features(S1, fw1)
+    _, n2, a2 = _varifold_features(S2, fw2)
+    b1 = n1 * np.sqrt(a1[:, None])
+    b2 = n2 * np.sqrt(a2[:, None])
+    obj1 = applyktensor(c1, c1, a1, a1, b1, b1, sigma, KP).sum()
+    obj2 = applyktensor(c1, c2, a1, a2, b1, b2, sigma, KP).sum()
+    return obj1 - 2 * obj2
+
+
+def varifoldNorm(fvDef, fv1, sigma, KP=-1):
+    """base/surfaceDistances.py:386-390."""
+    return varifoldNormDef(fvDef, fv1, sigma, KP) + varifoldNorm0(fv1, sigma, KP)
+
+
+def _scatter_faces(nv, faces, per_face_vertex):
+    px = np.zeros((nv, per_face_vertex[0].shape[1]))
+    for j in range(faces.shape[1]):
+        np.add.at(px, faces[:, j], per_face_vertex[j])
+    return px
+
+
+def varifoldNormGradient(fvDef, fv1, sigma, KP=-1):
+    """Gradient of |fvDef - fv1|^2 (varifold, f0(u) = 1 + u^2) wrt fvDef vertices.
+
+    Follows base/surfaceDistances.py:403-468 (varifoldNormGradient_numpy),
+    with the dense beta/u temporaries evaluated per row block.
+    """
+    xDef, f, c1, S1, fw1 = _surf(fvDef)
+    _, _, c2, S2, fw2 = _surf(fv1)
+    _, n1, a1w = _varifold_features(S1, fw1)
+    _, n2, a2w = _varifold_features(S2, fw2)
+    s, sKP, W = _scales(sigma, KP)
+    sKP1 = s ** (KP - 1)
+    F = c1.shape[0]
+    z1 = np.zeros((F, 3))
+    dz1 = np.zeros((F, 3))
+    for (cx, nx, ax, sign) in ((c1, n1, a1w, 1.), (c2, n2, a2w, -1.)):
+        for i0, i1 in _row_blocks(F, cx.shape[0]):
+            cc = n1[i0:i1] @ nx.T                                 # cr1cr_
+            beta = a1w[i0:i1, None] * ax[None, :] * (1 + cc ** 2)
+            u = (2 * cc[..., None] * (nx[None, :, :] - cc[..., None] * n1[i0:i1, None, :])
+                 + (1 + cc ** 2)[..., None] * n1[i0:i1, None, :]) * ax[None, :, None]
+            for si, sc in enumerate(s):
+                d = c1[i0:i1, None, :] / sc - cx[None, :, :] / sc
+                K = np.exp(-(d ** 2).sum(-1) / 2)
+                z1[i0:i1] += sign * (K[..., None] * u).sum(1) * sKP[si] / W
+                dz1[i0:i1] -= sign * (d * (K * beta)[..., None]).sum(1) * sKP1[si] / W
+    z1 *= fw1[:, None]
+    dz1 *= 2. / 3.
+    x1, x2, x3 = xDef[f[:, 0]], xDef[f[:, 1]], xDef[f[:, 2]]
+    return _scatter_faces(xDef.shape[0], f, (dz1 - np.cross(x3 - x2, z1),
+                                            dz1 - np.cross(x1 - x3, z1),
+                                            dz1 - np.cross(x2 - x1, z1)))
+
+
+def currentNorm0(fv1, sigma, KP=-1):
+    """base/surfaceDistances.py:21-25 (currentMagnitude) via :67-68."""
+    _, _, c2, S2, fw2 = _surf(fv1)
+    cr2 = S2 * fw2[:, None]
+    return (cr2 * applyK(c2, c2, cr2, sigma, KP)).sum()
+
+
+currentMagnitude = currentNorm0
+
+
+def currentNormDef(fvDef, fv1, sigma, KP=-1):
+    """base/surfaceDistances.py:72-78."""
+    _, _, c1, S1, fw1 = _surf(fvDef)
+    _, _, c2, S2, fw2 = _surf(fv1)
+    cr1 = fw1[:, None] * S1
+    cr2 = fw2[:, None] * S2
+    return (cr1 * applyK(c1, c1, cr1, sigma, KP)).sum() - 2 * (cr1 * applyK(c1, c2, cr2, sigma, KP)).sum()
+
+
+def currentNorm(fvDef, fv1, sigma, KP=-1):
+    """base/surfaceDistances.py:82-83."""
+    return currentNormDef(fvDef, fv1, sigma, KP) + currentNorm0(fv1, sigma, KP)
+
+
+def currentNormGradient(fvDef, fv1, sigma, KP=-1):
+    """base/surfaceDistances.py:87-139 (currentNormGradient)."""
+    xDef, f, c1, S1, w1 = _surf(fvDef)
+    _, _, c2, S2, w2 = _surf(fv1)
+    wcr1 = w1[:, None] * S1
+    wcr2 = w2[:, None] * S2
+    z1 = applyK(c1, c1, wcr1, sigma, KP) - applyK(c1, c2, wcr2, sigma, KP)
+    wz1 = w1[:, None] * z1
+    dz1 = (2 / 3) * (applyDiffKT(c1, c1, wcr1, wcr1, sigma, KP=KP) -
+                     applyDiffKT(c1, c2, wcr1, wcr2, sigma, KP=KP))
+    x1, x2, x3 = xDef[f[:, 0]], xDef[f[:, 1]], xDef[f[:, 2]]
+    return _scatter_faces(xDef.shape[0], f, (dz1 - np.cross(x3 - x2, wz1),
+                                            dz1 - np.cross(x1 - x3, wz1),
+                                            dz1 - np.cross(x2 - x1, wz1)))
+
+
+def currentMagnitudeGradient(fvDef, sigma, KP=-1):
+    """base/surfaceDistances.py:28-63 (currentMagnitudeGradient)."""
+    xDef, f, c1, S1, w1 = _surf(fvDef)
+    wcr1 = w1[:, None] * S1
+    z1 = applyK(c1, c1, wcr1, sigma, KP)
+    wz1 = w1[:, None] * z1
+    dz1 = (2 / 3) * applyDiffKT(c1, c1, wcr1, wcr1, sigma, KP=KP)
+    x1, x2, x3 = xDef[f[:, 0]], xDef[f[:, 1]], xDef[f[:, 2]]
+    return _scatter_faces(xDef.shape[0], f, (dz1 - np.cross(x3 - x2, wz1),
+                                            dz1 - np.cross(x1 - x3, wz1),
+                                            dz1 - np.cross(x2 - x1, wz1)))
+
+
+# Curves --------------------------------------------------------------------
+
+def curve_geometry(vertices, faces, weights=None):
+    """(centers, linel, line_weights); base/curves.py:117-125."""
+    v = _f64(vertices)
+    f = np.asarray(faces, dtype=np.int64)
+    x1, x2 = v[f[:, 0]], v[f[:, 1]]
+    w = np.ones(v.shape[0]) if weights is None else _f64(weights)
+    return (x1 + x2) / 2, x2 - x1, (w[f[:, 0]] + w[f[:, 1]]) / 2
+
+
+def _curve(fv):
+    v, f = fv[0], fv[1]
+    w = fv[2] if len(fv) > 2 else None
+    c, l, lw = curve_geometry(v, f, w)
+    return _f64(v), np.asarray(f, dtype=np.int64), c, l, lw
+
+
+def _curve_vfeat(cr):
+    a = np.sqrt((cr ** 2).sum(axis=1) + 1e-10)
+    return a, cr / a[:, None]
+
+
+def curveVarifoldNorm0(fv1, sigma, weight=1., KP=-1):
+    """base/curveDistances.py:186-200 (_varifoldNorm0 / varifoldNorm0)."""
+    _, _, c2, l2, w2 = _curve(fv1)
+    a2, n2 = _curve_vfeat(w2[:, None] * l2)
+    tot = 0.
+    for i0, i1 in _row_blocks(c2.shape[0], c2.shape[0]):
+        cc = n2[i0:i1] @ n2.T
+        beta = (1 + weight * cc ** 2) * a2[i0:i1, None] * a2[None, :]
+        tot += applykmat(c2[i0:i1], c2, beta, sigma, KP).sum()
+    return tot
+
+
+def curveVarifoldNormDef(fvDef, fv1, sigma, weight=1., KP=-1):
+    """base/curveDistances.py:218-242 (_varifoldNormDef / varifoldNormDef)."""
+    _, _, c1, l1, w1 = _curve(fvDef)
+    _, _, c2, l2, w2 = _curve(fv1)
+    a1, n1 = _curve_vfeat(w1[:, None] * l1)
+    a2, n2 = _curve_vfeat(w2[:, None] * l2)
+    tot = 0.
+    for (cx, nx, ax, sgn) in ((c1, n1, a1, 1.), (c2, n2, a2, -2.)):
+        for i0, i1 in _row_blocks(c1.shape[0], cx.shape[0]):
+            cc = n1[i0:i1] @ nx.T
+            beta = (1 + weight * cc ** 2) * a1[i0:i1, None] * ax[None, :]
+            tot += sgn * applykmat(c1[i0:i1], cx, beta, sigma, KP).sum()
+    return tot
+
+
+def curveVarifoldNormGradient(fvDef, fv1, sigma, weight=1., KP=-1):
+    """base/curveDistances.py:280-336 (_varifoldNormGradient / varifoldNormGradient)."""
+    xDef, f, c1, l1, w1 = _curve(fvDef)
+    _, _, c2, l2, w2 = _curve(fv1)
+    d = weight
+    a1, n1 = _curve_vfeat(l1)
+    a2, n2 = _curve_vfeat(l2)
+    a1w, a2w = a1 * w1, a2 * w2
+    F = c1.shape[0]
+    z1 = np.zeros((F, xDef.shape[1]))
+    dz1 = np.zeros((F, xDef.shape[1]))
+    for (cx, nx, ax, sgn) in ((c1, n1, a1w, 1.), (c2, n2, a2w, -1.)):
+        for i0, i1 in _row_blocks(F, cx.shape[0]):
+            cc = n1[i0:i1] @ nx.T
+            beta = a1w[i0:i1, None] * ax[None, :] * (1 + d * cc ** 2)
+            u = (2 * d * cc[..., None] * nx[None, ...]
+                 - d * (cc ** 2)[..., None] * n1[i0:i1, None, :]
+                 + n1[i0:i1, None, :]) * ax[None, :, None]
+            z1[i0:i1] += sgn * 2 * applykmat(c1[i0:i1], cx, u, sigma, KP)
+            dz1[i0:i1] += sgn * applykdiffmat(c1[i0:i1], cx, beta, sigma, KP)
+    wz1 = z1 * w1[:, None]
+    return _scatter_faces(xDef.shape[0], f, (dz1 - wz1, dz1 + wz1))
+
+
+def curveCurrentNorm0(fv1, sigma, KP=-1):
+    """base/curveDistances.py:30-37 (currentNorm0, weight=None)."""
+    _, _, c2, l2, w2 = _curve(fv1)
+    cr2 = l2 * w2[:, None]
+    return (cr2 * applyK(c2, c2, cr2, sigma, KP)).sum()
+
+
+def curveCurrentNormDef(fvDef, fv1, sigma, KP=-1):
+    """base/curveDistances.py:41-52 (currentNormDef, weight=None)."""
+    _, _, c1, l1, w1 = _curve(fvDef)
+    _, _, c2, l2, w2 = _curve(fv1)
+    cr1 = l1 * w1[:, None]
+    cr2 = l2 * w2[:, None]
+    return (cr1 * applyK(c1, c1, cr1, sigma, KP)).sum() - 2 * (cr1 * applyK(c1, c2, cr2, sigma, KP)).sum()
+
+
+def curveCurrentNormGradient(fvDef, fv1, sigma, KP=-1):
+    """base/curveDistances.py:61-110 (currentNormGradient, weight=None)."""
+    xDef, f, c1, l1, w1 = _curve(fvDef)
+    _, _, c2, l2, w2 = _curve(fv1)
+    wcr1 = w1[:, None] * l1
+    wcr2 = w2[:, None] * l2
+    z1 = 2 * (applyK(c1, c1, wcr1, sigma, KP) - applyK(c1, c2, wcr2, sigma, KP))
+    wz1 = w1[:, None] * z1
+    dz1 = applyDiffKT(c1, c1, wcr1, wcr1, sigma, KP=KP) - applyDiffKT(c1, c2, wcr1, wcr2, sigma, KP=KP)
+    return _scatter_faces(xDef.shape[0], f, (dz1 - wz1, dz1 + wz1))
+
+
+# ---------------------------------------------------------------------------
+# Landmark shooting (first order) and its discrete adjoint
+# ---------------------------------------------------------------------------
+
+def landmarkDirectEvolutionEuler(x0, at, sigma, KP=-1):
+    """x_{t+1} = x_t + (1/T) K(x_t, x_t) a_t; base/pointEvolution.py:88-164 (no affine)."""
+    x0, at = _f64(x0), _f64(at)
+    T = at.shape[0]
+    dt = 1.0 / T
+    xt = np.zeros((T + 1,) + x0.shape)
+    xt[0] = x0
+    for t in range(T):
+        xt[t + 1] = xt[t] + dt * applyK(xt[t], xt[t], at[t], sigma, KP)
+    return xt
+
+
+def landmarkHamiltonianCovector(x0, at, px1, sigma, regweight, KP=-1):
+    """base/pointEvolution.py:167-201 (no affine, no extra term)."""
+    xt = landmarkDirectEvolutionEuler(x0, at, sigma, KP)
+    T = at.shape[0]
+    dt = 1.0 / T
+    pxt = np.zeros(xt.shape)
+    pxt[T] = px1
+    for t in range(T):
+        px = pxt[T - t]
+        z = xt[T - t - 1]
+        a = at[T - t - 1]
+        zpx = applyDiffKT(z, z, px, a, sigma, regweight=regweight, lddmm=True, KP=KP)
+        pxt[T - t - 1] = px + dt * zpx
+    return pxt, xt
+
+
+def landmarkHamiltonianGradient(x0, at, px1, sigma, regweight, KP=-1):
+    """dat_t = 2 r a_t - p_{t+1}; base/pointEvolution.py:205-236 (no affine)."""
+    pxt, xt = landmarkHamiltonianCovector(x0, at, px1, sigma, regweight, KP)
+    at = _f64(at)
+    dat = 2 * regweight * at - pxt[1:]
+    return dat, xt, pxt
+
+
+def _data_term(kind, fvDef, fv1, sigma_dist):
+    if kind == 'varifold':
+        return varifoldNormDef(fvDef, fv1, sigma_dist), varifoldNorm0(fv1, sigma_dist)
+    if kind == 'current':
+        return currentNormDef(fvDef, fv1, sigma_dist), currentNorm0(fv1, sigma_dist)
+    raise ValueError(kind)
+
+
+def _data_grad(kind, fvDef, fv1, sigma_dist):
+    if kind == 'varifold':
+        return varifoldNormGradient(fvDef, fv1, sigma_dist)
+    if kind == 'current':
+        return currentNormGradient(fvDef, fv1, sigma_dist)
+    raise ValueError(kind)
+
+
+def matching_objective(x0, faces, at, target, sigma_diff, sigma_dist, kind='varifold',
+                       regweight=1., sigma_error=1.):
+    """Surface-matching objective at control `at` (plain, no affine, ds = 1).
+
+    sum_t r dt a_t.K(x_t)a_t  +  (|def|^2 - 2<def, tgt> + |tgt|^2) / sigma_err^2.
+    Follows base/surfaceMatching.py:468-565 (objectiveFunDef) and :638-675
+    (updateTry: obj0 + deformation + dataTerm), dataTerm :443-466.
+    """
+    at = _f64(at)
+    T = at.shape[0]
+    dt = 1.0 / T
+    xt = landmarkDirectEvolutionEuler(x0, at, sigma_diff)
+    obj = 0.
+    for t in range(T):
+        ra = applyK(xt[t], xt[t], at[t], sigma_diff)
+        obj += regweight * dt * (at[t] * ra).sum()
+    dterm, obj0 = _data_term(kind, (xt[-1], faces), target, sigma_dist)
+    return obj + (obj0 + dterm) / sigma_error ** 2, xt
+
+
+def matching_gradient(x0, faces, at, target, sigma_diff, sigma_dist, kind='varifold',
+                      regweight=1., sigma_error=1., euclidean=True):
+    """grad['at'] of the plain surface-matching objective as BFGS sees it.
+
+    px1 = -endPointGradient/sigma_err^2 (base/surfaceMatching.py:702-721,
+    :1077), covector by the lddmm adjoint (:783-818), dat_t = 2 r a_t - p_{t+1}
+    (:879), K(x_t) dat_t for the Euclidean gradient (:890-891), / T (:1086).
+    """
+    at = _f64(at)
+    T = at.shape[0]
+    xt = landmarkDirectEvolutionEuler(x0, at, sigma_diff)
+    px1 = -_data_grad(kind, (xt[-1], faces), target, sigma_dist) / sigma_error ** 2
+    dat, xt, pxt = landmarkHamiltonianGradient(x0, at, px1, sigma_diff, regweight)
+    if euclidean:
+        for t in range(T):
+            dat[t] = applyK(xt[t], xt[t], dat[t], sigma_diff)
+    return dat / T
+
+
+# ---------------------------------------------------------------------------
+# Geodesic shooting (second order) and its adjoint
+# ---------------------------------------------------------------------------
+
+def landmarkEPDiff(T, x0, a0, sigma, KP=-1):
+    """base/pointEvolution.py:320-379 (simple output)."""
+    x0, a0 = _f64(x0), _f64(a0)
+    dt = 1.0 / T
+    at = np.zeros((T,) + x0.shape)
+    xt = np.zeros((T + 1,) + x0.shape)
+    xt[0] = x0
+    at[0] = a0
+    for k in range(T):
+        z, a = xt[k], at[k]
+        xt[k + 1] = z + dt * applyK(z, z, a, sigma, KP)
+        if k < T - 1:
+            at[k + 1] = a - dt * applyDiffKT(z, z, a, a, sigma, KP=KP)
+    return xt, at
+
+
+def secondOrderEvolution(x0, a0, sigma, timeStep, KP=-1):
+    """base/pointEvolution.py:385-472 (no spline, no affine, no point set)."""
+    x0, a0 = _f64(x0), _f64(a0)
+    T = int(np.round(1 / timeStep))
+    xt = np.zeros((T + 1,) + x0.shape)
+    at = np.zeros((T + 1,) + x0.shape)
+    xt[0], at[0] = x0, a0
+    for k in range(T):
+        x, a = xt[k], at[k]
+        zx = applyK(x, x, a, sigma, KP)
+        za = -applyDiffKT(x, x, a, a, sigma, KP=KP)
+        xt[k + 1] = x + timeStep * zx
+        at[k + 1] = a + timeStep * za
+    return xt, at
+
+
+def secondOrderHamiltonian(x, a, px, pa, sigma, KP=-1):
+    """base/pointEvolution.py:475-486 (no spline, no affine)."""
+    Ht = (px * applyK(x, x, a, sigma, KP)).sum()
+    Ht += (pa * (-applyDiffKT(x, x, a, a, sigma, KP=KP))).sum()
+    return Ht
+
+
+def secondOrderCovector(x0, a0, px1, pa1, sigma, timeStep, KP=-1):
+    """base/pointEvolution.py:489-552 (single target, no affine)."""
+    T = int(np.round(1 / timeStep))
+    xt, at = secondOrderEvolution(x0, a0, sigma, timeStep, KP)
+    pxt = np.zeros(xt.shape)
+    pat = np.zeros(xt.shape)
+    pxt[T], pat[T] = px1, pa1
+    for t in range(T):
+        px, pa = pxt[T - t], pat[T - t]
+        x, a = xt[T - t - 1], at[T - t - 1]
+        zpx = applyDiffKT(x, x, px, a, sigma, sym=True, KP=KP) - applyDDiffK(x, a, a, pa, sigma, KP=KP)
+        # Kernel.applyDiffK1and2 -> applykdiff1and2, which forces KP = -1
+        zpa = applyK(x, x, px, sigma, KP) - applyDiffK(x, pa, a, sigma, KP=-1, option='1and2')
+        pxt[T - t - 1] = px + timeStep * zpx
+        pat[T - t - 1] = pa + timeStep * zpa
+    return pxt, pat, xt, at
+
+
+def secondOrderGradient(x0, a0, px1, pa1, sigma, timeStep, KP=-1):
+    """da0 = K(x0) a0 - pa_0; base/pointEvolution.py:555-618 (no affine)."""
+    pxt, pat, xt, at = secondOrderCovector(x0, a0, px1, pa1, sigma, timeStep, KP)
+    da0 = applyK(x0, x0, a0, sigma, KP) - pat[0]
+    return da0, xt, at, pxt, pat

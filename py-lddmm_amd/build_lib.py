@@ -1,0 +1,88 @@
+"""Builds liblddmm_amd.so (HIP, gfx950) in-tree.
+
+    python py-lddmm_amd/build_lib.py [--jobs N] [--save-temps]
+
+Objects go to py-lddmm_amd/_build/, the shared library to
+py-lddmm_amd/lddmm_amd/liblddmm_amd.so (loaded by lddmm_amd._backend).
+"""
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+OUT = os.path.join(HERE, "lddmm_amd", "liblddmm_amd.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
+    "-mcode-object-version=5",
+    # packed FP32 VALU ops stay disabled (precision notes)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+
+def sources():
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+
+
+def compile_one(src, extra):
+    obj = os.path.join(BUILD, os.path.basename(src).replace(".hip", ".o"))
+    cmd = [HIPCC, *FLAGS, *extra, "-c", src, "-o", obj]
+    t0 = time.time()
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    return src, obj, r, time.time() - t0
+
+
+def build(jobs=None, save_temps=False, verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    extra = []
+    if save_temps:
+        extra += ["--save-temps"]
+    srcs = sources()
+    objs = []
+    jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
+    failed = False
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        cwd = os.getcwd()
+        if save_temps:
+            os.chdir(BUILD)
+        try:
+            for src, obj, r, dt in ex.map(lambda s: compile_one(s, extra), srcs):
+                if verbose:
+                    print(f"[build] {os.path.basename(src)} {dt:.1f}s", flush=True)
+                if r.returncode != 0:
+                    failed = True
+                    sys.stderr.write(r.stdout + r.stderr)
+                elif verbose and r.stderr.strip():
+                    sys.stderr.write("\n".join(l for l in r.stderr.splitlines()
+                                               if "packed-fp32-ops" not in l) + "\n")
+                objs.append(obj)
+        finally:
+            os.chdir(cwd)
+    if failed:
+        raise RuntimeError("HIP compilation failed")
+    tmp = OUT + ".tmp"
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("link failed")
+    os.replace(tmp, OUT)
+    if verbose:
+        print(f"[build] wrote {OUT}", flush=True)
+    return OUT
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=None)
+    ap.add_argument("--save-temps", action="store_true")
+    a = ap.parse_args()
+    build(a.jobs, a.save_temps)

@@ -1,0 +1,149 @@
+// Shared host/device helpers of the lddmm_amd HIP library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <cstdio>
+#include <cmath>
+#include <string>
+
+#include "../../include/lddmm_amd.h"
+
+namespace lddmm {
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+void set_error(const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define LDDMM_HIP(call)                                                        \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) return ::lddmm::hip_fail(e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define LDDMM_CHECK_LAUNCH() LDDMM_HIP(hipGetLastError())
+
+#define LDDMM_REQUIRE(cond, ...)                                               \
+    do {                                                                       \
+        if (!(cond)) { ::lddmm::set_error(__VA_ARGS__); return LDDMM_EINVAL; } \
+    } while (0)
+
+// Runs `expr` (an int-returning call) and propagates a non-zero result.
+#define LDDMM_TRY(expr)                                                        \
+    do { int rc_ = (expr); if (rc_ != LDDMM_OK) return rc_; } while (0)
+
+// ---------------------------------------------------------------------------
+// Device facts and workspace
+// ---------------------------------------------------------------------------
+struct DeviceInfo {
+    int device = -1;
+    int num_cu = 0;
+};
+int device_info(DeviceInfo* out);
+
+// Grow-only per-device scratch.  The returned pointer stays valid until the
+// next call that needs more bytes on the same device; growth synchronises
+// `stream` before the old block is released.
+int workspace(size_t bytes, hipStream_t stream, void** ptr);
+
+// Bump allocator over one workspace request (256-byte aligned slices).
+struct Carve {
+    size_t off = 0;
+    size_t take(size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~size_t(255);
+        return o;
+    }
+};
+
+// Occupancy (blocks per CU) of a kernel, cached by function address.
+int blocks_per_cu(const void* func, int block, size_t dyn_lds);
+
+// ---------------------------------------------------------------------------
+// Kernel description -> per-precision scale constants
+// ---------------------------------------------------------------------------
+constexpr double kLog2e = 1.4426950408889634;
+
+template <typename T>
+struct ScaleSet {
+    int ns;                         // number of scales
+    T beta[LDDMM_MAX_SCALES];       // exp2 coefficient on |x - y|^2 (centred units)
+    T wK[LDDMM_MAX_SCALES];         // s^KP / W
+    T wD[LDDMM_MAX_SCALES];         // s^(KP-2) / W
+    T wDD[LDDMM_MAX_SCALES];        // s^(KP-4) / W
+    double cs;                      // coordinate factor (single scale: K = exp2(-|u_i - u_j|^2), else 1)
+};
+
+int validate_kernel(const lddmm_kernel* kd);
+
+// Constant factors a single-scale op applies at the end, in float64.
+struct SingleScaleConsts {
+    double sigma, cs;
+    double diff;      // -1 / (s^2 cs): derivative sums (diffKT, diffK, kdiffmat)
+    double ddiff;     //  1 / s^2: second-derivative sums
+    double gamma;     //  1 / (s^2 cs^2): (d.q) d term in DDiffK
+};
+
+template <typename T>
+ScaleSet<T> make_scales(const lddmm_kernel* kd, SingleScaleConsts* single) {
+    ScaleSet<T> s{};
+    s.ns = kd->nscale;
+    double W = 0;
+    for (int i = 0; i < kd->nscale; ++i) W += std::pow(kd->sigma[i], kd->kp);
+    for (int i = 0; i < kd->nscale; ++i) {
+        double sg = kd->sigma[i];
+        s.beta[i] = T(kLog2e / (2.0 * sg * sg));
+        s.wK[i] = T(std::pow(sg, kd->kp) / W);
+        s.wD[i] = T(std::pow(sg, kd->kp - 2) / W);
+        s.wDD[i] = T(std::pow(sg, kd->kp - 4) / W);
+    }
+    double sg = kd->sigma[0];
+    double cs = std::sqrt(kLog2e / 2.0) / sg;
+    s.cs = kd->nscale == 1 ? cs : 1.0;
+    if (single) {
+        single->sigma = sg;
+        single->cs = cs;
+        single->diff = -1.0 / (sg * sg * cs);
+        single->ddiff = 1.0 / (sg * sg);
+        single->gamma = 1.0 / (sg * sg * cs * cs);
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Device math
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ double fast_exp2(double x) { return exp2(x); }
+
+// Knuth two-sum: (s, e) with s + e == a + b exactly.
+template <typename T>
+__device__ __forceinline__ void two_sum_acc(T& hi, T& lo, T b) {
+    T s = hi + b;
+    T bp = s - hi;
+    T e = (hi - (s - bp)) + (b - bp);
+    hi = s;
+    lo += e;
+}
+
+template <typename T> struct Vec4 { T x, y, z, w; };
+
+template <typename T>
+__device__ __forceinline__ Vec4<T> lds4(const T* p);
+
+template <>
+__device__ __forceinline__ Vec4<float> lds4<float>(const float* p) {
+    float4 v = *reinterpret_cast<const float4*>(p);
+    return {v.x, v.y, v.z, v.w};
+}
+template <>
+__device__ __forceinline__ Vec4<double> lds4<double>(const double* p) {
+    double2 a = *reinterpret_cast<const double2*>(p);
+    double2 b = *reinterpret_cast<const double2*>(p + 2);
+    return {a.x, a.y, b.x, b.y};
+}
+
+}  // namespace lddmm

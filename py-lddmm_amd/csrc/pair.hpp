@@ -1,0 +1,284 @@
+// Tiled N-body pair-sum machinery shared by every Gaussian kernel sum.
+//
+// One workgroup (kBlock lanes = 4 wave64) owns kBlock*R consecutive targets
+// and one chunk of sources (grid.y).  Sources are staged through LDS in tiles
+// of kTile records: each lane loads one source from HBM (float64, coalesced
+// AoS), centres / scales / converts it and stores one LDS record; the next
+// tile's raw data is fetched into registers while the current tile is
+// consumed.  The inner loop reads each record with a wave-uniform address
+// (LDS broadcast) and applies the op's pair function to R targets held in
+// registers, so each LDS read feeds R pairs.
+//
+// Precision: T = float accumulates each tile in plain float32 and folds tile
+// sums into a (hi, lo) two-sum accumulator; chunk partials are combined by a
+// second two-sum pass in k_reduce.  T = double accumulates in float64.
+#pragma once
+
+#include "common.hpp"
+
+namespace lddmm {
+
+constexpr int kBlock = 256;   // lanes per workgroup
+constexpr int kTile = 256;    // sources per LDS tile (one per lane)
+
+template <typename T>
+struct Partials {
+    T* hi;       // [nchunk][nch][n]
+    T* lo;       // float: compensation terms, same layout; double: unused
+    int n;
+};
+
+template <typename T>
+constexpr bool kCompensated = sizeof(T) == 4;
+
+// Default chunk -> source range mapping: contiguous chunks of `chunk` sources.
+struct ContiguousChunks {
+    template <class A>
+    __device__ static void range(const A& a, int chunk_id, int chunk, int m, int& j0, int& j1) {
+        j0 = chunk_id * chunk;
+        j1 = min(m, j0 + chunk);
+    }
+};
+
+template <typename T, class OP, int R>
+__global__ void __launch_bounds__(kBlock)
+k_pair(const typename OP::template Args<T> args, int n, int m, int chunk, Partials<T> part) {
+    constexpr int NCH = OP::NCH;
+    constexpr int REC = OP::REC;
+    __shared__ __attribute__((aligned(16))) T lds[kTile * REC];
+
+    const int tid = threadIdx.x;
+    const int ibase = blockIdx.x * (kBlock * R) + tid;
+
+    typename OP::template Tgt<T> tg[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int i = ibase + r * kBlock;
+        OP::load_target(args, i < n ? i : n - 1, tg[r]);
+    }
+
+    T hi[R][NCH], lo[R][NCH];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) { hi[r][c] = T(0); lo[r][c] = T(0); }
+
+    int j0, j1;
+    OP::Chunks::range(args, blockIdx.y, chunk, m, j0, j1);
+
+    typename OP::Raw raw;
+    OP::load_raw(args, j0 + tid < j1 ? j0 + tid : -1, raw);
+
+    for (int jt = j0; jt < j1; jt += kTile) {
+        __syncthreads();                       // previous tile fully consumed
+        OP::store_rec(args, raw, &lds[tid * REC]);
+        __syncthreads();
+        {
+            int jn = jt + kTile + tid;         // prefetch the next tile
+            OP::load_raw(args, jn < j1 ? jn : -1, raw);
+        }
+        T acc[R][NCH];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) acc[r][c] = T(0);
+
+#pragma unroll 4
+        for (int t = 0; t < kTile; ++t) {
+            const T* rec = &lds[t * REC];
+#pragma unroll
+            for (int r = 0; r < R; ++r) OP::pair(args, tg[r], rec, acc[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                if constexpr (kCompensated<T>) two_sum_acc(hi[r][c], lo[r][c], acc[r][c]);
+                else hi[r][c] += acc[r][c];
+            }
+    }
+
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int i = ibase + r * kBlock;
+        if (i < n) {
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) {
+                size_t o = (size_t(blockIdx.y) * NCH + c) * size_t(n) + i;
+                part.hi[o] = hi[r][c];
+                if constexpr (kCompensated<T>) part.lo[o] = lo[r][c];
+            }
+        }
+    }
+}
+
+// Sum of chunk partials for (i, c) in float64 (two-sum in float32 first).
+template <typename T>
+__device__ __forceinline__ double sum_chunks(const Partials<T>& p, int nchunk, int nch, int i, int c,
+                                             const double* chunk_w) {
+    if constexpr (kCompensated<T>) {
+        float h = 0.f, l = 0.f;
+        for (int k = 0; k < nchunk; ++k) {
+            size_t o = (size_t(k) * nch + c) * size_t(p.n) + i;
+            float w = chunk_w ? float(chunk_w[k]) : 1.f;
+            two_sum_acc(h, l, p.hi[o] * w);
+            l += p.lo[o] * w;
+        }
+        return double(h) + double(l);
+    } else {
+        double s = 0.0;
+        for (int k = 0; k < nchunk; ++k) {
+            size_t o = (size_t(k) * nch + c) * size_t(p.n) + i;
+            s += chunk_w ? p.hi[o] * chunk_w[k] : p.hi[o];
+        }
+        return s;
+    }
+}
+
+struct ChanScale { double s[8]; };
+
+// out[i*ldo + col0 + c] = (base ? base[...] + alpha*v : (accumulate ? out + v : v)),
+// v = scale_c * sum over chunks of channel c; partials hold `nch` channels per
+// chunk, of which the first `nout` are written.
+template <typename T>
+__global__ void k_reduce(Partials<T> part, int nchunk, int nch, int nout, int n, ChanScale sc,
+                         double* out, int ldo, int col0, int accumulate, const double* base,
+                         double alpha) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < nout; ++c) {
+        double v = sum_chunks(part, nchunk, nch, i, c, nullptr) * sc.s[c];
+        size_t o = size_t(i) * ldo + col0 + c;
+        if (base) out[o] = base[o] + alpha * v;
+        else if (accumulate) out[o] += v;
+        else out[o] = v;
+    }
+}
+
+__global__ void k_fill(double* p, size_t count, double v);
+
+// Midpoint of the bounding box of y[n,dim] and x[m,dim] -> c0[0..2] (c0[d] = 0 for d >= dim).
+__global__ void k_center(const double* y, int n, const double* x, int m, int dim, double* c0);
+
+int launch_center(const double* y, int n, const double* x, int m, int dim, double* c0, hipStream_t st);
+int launch_fill(double* p, size_t count, double v, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// Launch planning: pick the number of source chunks so that the grid fills
+// the GPU in whole rounds (blocks run in rounds of num_cu * occupancy).
+// ---------------------------------------------------------------------------
+struct PairPlan {
+    int bx = 0;       // target blocks
+    int nchunk = 0;   // source chunks (grid.y)
+    int chunk = 0;    // sources per chunk (multiple of kTile)
+};
+
+PairPlan plan_pair(int n, int m, int R, int num_cu, int occ_blocks);
+
+template <typename T, class OP, int R>
+int run_pair(const typename OP::template Args<T>& args, int n, int m, const PairPlan& plan,
+             Partials<T> part, hipStream_t st) {
+    dim3 grid(plan.bx, plan.nchunk);
+    hipLaunchKernelGGL((k_pair<T, OP, R>), grid, dim3(kBlock), 0, st, args, n, m, plan.chunk, part);
+    LDDMM_CHECK_LAUNCH();
+    return LDDMM_OK;
+}
+
+template <typename T, class OP, int R>
+int plan_for(int n, int m, PairPlan* plan) {
+    DeviceInfo di;
+    LDDMM_TRY(device_info(&di));
+    int occ = blocks_per_cu(reinterpret_cast<const void*>(&k_pair<T, OP, R>), kBlock, 0);
+    *plan = plan_pair(n, m, R, di.num_cu, occ);
+    return LDDMM_OK;
+}
+
+template <typename T>
+size_t partial_bytes(const PairPlan& p, int nch, int n) {
+    size_t e = size_t(p.nchunk) * nch * size_t(n) * sizeof(T);
+    return kCompensated<T> ? 2 * e : e;
+}
+
+template <typename T>
+Partials<T> carve_partials(char* base, Carve& cv, const PairPlan& p, int nch, int n) {
+    size_t e = size_t(p.nchunk) * nch * size_t(n) * sizeof(T);
+    Partials<T> r;
+    r.hi = reinterpret_cast<T*>(base + cv.take(e));
+    r.lo = kCompensated<T> ? reinterpret_cast<T*>(base + cv.take(e)) : nullptr;
+    r.n = n;
+    return r;
+}
+
+template <typename T>
+int run_reduce(const Partials<T>& part, int nchunk, int nch, int nout, int n, const ChanScale& sc,
+               double* out, int ldo, int col0, int accumulate, const double* base, double alpha,
+               hipStream_t st) {
+    int nb = (n + 255) / 256;
+    hipLaunchKernelGGL((k_reduce<T>), dim3(nb), dim3(256), 0, st, part, nchunk, nch, nout, n, sc, out,
+                       ldo, col0, accumulate, base, alpha);
+    LDDMM_CHECK_LAUNCH();
+    return LDDMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Gaussian factors for one pair given r2 (centred units; single scale: scaled
+// so that K = exp2(-r2)).
+// ---------------------------------------------------------------------------
+template <typename T, bool MS>
+__device__ __forceinline__ T gK(const ScaleSet<T>& s, T r2) {
+    if constexpr (!MS) {
+        return fast_exp2(-r2);
+    } else {
+        T g = T(0);
+        for (int q = 0; q < s.ns; ++q) g += s.wK[q] * fast_exp2(-s.beta[q] * r2);
+        return g;
+    }
+}
+
+template <typename T, bool MS>
+__device__ __forceinline__ T gD(const ScaleSet<T>& s, T r2) {
+    if constexpr (!MS) {
+        return fast_exp2(-r2);
+    } else {
+        T g = T(0);
+        for (int q = 0; q < s.ns; ++q) g += s.wD[q] * fast_exp2(-s.beta[q] * r2);
+        return g;
+    }
+}
+
+// Both weightings from one set of exponentials.
+template <typename T, bool MS>
+__device__ __forceinline__ void gKD(const ScaleSet<T>& s, T r2, T& k, T& d) {
+    if constexpr (!MS) {
+        k = d = fast_exp2(-r2);
+    } else {
+        k = d = T(0);
+        for (int q = 0; q < s.ns; ++q) {
+            T e = fast_exp2(-s.beta[q] * r2);
+            k += s.wK[q] * e;
+            d += s.wD[q] * e;
+        }
+    }
+}
+
+template <typename T, bool MS>
+__device__ __forceinline__ void gDDD(const ScaleSet<T>& s, T r2, T& dd, T& d) {
+    if constexpr (!MS) {
+        dd = d = fast_exp2(-r2);
+    } else {
+        dd = d = T(0);
+        for (int q = 0; q < s.ns; ++q) {
+            T e = fast_exp2(-s.beta[q] * r2);
+            dd += s.wDD[q] * e;
+            d += s.wD[q] * e;
+        }
+    }
+}
+
+// Centred + scaled coordinate d of point i in a [*, dim] float64 array.
+template <typename T>
+__device__ __forceinline__ T coord(const double* p, int i, int dim, int d, const double* c0, double cs) {
+    return d < dim ? T((p[size_t(i) * dim + d] - c0[d]) * cs) : T(0);
+}
+
+}  // namespace lddmm

@@ -1,0 +1,161 @@
+// Library runtime: error text, per-device workspace, occupancy cache,
+// host-side adjacency builder.
+#include "common.hpp"
+
+#include <cmath>
+#include <cstdarg>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+namespace lddmm {
+
+namespace {
+thread_local std::string g_last_error;
+
+struct Workspace {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_ws_mutex;
+std::unordered_map<int, Workspace> g_ws;
+
+std::mutex g_info_mutex;
+std::unordered_map<int, DeviceInfo> g_info;
+
+std::mutex g_occ_mutex;
+std::unordered_map<const void*, int> g_occ;
+}  // namespace
+
+void set_error(const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+    set_error("HIP error %d (%s) in %s at %s:%d", int(e), hipGetErrorString(e), what, file, line);
+    return LDDMM_EHIP;
+}
+
+int device_info(DeviceInfo* out) {
+    int dev = 0;
+    LDDMM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(g_info_mutex);
+    auto it = g_info.find(dev);
+    if (it == g_info.end()) {
+        DeviceInfo d;
+        d.device = dev;
+        LDDMM_HIP(hipDeviceGetAttribute(&d.num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+        it = g_info.emplace(dev, d).first;
+    }
+    *out = it->second;
+    return LDDMM_OK;
+}
+
+int workspace(size_t bytes, hipStream_t stream, void** ptr) {
+    int dev = 0;
+    LDDMM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(g_ws_mutex);
+    Workspace& w = g_ws[dev];
+    if (bytes > w.bytes) {
+        if (w.ptr) {
+            // the previous block may still be read by work queued on `stream`
+            LDDMM_HIP(hipStreamSynchronize(stream));
+            LDDMM_HIP(hipFree(w.ptr));
+            w.ptr = nullptr;
+            w.bytes = 0;
+        }
+        size_t grow = bytes + bytes / 4;
+        grow = (grow + (size_t(1) << 20) - 1) & ~((size_t(1) << 20) - 1);
+        LDDMM_HIP(hipMalloc(&w.ptr, grow));
+        w.bytes = grow;
+    }
+    *ptr = w.ptr;
+    return LDDMM_OK;
+}
+
+int release_workspace() {
+    int dev = 0;
+    LDDMM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(g_ws_mutex);
+    auto it = g_ws.find(dev);
+    if (it != g_ws.end() && it->second.ptr) {
+        LDDMM_HIP(hipDeviceSynchronize());
+        LDDMM_HIP(hipFree(it->second.ptr));
+        g_ws.erase(it);
+    }
+    return LDDMM_OK;
+}
+
+int blocks_per_cu(const void* func, int block, size_t dyn_lds) {
+    std::lock_guard<std::mutex> g(g_occ_mutex);
+    auto it = g_occ.find(func);
+    if (it != g_occ.end()) return it->second;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, func, block, dyn_lds) != hipSuccess || nb < 1)
+        nb = 1;
+    g_occ[func] = nb;
+    return nb;
+}
+
+int validate_kernel(const lddmm_kernel* kd) {
+    LDDMM_REQUIRE(kd != nullptr, "null kernel description");
+    if (kd->kind != LDDMM_KERNEL_GAUSS) {
+        set_error("kernel kind %d not supported (only gauss)", kd->kind);
+        return LDDMM_EUNSUPPORTED;
+    }
+    LDDMM_REQUIRE(kd->nscale >= 1 && kd->nscale <= LDDMM_MAX_SCALES,
+                  "nscale %d outside [1, %d]", kd->nscale, LDDMM_MAX_SCALES);
+    for (int i = 0; i < kd->nscale; ++i)
+        LDDMM_REQUIRE(std::isfinite(kd->sigma[i]) && kd->sigma[i] > 0, "sigma[%d] must be > 0", i);
+    LDDMM_REQUIRE(kd->prec == LDDMM_PREC_F64 || kd->prec == LDDMM_PREC_F32C, "bad precision %d", kd->prec);
+    return LDDMM_OK;
+}
+
+}  // namespace lddmm
+
+extern "C" {
+
+const char* lddmm_last_error(void) { return lddmm::g_last_error.c_str(); }
+
+const char* lddmm_version(void) { return "lddmm_amd 0.1 (gfx950)"; }
+
+int lddmm_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int lddmm_release_workspace(void) { return lddmm::release_workspace(); }
+
+int lddmm_surface_adjacency(const int32_t* faces, int64_t nf, int64_t nv, int32_t* adj_ptr,
+                            int32_t* adj_idx) {
+    LDDMM_REQUIRE(nf >= 0 && nv >= 0, "negative sizes");
+    LDDMM_REQUIRE(3 * nf < (int64_t(1) << 31), "too many faces for int32 adjacency");
+    LDDMM_REQUIRE(nf == 0 || (faces && adj_idx), "null faces / adj_idx");
+    LDDMM_REQUIRE(adj_ptr != nullptr, "null adj_ptr");
+    std::vector<int32_t> cnt(size_t(nv) + 1, 0);
+    for (int64_t f = 0; f < nf; ++f)
+        for (int c = 0; c < 3; ++c) {
+            int32_t v = faces[3 * f + c];
+            LDDMM_REQUIRE(v >= 0 && v < nv, "face %lld references vertex %d outside [0, %lld)",
+                          (long long)f, v, (long long)nv);
+            cnt[size_t(v) + 1]++;
+        }
+    for (int64_t v = 0; v < nv; ++v) cnt[size_t(v) + 1] += cnt[size_t(v)];
+    for (int64_t v = 0; v <= nv; ++v) adj_ptr[v] = cnt[size_t(v)];
+    // faces are visited in increasing order, so each vertex's list is sorted
+    // by (face, corner): the gather sums in the same order on every call.
+    for (int64_t f = 0; f < nf; ++f)
+        for (int c = 0; c < 3; ++c) {
+            int32_t v = faces[3 * f + c];
+            adj_idx[cnt[size_t(v)]++] = int32_t(3 * f + c);
+        }
+    return LDDMM_OK;
+}
+
+}  // extern "C"

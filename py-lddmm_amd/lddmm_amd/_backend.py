@@ -1,0 +1,152 @@
+"""ctypes binding of liblddmm_amd.so (the HIP C ABI declared in include/lddmm_amd.h).
+
+torch supplies device memory and the current HIP stream; every compute call
+goes through the C ABI.  There is no CPU fallback: importing this module
+without the built library, or calling a kernel without a GPU, raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+import torch  # loaded first so the library binds to torch's HIP runtime
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblddmm_amd.so")
+
+MAX_SCALES = 8
+KERNEL_GAUSS = 0
+PREC_F64 = 0
+PREC_F32C = 1
+OK, EINVAL, EUNSUPPORTED, EHIP = 0, -1, -2, -3
+
+DIFFKT_PLAIN, DIFFKT_SYM, DIFFKT_LDDMM = 0, 1, 2
+DIFFK_1, DIFFK_2, DIFFK_1AND2 = 0, 1, 2
+DDIFFK_11, DDIFFK_12, DDIFFK_11AND12 = 0, 1, 2
+
+
+class LddmmKernel(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("nscale", ctypes.c_int32),
+                ("sigma", ctypes.c_double * MAX_SCALES), ("kp", ctypes.c_double),
+                ("prec", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class LddmmSurface(ctypes.Structure):
+    _fields_ = [("vertices", ctypes.c_void_p), ("nv", ctypes.c_int64),
+                ("faces", ctypes.c_void_p), ("nf", ctypes.c_int64),
+                ("vweights", ctypes.c_void_p),
+                ("adj_ptr", ctypes.c_void_p), ("adj_idx", ctypes.c_void_p)]
+
+
+class LddmmError(RuntimeError):
+    pass
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+_D = ctypes.c_double
+_KP = ctypes.POINTER(LddmmKernel)
+_SP = ctypes.POINTER(LddmmSurface)
+
+# name -> argtypes (restype is int for every compute entry point)
+SIGNATURES = {
+    "lddmm_applyK": [_KP, _P, _I64, _P, _I64, _I32, _P, _I32, _P, _P],
+    "lddmm_applyDiffKT": [_KP, _P, _I64, _P, _I64, _I32, _P, _P, _I32, _I32, _D, _P, _P],
+    "lddmm_applyktensor": [_KP, _P, _I64, _P, _I64, _I32, _P, _P, _P, _P, _I32, _P, _P],
+    "lddmm_applykmat": [_KP, _P, _I64, _P, _I64, _I32, _P, _I32, _P, _P],
+    "lddmm_applykdiffmat": [_KP, _P, _I64, _P, _I64, _I32, _P, _P, _P],
+    "lddmm_applyDiffK": [_KP, _P, _I64, _I32, _P, _P, _I32, _I32, _P, _P],
+    "lddmm_applyDDiffK": [_KP, _P, _I64, _I32, _P, _P, _I32, _P, _I32, _P, _P],
+    "lddmm_surface_adjacency": [_P, _I64, _I64, _P, _P],
+    "lddmm_surface_varifold": [_KP, _SP, _SP, _P, _P, _P],
+    "lddmm_surface_features": [_SP, _P, _P, _P, _P],
+    "lddmm_surface_face_gather": [_SP, _P, _P, _P, _P],
+    "lddmm_shoot_forward": [_KP, _P, _I32, _I64, _I32, _P, _D, _P, _P],
+    "lddmm_shoot_adjoint": [_KP, _P, _P, _I32, _I64, _I32, _P, _D, _I32, _P, _P, _P],
+    "lddmm_epdiff_forward": [_KP, _I32, _I64, _I32, _P, _P, _P],
+    "lddmm_epdiff_adjoint": [_KP, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P],
+    "lddmm_release_workspace": [],
+    "lddmm_device_count": [],
+}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"lddmm_amd: {LIB_PATH} is missing; build it with "
+            "`python py-lddmm_amd/build_lib.py` (hipcc, gfx950)")
+    lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+    lib.lddmm_last_error.restype = ctypes.c_char_p
+    lib.lddmm_last_error.argtypes = []
+    lib.lddmm_version.restype = ctypes.c_char_p
+    lib.lddmm_version.argtypes = []
+    for name, args in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    return lib
+
+
+LIB = _load()
+
+
+def check(rc, what=""):
+    if rc != OK:
+        msg = LIB.lddmm_last_error().decode(errors="replace")
+        kind = {EINVAL: "invalid argument", EUNSUPPORTED: "unsupported",
+                EHIP: "HIP error"}.get(rc, f"error {rc}")
+        if rc == EUNSUPPORTED:
+            raise NotImplementedError(f"lddmm_amd {what}: {kind}: {msg}")
+        raise LddmmError(f"lddmm_amd {what}: {kind}: {msg}")
+
+
+def device():
+    if not torch.cuda.is_available():
+        raise LddmmError("lddmm_amd needs a HIP GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_handle():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def to_dev(a, dtype=torch.float64):
+    """Contiguous device tensor of `dtype` (numpy arrays are copied host->device)."""
+    dev = device()
+    if isinstance(a, torch.Tensor):
+        return a.to(device=dev, dtype=dtype).contiguous()
+    arr = np.ascontiguousarray(np.asarray(a), dtype=torch.empty((), dtype=dtype).numpy().dtype)
+    return torch.from_numpy(arr).to(dev)
+
+
+def kernel_desc(sigma, kp=-1.0, prec='float64', kind=KERNEL_GAUSS):
+    s = np.atleast_1d(np.asarray(sigma, dtype=np.float64)).ravel()
+    if s.size < 1 or s.size > MAX_SCALES:
+        raise ValueError(f"lddmm_amd supports 1..{MAX_SCALES} scales (got {s.size})")
+    kd = LddmmKernel()
+    kd.kind = kind
+    kd.nscale = int(s.size)
+    for i, v in enumerate(s):
+        kd.sigma[i] = float(v)
+    kd.kp = float(kp)
+    kd.prec = precision_code(prec)
+    return kd
+
+
+def precision_code(prec):
+    if prec in ('float64', 'double', np.float64, torch.float64, PREC_F64):
+        return PREC_F64
+    if prec in ('float32', 'float', np.float32, torch.float32, PREC_F32C):
+        return PREC_F32C
+    raise ValueError(f"unknown precision {prec!r} (use 'float64' or 'float32')")
+
+
+def kernel_kind(name):
+    if name is None or 'gauss' not in str(name):
+        raise NotImplementedError(
+            f"lddmm_amd builds the Gaussian kernel family only (got name={name!r})")
+    return KERNEL_GAUSS
