@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Kernel-sum benchmark of the py-lddmm hot path on one MI355X.
+
+Workload (the reference's kernel-sum microbenchmark): applyK and applyDiffKT
+(plain) with N = M = 50,000 points drawn uniform in [0, 50]^3 (seeded), weights
+a, p ~ N(0, 1) of dimension 3, Gaussian sigma in {2.5, 5, 10}, each sigma
+single-scale.  One step = applyK + applyDiffKT for the three sigmas = 6 sums of
+N*M = 2.5e9 pair evaluations, in the compensated FP32 form.  Inputs are device
+resident before the timed region; calls go through the C ABI on the current
+stream.
+
+Prints one JSON line (the driver's contract) with `roofline` (dominant kernel:
+the applyDiffKT pair kernel, timed by HIP events around each launch on its own
+stream during the timed region) and `cpu_baseline` (the FP64 OpenMP
+restatement of the reference's numba loops on a bounded row sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "py-lddmm_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "Gaussian pair-evals/s; s per surface registration at matched energy vs plain"
+SIGMAS = (2.5, 5.0, 10.0)
+# algorithmic work per pair (FMA = 2 FLOP) and the MI355X published FP32 peak
+FLOP_PER_PAIR = {"applyK": 15, "applyDiffKT": 22}
+PEAK_FP32_TFLOPS = 157.3
+PEAK_EXP_PER_S = 256 * 4 * 16 / 4 * 2.4e9   # quarter-rate v_exp_f32 (assumed; see profiles/)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(y, x, a, p, target_s=12.0):
+    """FP64 OpenMP restatement timed on a row sample of the same workload."""
+    import numpy as np
+    from oracle import cpu
+    n = y.shape[0]
+    rows = 256
+    t0 = time.perf_counter()
+    cpu.applyK(y[:rows], x, a, [SIGMAS[0]])
+    cpu.applyDiffKT(y[:rows], x, p[:rows], a, [SIGMAS[0]])
+    dt = time.perf_counter() - t0
+    rate = 2 * rows * x.shape[0] / dt
+    rows = int(min(n, max(rows, target_s * rate / (2 * x.shape[0] * len(SIGMAS)))))
+    total_pairs = 0
+    t0 = time.perf_counter()
+    for s in SIGMAS:
+        cpu.applyK(y[:rows], x, a, [s])
+        cpu.applyDiffKT(y[:rows], x, p[:rows], a, [s])
+        total_pairs += 2 * rows * x.shape[0]
+    dt = time.perf_counter() - t0
+    return {
+        "value": total_pairs / dt, "unit": "pair-evals/s", "cores": cpu.threads(),
+        "kind": "port",
+        "sample": (f"first {rows} of {n} targets x all {x.shape[0]} sources, applyK + applyDiffKT "
+                   f"(plain) for sigma in {list(SIGMAS)}, float64, {total_pairs:.3g} pairs in {dt:.1f} s"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=50_000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference pass")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the kernel-sum path shards by target rows with no "
+              "communication (see DESIGN.md); multi-GPU runs are not part of this build.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from lddmm_amd import _backend as B
+    from lddmm_amd import synth
+
+    if not torch.cuda.is_available():
+        log("no GPU visible")
+        sys.exit(1)
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    N = M = args.n
+    y_h = synth.uniform_cloud(N, seed=1)
+    x_h = synth.uniform_cloud(M, seed=2)
+    rng = np.random.default_rng(3)
+    a_h = rng.normal(size=(M, 3))
+    p_h = rng.normal(size=(N, 3))
+    y, x, a, p = (torch.tensor(v, device=dev) for v in (y_h, x_h, a_h, p_h))
+    outK = torch.empty((N, 3), dtype=torch.float64, device=dev)
+    outD = torch.empty((N, 3), dtype=torch.float64, device=dev)
+    st = B.stream_handle()
+    kds = {(s, pr): B.kernel_desc([s], -1, pr) for s in SIGMAS for pr in ("float32", "float64")}
+
+    def applyK(s, pr, out):
+        B.check(B.LIB.lddmm_applyK(kds[(s, pr)], B.ptr(y), N, B.ptr(x), M, 3, B.ptr(a), 3,
+                                   B.ptr(out), st), "applyK")
+
+    def applyDiffKT(s, pr, out):
+        B.check(B.LIB.lddmm_applyDiffKT(kds[(s, pr)], B.ptr(y), N, B.ptr(x), M, 3, B.ptr(p),
+                                        B.ptr(a), 3, B.DIFFKT_PLAIN, 1.0, B.ptr(out), st),
+                "applyDiffKT")
+
+    def step(pr="float32"):
+        for s in SIGMAS:
+            applyK(s, pr, outK)
+            applyDiffKT(s, pr, outD)
+
+    log(f"warmup {args.warmup} steps, N=M={N}")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    B.profile_enable(True)
+    B.profile_read()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_each = B.profile_read()
+    B.profile_enable(False)
+    elapsed = t1 - t0
+    pairs_per_step = 2 * len(SIGMAS) * N * M
+    value = args.steps * pairs_per_step / elapsed
+    log(f"timed {args.steps} steps: {elapsed * 1e3 / args.steps:.2f} ms/step, {value:.4g} pair-evals/s")
+
+    # launch order per step: K(s0), D(s0), K(s1), D(s1), K(s2), D(s2)
+    per_step = 2 * len(SIGMAS)
+    kt = [t for i, t in enumerate(ms_each) if (i % per_step) % 2 == 0]
+    dt = [t for i, t in enumerate(ms_each) if (i % per_step) % 2 == 1]
+    avgK = sum(kt) / max(len(kt), 1)
+    avgD = sum(dt) / max(len(dt), 1)
+    achievedD = N * M * FLOP_PER_PAIR["applyDiffKT"] / (avgD * 1e-3) / 1e12
+    achievedK = N * M * FLOP_PER_PAIR["applyK"] / (avgK * 1e-3) / 1e12
+    log(f"pair kernels: applyK {avgK:.3f} ms ({achievedK:.1f} TFLOP/s), "
+        f"applyDiffKT {avgD:.3f} ms ({achievedD:.1f} TFLOP/s), {len(ms_each)} launches")
+
+    extra = {
+        "pair_kernel_ms": {"applyK": round(avgK, 4), "applyDiffKT": round(avgD, 4)},
+        "pair_evals_per_s_kernel": {"applyK": N * M / (avgK * 1e-3), "applyDiffKT": N * M / (avgD * 1e-3)},
+        "exp_per_s": N * M / (avgK * 1e-3),
+    }
+
+    if not args.no_fp64:
+        # FP64 plain path: rate and the FP32 error against it (norm-wise relative)
+        errs = {}
+        ref = torch.empty_like(outK)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in SIGMAS:
+            applyK(s, "float64", ref)
+            applyK(s, "float32", outK)
+            torch.cuda.synchronize()
+            errs[f"applyK_sigma{s}"] = float((outK - ref).norm() / ref.norm())
+            applyDiffKT(s, "float64", ref)
+            applyDiffKT(s, "float32", outD)
+            torch.cuda.synchronize()
+            errs[f"applyDiffKT_sigma{s}"] = float((outD - ref).norm() / ref.norm())
+        B.profile_enable(True)
+        B.profile_read()
+        t0 = time.perf_counter()
+        applyK(SIGMAS[0], "float64", ref)
+        applyDiffKT(SIGMAS[0], "float64", ref)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t0
+        ms64 = B.profile_read()
+        B.profile_enable(False)
+        extra["fp64_pair_evals_per_s"] = 2 * N * M / t64
+        extra["fp64_pair_kernel_ms"] = [round(v, 3) for v in ms64]
+        extra["fp32_rel_err_vs_fp64"] = errs
+        log(f"fp64: {2 * N * M / t64:.4g} pair-evals/s; max fp32 rel err {max(errs.values()):.2e}")
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pair-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded uniform points in [0,50]^3, N(0,1) weights)",
+        "config": {
+            "workload": ("kernel-sum microbenchmark: applyK + applyDiffKT (plain), N=M=50,000 uniform "
+                         "in [0,50]^3, sigma in {2.5,5,10} single-scale, compensated FP32"),
+            "n": N, "m": M, "sigmas": list(SIGMAS), "pairs_per_step": pairs_per_step,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "engine": "FP32 VALU + v_exp_f32 (compute-bound; packed FP32 disabled)",
+            "kernel": "k_pair<float, OpDiffKT<3,false,false>, 2> (applyDiffKT plain)",
+            "achieved": achievedD, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": achievedD / PEAK_FP32_TFLOPS, "traffic": None,
+            "flop_per_pair": FLOP_PER_PAIR["applyDiffKT"], "pairs_per_launch": N * M,
+            "avg_launch_ms": avgD,
+        },
+        "extra": extra,
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        result["cpu_baseline"] = cpu_baseline(y_h, x_h, a_h, p_h)
+        log(f"cpu baseline {result['cpu_baseline']['value']:.4g} pair-evals/s "
+            f"on {result['cpu_baseline']['cores']} threads")
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -78,6 +78,14 @@ int lddmm_device_count(void);
 /* Releases the calling device's workspace. */
 int lddmm_release_workspace(void);
 
+/* Instrumentation (no counterpart in the reference): when enabled, every
+ * pair-sum kernel launch is bracketed by HIP events on its own stream.
+ * lddmm_profile_read waits for the recorded events, writes the duration in ms
+ * of the first `cap` launches (in launch order) to ms_each, the number of
+ * launches since the last read to *launches, and clears the record. */
+int lddmm_profile_enable(int32_t on);
+int lddmm_profile_read(double* ms_each, int64_t cap, int64_t* launches);
+
 /* out[n,k] = sum_s w_s sum_j K_s(y_i, x_j) a[j,:]
  * Replaces ku.applyK / applyK_numba / applyK_pykeops
  * (base/kernelFunctions_util.py:443-466, :499-535), reached from

@@ -189,7 +189,8 @@ def applyDiffK(x, a1, a2, scale, KP=-1, option='1and2'):
     x, a1, a2 = _f64(x), _as2d(a1), _as2d(a2)
     s, sKP, W = _scales(scale, KP)
     sKP1 = s ** (KP - 1)
-    res = np.zeros(x.shape)
+    # the reference allocates x.shape, i.e. a2 has d columns; any width is accepted here
+    res = np.zeros((x.shape[0], a2.shape[1]))
     for si, sc in enumerate(s):
         xs = x / sc
         for i0, i1 in _row_blocks(x.shape[0], x.shape[0]):

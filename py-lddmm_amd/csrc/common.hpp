@@ -59,6 +59,11 @@ struct Carve {
     }
 };
 
+// Pair-kernel timing hooks (enabled by lddmm_profile_enable).
+bool profiling();
+int profile_begin(hipStream_t st);
+int profile_end(hipStream_t st);
+
 // Occupancy (blocks per CU) of a kernel, cached by function address.
 int blocks_per_cu(const void* func, int block, size_t dyn_lds);
 

@@ -1,0 +1,464 @@
+// Pair-sum operators (pair functions, LDS records, target state) of the
+// Gaussian kernel sums, shared by the single-call entry points and by the
+// device-resident shooting loops.
+#pragma once
+
+#include "pair.hpp"
+
+#include <type_traits>
+
+namespace lddmm {
+
+template <typename T>
+__device__ __forceinline__ T fmaT(T a, T b, T c) { return fma(a, b, c); }
+
+// ---------------------------------------------------------------------------
+// applyK:  out_i = sum_j g(r2) a_j          (base/kernelFunctions_util.py:451-466)
+// ---------------------------------------------------------------------------
+template <int KW, bool MS>
+struct OpApplyK {
+    static constexpr int NCH = KW;
+    static constexpr int REC = 8;
+    using Chunks = ContiguousChunks;
+    template <typename T>
+    struct Args {
+        const double *y, *x, *a, *c0;
+        int dim, lda, col0;
+        ScaleSet<T> sc;
+    };
+    template <typename T>
+    struct Tgt { T u0, u1, u2; };
+    struct Raw { double x0, x1, x2, a[KW]; };
+
+    template <typename T>
+    __device__ static void load_target(const Args<T>& A, int i, Tgt<T>& t) {
+        t.u0 = coord<T>(A.y, i, A.dim, 0, A.c0, A.sc.cs);
+        t.u1 = coord<T>(A.y, i, A.dim, 1, A.c0, A.sc.cs);
+        t.u2 = coord<T>(A.y, i, A.dim, 2, A.c0, A.sc.cs);
+    }
+    template <typename T>
+    __device__ static void load_raw(const Args<T>& A, int j, Raw& r) {
+        if (j < 0) {
+            r.x0 = A.c0[0]; r.x1 = A.c0[1]; r.x2 = A.c0[2];
+#pragma unroll
+            for (int c = 0; c < KW; ++c) r.a[c] = 0.0;
+            return;
+        }
+        const double* xp = A.x + size_t(j) * A.dim;
+        r.x0 = xp[0];
+        r.x1 = A.dim > 1 ? xp[1] : 0.0;
+        r.x2 = A.dim > 2 ? xp[2] : 0.0;
+        const double* ap = A.a + size_t(j) * A.lda + A.col0;
+#pragma unroll
+        for (int c = 0; c < KW; ++c) r.a[c] = ap[c];
+    }
+    template <typename T>
+    __device__ static void store_rec(const Args<T>& A, const Raw& r, T* rec) {
+        rec[0] = T((r.x0 - A.c0[0]) * A.sc.cs);
+        rec[1] = T((r.x1 - A.c0[1]) * A.sc.cs);
+        rec[2] = T((r.x2 - A.c0[2]) * A.sc.cs);
+        rec[3] = T(0);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) rec[4 + c] = c < KW ? T(r.a[c < KW ? c : 0]) : T(0);
+    }
+    template <typename T>
+    __device__ static void pair(const Args<T>& A, const Tgt<T>& t, const T* rec, T* acc) {
+        Vec4<T> p = lds4(rec);
+        T dx = t.u0 - p.x, dy = t.u1 - p.y, dz = t.u2 - p.z;
+        T r2 = fmaT(dz, dz, fmaT(dy, dy, dx * dx));
+        T K = gK<T, MS>(A.sc, r2);
+        Vec4<T> w = lds4(rec + 4);
+        acc[0] = fmaT(K, w.x, acc[0]);
+        if constexpr (KW > 1) acc[1] = fmaT(K, w.y, acc[1]);
+        if constexpr (KW > 2) acc[2] = fmaT(K, w.z, acc[2]);
+        if constexpr (KW > 3) acc[3] = fmaT(K, w.w, acc[3]);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// applyDiffKT:  out_i = sum_j g'(r2) c_ij (y_i - x_j)
+//   plain: c_ij = p_i.a_j;  sym/lddmm: c_ij = p_i.a_j + a_i.q_j with
+//   q_j = p_j - 2 r a_j (r = 0 for sym)      (base/kernelFunctions_util.py:773-793)
+// ---------------------------------------------------------------------------
+template <int KW, bool SYM, bool MS>
+struct OpDiffKT {
+    static constexpr int NCH = 3;
+    static constexpr int REC = SYM ? 12 : 8;
+    using Chunks = ContiguousChunks;
+    template <typename T>
+    struct Args {
+        const double *y, *x, *p, *a, *c0;
+        int dim, lda, col0;
+        double r2w;       // 2 * regweight (lddmm) or 0 (sym)
+        ScaleSet<T> sc;
+    };
+    template <typename T>
+    struct Tgt { T u0, u1, u2, p[KW], a[SYM ? KW : 1]; };
+    struct Raw { double x0, x1, x2, a[KW], p[SYM ? KW : 1]; };
+
+    template <typename T>
+    __device__ static void load_target(const Args<T>& A, int i, Tgt<T>& t) {
+        t.u0 = coord<T>(A.y, i, A.dim, 0, A.c0, A.sc.cs);
+        t.u1 = coord<T>(A.y, i, A.dim, 1, A.c0, A.sc.cs);
+        t.u2 = coord<T>(A.y, i, A.dim, 2, A.c0, A.sc.cs);
+        const double* pp = A.p + size_t(i) * A.lda + A.col0;
+#pragma unroll
+        for (int c = 0; c < KW; ++c) t.p[c] = T(pp[c]);
+        if constexpr (SYM) {
+            const double* ap = A.a + size_t(i) * A.lda + A.col0;
+#pragma unroll
+            for (int c = 0; c < KW; ++c) t.a[c] = T(ap[c]);
+        }
+    }
+    template <typename T>
+    __device__ static void load_raw(const Args<T>& A, int j, Raw& r) {
+        if (j < 0) {
+            r.x0 = A.c0[0]; r.x1 = A.c0[1]; r.x2 = A.c0[2];
+#pragma unroll
+            for (int c = 0; c < KW; ++c) r.a[c] = 0.0;
+            if constexpr (SYM) {
+#pragma unroll
+                for (int c = 0; c < KW; ++c) r.p[c] = 0.0;
+            }
+            return;
+        }
+        const double* xp = A.x + size_t(j) * A.dim;
+        r.x0 = xp[0];
+        r.x1 = A.dim > 1 ? xp[1] : 0.0;
+        r.x2 = A.dim > 2 ? xp[2] : 0.0;
+        const double* ap = A.a + size_t(j) * A.lda + A.col0;
+#pragma unroll
+        for (int c = 0; c < KW; ++c) r.a[c] = ap[c];
+        if constexpr (SYM) {
+            const double* pp = A.p + size_t(j) * A.lda + A.col0;
+#pragma unroll
+            for (int c = 0; c < KW; ++c) r.p[c] = pp[c];
+        }
+    }
+    template <typename T>
+    __device__ static void store_rec(const Args<T>& A, const Raw& r, T* rec) {
+        rec[0] = T((r.x0 - A.c0[0]) * A.sc.cs);
+        rec[1] = T((r.x1 - A.c0[1]) * A.sc.cs);
+        rec[2] = T((r.x2 - A.c0[2]) * A.sc.cs);
+        rec[3] = T(0);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) rec[4 + c] = c < KW ? T(r.a[c < KW ? c : 0]) : T(0);
+        if constexpr (SYM) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                int cc = c < KW ? c : 0;
+                rec[8 + c] = c < KW ? T(r.p[cc] - A.r2w * r.a[cc]) : T(0);
+            }
+        }
+    }
+    template <typename T>
+    __device__ static void pair(const Args<T>& A, const Tgt<T>& t, const T* rec, T* acc) {
+        Vec4<T> p = lds4(rec);
+        T dx = t.u0 - p.x, dy = t.u1 - p.y, dz = t.u2 - p.z;
+        T r2 = fmaT(dz, dz, fmaT(dy, dy, dx * dx));
+        T g = gD<T, MS>(A.sc, r2);
+        Vec4<T> a = lds4(rec + 4);
+        T c = t.p[0] * a.x;
+        if constexpr (KW > 1) c = fmaT(t.p[1], a.y, c);
+        if constexpr (KW > 2) c = fmaT(t.p[2], a.z, c);
+        if constexpr (KW > 3) c = fmaT(t.p[3], a.w, c);
+        if constexpr (SYM) {
+            Vec4<T> q = lds4(rec + 8);
+            c = fmaT(t.a[0], q.x, c);
+            if constexpr (KW > 1) c = fmaT(t.a[1], q.y, c);
+            if constexpr (KW > 2) c = fmaT(t.a[2], q.z, c);
+            if constexpr (KW > 3) c = fmaT(t.a[3], q.w, c);
+        }
+        T w = g * c;
+        acc[0] = fmaT(w, dx, acc[0]);
+        acc[1] = fmaT(w, dy, acc[1]);
+        acc[2] = fmaT(w, dz, acc[2]);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// applyktensor: out_i = sum_j g(r2)(ay_i ax_j + (by_i.bx_j)^2)
+//                                          (base/kernelFunctions_util.py:1475-1506)
+// ---------------------------------------------------------------------------
+template <int KB, bool MS>
+struct OpKTensor {
+    static constexpr int NCH = 1;
+    static constexpr int REC = 8;
+    using Chunks = ContiguousChunks;
+    template <typename T>
+    struct Args {
+        const double *y, *x, *ay, *ax, *by, *bx, *c0;
+        int dim, kb;
+        ScaleSet<T> sc;
+    };
+    template <typename T>
+    struct Tgt { T u0, u1, u2, ay, b[KB]; };
+    struct Raw { double x0, x1, x2, ax, b[KB]; };
+
+    template <typename T>
+    __device__ static void load_target(const Args<T>& A, int i, Tgt<T>& t) {
+        t.u0 = coord<T>(A.y, i, A.dim, 0, A.c0, A.sc.cs);
+        t.u1 = coord<T>(A.y, i, A.dim, 1, A.c0, A.sc.cs);
+        t.u2 = coord<T>(A.y, i, A.dim, 2, A.c0, A.sc.cs);
+        t.ay = T(A.ay[i]);
+#pragma unroll
+        for (int c = 0; c < KB; ++c) t.b[c] = T(A.by[size_t(i) * A.kb + c]);
+    }
+    template <typename T>
+    __device__ static void load_raw(const Args<T>& A, int j, Raw& r) {
+        if (j < 0) {
+            r.x0 = A.c0[0]; r.x1 = A.c0[1]; r.x2 = A.c0[2]; r.ax = 0.0;
+#pragma unroll
+            for (int c = 0; c < KB; ++c) r.b[c] = 0.0;
+            return;
+        }
+        const double* xp = A.x + size_t(j) * A.dim;
+        r.x0 = xp[0];
+        r.x1 = A.dim > 1 ? xp[1] : 0.0;
+        r.x2 = A.dim > 2 ? xp[2] : 0.0;
+        r.ax = A.ax[j];
+#pragma unroll
+        for (int c = 0; c < KB; ++c) r.b[c] = A.bx[size_t(j) * A.kb + c];
+    }
+    template <typename T>
+    __device__ static void store_rec(const Args<T>& A, const Raw& r, T* rec) {
+        rec[0] = T((r.x0 - A.c0[0]) * A.sc.cs);
+        rec[1] = T((r.x1 - A.c0[1]) * A.sc.cs);
+        rec[2] = T((r.x2 - A.c0[2]) * A.sc.cs);
+        rec[3] = T(r.ax);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) rec[4 + c] = c < KB ? T(r.b[c < KB ? c : 0]) : T(0);
+    }
+    template <typename T>
+    __device__ static void pair(const Args<T>& A, const Tgt<T>& t, const T* rec, T* acc) {
+        Vec4<T> p = lds4(rec);
+        T dx = t.u0 - p.x, dy = t.u1 - p.y, dz = t.u2 - p.z;
+        T r2 = fmaT(dz, dz, fmaT(dy, dy, dx * dx));
+        T K = gK<T, MS>(A.sc, r2);
+        Vec4<T> b = lds4(rec + 4);
+        T g = t.b[0] * b.x;
+        if constexpr (KB > 1) g = fmaT(t.b[1], b.y, g);
+        if constexpr (KB > 2) g = fmaT(t.b[2], b.z, g);
+        if constexpr (KB > 3) g = fmaT(t.b[3], b.w, g);
+        acc[0] = fmaT(K, fmaT(g, g, t.ay * p.w), acc[0]);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// applyDiffK: out_i = sum_l -g'(r2) (d_il . A_il) a2_l with A = a1_i (opt 1),
+// a1_l (opt 2, overall sign flipped), a1_i - a1_l (opt 1and2)
+//                                          (base/kernelFunctions_util.py:655-680)
+// ---------------------------------------------------------------------------
+template <int KW, int OPT, bool MS>
+struct OpDiffK {
+    static constexpr int NCH = KW;
+    static constexpr int REC = 12;
+    using Chunks = ContiguousChunks;
+    template <typename T>
+    struct Args {
+        const double *x, *a1, *a2, *c0;
+        int dim, lda, col0;
+        ScaleSet<T> sc;
+    };
+    template <typename T>
+    struct Tgt { T u0, u1, u2, b0, b1, b2; };
+    struct Raw { double x0, x1, x2, b0, b1, b2, a[KW]; };
+
+    template <typename T>
+    __device__ static void load_target(const Args<T>& A, int i, Tgt<T>& t) {
+        t.u0 = coord<T>(A.x, i, A.dim, 0, A.c0, A.sc.cs);
+        t.u1 = coord<T>(A.x, i, A.dim, 1, A.c0, A.sc.cs);
+        t.u2 = coord<T>(A.x, i, A.dim, 2, A.c0, A.sc.cs);
+        const double* bp = A.a1 + size_t(i) * A.dim;
+        t.b0 = T(bp[0]);
+        t.b1 = A.dim > 1 ? T(bp[1]) : T(0);
+        t.b2 = A.dim > 2 ? T(bp[2]) : T(0);
+    }
+    template <typename T>
+    __device__ static void load_raw(const Args<T>& A, int j, Raw& r) {
+        if (j < 0) {
+            r.x0 = A.c0[0]; r.x1 = A.c0[1]; r.x2 = A.c0[2];
+            r.b0 = r.b1 = r.b2 = 0.0;
+#pragma unroll
+            for (int c = 0; c < KW; ++c) r.a[c] = 0.0;
+            return;
+        }
+        const double* xp = A.x + size_t(j) * A.dim;
+        const double* bp = A.a1 + size_t(j) * A.dim;
+        r.x0 = xp[0];
+        r.x1 = A.dim > 1 ? xp[1] : 0.0;
+        r.x2 = A.dim > 2 ? xp[2] : 0.0;
+        r.b0 = bp[0];
+        r.b1 = A.dim > 1 ? bp[1] : 0.0;
+        r.b2 = A.dim > 2 ? bp[2] : 0.0;
+        const double* ap = A.a2 + size_t(j) * A.lda + A.col0;
+#pragma unroll
+        for (int c = 0; c < KW; ++c) r.a[c] = ap[c];
+    }
+    template <typename T>
+    __device__ static void store_rec(const Args<T>& A, const Raw& r, T* rec) {
+        rec[0] = T((r.x0 - A.c0[0]) * A.sc.cs);
+        rec[1] = T((r.x1 - A.c0[1]) * A.sc.cs);
+        rec[2] = T((r.x2 - A.c0[2]) * A.sc.cs);
+        rec[3] = T(0);
+        rec[4] = T(r.b0); rec[5] = T(r.b1); rec[6] = T(r.b2); rec[7] = T(0);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) rec[8 + c] = c < KW ? T(r.a[c < KW ? c : 0]) : T(0);
+    }
+    template <typename T>
+    __device__ static void pair(const Args<T>& A, const Tgt<T>& t, const T* rec, T* acc) {
+        Vec4<T> p = lds4(rec);
+        T dx = t.u0 - p.x, dy = t.u1 - p.y, dz = t.u2 - p.z;
+        T r2 = fmaT(dz, dz, fmaT(dy, dy, dx * dx));
+        T g = gD<T, MS>(A.sc, r2);
+        Vec4<T> b = lds4(rec + 4);
+        T s;
+        if constexpr (OPT == LDDMM_DIFFK_1) s = fmaT(dz, t.b2, fmaT(dy, t.b1, dx * t.b0));
+        else if constexpr (OPT == LDDMM_DIFFK_2) s = fmaT(dz, b.z, fmaT(dy, b.y, dx * b.x));
+        else s = fmaT(dz, t.b2 - b.z, fmaT(dy, t.b1 - b.y, dx * (t.b0 - b.x)));
+        T w = g * s;
+        Vec4<T> a = lds4(rec + 8);
+        acc[0] = fmaT(w, a.x, acc[0]);
+        if constexpr (KW > 1) acc[1] = fmaT(w, a.y, acc[1]);
+        if constexpr (KW > 2) acc[2] = fmaT(w, a.z, acc[2]);
+        if constexpr (KW > 3) acc[3] = fmaT(w, a.w, acc[3]);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// applyDDiffK: out_i = sum_l ((d.q) d gamma - q) g(r2) (a1_i . a2_l)
+//   q = p_i (11), -p_l (12), p_i - p_l (11and12)
+//                                          (base/kernelFunctions_util.py:1092-1129)
+// ---------------------------------------------------------------------------
+template <int KW, int OPT, bool MS>
+struct OpDDiffK {
+    static constexpr int NCH = 3;
+    static constexpr int REC = 12;
+    using Chunks = ContiguousChunks;
+    template <typename T>
+    struct Args {
+        const double *x, *a1, *a2, *p, *c0;
+        int dim, lda, col0;
+        T gamma;
+        ScaleSet<T> sc;
+    };
+    template <typename T>
+    struct Tgt { T u0, u1, u2, a[KW], p0, p1, p2; };
+    struct Raw { double x0, x1, x2, a[KW], p0, p1, p2; };
+
+    template <typename T>
+    __device__ static void load_target(const Args<T>& A, int i, Tgt<T>& t) {
+        t.u0 = coord<T>(A.x, i, A.dim, 0, A.c0, A.sc.cs);
+        t.u1 = coord<T>(A.x, i, A.dim, 1, A.c0, A.sc.cs);
+        t.u2 = coord<T>(A.x, i, A.dim, 2, A.c0, A.sc.cs);
+        const double* ap = A.a1 + size_t(i) * A.lda + A.col0;
+#pragma unroll
+        for (int c = 0; c < KW; ++c) t.a[c] = T(ap[c]);
+        const double* pp = A.p + size_t(i) * A.dim;
+        t.p0 = T(pp[0]);
+        t.p1 = A.dim > 1 ? T(pp[1]) : T(0);
+        t.p2 = A.dim > 2 ? T(pp[2]) : T(0);
+    }
+    template <typename T>
+    __device__ static void load_raw(const Args<T>& A, int j, Raw& r) {
+        if (j < 0) {
+            r.x0 = A.c0[0]; r.x1 = A.c0[1]; r.x2 = A.c0[2];
+            r.p0 = r.p1 = r.p2 = 0.0;
+#pragma unroll
+            for (int c = 0; c < KW; ++c) r.a[c] = 0.0;
+            return;
+        }
+        const double* xp = A.x + size_t(j) * A.dim;
+        const double* pp = A.p + size_t(j) * A.dim;
+        r.x0 = xp[0];
+        r.x1 = A.dim > 1 ? xp[1] : 0.0;
+        r.x2 = A.dim > 2 ? xp[2] : 0.0;
+        r.p0 = pp[0];
+        r.p1 = A.dim > 1 ? pp[1] : 0.0;
+        r.p2 = A.dim > 2 ? pp[2] : 0.0;
+        const double* ap = A.a2 + size_t(j) * A.lda + A.col0;
+#pragma unroll
+        for (int c = 0; c < KW; ++c) r.a[c] = ap[c];
+    }
+    template <typename T>
+    __device__ static void store_rec(const Args<T>& A, const Raw& r, T* rec) {
+        rec[0] = T((r.x0 - A.c0[0]) * A.sc.cs);
+        rec[1] = T((r.x1 - A.c0[1]) * A.sc.cs);
+        rec[2] = T((r.x2 - A.c0[2]) * A.sc.cs);
+        rec[3] = T(0);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) rec[4 + c] = c < KW ? T(r.a[c < KW ? c : 0]) : T(0);
+        rec[8] = T(r.p0); rec[9] = T(r.p1); rec[10] = T(r.p2); rec[11] = T(0);
+    }
+    template <typename T>
+    __device__ static void pair(const Args<T>& A, const Tgt<T>& t, const T* rec, T* acc) {
+        Vec4<T> p = lds4(rec);
+        T dx = t.u0 - p.x, dy = t.u1 - p.y, dz = t.u2 - p.z;
+        T r2 = fmaT(dz, dz, fmaT(dy, dy, dx * dx));
+        Vec4<T> a = lds4(rec + 4);
+        T aa = t.a[0] * a.x;
+        if constexpr (KW > 1) aa = fmaT(t.a[1], a.y, aa);
+        if constexpr (KW > 2) aa = fmaT(t.a[2], a.z, aa);
+        if constexpr (KW > 3) aa = fmaT(t.a[3], a.w, aa);
+        Vec4<T> pl = lds4(rec + 8);
+        T q0, q1, q2;
+        if constexpr (OPT == LDDMM_DDIFFK_11) { q0 = t.p0; q1 = t.p1; q2 = t.p2; }
+        else if constexpr (OPT == LDDMM_DDIFFK_12) { q0 = -pl.x; q1 = -pl.y; q2 = -pl.z; }
+        else { q0 = t.p0 - pl.x; q1 = t.p1 - pl.y; q2 = t.p2 - pl.z; }
+        T dq = fmaT(dz, q2, fmaT(dy, q1, dx * q0));
+        if constexpr (!MS) {
+            T K = fast_exp2(-r2) * aa;
+            T tq = A.gamma * dq;
+            acc[0] = fmaT(K, fmaT(tq, dx, -q0), acc[0]);
+            acc[1] = fmaT(K, fmaT(tq, dy, -q1), acc[1]);
+            acc[2] = fmaT(K, fmaT(tq, dz, -q2), acc[2]);
+        } else {
+            T gdd, gd;
+            gDDD<T, MS>(A.sc, r2, gdd, gd);
+            T tq = dq * gdd * aa, tp = gd * aa;
+            acc[0] = fmaT(tq, dx, fmaT(-tp, q0, acc[0]));
+            acc[1] = fmaT(tq, dy, fmaT(-tp, q1, acc[1]));
+            acc[2] = fmaT(tq, dz, fmaT(-tp, q2, acc[2]));
+        }
+    }
+};
+
+
+// ---------------------------------------------------------------------------
+// Host helpers
+// ---------------------------------------------------------------------------
+template <class F>
+int with_kw(int kw, F&& f) {
+    switch (kw) {
+        case 1: return f(std::integral_constant<int, 1>{});
+        case 2: return f(std::integral_constant<int, 2>{});
+        case 3: return f(std::integral_constant<int, 3>{});
+        default: return f(std::integral_constant<int, 4>{});
+    }
+}
+
+// targets per lane: more targets amortise each LDS record over more pairs
+inline int pick_R(int n) { return n >= 8192 ? 2 : 1; }
+
+// One planned pair launch of an op.
+template <typename T, class OP>
+struct Job {
+    typename OP::template Args<T> args;
+    PairPlan plan;
+    int R = 1;
+};
+
+template <typename T, class OP>
+int plan_job(Job<T, OP>& j, int n, int m, bool allow_r2) {
+    j.R = allow_r2 ? pick_R(n) : 1;
+    if (j.R == 2) return plan_for<T, OP, 2>(n, m, &j.plan);
+    return plan_for<T, OP, 1>(n, m, &j.plan);
+}
+
+template <typename T, class OP>
+int run_job(const Job<T, OP>& j, int n, int m, Partials<T> part, hipStream_t st) {
+    if (j.R == 2) return run_pair<T, OP, 2>(j.args, n, m, j.plan, part, st);
+    return run_pair<T, OP, 1>(j.args, n, m, j.plan, part, st);
+}
+
+}  // namespace lddmm

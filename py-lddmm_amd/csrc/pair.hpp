@@ -137,9 +137,9 @@ __device__ __forceinline__ double sum_chunks(const Partials<T>& p, int nchunk, i
 
 struct ChanScale { double s[8]; };
 
-// out[i*ldo + col0 + c] = (base ? base[...] + alpha*v : (accumulate ? out + v : v)),
-// v = scale_c * sum over chunks of channel c; partials hold `nch` channels per
-// chunk, of which the first `nout` are written.
+// out[o] = (base ? base[o] : (accumulate ? out[o] : 0)) + alpha * scale_c * v,
+// o = i*ldo + col0 + c, v = sum over chunks of channel c; partials hold `nch`
+// channels per chunk, of which the first `nout` are written.
 template <typename T>
 __global__ void k_reduce(Partials<T> part, int nchunk, int nch, int nout, int n, ChanScale sc,
                          double* out, int ldo, int col0, int accumulate, const double* base,
@@ -147,9 +147,9 @@ __global__ void k_reduce(Partials<T> part, int nchunk, int nch, int nout, int n,
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     for (int c = 0; c < nout; ++c) {
-        double v = sum_chunks(part, nchunk, nch, i, c, nullptr) * sc.s[c];
+        double v = alpha * sum_chunks(part, nchunk, nch, i, c, nullptr) * sc.s[c];
         size_t o = size_t(i) * ldo + col0 + c;
-        if (base) out[o] = base[o] + alpha * v;
+        if (base) out[o] = base[o] + v;
         else if (accumulate) out[o] += v;
         else out[o] = v;
     }
@@ -179,8 +179,11 @@ template <typename T, class OP, int R>
 int run_pair(const typename OP::template Args<T>& args, int n, int m, const PairPlan& plan,
              Partials<T> part, hipStream_t st) {
     dim3 grid(plan.bx, plan.nchunk);
+    const bool prof = profiling();
+    if (prof) LDDMM_TRY(profile_begin(st));
     hipLaunchKernelGGL((k_pair<T, OP, R>), grid, dim3(kBlock), 0, st, args, n, m, plan.chunk, part);
     LDDMM_CHECK_LAUNCH();
+    if (prof) LDDMM_TRY(profile_end(st));
     return LDDMM_OK;
 }
 

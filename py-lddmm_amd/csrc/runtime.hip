@@ -102,6 +102,66 @@ int blocks_per_cu(const void* func, int block, size_t dyn_lds) {
     return nb;
 }
 
+// ---------------------------------------------------------------------------
+// Pair-kernel timing: event pairs are recorded around each launch and read
+// back (synchronously) by lddmm_profile_read.
+// ---------------------------------------------------------------------------
+namespace {
+struct ProfState {
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> open;
+    std::vector<hipEvent_t> pool;
+};
+std::mutex g_prof_mutex;
+ProfState g_prof;
+
+hipEvent_t take_event() {
+    if (!g_prof.pool.empty()) {
+        hipEvent_t e = g_prof.pool.back();
+        g_prof.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+}  // namespace
+
+bool profiling() { return g_prof.on; }
+
+int profile_begin(hipStream_t st) {
+    std::lock_guard<std::mutex> g(g_prof_mutex);
+    hipEvent_t a = take_event(), b = take_event();
+    if (!a || !b) { set_error("hipEventCreate failed"); return LDDMM_EHIP; }
+    LDDMM_HIP(hipEventRecord(a, st));
+    g_prof.open.push_back({a, b});
+    return LDDMM_OK;
+}
+
+int profile_end(hipStream_t st) {
+    std::lock_guard<std::mutex> g(g_prof_mutex);
+    if (g_prof.open.empty()) return LDDMM_OK;
+    LDDMM_HIP(hipEventRecord(g_prof.open.back().second, st));
+    return LDDMM_OK;
+}
+
+int profile_read(double* ms_each, int64_t cap, int64_t* launches) {
+    std::lock_guard<std::mutex> g(g_prof_mutex);
+    int64_t i = 0;
+    for (auto& ev : g_prof.open) {
+        LDDMM_HIP(hipEventSynchronize(ev.second));
+        float t = 0.f;
+        LDDMM_HIP(hipEventElapsedTime(&t, ev.first, ev.second));
+        if (ms_each && i < cap) ms_each[i] = t;
+        ++i;
+        g_prof.pool.push_back(ev.first);
+        g_prof.pool.push_back(ev.second);
+    }
+    if (launches) *launches = i;
+    g_prof.open.clear();
+    return LDDMM_OK;
+}
+
 int validate_kernel(const lddmm_kernel* kd) {
     LDDMM_REQUIRE(kd != nullptr, "null kernel description");
     if (kd->kind != LDDMM_KERNEL_GAUSS) {
@@ -131,6 +191,15 @@ int lddmm_device_count(void) {
 }
 
 int lddmm_release_workspace(void) { return lddmm::release_workspace(); }
+
+int lddmm_profile_enable(int32_t on) {
+    lddmm::g_prof.on = on != 0;
+    return LDDMM_OK;
+}
+
+int lddmm_profile_read(double* ms_each, int64_t cap, int64_t* launches) {
+    return lddmm::profile_read(ms_each, cap, launches);
+}
 
 int lddmm_surface_adjacency(const int32_t* faces, int64_t nf, int64_t nv, int32_t* adj_ptr,
                             int32_t* adj_idx) {

@@ -3,8 +3,7 @@
 Module names mirror the reference package (py-lddmm/base): kernelFunctions,
 kernelFunctions_util, surfaces, curves, surfaceDistances, curveDistances,
 pointEvolution, bfgs, linesearch, surfaceMatching, curveMatching,
-secondOrderPointSetMatching.  All kernel sums run through liblddmm_amd.so.
+secondOrderPointSetMatching.  All kernel sums run through liblddmm_amd.so (loaded by _backend on first
+use of a kernel module; a missing library raises ImportError).
 """
-from . import _backend  # noqa: F401  (fails loudly when the HIP library is missing)
-
 __version__ = "0.1.0"

@@ -66,6 +66,8 @@ SIGNATURES = {
     "lddmm_epdiff_forward": [_KP, _I32, _I64, _I32, _P, _P, _P],
     "lddmm_epdiff_adjoint": [_KP, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P],
     "lddmm_release_workspace": [],
+    "lddmm_profile_enable": [_I32],
+    "lddmm_profile_read": [_P, _I64, _P],
     "lddmm_device_count": [],
 }
 
@@ -121,6 +123,18 @@ def to_dev(a, dtype=torch.float64):
         return a.to(device=dev, dtype=dtype).contiguous()
     arr = np.ascontiguousarray(np.asarray(a), dtype=torch.empty((), dtype=dtype).numpy().dtype)
     return torch.from_numpy(arr).to(dev)
+
+
+def profile_enable(on=True):
+    check(LIB.lddmm_profile_enable(1 if on else 0), "profile_enable")
+
+
+def profile_read(cap=4096):
+    """Durations (ms) of the pair-kernel launches recorded since the last read."""
+    buf = (ctypes.c_double * cap)()
+    n = ctypes.c_int64(0)
+    check(LIB.lddmm_profile_read(buf, cap, ctypes.byref(n)), "profile_read")
+    return [buf[i] for i in range(min(n.value, cap))]
 
 
 def kernel_desc(sigma, kp=-1.0, prec='float64', kind=KERNEL_GAUSS):

@@ -1,0 +1,202 @@
+"""Landmark evolution and adjoints with the reference's signatures
+(py-lddmm base/pointEvolution.py): landmarkDirectEvolutionEuler,
+landmarkHamiltonianCovector, landmarkHamiltonianGradient, landmarkEPDiff,
+secondOrderEvolution, secondOrderHamiltonian, secondOrderCovector,
+secondOrderGradient.
+
+The T-step loops run on the GPU (lddmm_shoot_* / lddmm_epdiff_*): the
+trajectory stays in HBM and only the requested outputs come back.  numpy
+inputs return numpy arrays; CUDA tensors return CUDA tensors.  Affine
+controls, Jacobians, normals, passenger point sets and splines are not built
+(NotImplementedError), as in the configurations this build targets.
+"""
+import numpy as np
+import torch
+
+from . import _backend as B
+from . import kernelFunctions_util as ku
+
+
+def _kd(K):
+    B.kernel_kind(K.name)
+    if getattr(K, 'affine', 'none') not in ('none', None):
+        raise NotImplementedError("device shooting supports affine='none' kernels only")
+    return B.kernel_desc(K.sigma, K.ms_exponent, K.pk_dtype)
+
+
+def _is_torch(*a):
+    return any(isinstance(v, torch.Tensor) for v in a)
+
+
+def _ret(t, tt):
+    return t if tt else t.cpu().numpy()
+
+
+def _no_affine(affine):
+    if affine is not None and len(affine) > 0 and affine[0] is not None and len(affine[0]) > 0:
+        raise NotImplementedError("affine controls are not built in lddmm_amd shooting")
+
+
+def shoot(x0, at, KparDiff, regweight=1.0, energy=False):
+    """Device forward flow: returns (xt [T+1,N,d] tensor, energy tensor [1] or None)."""
+    at_ = B.to_dev(at)
+    T, n, dim = at_.shape
+    xt = torch.empty((T + 1, n, dim), dtype=torch.float64, device=at_.device)
+    xt[0] = B.to_dev(x0)
+    e = torch.empty(1, dtype=torch.float64, device=at_.device) if energy else None
+    B.check(B.LIB.lddmm_shoot_forward(_kd(KparDiff), B.ptr(at_), T, n, dim, B.ptr(xt), float(regweight),
+                                      B.ptr(e), B.stream_handle()), "shoot_forward")
+    return xt, e
+
+
+def shoot_adjoint(xt, at, px1, KparDiff, regweight, euclidean=True, covector=False):
+    """Device adjoint: returns (grad [T,N,d] (already / T), pxt or None)."""
+    at_ = B.to_dev(at)
+    T, n, dim = at_.shape
+    xt_ = B.to_dev(xt)
+    px1_ = B.to_dev(px1)
+    g = torch.empty_like(at_)
+    pxt = torch.empty((T + 1, n, dim), dtype=torch.float64, device=at_.device) if covector else None
+    B.check(B.LIB.lddmm_shoot_adjoint(_kd(KparDiff), B.ptr(at_), B.ptr(xt_), T, n, dim, B.ptr(px1_),
+                                      float(regweight), 1 if euclidean else 0, B.ptr(pxt), B.ptr(g),
+                                      B.stream_handle()), "shoot_adjoint")
+    return g, pxt
+
+
+def landmarkDirectEvolutionEuler(x0, at, KparDiff, affine=None, options=None):
+    """x_{t+1} = x_t + (1/T) K(x_t) a_t   (base/pointEvolution.py:88-164)."""
+    _no_affine(affine)
+    opt = {'withJacobian': False, 'withNormals': None, 'withPointSet': None}
+    if options:
+        opt.update(options)
+    if opt['withJacobian'] or opt['withNormals'] is not None or opt['withPointSet'] is not None:
+        raise NotImplementedError("Jacobian / normals / point-set tracking are not built")
+    tt = _is_torch(x0, at)
+    xt, _ = shoot(x0, at, KparDiff)
+    return {'xt': _ret(xt, tt), 'Jt': None, 'yt': None, 'nt': None}
+
+
+def landmarkHamiltonianCovector(x0, at, px1, Kpardiff, regweight, affine=None, extraTerm=None):
+    """Backward covector p_t (base/pointEvolution.py:167-201); returns (pxt, xt)."""
+    _no_affine(affine)
+    if extraTerm is not None:
+        raise NotImplementedError("extra (internal-cost) terms are not built")
+    tt = _is_torch(x0, at, px1)
+    xt, _ = shoot(x0, at, Kpardiff)
+    _, pxt = shoot_adjoint(xt, at, px1, Kpardiff, regweight, euclidean=False, covector=True)
+    return _ret(pxt, tt), _ret(xt, tt)
+
+
+def landmarkHamiltonianGradient(x0, at, px1, KparDiff, regweight, getCovector=False, affine=None,
+                                extraTerm=None):
+    """dat_t = 2 r a_t - p_{t+1}  (base/pointEvolution.py:205-236)."""
+    _no_affine(affine)
+    if extraTerm is not None:
+        raise NotImplementedError("extra (internal-cost) terms are not built")
+    tt = _is_torch(x0, at, px1)
+    at_ = B.to_dev(at)
+    xt, _ = shoot(x0, at_, KparDiff)
+    _, pxt = shoot_adjoint(xt, at_, px1, KparDiff, regweight, euclidean=False, covector=True)
+    dat = 2 * regweight * at_ - pxt[1:]
+    if getCovector:
+        return _ret(dat, tt), _ret(xt, tt), _ret(pxt, tt)
+    return _ret(dat, tt), _ret(xt, tt)
+
+
+# --------------------------------------------------------------------------
+# Second order (geodesic shooting)
+# --------------------------------------------------------------------------
+def _epdiff(x0, a0, KparDiff, T):
+    x0_ = B.to_dev(x0)
+    n, dim = x0_.shape
+    xt = torch.empty((T + 1, n, dim), dtype=torch.float64, device=x0_.device)
+    at = torch.empty_like(xt)
+    xt[0] = x0_
+    at[0] = B.to_dev(a0)
+    B.check(B.LIB.lddmm_epdiff_forward(_kd(KparDiff), T, n, dim, B.ptr(xt), B.ptr(at), B.stream_handle()),
+            "epdiff_forward")
+    return xt, at
+
+
+def landmarkEPDiff(T, x0, a0, KparDiff, affine=None, withJacobian=False, withNormals=None, withPointSet=None):
+    """Geodesic shooting of (x0, a0) over T steps (base/pointEvolution.py:320-379).
+
+    Returns (xt [T+1], at [T]); at[T-1] is the last momentum used (the
+    reference does not update it past T-1)."""
+    _no_affine(affine)
+    if withJacobian or withNormals is not None or withPointSet is not None:
+        raise NotImplementedError("Jacobian / normals / point-set tracking are not built")
+    tt = _is_torch(x0, a0)
+    xt, at = _epdiff(x0, a0, KparDiff, T)
+    return _ret(xt, tt), _ret(at[:T], tt)
+
+
+def secondOrderEvolution(x0, a0, KparDiff, timeStep, withSpline=None, affine=None, options=None):
+    """base/pointEvolution.py:385-472 (no spline, no affine)."""
+    if withSpline is not None:
+        raise NotImplementedError("spline controls are not built")
+    _no_affine(affine)
+    if options and (options.get('withJacobian') or options.get('withPointSet') is not None):
+        raise NotImplementedError("Jacobian / point-set tracking are not built")
+    T = int(np.round(1 / timeStep))
+    tt = _is_torch(x0, a0)
+    xt, at = _epdiff(x0, a0, KparDiff, T)
+    return {'xt': _ret(xt, tt), 'at': _ret(at, tt), 'Jt': None, 'yt': None}
+
+
+def secondOrderHamiltonian(x, a, px, pa, KparDiff, withSpline=None, affine=None):
+    """H = <px, K(x)a> - <pa, DiffKT(x, a, a)>  (base/pointEvolution.py:475-486); float64."""
+    if withSpline is not None:
+        raise NotImplementedError("spline controls are not built")
+    _no_affine(affine)
+    kw = dict(name=KparDiff.name, scale=KparDiff.sigma, order=KparDiff.order, dtype=KparDiff.pk_dtype,
+              KP=KparDiff.ms_exponent)
+    x_, a_ = B.to_dev(x), B.to_dev(a)
+    Ht = (B.to_dev(px) * ku.applyK(x_, x_, a_, **kw)).sum()
+    Ht = Ht - (B.to_dev(pa) * ku.applyDiffKT(x_, x_, a_, a_, **kw)).sum()
+    return float(Ht.item())
+
+
+def secondOrderCovector(x0, a0, px1, pa1, KparDiff, timeStep, withSpline=None, affine=None, isjump=None,
+                        forwardState=None):
+    """Backward (px, pa) covectors (base/pointEvolution.py:489-552, single target)."""
+    if withSpline is not None or isjump is not None:
+        raise NotImplementedError("splines / multiple targets are not built")
+    _no_affine(affine)
+    if isinstance(px1, (list, tuple)):
+        if len(px1) != 1:
+            raise NotImplementedError("multiple targets are not built")
+        px1, pa1 = px1[0], pa1[0]
+    T = int(np.round(1 / timeStep))
+    tt = _is_torch(x0, a0, px1, pa1)
+    if forwardState is None:
+        xt, at = _epdiff(x0, a0, KparDiff, T)
+    else:
+        xt, at = B.to_dev(forwardState[0]), B.to_dev(forwardState[1])
+    pxt, pat, da0 = _epdiff_adjoint(xt, at, px1, pa1, KparDiff, T)
+    return _ret(pxt, tt), _ret(pat, tt), _ret(xt, tt), _ret(at, tt)
+
+
+def _epdiff_adjoint(xt, at, px1, pa1, KparDiff, T):
+    n, dim = xt.shape[1], xt.shape[2]
+    pxt = torch.empty_like(xt)
+    pat = torch.empty_like(xt)
+    da0 = torch.empty((n, dim), dtype=torch.float64, device=xt.device)
+    B.check(B.LIB.lddmm_epdiff_adjoint(_kd(KparDiff), T, n, dim, B.ptr(xt), B.ptr(at), B.ptr(B.to_dev(px1)),
+                                       B.ptr(B.to_dev(pa1)), B.ptr(pxt), B.ptr(pat), B.ptr(da0),
+                                       B.stream_handle()), "epdiff_adjoint")
+    return pxt, pat, da0
+
+
+def secondOrderGradient(x0, a0, px1, pa1, KparDiff, timeStep, isjump=None, withSpline=None, affine=None,
+                        controlWeight=1.0):
+    """da0 = K(x0) a0 - pa_0  (base/pointEvolution.py:555-618)."""
+    if withSpline is not None or isjump is not None:
+        raise NotImplementedError("splines / multiple targets are not built")
+    _no_affine(affine)
+    T = int(np.round(1 / timeStep))
+    tt = _is_torch(x0, a0, px1, pa1)
+    xt, at = _epdiff(x0, a0, KparDiff, T)
+    pxt, pat, da0 = _epdiff_adjoint(xt, at, px1, pa1, KparDiff, T)
+    return {'da0': _ret(da0, tt), 'drhot': None, 'xt': _ret(xt, tt), 'at': _ret(at, tt), 'dA': None,
+            'db': None, 'pxt': _ret(pxt, tt), 'pat': _ret(pat, tt)}

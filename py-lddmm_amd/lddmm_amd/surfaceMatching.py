@@ -1,0 +1,384 @@
+"""Surface matching (plain LDDMM with a time-dependent momentum control),
+restating py-lddmm base/surfaceMatching.py + base/pointSetMatching.py +
+base/basicMatching.py for the configuration the hot path serves:
+`SurfaceMatching(Template, Target, options).optimizeMatching()` with
+errorType 'varifold' or 'current', algorithm 'bfgs' (weak-Wolfe line search),
+affine None, no internal cost, not symmetric, not unreduced.
+
+The control `at` [T, N, 3], the trajectories and the BFGS memory live on
+the GPU (float64 tensors); every objective is one device forward shoot with
+its energy fused in (lddmm_shoot_forward) plus one fused data-term pass, and
+every gradient is one fused data-term gradient plus one device adjoint
+(lddmm_shoot_adjoint with the Euclidean K(x_t) correction).  Scalars cross to
+the host once per evaluation.  `control_numpy()` / `fvDef` give host copies.
+Unsupported options raise NotImplementedError.
+"""
+import logging
+from copy import deepcopy
+
+import numpy as np
+import torch
+
+from . import _backend as B
+from . import bfgs as bfgs_mod
+from . import pointEvolution as evol
+from . import surfaceDistances as sd
+from .kernelFunctions import Kernel
+from .surfaces import Surface
+
+
+class Control(dict):
+    def __init__(self):
+        super().__init__()
+        self['at'] = None
+        self['Afft'] = None
+        self['x0'] = None
+        self['ct'] = None
+
+
+class State(dict):
+    def __init__(self):
+        super().__init__()
+        self['xt'] = None
+        self['yt'] = None
+        self['Jt'] = None
+        self['x0'] = None
+
+
+class SurfaceMatching:
+    def __init__(self, Template=None, Target=None, options=None):
+        self.setInitialOptions(options)
+        if self.options['algorithm'] != 'bfgs':
+            raise NotImplementedError("lddmm_amd drives the matching with algorithm='bfgs' only")
+        self.euclideanGradient = True
+        self.obj = None
+        self.objTry = None
+        self.obj0 = 0.
+        self.control = Control()
+        self.controlTry = Control()
+        self.state = State()
+        self.stateTry = State()
+        self.set_template_and_target(Template, Target)
+        self.burnIn = self.options['burnIn']
+        self.reset = False
+        self.Kdiff_dtype = self.options['pk_dtype']
+        self.Kdist_dtype = self.options['pk_dtype']
+        self.set_parameters()
+        self.set_fun(self.options['errorType'])
+        self.initialize_variables()
+        self.gradCoeff = 1.
+        self.ds = 1.
+        self._last_try = None
+
+    # ------------------------------------------------------------- options
+    def getDefaultOptions(self):
+        """base/basicMatching.py:84-126 + base/surfaceMatching.py:136-142"""
+        return {
+            'timeStep': 0.1, 'algorithm': 'bfgs', 'unreduced': False, 'Wolfe': True, 'burnIn': 10,
+            'epsInit': 1., 'KparDiff': None, 'KparDist': None, 'pk_dtype': 'float32', 'sigmaError': 1.0,
+            'errorType': 'measure', 'vfun': None, 'maxIter': 1000, 'regWeight': 1.0,
+            'unreducedWeight': 1.0, 'affineKernel': False, 'affineWeight': 10.0, 'rotWeight': 0.01,
+            'scaleWeight': None, 'transWeight': 0.1, 'testGradient': False, 'subsampleTargetSize': -1,
+            'affineOnly': False, 'passenger': None, 'Landmarks': None, 'saveFile': 'evolution',
+            'saveRate': 10, 'mode': 'normal', 'verb': True, 'pplot': False, 'saveTrajectories': False,
+            'affine': None, 'outputDir': '.', 'symmetric': False, 'internalCost': None,
+            'internalWeight': 1.0, 'lineSearch': 'Weak_Wolfe', 'gradTol': -1., 'gradLB': 0.001,
+            'reweightCells': False, 'unreducedResetRate': -1, 'fidelityWeight': 0.,
+        }
+
+    def setInitialOptions(self, options):
+        self.options = self.getDefaultOptions()
+        if options is not None:
+            for k in options.keys():
+                self.options[k] = options[k]
+        o = self.options
+        if o['affine'] not in (None, 'none'):
+            raise NotImplementedError("affine registration is not built in lddmm_amd")
+        for key in ('symmetric', 'unreduced', 'affineKernel', 'pplot'):
+            if o[key]:
+                raise NotImplementedError(f"option {key}=True is not built in lddmm_amd")
+        if o['internalCost'] is not None:
+            raise NotImplementedError("internal costs are not built in lddmm_amd")
+        if o['Landmarks'] is not None or o['passenger'] is not None:
+            raise NotImplementedError("landmark / passenger tracking is not built in lddmm_amd")
+        if o['errorType'] not in ('varifold', 'current'):
+            raise NotImplementedError(f"errorType {o['errorType']!r}: lddmm_amd builds 'varifold' and 'current'")
+        if o['vfun'] is not None:
+            raise NotImplementedError("custom varifold functions are not built")
+
+    def set_template_and_target(self, Template, Target):
+        if Template is None or Target is None:
+            raise ValueError("SurfaceMatching needs a template and a target surface")
+        self.fv0 = Surface(surf=Template)
+        self.fv1 = Surface(surf=Target)
+        self.fvInit = Surface(surf=self.fv0)
+        self.fix_orientation()
+        self.dim = self.fv0.vertices.shape[1]
+
+    def fix_orientation(self):
+        """Flip the target's faces when the signed volumes disagree (base/surfaceMatching.py:284-318)."""
+        self.fv0.getEdges()
+        self.fv1.getEdges()
+        self.closed = self.fv0.bdry.max() == 0 and self.fv1.bdry.max() == 0
+        if self.closed:
+            v0 = self.fv0.surfVolume()
+            v1 = self.fv1.surfVolume()
+            if v0 * v1 < 0:
+                self.fv1.flipFaces()
+            self.fv0ori = -1 if self.fvInit.surfVolume() < 0 else 1
+            self.fv1ori = -1 if self.fv1.surfVolume() < 0 else 1
+        else:
+            self.fv0ori = 1
+            self.fv1ori = 1
+
+    def set_parameters(self):
+        """base/basicMatching.py:143-189, base/pointSetMatching.py:158-229, base/surfaceMatching.py:175-181"""
+        o = self.options
+        if o['mode'] == 'debug':
+            o['verb'] = True
+            o['testGradient'] = True
+            o['pk_dtype'] = 'float64'
+        elif o['mode'] == 'quiet':
+            o['verb'] = False
+            o['testGradient'] = False
+        else:
+            o['testGradient'] = False
+            o['verb'] = True
+        self.Kdiff_dtype = o['pk_dtype']
+        self.Kdist_dtype = o['pk_dtype']
+        self.saveRate = o['saveRate']
+        self.gradEps = o['gradTol']
+        self.iter = 0
+        self.pkBuffer = 0
+        typeKernel, sigmaKernel = 'gauss', 6.5
+        typeKDist, sigmaDist = 'gauss', 2.5
+        if isinstance(o['KparDiff'], (list, tuple)):
+            typeKernel, sigmaKernel = o['KparDiff'][0], o['KparDiff'][1]
+            o['KparDiff'] = None
+        if o['KparDiff'] is None:
+            o['KparDiff'] = Kernel(name=typeKernel, sigma=sigmaKernel)
+        if isinstance(o['KparDist'], (list, tuple)):
+            typeKDist, sigmaDist = o['KparDist'][0], o['KparDist'][1]
+            o['KparDist'] = None
+        if o['KparDist'] is None:
+            o['KparDist'] = Kernel(name=typeKDist, sigma=sigmaDist)
+        o['KparDiff'].pk_dtype = o['pk_dtype']
+        o['KparDist'].pk_dtype = o['pk_dtype']
+        self.options['epsInit'] /= self.fv0.vertices.shape[0]
+
+    def set_fun(self, errorType):
+        self.options['errorType'] = errorType
+
+    def initialize_variables(self):
+        self.Tsize = int(round(1.0 / self.options['timeStep']))
+        self.nvert = self.fvInit.vertices.shape[0]
+        self.npt = self.nvert
+        dev = B.device()
+        self.x0_dev = B.to_dev(self.fvInit.vertices)
+        self.control['x0'] = None
+        self.control['at'] = torch.zeros((self.Tsize, self.npt, self.dim), dtype=torch.float64, device=dev)
+        self.controlTry['at'] = torch.zeros_like(self.control['at'])
+        self.state['xt'] = self.x0_dev.expand(self.Tsize + 1, -1, -1).clone()
+        self.fvDef = Surface(surf=self.fvInit)
+
+    def resetPK(self, newType=None):
+        t = newType
+        self.options['KparDiff'].pk_dtype = self.Kdiff_dtype if t is None else t
+        self.options['KparDist'].pk_dtype = self.Kdist_dtype if t is None else t
+        self.pkBuffer = 0
+
+    # ----------------------------------------------------------- data term
+    def _dist_kd(self):
+        K = self.options['KparDist']
+        return B.kernel_desc(K.sigma, K.ms_exponent, K.pk_dtype)
+
+    def fun_obj0(self, fv1):
+        K = self.options['KparDist']
+        if self.options['errorType'] == 'varifold':
+            return sd.varifoldNorm0(fv1, K)
+        return sd.currentNorm0(fv1, K)
+
+    def dataTerm_device(self, xT):
+        """(|def|^2 - 2<def, tgt>) / sigmaError^2 as a device scalar tensor."""
+        K = self.options['KparDist']
+        if self.options['errorType'] == 'varifold':
+            e, _ = sd.varifold_device(self._dist_kd(), self.fv0, self.fv1, energy=True, grad=False, v1=xT)
+            e = e[0]
+        else:
+            c1, cr1, _, _ = sd._face_data(self.fv0, xT)
+            c2, cr2, _, _ = sd._face_data(self.fv1)
+            kw = sd._ap(K)
+            from . import kernelFunctions_util as ku
+            e = ((cr1 * ku.applyK(c1, c1, cr1, **kw)).sum() - 2 * (cr1 * ku.applyK(c1, c2, cr2, **kw)).sum())
+        return e / self.options['sigmaError'] ** 2
+
+    def dataTerm(self, _fvDef, var=None):
+        return float(self.dataTerm_device(B.to_dev(_fvDef.vertices)).item())
+
+    def endPointGradient_device(self, xT):
+        K = self.options['KparDist']
+        if self.options['errorType'] == 'varifold':
+            _, g = sd.varifold_device(self._dist_kd(), self.fv0, self.fv1, energy=False, grad=True, v1=xT)
+        else:
+            g = sd.currentNormGradient_device(self.fv0, self.fv1, K, vertices=xT)
+        return g / self.options['sigmaError'] ** 2
+
+    def endPointGradient(self, endPoint=None):
+        xT = self.state['xt'][-1] if endPoint is None else B.to_dev(endPoint.vertices)
+        return self.endPointGradient_device(xT).cpu().numpy()
+
+    # ----------------------------------------------------------- objective
+    def objectiveFunDef(self, control, var=None, withTrajectory=True, withJacobian=False):
+        """Deformation energy sum_t r dt a_t.K(x_t)a_t ds^2 and the trajectory
+        (base/surfaceMatching.py:468-565); energy fused in the device shoot."""
+        if withJacobian:
+            raise NotImplementedError("Jacobian tracking is not built")
+        xt, e = evol.shoot(self.x0_dev, control['at'], self.options['KparDiff'],
+                           regweight=self.options['regWeight'], energy=True)
+        st = State()
+        st['xt'] = xt
+        obj = e[0] * self.ds ** 2
+        return (obj, st) if withTrajectory else obj
+
+    def objectiveFun(self):
+        """base/surfaceMatching.py:568-594"""
+        if self.obj is None:
+            self.obj0 = self.fun_obj0(self.fv1) / self.options['sigmaError'] ** 2
+            obj, self.state = self.objectiveFunDef(self.control, withTrajectory=True)
+            total = obj + self.dataTerm_device(self.state['xt'][-1])
+            self.obj = self.obj0 + float(total.item())
+        return self.obj
+
+    def getVariable(self):
+        return self.control
+
+    def updateTry(self, dr, eps, objRef=None):
+        """Objective at control - eps * dr (base/surfaceMatching.py:638-675)."""
+        controlTry = Control()
+        for k in dr.keys():
+            if dr[k] is not None:
+                controlTry[k] = self.control[k] - eps * dr[k]
+        obj_, stateTry = self.objectiveFunDef(controlTry, withTrajectory=True)
+        objTry = self.obj0 + float((obj_ + self.dataTerm_device(stateTry['xt'][-1])).item())
+        if np.isnan(objTry):
+            return 1e500
+        if objRef is None or objTry < objRef:
+            self.controlTry = controlTry
+            self.stateTry = stateTry
+            self.objTry = objTry
+        self._last_try = (dr, eps, controlTry, stateTry)
+        return objTry
+
+    def acceptVarTry(self):
+        self.obj = self.objTry
+        self.control = self.controlTry
+        self.state = self.stateTry
+
+    # ------------------------------------------------------------ gradient
+    def getGradient(self, coeff=1.0, update=None):
+        """Euclidean gradient wrt at (base/surfaceMatching.py:1040-1105)."""
+        if update is None:
+            control = self.control
+            xt = self.state['xt']
+        else:
+            lt = self._last_try
+            if lt is not None and lt[0] is update[0] and lt[1] == update[1]:
+                control, xt = lt[2], lt[3]['xt']          # trajectory of this trial control
+            else:
+                control = Control()
+                for k in update[0].keys():
+                    if update[0][k] is not None:
+                        control[k] = self.control[k] - update[1] * update[0][k]
+                xt, _ = evol.shoot(self.x0_dev, control['at'], self.options['KparDiff'])
+        px1 = -self.endPointGradient_device(xt[-1])
+        g, _ = evol.shoot_adjoint(xt, control['at'], px1, self.options['KparDiff'], self.options['regWeight'],
+                                  euclidean=self.euclideanGradient)
+        grd = Control()
+        grd['at'] = g / coeff
+        return grd
+
+    def dotProduct_euclidean(self, g1, g2):
+        res = np.zeros(len(g2))
+        for k in g1.keys():
+            if g1[k] is not None:
+                for ll, gr in enumerate(g2):
+                    res[ll] += float((g1[k] * gr[k]).sum())
+        return res
+
+    def addProd(self, dir1, dir2, beta):
+        dr = Control()
+        for k in dir1.keys():
+            if dir1[k] is not None and dir2[k] is not None:
+                dr[k] = dir1[k] + beta * dir2[k]
+        return dr
+
+    def prod(self, dir1, beta):
+        dr = Control()
+        for k in dir1.keys():
+            if dir1[k] is not None:
+                dr[k] = beta * dir1[k]
+        return dr
+
+    def copyDir(self, dir0):
+        dr = Control()
+        for k in dir0.keys():
+            if dir0[k] is not None:
+                dr[k] = dir0[k].clone()
+        return dr
+
+    def randomDir(self):
+        d = Control()
+        d['at'] = torch.randn_like(self.control['at'])
+        return d
+
+    # ----------------------------------------------------------- iteration
+    def startOfIteration(self):
+        if self.reset:
+            logging.info('switching to float64')
+            self.resetPK('float64')
+            self.pkBuffer = 0
+
+    def endOfIteration(self, forceSave=False):
+        """base/surfaceMatching.py:1348-1503 without file output."""
+        self.iter += 1
+        if self.reset:
+            obj0 = self.objectiveFun()
+            self.obj = None
+            obj = self.objectiveFun()
+            self.obj = None
+            self.resetPK('float64')
+            obj2 = self.objectiveFun()
+            self.resetPK()
+            logging.info(f"recomputing Objective {obj0:.5f} {obj:.5f} {obj2:.5f}")
+        if self.pkBuffer > 50:
+            if self.options['KparDiff'].pk_dtype != self.Kdiff_dtype:
+                logging.info("return to original pk_dtype")
+            self.resetPK()
+        else:
+            self.pkBuffer += 1
+
+    def endOfProcedure(self):
+        self.endOfIteration(forceSave=True)
+        self.fvDef.updateVertices(self.state['xt'][-1].cpu().numpy())
+
+    def optimizeMatching(self):
+        """base/surfaceMatching.py:1562-1583 (bfgs branch)."""
+        grd = self.getGradient(self.gradCoeff)
+        [grd2] = self.dotProduct_euclidean(grd, [grd])
+        if self.gradEps < 0:
+            self.gradEps = max(1e-5, np.sqrt(grd2) / 10000)
+        self.epsMax = 5.
+        logging.info(f'Gradient lower bound: {self.gradEps:.5f}')
+        bfgs_mod.bfgs(self, verb=self.options['verb'], maxIter=self.options['maxIter'],
+                      TestGradient=self.options['testGradient'], epsInit=self.options['epsInit'],
+                      Wolfe=self.options['Wolfe'], lineSearch=self.options['lineSearch'], memory=50)
+        self.fvDef.updateVertices(self.state['xt'][-1].cpu().numpy())
+        return self.control
+
+    # ------------------------------------------------------------- helpers
+    def control_numpy(self):
+        return self.control['at'].cpu().numpy()
+
+    def varifold_data(self):
+        """Current data term |def - tgt|^2 / sigmaError^2 (host float)."""
+        return self.obj0 + float(self.dataTerm_device(self.state['xt'][-1]).item())

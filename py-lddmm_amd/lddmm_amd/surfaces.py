@@ -1,0 +1,201 @@
+"""Triangulated surfaces (py-lddmm base/surfaces.py:226-351, :603-605, :1645-1651).
+
+`Surface((faces, vertices))`, `Surface(surf=other)` and `Surface('file.vtk')`
+(legacy ASCII VTK POLYDATA, the format of the reference's TestData) build
+the same attributes as the reference: vertices, faces, weights, centers,
+surfel, face_weights, component.  `device()` returns the C-ABI view of the
+surface (float64 vertices, int32 faces, vertex -> face adjacency) on the GPU,
+rebuilt only when the vertices or topology change.
+"""
+import os
+
+import numpy as np
+
+from . import vtk_io
+
+
+class Surface:
+    def __init__(self, surf=None, weights=None):
+        self._dev = None
+        self._topo = None
+        if type(surf) in (list, tuple):
+            if len(surf) > 0 and isinstance(surf[0], Surface):
+                self.concatenate(surf)
+            elif len(surf) > 0 and isinstance(surf[0], str):
+                self.concatenate([Surface(surf=name) for name in surf])
+            else:
+                self.vertices = np.array(surf[1], dtype=np.float64, copy=True)
+                self.faces = np.array(surf[0], dtype=np.int64, copy=True)
+                self.component = np.zeros(self.faces.shape[0], dtype=int)
+                if weights is None:
+                    self.weights = np.ones(self.vertices.shape[0], dtype=int)
+                    self.face_weights = np.ones(self.faces.shape[0], dtype=int)
+                else:
+                    self.updateWeights(weights)
+                self.computeCentersAreas()
+        elif isinstance(surf, str):
+            self.read(surf)
+        elif isinstance(surf, Surface):
+            self.vertices = np.copy(surf.vertices)
+            self.surfel = np.copy(surf.surfel)
+            self.faces = np.copy(surf.faces)
+            self.centers = np.copy(surf.centers)
+            self.component = np.copy(surf.component)
+            if weights is None:
+                self.weights = np.copy(surf.weights)
+                self.face_weights = np.copy(surf.face_weights)
+            else:
+                self.updateWeights(weights)
+            self._topo = surf._topo
+        else:
+            self.vertices = np.empty(0)
+            self.centers = np.empty(0)
+            self.faces = np.empty(0, dtype=np.int64)
+            self.surfel = np.empty(0)
+            self.component = np.empty(0)
+            self.weights = np.empty(0)
+            self.face_weights = np.empty(0)
+        self.edges = None
+        self.edgeFaces = None
+        self.faceEdges = None
+
+    # ------------------------------------------------------------------ I/O
+    def read(self, filename):
+        ext = os.path.splitext(filename)[1]
+        if ext != '.vtk':
+            raise NotImplementedError(f"lddmm_amd reads legacy ASCII .vtk surfaces only ({filename})")
+        v, f, w = vtk_io.read_polydata(filename)
+        self.vertices = v
+        self.faces = f
+        self.component = np.zeros(f.shape[0], dtype=int)
+        if w is None:
+            self.weights = np.ones(v.shape[0], dtype=int)
+        else:
+            self.weights = w
+        self.computeCentersAreas()
+
+    def saveVTK(self, fileName, scalars=None, normals=None, tensors=None, scal_name='scalars',
+                vectors=None, vect_name='vectors'):
+        vtk_io.write_polydata(fileName, self.vertices, self.faces, weights=self.weights,
+                              scalars=scalars, scal_name=scal_name, vectors=vectors, vect_name=vect_name)
+
+    def concatenate(self, fvl):
+        nv = 0
+        vs, fs, ws, cs = [], [], [], []
+        for k, fv in enumerate(fvl):
+            vs.append(fv.vertices)
+            fs.append(fv.faces + nv)
+            ws.append(fv.weights)
+            cs.append(np.full(fv.faces.shape[0], k, dtype=int))
+            nv += fv.vertices.shape[0]
+        self.vertices = np.concatenate(vs)
+        self.faces = np.concatenate(fs).astype(np.int64)
+        self.weights = np.concatenate(ws)
+        self.component = np.concatenate(cs)
+        self.computeCentersAreas()
+
+    # ------------------------------------------------------------- geometry
+    def computeCentersAreas(self):
+        """base/surfaces.py:326-335"""
+        x1 = self.vertices[self.faces[:, 0], :]
+        x2 = self.vertices[self.faces[:, 1], :]
+        x3 = self.vertices[self.faces[:, 2], :]
+        self.centers = (x1 + x2 + x3) / 3
+        self.surfel = np.cross(x2 - x1, x3 - x1) / 2
+        w = self.weights
+        self.face_weights = (w[self.faces[:, 0]] + w[self.faces[:, 1]] + w[self.faces[:, 2]]) / 3
+        self._dev = None
+
+    def updateWeights(self, w0):
+        self.weights = np.copy(w0)
+        w = self.weights
+        self.face_weights = (w[self.faces[:, 0]] + w[self.faces[:, 1]] + w[self.faces[:, 2]]) / 3
+        self._dev = None
+
+    def updateVertices(self, x0):
+        """base/surfaces.py:345-351"""
+        self.vertices = np.array(x0, dtype=np.float64, copy=True)
+        x1 = self.vertices[self.faces[:, 0], :]
+        x2 = self.vertices[self.faces[:, 1], :]
+        x3 = self.vertices[self.faces[:, 2], :]
+        self.centers = (x1 + x2 + x3) / 3
+        self.surfel = np.cross(x2 - x1, x3 - x1) / 2
+        self._dev = None
+
+    def flipFaces(self):
+        """base/surfaces.py:603-605"""
+        self.faces = self.faces[:, [0, 2, 1]]
+        self._topo = None
+        self.computeCentersAreas()
+
+    def surfVolume(self):
+        """sum_f det(v[f]) / 6  (base/surfaces.py:1645-1651)"""
+        return float(np.linalg.det(self.vertices[self.faces]).sum() / 6)
+
+    def surfArea(self):
+        return float(np.sqrt((self.surfel ** 2).sum(axis=1)).sum())
+
+    def getEdges(self):
+        """Edges, edge -> faces and boundary flags (sparse; the reference builds
+        a dense V x V table, base/surfaces.py:61-116)."""
+        f = self.faces
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        es = np.sort(e, axis=1)
+        uniq, inv, cnt = np.unique(es, axis=0, return_inverse=True, return_counts=True)
+        self.edges = uniq
+        nf = f.shape[0]
+        self.faceEdges = inv.reshape(3, nf).T
+        self.edgeFaces = -np.ones((uniq.shape[0], 2), dtype=int)
+        fid = np.tile(np.arange(nf), 3)
+        order = np.argsort(inv, kind='stable')
+        first = np.ones(len(order), dtype=bool)
+        first[1:] = inv[order][1:] != inv[order][:-1]
+        self.edgeFaces[inv[order][first], 0] = fid[order][first]
+        sec = ~first
+        self.edgeFaces[inv[order][sec], 1] = fid[order][sec]
+        self.bdry = (cnt == 1).astype(int)
+        vb = np.zeros(self.vertices.shape[0], dtype=int)
+        vb[uniq[cnt == 1].ravel()] = 1
+        self.vertices_bdry = vb
+        return self.edges
+
+    # ---------------------------------------------------------- device view
+    def topology(self):
+        """int32 faces and CSR adjacency on the device (cached per topology)."""
+        import torch
+        from . import _backend as B
+        if self._topo is None or self._topo[0].shape[0] != self.faces.shape[0]:
+            f32 = np.ascontiguousarray(self.faces, dtype=np.int32)
+            nv = self.vertices.shape[0]
+            ptr = np.zeros(nv + 1, dtype=np.int32)
+            idx = np.zeros(3 * f32.shape[0], dtype=np.int32)
+            B.check(B.LIB.lddmm_surface_adjacency(f32.ctypes.data, f32.shape[0], nv, ptr.ctypes.data,
+                                                  idx.ctypes.data), "surface_adjacency")
+            dev = B.device()
+            self._topo = (torch.from_numpy(f32).to(dev), torch.from_numpy(ptr).to(dev),
+                          torch.from_numpy(idx).to(dev))
+        return self._topo
+
+    def device(self, vertices=None):
+        """LddmmSurface view (+ keep-alive tensors).  `vertices` may be a device
+        tensor [nv, 3] that replaces this surface's vertices for the call."""
+        from . import _backend as B
+        faces, ptr, idx = self.topology()
+        if vertices is None:
+            if self._dev is None:
+                self._dev = (B.to_dev(self.vertices), B.to_dev(np.asarray(self.weights, dtype=np.float64)))
+            v, w = self._dev
+        else:
+            v = B.to_dev(vertices)
+            if self._dev is None:
+                self._dev = (None, B.to_dev(np.asarray(self.weights, dtype=np.float64)))
+            w = self._dev[1]
+        s = B.LddmmSurface()
+        s.vertices = v.data_ptr()
+        s.nv = v.shape[0]
+        s.faces = faces.data_ptr()
+        s.nf = faces.shape[0]
+        s.vweights = w.data_ptr() if not np.all(self.weights == 1) else None
+        s.adj_ptr = ptr.data_ptr()
+        s.adj_idx = idx.data_ptr()
+        return s, (v, w, faces, ptr, idx)

@@ -1,0 +1,89 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry point
+declared in include/lddmm_amd.h, and rejects bad arguments before any device
+work (those paths return before a HIP call)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "lddmm_amd.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lddmm_[a-zA-Z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    for s in ("lddmm_applyK", "lddmm_applyDiffKT", "lddmm_applyktensor", "lddmm_surface_varifold",
+              "lddmm_shoot_forward", "lddmm_shoot_adjoint", "lddmm_epdiff_forward",
+              "lddmm_epdiff_adjoint"):
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol(lib):
+    missing = [s for s in declared_symbols() if not hasattr(lib.LIB, s)]
+    assert not missing, missing
+
+
+def test_python_signature_table_matches_header(lib):
+    declared = set(declared_symbols())
+    for name in lib.SIGNATURES:
+        assert name in declared, name
+
+
+def test_bad_arguments_are_rejected(lib):
+    B = lib
+    null = ctypes.c_void_p(0)
+    # null kernel description
+    rc = B.LIB.lddmm_applyK(None, null, 1, null, 1, 3, null, 3, null, null)
+    assert rc == B.EINVAL
+    assert b"kernel" in B.LIB.lddmm_last_error()
+    kd = B.kernel_desc([1.0])
+    kd.nscale = 0
+    assert B.LIB.lddmm_applyK(kd, null, 1, null, 1, 3, null, 3, null, null) == B.EINVAL
+    kd = B.kernel_desc([1.0])
+    kd.kind = 7
+    assert B.LIB.lddmm_applyK(kd, null, 1, null, 1, 3, null, 3, null, null) == B.EUNSUPPORTED
+    kd = B.kernel_desc([1.0])
+    assert B.LIB.lddmm_applyK(kd, null, 1, null, 1, 5, null, 3, null, null) == B.EINVAL   # dim
+    assert B.LIB.lddmm_applyK(kd, null, -1, null, 1, 3, null, 3, null, null) == B.EINVAL  # n < 0
+    assert B.LIB.lddmm_applyDiffKT(kd, null, 4, null, 5, 3, null, null, 3, B.DIFFKT_LDDMM, 1.0,
+                                   null, null) == B.EINVAL   # lddmm needs n == m
+    assert B.LIB.lddmm_applyktensor(kd, null, 4, null, 5, 3, null, null, null, null, 9, null,
+                                    null) == B.EUNSUPPORTED
+    kd.sigma[0] = -1.0
+    assert B.LIB.lddmm_applyK(kd, null, 1, null, 1, 3, null, 3, null, null) == B.EINVAL
+    # n == 0 is a no-op
+    kd = B.kernel_desc([1.0])
+    assert B.LIB.lddmm_applyK(kd, null, 0, null, 0, 3, null, 3, null, null) == B.OK
+
+
+def test_adjacency_builder(lib):
+    B = lib
+    f = np.array([[0, 1, 2], [0, 2, 3], [3, 2, 1]], dtype=np.int32)
+    ptr = np.zeros(5, dtype=np.int32)
+    idx = np.zeros(9, dtype=np.int32)
+    rc = B.LIB.lddmm_surface_adjacency(f.ctypes.data, 3, 4, ptr.ctypes.data, idx.ctypes.data)
+    assert rc == B.OK
+    assert ptr.tolist() == [0] + np.cumsum(np.bincount(f.ravel(), minlength=4)).tolist()
+    for v in range(4):
+        got = sorted((e // 3, e % 3) for e in idx[ptr[v]:ptr[v + 1]])
+        want = sorted((fi, c) for fi in range(3) for c in range(3) if f[fi, c] == v)
+        assert got == want
+    bad = np.array([[0, 1, 9]], dtype=np.int32)
+    assert B.LIB.lddmm_surface_adjacency(bad.ctypes.data, 1, 4, ptr.ctypes.data, idx.ctypes.data) == B.EINVAL
+
+
+def test_kernel_desc_mirrors_kernelspec(lib):
+    kd = lib.kernel_desc([2.5, 5.0], kp=-1, prec='float32')
+    assert kd.nscale == 2 and kd.sigma[1] == 5.0 and kd.prec == lib.PREC_F32C
+    with pytest.raises(ValueError):
+        lib.kernel_desc([1.0], prec='float16')
+    with pytest.raises(NotImplementedError):
+        lib.kernel_kind('laplacian')

@@ -1,0 +1,351 @@
+"""Pins the FP64 oracle (oracle/oracle.py) without retyping its formulas.
+
+Each check is chosen so that a plausible mistake fails it: closed forms catch
+a dropped 1/2 or a wrong sigma power, finite differences catch wrong signs
+and missing terms of the derivative sums, symmetry identities catch swapped
+operands, and invariances catch orientation / frame mistakes in the data
+terms.  The reference's own finite-difference self-checks are mirrored where
+they exist (Kernel.testDiffKT, base/kernelFunctions.py:629-635;
+testEndpointGradient, base/surfaceMatching.py:678-698; the BFGS gradient
+test, base/bfgs.py:167-176).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from lddmm_amd import synth as S
+
+RNG = np.random.default_rng(1234)
+
+
+def rnd(*shape, scale=1.0):
+    return RNG.normal(size=shape) * scale
+
+
+# ----------------------------------------------------------------- applyK
+def test_applyK_single_point_returns_weight():
+    x = rnd(1, 3)
+    a = rnd(1, 3)
+    np.testing.assert_allclose(O.applyK(x, x, a, [1.7]), a, rtol=1e-15)
+
+
+def test_applyK_two_points_closed_form():
+    s = 1.3
+    y = np.array([[0., 0., 0.]])
+    x = np.array([[0.4, -1.2, 0.7]])
+    a = np.array([[2.0, -1.0, 0.5]])
+    r2 = (x ** 2).sum()
+    np.testing.assert_allclose(O.applyK(y, x, a, [s]), np.exp(-r2 / (2 * s * s)) * a, rtol=1e-14)
+
+
+def test_applyK_multiscale_weights():
+    # w_s = s^-1 / sum s^-1 ; checked on a two-point pair against the closed form
+    y = np.zeros((1, 3))
+    x = np.array([[1.0, 0.5, -0.3]])
+    a = np.ones((1, 1))
+    s = np.array([0.8, 2.0, 5.0])
+    r2 = (x ** 2).sum()
+    w = (1 / s) / (1 / s).sum()
+    expect = (w * np.exp(-r2 / (2 * s ** 2))).sum()
+    np.testing.assert_allclose(O.applyK(y, x, a, s)[0, 0], expect, rtol=1e-14)
+    # a repeated sigma is the same kernel
+    np.testing.assert_allclose(O.applyK(y, x, a, [2.0, 2.0]), O.applyK(y, x, a, [2.0]), rtol=1e-15)
+
+
+def test_applyK_large_sigma_limit_is_column_sum():
+    y, x, a = rnd(5, 3), rnd(7, 3), rnd(7, 2)
+    np.testing.assert_allclose(O.applyK(y, x, a, [1e7]), np.tile(a.sum(0), (5, 1)), rtol=1e-10)
+
+
+def test_applyK_adjoint_symmetry():
+    y, x = rnd(40, 3), rnd(60, 3)
+    p, a = rnd(40, 3), rnd(60, 3)
+    lhs = (p * O.applyK(y, x, a, [1.5])).sum()
+    rhs = (a * O.applyK(x, y, p, [1.5])).sum()
+    np.testing.assert_allclose(lhs, rhs, rtol=1e-13)
+
+
+def test_applyK_translation_rotation_invariance():
+    y, x, a = rnd(20, 3), rnd(30, 3), rnd(30, 1)
+    Q, _ = np.linalg.qr(rnd(3, 3))
+    t = rnd(1, 3) * 10
+    np.testing.assert_allclose(O.applyK(y @ Q.T + t, x @ Q.T + t, a, [1.1]),
+                               O.applyK(y, x, a, [1.1]), rtol=1e-12)
+
+
+# ------------------------------------------------------------ applyDiffKT
+def _fd(f, x, dx, eps=1e-6):
+    return (f(x + eps * dx) - f(x - eps * dx)) / (2 * eps)
+
+
+@pytest.mark.parametrize("sig", [[1.2], [0.9, 2.1]])
+def test_diffKT_plain_is_gradient_wrt_targets(sig):
+    y, x, p, a = rnd(15, 3), rnd(20, 3), rnd(15, 3), rnd(20, 3)
+    dy = rnd(15, 3)
+    fd = _fd(lambda yy: (p * O.applyK(yy, x, a, sig)).sum(), y, dy)
+    g = O.applyDiffKT(y, x, p, a, sig)
+    np.testing.assert_allclose((g * dy).sum(), fd, rtol=1e-7)
+
+
+def test_diffKT_sym_is_gradient_of_pKa():
+    x, p, a = rnd(18, 3), rnd(18, 3), rnd(18, 3)
+    dx = rnd(18, 3)
+    fd = _fd(lambda xx: (p * O.applyK(xx, xx, a, [1.4])).sum(), x, dx)
+    g = O.applyDiffKT(x, x, p, a, [1.4], sym=True)
+    np.testing.assert_allclose((g * dx).sum(), fd, rtol=1e-7)
+
+
+def test_diffKT_lddmm_is_gradient_of_pKa_minus_raKa():
+    x, p, a = rnd(18, 3), rnd(18, 3), rnd(18, 3)
+    dx = rnd(18, 3)
+    r = 0.7
+    f = lambda xx: (p * O.applyK(xx, xx, a, [1.4])).sum() - r * (a * O.applyK(xx, xx, a, [1.4])).sum()
+    g = O.applyDiffKT(x, x, p, a, [1.4], regweight=r, lddmm=True)
+    np.testing.assert_allclose((g * dx).sum(), _fd(f, x, dx), rtol=1e-7)
+
+
+# ------------------------------------------------------------ applyktensor
+def test_ktensor_single_pair_closed_form():
+    y = np.zeros((1, 3))
+    x = np.array([[0.3, 0.2, -0.1]])
+    by, bx = np.array([[1.0, 2.0, 0.5]]), np.array([[-0.5, 1.0, 2.0]])
+    ay, ax = np.array([1.5]), np.array([0.7])
+    K = np.exp(-(x ** 2).sum() / 2 / 0.8 ** 2)
+    expect = K * (1.5 * 0.7 + (by * bx).sum() ** 2)
+    np.testing.assert_allclose(O.applyktensor(y, x, ay, ax, by, bx, [0.8]), [expect], rtol=1e-14)
+
+
+def test_ktensor_symmetry():
+    y, x = rnd(25, 3), rnd(30, 3)
+    ay, ax, by, bx = rnd(25), rnd(30), rnd(25, 3), rnd(30, 3)
+    np.testing.assert_allclose(O.applyktensor(y, x, ay, ax, by, bx, [1.0]).sum(),
+                               O.applyktensor(x, y, ax, ay, bx, by, [1.0]).sum(), rtol=1e-13)
+
+
+# --------------------------------------------------- dense-weight helpers
+def test_kmat_reduces_to_applyK():
+    y, x, a = rnd(12, 3), rnd(17, 3), rnd(17, 3)
+    beta = np.broadcast_to(a[None], (12, 17, 3))
+    np.testing.assert_allclose(O.applykmat(y, x, beta, [1.3, 2.0]), O.applyK(y, x, a, [1.3, 2.0]),
+                               rtol=1e-13)
+
+
+def test_kdiffmat_reduces_to_diffKT_plain():
+    y, x, p, a = rnd(12, 3), rnd(17, 3), rnd(12, 3), rnd(17, 3)
+    beta = p @ a.T
+    np.testing.assert_allclose(O.applykdiffmat(y, x, beta, [1.3]), O.applyDiffKT(y, x, p, a, [1.3]),
+                               rtol=1e-12)
+
+
+# ------------------------------------------- first / second derivatives
+@pytest.mark.parametrize("sig", [[1.1], [0.8, 1.9]])
+def test_diffK_1and2_is_directional_derivative_of_applyK(sig):
+    x, a1, a2 = rnd(16, 3), rnd(16, 3), rnd(16, 2)
+    fd = _fd(lambda t: O.applyK(x + t * a1, x + t * a1, a2, sig), 0.0, 1.0)
+    np.testing.assert_allclose(O.applyDiffK(x, a1, a2, sig, option='1and2'), fd, rtol=1e-6, atol=1e-9)
+
+
+def test_diffK_options_1_plus_2_make_1and2():
+    x, a1, a2 = rnd(16, 3), rnd(16, 3), rnd(16, 3)
+    tot = O.applyDiffK(x, a1, a2, [1.2], option='1') + O.applyDiffK(x, a1, a2, [1.2], option='2')
+    np.testing.assert_allclose(tot, O.applyDiffK(x, a1, a2, [1.2], option='1and2'), rtol=1e-12)
+    # option 1 only moves the target point: derivative of sum_l K(x_i + t a1_i, x_l) a2_l
+    i = 3
+    def row(t):
+        xx = x.copy()
+        yi = x[i:i + 1] + t * a1[i:i + 1]
+        return O.applyK(yi, xx, a2, [1.2])[0]
+    np.testing.assert_allclose(O.applyDiffK(x, a1, a2, [1.2], option='1')[i], _fd(row, 0.0, 1.0),
+                               rtol=1e-6, atol=1e-10)
+
+
+@pytest.mark.parametrize("sig", [[1.3], [0.9, 2.2]])
+def test_ddiffK_11and12_is_derivative_of_diffKT_plain(sig):
+    x, a1, a2, p = rnd(14, 3), rnd(14, 3), rnd(14, 3), rnd(14, 3)
+    fd = _fd(lambda t: O.applyDiffKT(x + t * p, x + t * p, a1, a2, sig), 0.0, 1.0)
+    np.testing.assert_allclose(O.applyDDiffK(x, a1, a2, p, sig, option='11and12'), fd,
+                               rtol=1e-6, atol=1e-9)
+
+
+def test_ddiffK_options_11_plus_12_make_11and12():
+    x, a1, a2, p = rnd(14, 3), rnd(14, 3), rnd(14, 3), rnd(14, 3)
+    tot = O.applyDDiffK(x, a1, a2, p, [1.2], option='11') + O.applyDDiffK(x, a1, a2, p, [1.2], option='12')
+    np.testing.assert_allclose(tot, O.applyDDiffK(x, a1, a2, p, [1.2], option='11and12'), rtol=1e-12)
+
+
+# --------------------------------------------------------- surface terms
+@pytest.fixture(scope="module")
+def spheres():
+    v, f = S.icosphere(2, radius=3.0)
+    v2 = S.bump_deform(v, f, nbumps=6, width=1.0, amp_sd=0.3, seed=3)
+    return (v, f), (v2, f)
+
+
+def test_icosphere_closed_forms():
+    v, f = S.icosphere(3, radius=2.0)
+    assert v.shape == (642, 3) and f.shape == (1280, 3)
+    np.testing.assert_allclose(np.linalg.norm(v, axis=1), 2.0, rtol=1e-14)
+    vol = O.surface_volume(v, f)
+    assert vol > 0  # outward orientation
+    assert abs(vol - 4 / 3 * np.pi * 8) / (4 / 3 * np.pi * 8) < 0.02
+    c, Sf, fw = O.surface_geometry(v, f)
+    area = np.linalg.norm(Sf, axis=1).sum()
+    assert abs(area - 4 * np.pi * 4) / (4 * np.pi * 4) < 0.02
+
+
+def test_varifold_self_distance_zero(spheres):
+    A, _ = spheres
+    np.testing.assert_allclose(O.varifoldNorm(A, A, [1.0]), 0.0, atol=1e-9 * O.varifoldNorm0(A, [1.0]))
+
+
+def test_varifold_rigid_and_flip_invariance(spheres):
+    A, Bs = spheres
+    d0 = O.varifoldNorm(A, Bs, [0.8])
+    Q, _ = np.linalg.qr(rnd(3, 3))
+    t = rnd(1, 3)
+    A2 = (A[0] @ Q.T + t, A[1])
+    B2 = (Bs[0] @ Q.T + t, Bs[1])
+    np.testing.assert_allclose(O.varifoldNorm(A2, B2, [0.8]), d0, rtol=1e-11)
+    Bflip = (Bs[0], Bs[1][:, [0, 2, 1]])
+    np.testing.assert_allclose(O.varifoldNorm(A, Bflip, [0.8]), d0, rtol=1e-12)
+    assert d0 > 0
+
+
+def test_varifold_single_triangle_pair_closed_form():
+    f = np.array([[0, 1, 2]])
+    A = (np.array([[0., 0, 0], [1, 0, 0], [0, 1, 0]]), f)
+    B = (np.array([[0., 0, 1], [1, 0, 1], [0, 0, 2]]), f)
+    cA, SA, _ = O.surface_geometry(*A)
+    cB, SB, _ = O.surface_geometry(*B)
+    aA, aB = np.linalg.norm(SA), np.linalg.norm(SB)
+    cos = (SA[0] @ SB[0]) / (aA * aB)
+    K = np.exp(-((cA - cB) ** 2).sum() / 2)
+    cross = aA * aB * K * (1 + cos ** 2)
+    expect = 2 * aA ** 2 + 2 * aB ** 2 - 2 * cross   # E(A,A) = 2 a^2 (cos = 1)
+    np.testing.assert_allclose(O.varifoldNorm(A, B, [1.0]), expect, rtol=1e-9)
+
+
+@pytest.mark.parametrize("sig", [[0.7], [0.5, 1.5]])
+def test_varifold_gradient_fd(spheres, sig):
+    A, Bs = spheres
+    v, f = A
+    dv = rnd(*v.shape)
+    fd = _fd(lambda vv: O.varifoldNorm((vv, f), Bs, sig), v, dv, eps=1e-6)
+    g = O.varifoldNormGradient(A, Bs, sig)
+    np.testing.assert_allclose((g * dv).sum(), fd, rtol=1e-6)
+
+
+def test_current_orientation_and_gradient(spheres):
+    A, Bs = spheres
+    v, f = A
+    flip = (Bs[0], Bs[1][:, [0, 2, 1]])
+    # flipping the target reverses the cross term only
+    cross = -2 * (O.currentNormDef(A, Bs, [0.8]) - O.currentNorm0(A, [0.8]))
+    cross_f = -2 * (O.currentNormDef(A, flip, [0.8]) - O.currentNorm0(A, [0.8]))
+    np.testing.assert_allclose(cross_f, -cross, rtol=1e-12)
+    np.testing.assert_allclose(O.currentNorm(A, A, [0.8]), 0.0, atol=1e-10 * O.currentNorm0(A, [0.8]))
+    dv = rnd(*v.shape)
+    fd = _fd(lambda vv: O.currentNorm((vv, f), Bs, [0.8]), v, dv)
+    np.testing.assert_allclose((O.currentNormGradient(A, Bs, [0.8]) * dv).sum(), fd, rtol=1e-6)
+    fd = _fd(lambda vv: O.currentMagnitude((vv, f), [0.8]), v, dv)
+    np.testing.assert_allclose((O.currentMagnitudeGradient(A, [0.8]) * dv).sum(), fd, rtol=1e-6)
+
+
+# ----------------------------------------------------------- curve terms
+@pytest.fixture(scope="module")
+def curves():
+    t = np.linspace(0, 2 * np.pi, 41)[:-1]
+    v1 = np.stack([np.cos(t), 0.7 * np.sin(t)], 1) * 2
+    v2 = np.stack([np.cos(t) + 0.1 * np.cos(3 * t), 0.6 * np.sin(t)], 1) * 2.1
+    f = np.stack([np.arange(40), (np.arange(40) + 1) % 40], 1)
+    return (v1, f), (v2, f)
+
+
+def test_curve_varifold(curves):
+    A, Bc = curves
+    v, f = A
+    assert abs(O.curveVarifoldNormDef(A, A, [0.5]) + O.curveVarifoldNorm0(A, [0.5])) < 1e-9
+    dv = rnd(*v.shape)
+    fd = _fd(lambda vv: O.curveVarifoldNormDef((vv, f), Bc, [0.5]), v, dv)
+    np.testing.assert_allclose((O.curveVarifoldNormGradient(A, Bc, [0.5]) * dv).sum(), fd, rtol=1e-6)
+    flip = (Bc[0], Bc[1][:, [1, 0]])
+    np.testing.assert_allclose(O.curveVarifoldNormDef(A, flip, [0.5]), O.curveVarifoldNormDef(A, Bc, [0.5]),
+                               rtol=1e-12)
+
+
+def test_curve_current(curves):
+    A, Bc = curves
+    v, f = A
+    dv = rnd(*v.shape)
+    fd = _fd(lambda vv: O.curveCurrentNormDef((vv, f), Bc, [0.5]), v, dv)
+    np.testing.assert_allclose((O.curveCurrentNormGradient(A, Bc, [0.5]) * dv).sum(), fd, rtol=1e-6)
+
+
+# ------------------------------------------------------------- shooting
+def test_zero_control_is_identity():
+    x0 = rnd(10, 3)
+    xt = O.landmarkDirectEvolutionEuler(x0, np.zeros((5, 10, 3)), [1.0])
+    np.testing.assert_array_equal(xt, np.broadcast_to(x0, xt.shape))
+
+
+def test_one_step_closed_form():
+    x0, a = rnd(6, 3), rnd(1, 6, 3)
+    xt = O.landmarkDirectEvolutionEuler(x0, a, [1.2])
+    np.testing.assert_allclose(xt[1], x0 + O.applyK(x0, x0, a[0], [1.2]), rtol=1e-14)
+
+
+def test_matching_gradient_fd(spheres):
+    A, Bs = spheres
+    v, f = A
+    T = 4
+    at = rnd(T, v.shape[0], 3, scale=0.05)
+    d = rnd(*at.shape)
+    obj = lambda aa: O.matching_objective(v, f, aa, Bs, [1.5], [0.8], regweight=0.5)[0]
+    fd = _fd(obj, at, d)
+    # the Euclidean gradient K(x_t)(2 r a_t - p_{t+1}) / T is the exact discrete adjoint
+    g = O.matching_gradient(v, f, at, Bs, [1.5], [0.8], regweight=0.5, euclidean=True)
+    np.testing.assert_allclose((g * d).sum(), fd, rtol=1e-6)
+
+
+def test_matching_gradient_current_fd(spheres):
+    A, Bs = spheres
+    v, f = A
+    at = rnd(3, v.shape[0], 3, scale=0.05)
+    d = rnd(*at.shape)
+    obj = lambda aa: O.matching_objective(v, f, aa, Bs, [1.5], [0.8], kind='current')[0]
+    g = O.matching_gradient(v, f, at, Bs, [1.5], [0.8], kind='current', euclidean=True)
+    np.testing.assert_allclose((g * d).sum(), _fd(obj, at, d), rtol=1e-6)
+
+
+# ------------------------------------------------------ geodesic shooting
+def test_epdiff_matches_second_order_evolution():
+    x0, a0 = rnd(12, 3), rnd(12, 3, scale=0.3)
+    xe, ae = O.landmarkEPDiff(8, x0, a0, [1.3])
+    xs, as_ = O.secondOrderEvolution(x0, a0, [1.3], 1 / 8)
+    np.testing.assert_allclose(xe, xs, rtol=1e-14)
+    np.testing.assert_allclose(ae, as_[:8], rtol=1e-14)
+
+
+def test_hamiltonian_drift_shrinks_with_dt():
+    x0, a0 = rnd(12, 3), rnd(12, 3, scale=0.4)
+    H = lambda x, a: 0.5 * (a * O.applyK(x, x, a, [1.3])).sum()
+    drift = []
+    for T in (10, 40):
+        xt, at = O.secondOrderEvolution(x0, a0, [1.3], 1 / T)
+        drift.append(abs(H(xt[-1], at[-1]) - H(x0, a0)))
+    assert drift[1] < drift[0] / 2.5   # first-order scheme: O(dt)
+
+
+def test_second_order_gradient_fd():
+    x0, a0 = rnd(10, 3), rnd(10, 3, scale=0.3)
+    tgt = rnd(10, 3)
+    T = 5
+    # J(a0) = 1/2 a0.K a0 + |x_T - tgt|^2 ; px1 = -2(x_T - tgt), pa1 = 0
+    def J(a):
+        xt, at = O.secondOrderEvolution(x0, a, [1.2], 1 / T)
+        return 0.5 * (a * O.applyK(x0, x0, a, [1.2])).sum() + ((xt[-1] - tgt) ** 2).sum()
+    xt, _ = O.secondOrderEvolution(x0, a0, [1.2], 1 / T)
+    px1 = -2 * (xt[-1] - tgt)
+    da0 = O.secondOrderGradient(x0, a0, px1, np.zeros_like(a0), [1.2], 1 / T)[0]
+    d = rnd(10, 3)
+    np.testing.assert_allclose((da0 * d).sum(), _fd(J, a0, d), rtol=1e-6)
