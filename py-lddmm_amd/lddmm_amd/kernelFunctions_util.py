@@ -38,7 +38,8 @@ def _w(a, n):
     t = B.to_dev(a)
     if t.dim() == 1:
         t = t[:, None]
-    return t.reshape(n, -1).contiguous()
+    width = t.shape[-1] if n == 0 else -1   # reshape(0, -1) is ambiguous
+    return t.reshape(n, width).contiguous()
 
 
 def applyK(y, x, a, name, scale, order, cpu=False, dtype='float64', KP=-1):

@@ -182,8 +182,11 @@ def _epdiff_adjoint(xt, at, px1, pa1, KparDiff, T):
     pxt = torch.empty_like(xt)
     pat = torch.empty_like(xt)
     da0 = torch.empty((n, dim), dtype=torch.float64, device=xt.device)
-    B.check(B.LIB.lddmm_epdiff_adjoint(_kd(KparDiff), T, n, dim, B.ptr(xt), B.ptr(at), B.ptr(B.to_dev(px1)),
-                                       B.ptr(B.to_dev(pa1)), B.ptr(pxt), B.ptr(pat), B.ptr(da0),
+    # device copies stay referenced until the launch is enqueued (a freed
+    # temporary would be handed to the next allocation before the kernel reads it)
+    px1_, pa1_ = B.to_dev(px1), B.to_dev(pa1)
+    B.check(B.LIB.lddmm_epdiff_adjoint(_kd(KparDiff), T, n, dim, B.ptr(xt), B.ptr(at), B.ptr(px1_),
+                                       B.ptr(pa1_), B.ptr(pxt), B.ptr(pat), B.ptr(da0),
                                        B.stream_handle()), "epdiff_adjoint")
     return pxt, pat, da0
 

@@ -103,8 +103,9 @@ def _face_data(fv, vertices=None):
 def _gather(dk, dz1, z1):
     d, keep = dk
     px = torch.empty((d.nv, 3), dtype=torch.float64, device=dz1.device)
-    B.check(B.LIB.lddmm_surface_face_gather(ctypes.byref(d), B.ptr(dz1.contiguous()), B.ptr(z1.contiguous()),
-                                            B.ptr(px), B.stream_handle()), "surface_face_gather")
+    dz1_, z1_ = dz1.contiguous(), z1.contiguous()
+    B.check(B.LIB.lddmm_surface_face_gather(ctypes.byref(d), B.ptr(dz1_), B.ptr(z1_), B.ptr(px),
+                                            B.stream_handle()), "surface_face_gather")
     return px
 
 
