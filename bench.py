@@ -30,7 +30,7 @@ SIGMAS = (2.5, 5.0, 10.0)
 # algorithmic work per pair (FMA = 2 FLOP) and the MI355X published FP32 peak
 FLOP_PER_PAIR = {"applyK": 15, "applyDiffKT": 22}
 PEAK_FP32_TFLOPS = 157.3
-PEAK_EXP_PER_S = 256 * 4 * 16 / 4 * 2.4e9   # quarter-rate v_exp_f32 (assumed; see profiles/)
+PEAK_EXP_PER_S = 256 * 4 * 16 / 2 * 2.4e9   # v_exp_f32 issues at half the FMA rate (profiles/gfx950_rates.md)
 
 
 def log(msg):
@@ -64,6 +64,65 @@ def cpu_baseline(y, x, a, p, target_s=12.0):
     }
 
 
+def varifold_microbench(B, torch, synth, np):
+    """Varifold norm between two seeded random triangulated surfaces of 100,000
+    triangles each (50,002 points on a radius-50 sphere -> convex hull; the
+    second with another seed and radial bumps), spatial sigma = 1: three blocks
+    of 1e10 face pairs (|S1|^2, -2<S1,S2>, |S2|^2)."""
+    from lddmm_amd.surfaces import Surface
+    from lddmm_amd import surfaceDistances as sd
+    v1, f1 = synth.sphere_hull(50_002, 50.0, seed=11)
+    v2, f2 = synth.sphere_hull(50_002, 50.0, seed=12)
+    v2 = synth.bump_deform(v2, f2, 30, 8.0, 2.0, seed=13)
+    S1, S2 = Surface((f1, v1)), Surface((f2, v2))
+    res = {"faces": [int(f1.shape[0]), int(f2.shape[0])], "sigma": 1.0}
+    vals = {}
+    for pr in ("float32", "float64"):
+        kd = B.kernel_desc([1.0], -1, pr)
+        sd.varifold_device(kd, S1, S2)          # warm-up (features, adjacency, workspace)
+        sd.varifold_device(kd, S2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eD, _ = sd.varifold_device(kd, S1, S2)
+        e0, _ = sd.varifold_device(kd, S2)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        vals[pr] = float(eD.item()) + float(e0.item())
+        pairs = f1.shape[0] * (f1.shape[0] + f2.shape[0]) + f2.shape[0] ** 2
+        res[pr] = {"seconds": dt, "face_pairs_per_s": pairs / dt, "varifold_norm": vals[pr]}
+        log(f"varifold 100k ({pr}): {dt * 1e3:.1f} ms, {pairs / dt:.3g} face-pairs/s, norm {vals[pr]:.6g}")
+    res["fp32_rel_err_vs_fp64"] = abs(vals["float32"] - vals["float64"]) / abs(vals["float64"])
+    return res
+
+
+def registration_cfg1(torch):
+    """Plain surface matching (icosphere 4, R=10, 20 bumps width 3 / amp N(0,1),
+    sigma_def 2.5, varifold sigma 1, T = 10, zero initial momentum, BFGS 50
+    iterations, float64): wall seconds per registration."""
+    import logging
+    from lddmm_amd import synth
+    from lddmm_amd.surfaces import Surface
+    from lddmm_amd.kernelFunctions import Kernel
+    from lddmm_amd.surfaceMatching import SurfaceMatching
+    logging.getLogger().setLevel(logging.WARNING)
+    (v, f), (v2, _) = synth.plain_matching_pair(seed=0)
+    out = {}
+    for pr in ("float64", "float32"):
+        opts = {'KparDiff': Kernel(name='gauss', sigma=2.5), 'KparDist': Kernel(name='gauss', sigma=1.0),
+                'errorType': 'varifold', 'timeStep': 0.1, 'pk_dtype': pr, 'maxIter': 50, 'mode': 'quiet'}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sm = SurfaceMatching(Template=Surface((f, v)), Target=Surface((f, v2)), options=opts)
+        sm.optimizeMatching()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[pr] = {"seconds": dt, "iterations": int(sm.bfgs_iterations), "final_objective": sm.obj,
+                   "final_data_term": sm.varifold_data(), "initial_objective": sm.bfgs_history[0]}
+        log(f"registration cfg1 ({pr}): {dt:.2f} s, {sm.bfgs_iterations} iterations, "
+            f"objective {sm.bfgs_history[0]:.4f} -> {sm.obj:.4f}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +131,8 @@ def main():
     ap.add_argument("--n", type=int, default=50_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference pass")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the varifold and registration measurements reported under `extra`")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -151,7 +212,26 @@ def main():
         "pair_kernel_ms": {"applyK": round(avgK, 4), "applyDiffKT": round(avgD, 4)},
         "pair_evals_per_s_kernel": {"applyK": N * M / (avgK * 1e-3), "applyDiffKT": N * M / (avgD * 1e-3)},
         "exp_per_s": N * M / (avgK * 1e-3),
+        "exp_frac_of_peak": N * M / (avgK * 1e-3) / PEAK_EXP_PER_S,
     }
+
+    # A/B: the same step with MFMA accumulation (applyK, plain applyDiffKT)
+    B.set_pair_engine('mfma')
+    step()
+    torch.cuda.synchronize()
+    B.profile_enable(True)
+    B.profile_read()
+    t0 = time.perf_counter()
+    step()
+    torch.cuda.synchronize()
+    t_mfma = time.perf_counter() - t0
+    ms_m = B.profile_read()
+    B.profile_enable(False)
+    B.set_pair_engine('valu')
+    extra["mfma_engine"] = {"ms_per_step": t_mfma * 1e3,
+                            "pair_kernel_ms": {"applyK": round(sum(ms_m[0::2]) / max(len(ms_m[0::2]), 1), 4),
+                                               "applyDiffKT": round(sum(ms_m[1::2]) / max(len(ms_m[1::2]), 1), 4)}}
+    log(f"mfma engine: {t_mfma * 1e3:.2f} ms/step, kernels {extra['mfma_engine']['pair_kernel_ms']}")
 
     if not args.no_fp64:
         # FP64 plain path: rate and the FP32 error against it (norm-wise relative)
@@ -211,6 +291,10 @@ def main():
         },
         "extra": extra,
     }
+    if not args.no_extras:
+        extra["varifold_100k"] = varifold_microbench(B, torch, synth, np)
+        extra["registration_plain_cfg1"] = registration_cfg1(torch)
+
     if not args.no_cpu:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(y_h, x_h, a_h, p_h)

@@ -86,6 +86,16 @@ int lddmm_release_workspace(void);
 int lddmm_profile_enable(int32_t on);
 int lddmm_profile_read(double* ms_each, int64_t cap, int64_t* launches);
 
+/* Accumulation engine of the float32 (LDDMM_PREC_F32C) applyK and plain
+ * applyDiffKT sums (no counterpart in the reference): LDDMM_ENGINE_VALU
+ * (default) accumulates each pair on the vector ALUs; LDDMM_ENGINE_MFMA
+ * computes the Gaussian values on the VALU and accumulates them against the
+ * per-source weight columns on the matrix cores (v_mfma_f32_16x16x4f32).
+ * Process-wide setting. */
+#define LDDMM_ENGINE_VALU 0
+#define LDDMM_ENGINE_MFMA 1
+int lddmm_set_pair_engine(int32_t engine);
+
 /* out[n,k] = sum_s w_s sum_j K_s(y_i, x_j) a[j,:]
  * Replaces ku.applyK / applyK_numba / applyK_pykeops
  * (base/kernelFunctions_util.py:443-466, :499-535), reached from

@@ -1,7 +1,7 @@
 // Gaussian kernel sums on point sets: applyK, applyDiffKT, applyktensor,
 // applyDiffK, applyDDiffK (pair machinery in pair.hpp) and the dense-weight
 // sums applykmat / applykdiffmat.
-#include "ops.hpp"
+#include "ops_mfma.hpp"
 
 #include <algorithm>
 #include <vector>
@@ -395,6 +395,8 @@ extern "C" int lddmm_applyK(const lddmm_kernel* kd, const double* y, int64_t n, 
     if (m == 0) return launch_fill(out, size_t(n) * k, 0.0, st);
     LDDMM_REQUIRE(x && a, "null x/a");
     const bool ms = kd->nscale > 1;
+    if (kd->prec == LDDMM_PREC_F32C && pair_engine() == LDDMM_ENGINE_MFMA)
+        return mfma_applyK(kd, y, int(n), x, int(m), dim, a, k, out, st);
     if (kd->prec == LDDMM_PREC_F32C)
         return ms ? applyK_t<float, true>(kd, y, n, x, m, dim, a, k, out, st)
                   : applyK_t<float, false>(kd, y, n, x, m, dim, a, k, out, st);
@@ -418,6 +420,8 @@ extern "C" int lddmm_applyDiffKT(const lddmm_kernel* kd, const double* y, int64_
     const bool ms = kd->nscale > 1;
     const bool sym = mode != LDDMM_DIFFKT_PLAIN;
     const double r2w = mode == LDDMM_DIFFKT_LDDMM ? 2.0 * regweight : 0.0;
+    if (kd->prec == LDDMM_PREC_F32C && !sym && pair_engine() == LDDMM_ENGINE_MFMA)
+        return mfma_diffKT_plain(kd, y, int(n), x, int(m), dim, p, a, k, out, st);
 #define LDDMM_DKT(T, S, M) diffKT_t<T, S, M>(kd, y, n, x, m, dim, p, a, k, r2w, out, st)
     if (kd->prec == LDDMM_PREC_F32C) {
         if (sym) return ms ? LDDMM_DKT(float, true, true) : LDDMM_DKT(float, true, false);

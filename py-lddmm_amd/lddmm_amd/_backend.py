@@ -69,6 +69,7 @@ SIGNATURES = {
     "lddmm_profile_enable": [_I32],
     "lddmm_profile_read": [_P, _I64, _P],
     "lddmm_device_count": [],
+    "lddmm_set_pair_engine": [_I32],
 }
 
 
@@ -123,6 +124,15 @@ def to_dev(a, dtype=torch.float64):
         return a.to(device=dev, dtype=dtype).contiguous()
     arr = np.ascontiguousarray(np.asarray(a), dtype=torch.empty((), dtype=dtype).numpy().dtype)
     return torch.from_numpy(arr).to(dev)
+
+
+ENGINE_VALU, ENGINE_MFMA = 0, 1
+
+
+def set_pair_engine(engine):
+    """Accumulation engine of the float32 applyK / plain applyDiffKT sums."""
+    code = {'valu': ENGINE_VALU, 'mfma': ENGINE_MFMA}.get(engine, engine)
+    check(LIB.lddmm_set_pair_engine(int(code)), "set_pair_engine")
 
 
 def profile_enable(on=True):

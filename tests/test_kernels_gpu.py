@@ -188,3 +188,22 @@ def test_kernel_class_dropin(ku):
     xx = x - 0.0
     expect = O.applyK(x, x, a, [1.7]) + 0.3 * (xx @ (xx.T @ a)) + 0.2 * a.sum(0)
     assert rel(Ka.applyK(x, a), expect) < 1e-12
+
+
+@pytest.mark.parametrize("n,m,dim,k,sig", [(1, 1, 3, 3, [1.0]), (300, 700, 3, 1, [1.5]), (513, 257, 2, 2, [1.0]),
+                                           (1000, 1000, 3, 6, [1.0, 2.5]), (2000, 3000, 3, 3, [2.5]),
+                                           (700, 900, 3, 18, [1.2])])
+def test_mfma_engine_parity(ku, n, m, dim, k, sig):
+    """FP32 applyK / plain applyDiffKT with MFMA accumulation against the oracle."""
+    from lddmm_amd import _backend as B
+    rng = np.random.default_rng(n + 5 * m + k)
+    y, x = pts(rng, n, dim, shift=20.0), pts(rng, m, dim, shift=20.0)
+    a, p = rng.normal(size=(m, k)), rng.normal(size=(n, k))
+    B.set_pair_engine('mfma')
+    try:
+        gK = ku.applyK(y, x, a, 'gauss', sig, 10, dtype='float32')
+        gD = ku.applyDiffKT(y, x, p, a, 'gauss', sig, 10, dtype='float32')
+    finally:
+        B.set_pair_engine('valu')
+    assert rel(gK, O.applyK(y, x, a, sig)) < 1e-5
+    assert rel(gD, O.applyDiffKT(y, x, p, a, sig)) < 1e-5
