@@ -233,6 +233,39 @@ int lddmm_epdiff_adjoint(const lddmm_kernel* kd, int32_t T, int64_t n, int32_t d
                          const double* pa1, double* pxt, double* pat, double* da0,
                          void* stream);
 
+/* ------------------------------------------------------------------------
+ * Point-cloud network primitives of the learned warm start (no counterpart
+ * in the reference; they implement the momentum predictor that feeds
+ * SurfaceMatching's initial control).  Float32 activations, BF16 weights
+ * (uint16 bit patterns), FP32 accumulation; device pointers; a segment is
+ * the contiguous point range [off[s], off[s+1]) of one template-target pair.
+ * ---------------------------------------------------------------------- */
+/* Y[P,N] = act(X[P,K] W + b + segbias[seg[p]]); W is [K,N] (or [N,K] when transW);
+ * b, segbias, seg may be NULL; relu != 0 applies max(., 0). */
+int lddmm_nn_linear(const float* X, int64_t P, int32_t K, const uint16_t* W, int32_t transW,
+                    const float* b, const float* segbias, const int32_t* seg, int32_t N,
+                    int32_t relu, float* Y, void* stream);
+/* dZ = dY * (Y > 0) (ReLU backward on post-activation values). */
+int lddmm_nn_relu_bwd(const float* Y, const float* dY, float* dZ, int64_t count, void* stream);
+/* dW[K,N] = X^T dZ, reduced over point chunks in a fixed order (deterministic). */
+int lddmm_nn_wgrad(const float* X, const float* dZ, int64_t P, int32_t K, int32_t N, float* dW,
+                   void* stream);
+/* out[s,n] = sum over segment s of dZ[p,n] (bias / segment-bias gradients). */
+int lddmm_nn_seg_colsum(const float* dZ, const int32_t* off, int32_t nseg, int32_t N, float* out,
+                        void* stream);
+/* g[s,n] = max over segment s of H[p,n]; arg[s,n] = first point attaining it. */
+int lddmm_nn_segmax(const float* H, const int32_t* off, int32_t nseg, int32_t N, float* g,
+                    int32_t* arg, void* stream);
+/* dH[arg[s,n], n] += dg[s,n]. */
+int lddmm_nn_segmax_bwd(const float* dg, const int32_t* off, const int32_t* arg, int32_t nseg,
+                        int32_t N, float* dH, void* stream);
+/* Adam step on float32 master weights w (gradient g, moments m, v; bias
+ * corrections for `step` >= 1; decoupled-free L2 term wd*w added to g);
+ * wbf (may be NULL) receives the BF16 copy. */
+int lddmm_nn_adam(float* w, uint16_t* wbf, const float* g, float* m, float* v, int64_t count,
+                  float lr, float b1, float b2, float eps, int32_t step, float wd, void* stream);
+int lddmm_nn_to_bf16(const float* w, uint16_t* wbf, int64_t count, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,6 +70,15 @@ SIGNATURES = {
     "lddmm_profile_read": [_P, _I64, _P],
     "lddmm_device_count": [],
     "lddmm_set_pair_engine": [_I32],
+    "lddmm_nn_linear": [_P, _I64, _I32, _P, _I32, _P, _P, _P, _I32, _I32, _P, _P],
+    "lddmm_nn_relu_bwd": [_P, _P, _P, _I64, _P],
+    "lddmm_nn_wgrad": [_P, _P, _I64, _I32, _I32, _P, _P],
+    "lddmm_nn_seg_colsum": [_P, _P, _I32, _I32, _P, _P],
+    "lddmm_nn_segmax": [_P, _P, _I32, _I32, _P, _P, _P],
+    "lddmm_nn_segmax_bwd": [_P, _P, _P, _I32, _I32, _P, _P],
+    "lddmm_nn_adam": [_P, _P, _P, _P, _P, _I64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                      ctypes.c_float, _I32, ctypes.c_float, _P],
+    "lddmm_nn_to_bf16": [_P, _P, _I64, _P],
 }
 
 

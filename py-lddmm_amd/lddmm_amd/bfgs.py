@@ -233,6 +233,11 @@ def bfgs(opt, verb=True, maxIter=1000, TestGradient=False, epsInit=0.01, memory=
         if not opt.reset:
             it += 1
             history.append(obj)
+        # optional early stop requested by the optimisable object (no
+        # counterpart in the reference loop; absent -> identical behaviour)
+        if hasattr(opt, 'stopNow') and opt.stopNow():
+            opt.converged = True
+            break
 
     if it == maxIter and hasattr(opt, 'endOfProcedure'):
         opt.endOfProcedure()

@@ -376,6 +376,32 @@ class SurfaceMatching:
         return self.control
 
     # ------------------------------------------------------------- helpers
+    def setInitialControl(self, at):
+        """Start the optimisation from the control `at` [T, N, 3] instead of zero
+        (the first-order counterpart of the reference's second-order
+        'initialMomentum' option, base/secondOrderPointSetMatching.py:82-88)."""
+        at_ = B.to_dev(at)
+        if tuple(at_.shape) != tuple(self.control['at'].shape):
+            raise ValueError(f"control shape {tuple(at_.shape)} != {tuple(self.control['at'].shape)}")
+        self.control['at'] = at_.clone()
+        self.obj = None
+        self.state['xt'], _ = evol.shoot(self.x0_dev, self.control['at'], self.options['KparDiff'])
+
+    def setTargetDataEnergy(self, value, rtol=1e-4):
+        """Record the data term (varifold / current distance / sigmaError^2)
+        after every iteration in `data_trace`; with a value, stop once it is
+        within rtol of it (iteration counts at matched energy)."""
+        self._target_data = None if value is None else value * (1 + rtol)
+        self.data_trace = []
+        self._trace_on = True
+
+    def stopNow(self):
+        if not getattr(self, '_trace_on', False):
+            return False
+        d = self.varifold_data()
+        self.data_trace.append(d)
+        return self._target_data is not None and d <= self._target_data
+
     def control_numpy(self):
         return self.control['at'].cpu().numpy()
 

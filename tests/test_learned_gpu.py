@@ -1,0 +1,62 @@
+"""GPU checks of the learned warm start: the HIP forward / backward of
+MomentumNet against the float64 numpy network (same BF16-rounded weights),
+and one Adam step against its closed form."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def net_mod():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from lddmm_amd.learned import model
+    return model, torch
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300)
+
+
+@pytest.mark.parametrize("counts", [[1], [37, 64, 130], [2562, 2562]])
+def test_forward_backward_match_oracle(net_mod, counts):
+    model, torch = net_mod
+    from oracle import mlp
+    rng = np.random.default_rng(sum(counts))
+    w = model.init_params(5)
+    lay, _ = model.layout()
+    for k, (o, shp) in lay.items():
+        if k.endswith(".b"):
+            w[o:o + shp[0]] = rng.normal(0, 0.1, shp[0])
+    net = model.MomentumNet(params=w)
+    P = sum(counts)
+    X = rng.normal(size=(P, 3)).astype(np.float32)
+    T = rng.normal(size=(P, 3)).astype(np.float32)
+    segs = model.Segments(counts, net.dev)
+    Xd, Td = torch.tensor(X, device="cuda"), torch.tensor(T, device="cuda")
+    loss = net.loss_and_grad(Xd, segs, Td)
+    out = net.forward(Xd, segs).cpu().numpy()
+    g = net.g.cpu().numpy()
+    wbf = model.bf16_round(w)                     # weights as the kernels read them
+    out_r, _ = mlp.forward(wbf, w, X.astype(np.float64), counts)
+    L_r, g_r = mlp.loss_and_grad(wbf, w, X.astype(np.float64), counts, T.astype(np.float64))
+    assert rel(out, out_r) < 1e-5
+    assert abs(float(loss) - L_r) / L_r < 1e-5
+    assert rel(g, g_r) < 1e-4
+
+
+def test_adam_step(net_mod):
+    model, torch = net_mod
+    net = model.MomentumNet(seed=1)
+    w0 = net.params_numpy()
+    g = np.random.default_rng(2).normal(size=w0.size).astype(np.float32)
+    net.g.copy_(torch.tensor(g, device="cuda"))
+    net.adam(lr=1e-2)
+    w1 = net.params_numpy()
+    # first step: m = 0.1 g, v = 0.001 g^2 -> update = lr * g / (|g| + eps) after bias correction
+    expect = w0 - 1e-2 * g / (np.abs(g) + 1e-8)
+    assert rel(w1, expect) < 1e-5
+    bf = net.wbf.cpu().numpy().view(np.uint16).astype(np.uint32) << 16
+    np.testing.assert_array_equal(bf.view(np.float32), model.bf16_round(w1))

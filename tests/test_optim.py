@@ -110,3 +110,26 @@ def test_nan_objective_guard_and_reset_stop():
     bfgs_mod.bfgs(P, maxIter=20, verb=False)
     assert P.converged is False
     assert np.array_equal(P.x, [1.0, 2.0])
+
+
+def test_mlp_oracle_gradient_fd():
+    """The numpy MomentumNet backward against central finite differences."""
+    from oracle import mlp
+    from lddmm_amd.learned.model import init_params
+    rng = np.random.default_rng(0)
+    w = init_params(3).astype(np.float64) * 0.5
+    # non-zero biases so that every ReLU sees both signs
+    lay = mlp.unpack(w)
+    for k in lay:
+        if k.endswith(".b"):
+            lay[k][:] = rng.normal(0, 0.1, lay[k].shape)
+    counts = [30, 45]
+    X = rng.normal(size=(75, 3))
+    T = rng.normal(size=(75, 3))
+    L, g = mlp.loss_and_grad(w, w, X, counts, T)
+    idx = rng.choice(w.size, 40, replace=False)
+    for i in idx:
+        e = np.zeros_like(w)
+        e[i] = 1e-6
+        fd = (mlp.loss_and_grad(w + e, w + e, X, counts, T)[0] - mlp.loss_and_grad(w - e, w - e, X, counts, T)[0]) / 2e-6
+        assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(fd)
