@@ -240,6 +240,46 @@ class Kernel(KernelSpec):
         logging.info('test diffKT: {0:.5f}, {1:.5f}'.format((k1 - k0) / eps, (grd * dy).sum()))
         return (k1 - k0) / eps, (grd * dy).sum()
 
+    def solve(self, x, xi, b=None, iterMax=100):
+        """Solve K(x, x) a = xi by conjugate gradients (base/kernelFunctions.py:676-680;
+        `b` is unused there too).  Follows linearcg (base/conjugateGradient.py:351-408):
+        zero start; return at once when |xi|^2 < 1e-10; stop when
+        alpha * mean|p| < 1e-10, when the quadratic energy a.Ka - 2 xi.a stalls
+        (relative change < 1e-20), or after iterMax products.  The products run
+        on the device; the reference's per-iteration residual (an extra product
+        used only for logging) is not computed."""
+        tt = isinstance(x, torch.Tensor) or isinstance(xi, torch.Tensor)
+        from . import _backend as B
+        x_ = B.to_dev(x)
+        rhs = B.to_dev(xi)
+        a = torch.zeros_like(rhs)
+        z = -2 * rhs                      # K a - 2 xi, tracked with the iterate
+        r = rhs.clone()
+        ener_old = 0.0
+        p = None
+        mu = None
+        for it in range(iterMax):
+            mu_new = float((r * r).sum())
+            if it == 0:
+                if mu_new < 1e-10:
+                    break
+                p = r.clone()
+            else:
+                p = r + (mu_new / mu) * p
+            mu = mu_new
+            q = self.applyK(x_, p)
+            alpha = mu / float((p * q).sum())
+            a = a + alpha * p
+            r = r - alpha * q
+            z = z + alpha * q
+            ener = float((z * a).sum())
+            if alpha * float(p.abs().mean()) < 1e-10:
+                break
+            if abs(ener - ener_old) < 1e-20 * abs(ener_old):
+                break
+            ener_old = ener
+        return a if tt else a.cpu().numpy()
+
     # (base/kernelFunctions.py:665-674)
     def applyDDiffK11and12(self, x, n, a, p):
         z = ku.applykdiff11and12(x, n, a, p, self.name, self.sigma, self.order, KP=self.ms_exponent,

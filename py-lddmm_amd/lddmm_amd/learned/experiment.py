@@ -56,6 +56,11 @@ def main(argv=None):
     ap.add_argument("--out", default="momentum_net.npz")
     ap.add_argument("--report", default=None)
     ap.add_argument("--time-budget", type=float, default=600.0, help="seconds for data generation")
+    ap.add_argument("--cg-iters", type=int, default=20)
+    ap.add_argument("--oracle-check", type=int, default=5,
+                    help="held-out pairs also warm-started from the plain run's own a0")
+    ap.add_argument("--targets", default="momentum,velocity",
+                    help="comma-separated network targets: momentum (a0) and/or velocity (K a0)")
     a = ap.parse_args(argv)
     logging.getLogger().setLevel(logging.WARNING)
     cfg = dict(W.DEFAULT)
@@ -66,7 +71,9 @@ def main(argv=None):
     for k in range(a.train):
         seed = 10000 + k
         sm, feats, sec = plain_with_trace(seed, cfg)
-        samples.append({"features": feats.cpu().numpy(), "a0": sm.control['at'][0].cpu().numpy()})
+        a0 = sm.control['at'][0]
+        samples.append({"features": feats.cpu().numpy(), "a0": a0.cpu().numpy(),
+                        "v0": W.velocity(sm, a0).cpu().numpy()})
         if k % 100 == 0:
             log(f"train registration {k}: {sec:.2f} s, data {sm.varifold_data():.4f}")
         if time.perf_counter() - t0 > a.time_budget:
@@ -75,13 +82,17 @@ def main(argv=None):
     rep["train_pairs"] = len(samples)
     rep["generation_seconds"] = time.perf_counter() - t0
 
-    t1 = time.perf_counter()
-    net, losses = W.train(samples, epochs=a.epochs, batch=a.batch, lr=a.lr, log_every=0)
-    torch.cuda.synchronize()
-    rep["train_seconds"] = time.perf_counter() - t1
-    rep["loss_first_last"] = [losses[0], losses[-1]]
-    log(f"trained on {len(samples)} pairs in {rep['train_seconds']:.1f} s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
-    net.save(a.out)
+    nets = {}
+    for target in a.targets.split(","):
+        t1 = time.perf_counter()
+        net, losses = W.train(samples, epochs=a.epochs, batch=a.batch, lr=a.lr, log_every=0, target=target)
+        torch.cuda.synchronize()
+        rep[f"train_seconds_{target}"] = time.perf_counter() - t1
+        rep[f"loss_first_last_{target}"] = [losses[0], losses[-1]]
+        log(f"[{target}] trained on {len(samples)} pairs in {rep[f'train_seconds_{target}']:.1f} s, "
+            f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+        net.save(a.out.replace(".npz", f"_{target}.npz"))
+        nets[target] = net
 
     rows = []
     for k in range(a.test):
@@ -90,25 +101,49 @@ def main(argv=None):
         E = sm.varifold_data()
         it_plain = iterations_to(sm.data_trace, E)
         sec_plain_to = sec_plain * it_plain / max(len(sm.data_trace), 1)
-        r = W.register(seed, net=net, target_data=E, cfg=cfg)
-        rows.append({"seed": seed, "E_plain": E, "it_plain": it_plain, "it_plain_total": len(sm.data_trace),
-                     "sec_plain_total": sec_plain, "sec_plain_to_E": sec_plain_to, "it_warm": r["iterations"],
-                     "sec_warm": r["seconds"], "E_warm": r["data"], "reached": r["reached"],
-                     "used_warm": r["used_warm"]})
-        if k % 10 == 0:
-            log(f"test {k}: plain {it_plain} it ({sec_plain:.2f} s) -> warm {r['iterations']} it "
-                f"({r['seconds']:.2f} s), reached {r['reached']}")
+        row = {"seed": seed, "E_plain": E, "it_plain": it_plain, "it_plain_total": len(sm.data_trace),
+               "sec_plain_total": sec_plain, "sec_plain_to_E": sec_plain_to}
+        if k < 3:
+            row["plain_trace"] = list(sm.data_trace)
+        for target, net in nets.items():
+            r = W.register(seed, net=net, target_data=E, cfg=cfg, cg_iters=a.cg_iters)
+            row.update({f"it_warm_{target}": r["iterations"], f"sec_warm_{target}": r["seconds"],
+                        f"E_warm_{target}": r["data"], f"E0_warm_{target}": r["data0"],
+                        f"reached_{target}": r["reached"], f"used_warm_{target}": r["used_warm"]})
+            if target == 'velocity':
+                feats = W.features(W.make_matching(seed, cfg))
+                a0_true = sm.control['at'][0]
+                v_true = W.velocity(sm, a0_true)
+                v_pred = net.predict(feats, [feats.shape[0]]).to(v_true.dtype)
+                row["v_rel_err"] = float((v_pred - v_true).norm() / v_true.norm())
+        if k < a.oracle_check:
+            # upper bound: warm start from the plain run's own a0 (perfect predictor)
+            r = W.register(seed, a0=sm.control['at'][0], target_data=E, cfg=cfg)
+            row.update({"it_warm_true_a0": r["iterations"], "E0_true_a0": r["data0"]})
+        rows.append(row)
+        if k % 10 == 0 or k < a.oracle_check:
+            log(f"test {k}: plain {it_plain} it ({sec_plain:.2f} s) -> warm "
+                + ", ".join(f"{t}: {row[f'it_warm_{t}']} it ({row[f'sec_warm_{t}']:.2f} s)" for t in nets))
     itp = np.array([r["it_plain"] for r in rows], dtype=float)
-    itw = np.array([r["it_warm"] for r in rows], dtype=float)
-    rep["test"] = {
-        "pairs": len(rows),
-        "mean_it_plain": float(itp.mean()), "mean_it_warm": float(itw.mean()),
-        "iteration_ratio": float(itp.mean() / max(itw.mean(), 1e-9)),
-        "mean_sec_plain_to_E": float(np.mean([r["sec_plain_to_E"] for r in rows])),
-        "mean_sec_warm": float(np.mean([r["sec_warm"] for r in rows])),
-        "reached_fraction": float(np.mean([r["reached"] for r in rows])),
-        "warm_used_fraction": float(np.mean([r["used_warm"] for r in rows])),
-    }
+    rep["test"] = {"pairs": len(rows), "mean_it_plain": float(itp.mean()),
+                   "mean_sec_plain_to_E": float(np.mean([r["sec_plain_to_E"] for r in rows]))}
+    for target in nets:
+        itw = np.array([r[f"it_warm_{target}"] for r in rows], dtype=float)
+        rep["test"][target] = {
+            "mean_it_warm": float(itw.mean()),
+            "iteration_ratio": float(itp.mean() / max(itw.mean(), 1e-9)),
+            "mean_sec_warm": float(np.mean([r[f"sec_warm_{target}"] for r in rows])),
+            "reached_fraction": float(np.mean([r[f"reached_{target}"] for r in rows])),
+            "warm_used_fraction": float(np.mean([r[f"used_warm_{target}"] for r in rows])),
+            "mean_E0_over_E": float(np.mean([r[f"E0_warm_{target}"] / r["E_plain"] for r in rows])),
+        }
+    if "v_rel_err" in rows[0]:
+        rep["test"]["mean_v_rel_err"] = float(np.mean([r["v_rel_err"] for r in rows]))
+    ora = [r for r in rows if "it_warm_true_a0" in r]
+    if ora:
+        rep["test"]["true_a0"] = {"pairs": len(ora),
+                                  "mean_it": float(np.mean([r["it_warm_true_a0"] for r in ora])),
+                                  "mean_E0_over_E": float(np.mean([r["E0_true_a0"] / r["E_plain"] for r in ora]))}
     rep["rows"] = rows
     log(json.dumps(rep["test"]))
     if a.report:

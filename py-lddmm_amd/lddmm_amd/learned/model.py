@@ -69,7 +69,8 @@ class Segments:
 
 
 class MomentumNet:
-    def __init__(self, params=None, seed=0, feat_scale=1.0, out_scale=1.0):
+    def __init__(self, params=None, seed=0, feat_scale=1.0, out_scale=1.0, target='momentum'):
+        self.target = target          # 'momentum' (a0) or 'velocity' (v0 = K(x0) a0)
         self.dev = B.device()
         self.lay, self.total = layout()
         p = init_params(seed) if params is None else np.asarray(params, dtype=np.float32)
@@ -201,9 +202,12 @@ class MomentumNet:
         return self.w.cpu().numpy().copy()
 
     def save(self, path):
-        np.savez(path, params=self.params_numpy(), feat_scale=self.feat_scale, out_scale=self.out_scale)
+        np.savez(path, params=self.params_numpy(), feat_scale=self.feat_scale, out_scale=self.out_scale,
+                 target=np.array(self.target))
 
     @classmethod
     def load(cls, path):
         d = np.load(path)
-        return cls(params=d["params"], feat_scale=float(d["feat_scale"]), out_scale=float(d["out_scale"]))
+        target = str(d["target"]) if "target" in d.files else 'momentum'
+        return cls(params=d["params"], feat_scale=float(d["feat_scale"]), out_scale=float(d["out_scale"]),
+                   target=target)

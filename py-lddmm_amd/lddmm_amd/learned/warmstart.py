@@ -42,6 +42,11 @@ def features(sm):
     return -sm.endPointGradient_device(sm.x0_dev)
 
 
+def velocity(sm, a0):
+    """v0 = K(x0) a0 (device float64)."""
+    return sm.options['KparDiff'].applyK(sm.x0_dev, B.to_dev(a0))
+
+
 def plain_registration(seed, cfg=DEFAULT):
     """Zero-momentum plain registration: returns dict with features, a0 target,
     final data term, iterations, seconds."""
@@ -55,17 +60,19 @@ def plain_registration(seed, cfg=DEFAULT):
             "seconds": time.perf_counter() - t0, "history": list(sm.bfgs_history)}
 
 
-def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50):
-    """Fit MomentumNet on (features -> a0) pairs with the package's HIP kernels."""
+def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50, target='momentum'):
+    """Fit MomentumNet on (features -> a0, or -> v0 = K(x0) a0) pairs with the package's HIP kernels."""
     rng = np.random.default_rng(seed)
+    key = "a0" if target == 'momentum' else "v0"
     F = np.concatenate([s["features"] for s in samples])
-    A = np.concatenate([s["a0"] for s in samples])
+    A = np.concatenate([s[key] for s in samples])
     feat_scale = float(np.sqrt((F ** 2).mean())) or 1.0
     out_scale = float(np.sqrt((A ** 2).mean())) or 1.0
     net = MomentumNet(seed=seed, feat_scale=feat_scale, out_scale=out_scale)
+    net.target = target
     dev = net.dev
     Xs = [torch.tensor(s["features"] / feat_scale, dtype=torch.float32, device=dev) for s in samples]
-    Ys = [torch.tensor(s["a0"] / out_scale, dtype=torch.float32, device=dev) for s in samples]
+    Ys = [torch.tensor(s[key] / out_scale, dtype=torch.float32, device=dev) for s in samples]
     losses = []
     for ep in range(epochs):
         order = rng.permutation(len(samples))
@@ -84,17 +91,30 @@ def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50):
     return net, losses
 
 
-def warm_control(sm, net):
-    """EPDiff control from the predicted momentum, or None when the zero control is better."""
+def warm_control(sm, net, cg_iters=20):
+    """EPDiff control from the predicted initial momentum.
+
+    A net trained on velocities (net.target == 'velocity') predicts v0 = K(x0) a0,
+    which is local in the data force; a0 then comes from a conjugate-gradient
+    solve of K(x0) a0 = v0 (Kernel.solve), truncated at cg_iters products."""
     feats = features(sm)
-    a0 = net.predict(feats, [feats.shape[0]])
+    pred = net.predict(feats, [feats.shape[0]])
+    if getattr(net, 'target', 'momentum') == 'velocity':
+        a0 = sm.options['KparDiff'].solve(sm.x0_dev, pred, iterMax=cg_iters)
+    else:
+        a0 = pred
     T = sm.Tsize
     _, at = evol.landmarkEPDiff(T, sm.x0_dev, a0, sm.options['KparDiff'])
     return at, a0
 
 
-def register(seed, net=None, target_data=None, cfg=DEFAULT, max_iter=None):
-    """Registration (plain or warm-started) until the data term reaches target_data (1e-4 rel)."""
+def register(seed, net=None, target_data=None, cfg=DEFAULT, max_iter=None, a0=None, cg_iters=20):
+    """Registration (plain, warm-started by `net`, or warm-started from a given
+    initial momentum `a0`) until the data term reaches target_data (1e-4 rel).
+
+    The warm control is kept only when its objective is below the zero
+    control's (zero-momentum fallback).  `data0` is the data term at the
+    starting control; a start already within tolerance counts 0 iterations."""
     c = dict(cfg)
     if max_iter is not None:
         c['maxIter'] = max_iter
@@ -102,8 +122,11 @@ def register(seed, net=None, target_data=None, cfg=DEFAULT, max_iter=None):
     t0 = time.perf_counter()
     sm = make_matching(seed, c)
     used_warm = False
-    if net is not None:
-        at, _ = warm_control(sm, net)
+    if net is not None or a0 is not None:
+        if a0 is None:
+            at, _ = warm_control(sm, net, cg_iters=cg_iters)
+        else:
+            _, at = evol.landmarkEPDiff(sm.Tsize, sm.x0_dev, B.to_dev(a0), sm.options['KparDiff'])
         obj_zero = sm.objectiveFun()
         sm.setInitialControl(at)
         obj_warm = sm.objectiveFun()
@@ -111,11 +134,17 @@ def register(seed, net=None, target_data=None, cfg=DEFAULT, max_iter=None):
             used_warm = True
         else:                                   # fallback: zero momentum
             sm.setInitialControl(torch.zeros_like(at))
-    if target_data is not None:
-        sm.setTargetDataEnergy(target_data, 1e-4)
-    sm.optimizeMatching()
+    data0 = sm.varifold_data()
+    tol = None if target_data is None else target_data * (1 + 1e-4)
+    if tol is not None and data0 <= tol:
+        iterations = 0
+    else:
+        if target_data is not None:
+            sm.setTargetDataEnergy(target_data, 1e-4)
+        sm.optimizeMatching()
+        iterations = int(sm.bfgs_iterations)
     torch.cuda.synchronize()
     d = sm.varifold_data()
-    reached = target_data is None or d <= target_data * (1 + 1e-4)
-    return {"seed": seed, "iterations": int(sm.bfgs_iterations), "seconds": time.perf_counter() - t0,
-            "data": d, "reached": bool(reached), "used_warm": used_warm}
+    reached = tol is None or d <= tol
+    return {"seed": seed, "iterations": iterations, "seconds": time.perf_counter() - t0,
+            "data": d, "data0": data0, "reached": bool(reached), "used_warm": used_warm}

@@ -207,3 +207,21 @@ def test_mfma_engine_parity(ku, n, m, dim, k, sig):
         B.set_pair_engine('valu')
     assert rel(gK, O.applyK(y, x, a, sig)) < 1e-5
     assert rel(gD, O.applyDiffKT(y, x, p, a, sig)) < 1e-5
+
+
+def test_kernel_solve_matches_dense_solve(ku):
+    """Kernel.solve (conjugate gradients on device products) against a dense
+    LU solve of the Gram matrix; K(x, solve(xi)) must reproduce xi."""
+    from lddmm_amd.kernelFunctions import Kernel
+    rng = np.random.default_rng(21)
+    x = rng.uniform(0, 12, size=(250, 3))
+    xi = rng.normal(size=(250, 3))
+    K = Kernel(name='gauss', sigma=1.0)
+    a = K.solve(x, xi, None, iterMax=400)
+    d2 = ((x[:, None, :] - x[None, :, :]) ** 2).sum(-1)
+    G = np.exp(-d2 / 2.0)
+    expect = np.linalg.solve(G, xi)
+    assert isinstance(a, np.ndarray)
+    assert rel(G @ a, xi) < 1e-6
+    assert rel(a, expect) < 1e-4
+    assert np.all(K.solve(x, np.zeros_like(xi), None) == 0)
