@@ -200,6 +200,49 @@ int lddmm_surface_features(const lddmm_surface* s, double* centers, double* wsur
 int lddmm_surface_face_gather(const lddmm_surface* s, const double* dz1, const double* z1,
                               double* px, void* stream);
 
+/* Polygonal curve on the device: vertices [nv,dim] f64 (dim 2 or 3),
+ * segments [ns,2] int32 (the reference's `faces`), vertex weights [nv] f64
+ * (NULL = all ones; line weight = mean of the two, base/curves.py:117-125). */
+typedef struct lddmm_curve {
+    const double*  vertices;
+    int64_t        nv;
+    int32_t        dim;
+    int32_t        reserved;
+    const int32_t* segments;
+    int64_t        ns;
+    const double*  vweights;
+    /* vertex -> incident (segment*2 + end) adjacency, CSR, built by
+     * lddmm_curve_adjacency; required only for gradients. */
+    const int32_t* adj_ptr;            /* [nv+1] */
+    const int32_t* adj_idx;            /* [2*ns] */
+} lddmm_curve;
+
+/* Host helper: CSR vertex->(segment,end) adjacency of int32 segments. */
+int lddmm_curve_adjacency(const int32_t* segments_host, int64_t ns, int64_t nv,
+                          int32_t* adj_ptr_host, int32_t* adj_idx_host);
+
+/* Curve varifold data term, f(u) = 1 + weight*u^2 (weight >= 0):
+ *   tgt == NULL : *energy = |C1|^2                 (varifoldNorm0)
+ *   tgt != NULL : *energy = |C1|^2 - 2 <C1, tgt>   (varifoldNormDef)
+ * energy: DEVICE f64[1].  grad (may be NULL): d/dx |C1 - tgt|^2, [nv,dim] f64
+ * (varifoldNormGradient, with_weights=False); requires C1's adjacency.  With
+ * grad the energy uses the gradient form's segment weight |l| w instead of
+ * |w l| (identical for unit weights).
+ * Replaces base/curveDistances.py:186-336. */
+int lddmm_curve_varifold(const lddmm_kernel* kd,
+                         const lddmm_curve* c1, const lddmm_curve* tgt, double weight,
+                         double* energy, double* grad, void* stream);
+
+/* Segment features: centers [ns,dim], linel [ns,dim], line weights [ns]
+ * (base/curves.py:117-125).  Any output may be NULL. */
+int lddmm_curve_features(const lddmm_curve* c, double* centers, double* linel,
+                         double* seg_weights, void* stream);
+
+/* px[v] = sum over incident segments of dz1_s -/+ wz1_s (start / end vertex),
+ * dz1 and wz1 [ns,dim] (base/curveDistances.py:97-103, :324-326). */
+int lddmm_curve_seg_gather(const lddmm_curve* c, const double* dz1, const double* wz1,
+                           double* px, void* stream);
+
 /* Forward Euler landmark flow x_{t+1} = x_t + (1/T) K(x_t,x_t) a_t, t < T.
  * at [T,n,dim], xt [T+1,n,dim] (xt[0] is read, xt[1..T] written).
  * If energy != NULL (device f64[1]) it receives sum_t r dt a_t.K(x_t)a_t.

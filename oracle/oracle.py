@@ -556,6 +556,10 @@ def _data_term(kind, fvDef, fv1, sigma_dist):
         return varifoldNormDef(fvDef, fv1, sigma_dist), varifoldNorm0(fv1, sigma_dist)
     if kind == 'current':
         return currentNormDef(fvDef, fv1, sigma_dist), currentNorm0(fv1, sigma_dist)
+    if kind == 'curve_varifold':
+        return curveVarifoldNormDef(fvDef, fv1, sigma_dist), curveVarifoldNorm0(fv1, sigma_dist)
+    if kind == 'curve_current':
+        return curveCurrentNormDef(fvDef, fv1, sigma_dist), curveCurrentNorm0(fv1, sigma_dist)
     raise ValueError(kind)
 
 
@@ -564,12 +568,18 @@ def _data_grad(kind, fvDef, fv1, sigma_dist):
         return varifoldNormGradient(fvDef, fv1, sigma_dist)
     if kind == 'current':
         return currentNormGradient(fvDef, fv1, sigma_dist)
+    if kind == 'curve_varifold':
+        return curveVarifoldNormGradient(fvDef, fv1, sigma_dist)
+    if kind == 'curve_current':
+        return curveCurrentNormGradient(fvDef, fv1, sigma_dist)
     raise ValueError(kind)
 
 
 def matching_objective(x0, faces, at, target, sigma_diff, sigma_dist, kind='varifold',
                        regweight=1., sigma_error=1.):
-    """Surface-matching objective at control `at` (plain, no affine, ds = 1).
+    """Surface- or curve-matching objective at control `at` (plain, no affine, ds = 1).
+    kind: 'varifold' / 'current' (surfaces), 'curve_varifold' / 'curve_current'
+    (curves, base/curveMatching.py:272-313 has the same structure).
 
     sum_t r dt a_t.K(x_t)a_t  +  (|def|^2 - 2<def, tgt> + |tgt|^2) / sigma_err^2.
     Follows base/surfaceMatching.py:468-565 (objectiveFunDef) and :638-675

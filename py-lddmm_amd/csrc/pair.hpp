@@ -40,21 +40,23 @@ struct ContiguousChunks {
     }
 };
 
+// One workgroup's share: targets [i0, i0 + kBlock*R) clipped to `ilim`,
+// sources [j0, j1); channel c of target i goes to ph[c*ldn + (i - ioff)]
+// (and pl[...] for the compensation term).
 template <typename T, class OP, int R>
-__global__ void __launch_bounds__(kBlock)
-k_pair(const typename OP::template Args<T> args, int n, int m, int chunk, Partials<T> part) {
+__device__ __forceinline__ void pair_span(const typename OP::template Args<T>& args, int i0, int ilim, int j0,
+                                          int j1, T* __restrict__ ph, T* __restrict__ pl, size_t ldn, int ioff,
+                                          T* lds) {
     constexpr int NCH = OP::NCH;
     constexpr int REC = OP::REC;
-    __shared__ __attribute__((aligned(16))) T lds[kTile * REC];
-
     const int tid = threadIdx.x;
-    const int ibase = blockIdx.x * (kBlock * R) + tid;
+    const int ibase = i0 + tid;
 
     typename OP::template Tgt<T> tg[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int i = ibase + r * kBlock;
-        OP::load_target(args, i < n ? i : n - 1, tg[r]);
+        OP::load_target(args, i < ilim ? i : ilim - 1, tg[r]);
     }
 
     T hi[R][NCH], lo[R][NCH];
@@ -62,9 +64,6 @@ k_pair(const typename OP::template Args<T> args, int n, int m, int chunk, Partia
     for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int c = 0; c < NCH; ++c) { hi[r][c] = T(0); lo[r][c] = T(0); }
-
-    int j0, j1;
-    OP::Chunks::range(args, blockIdx.y, chunk, m, j0, j1);
 
     typename OP::Raw raw;
     OP::load_raw(args, j0 + tid < j1 ? j0 + tid : -1, raw);
@@ -101,15 +100,26 @@ k_pair(const typename OP::template Args<T> args, int n, int m, int chunk, Partia
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         int i = ibase + r * kBlock;
-        if (i < n) {
+        if (i < ilim) {
 #pragma unroll
             for (int c = 0; c < NCH; ++c) {
-                size_t o = (size_t(blockIdx.y) * NCH + c) * size_t(n) + i;
-                part.hi[o] = hi[r][c];
-                if constexpr (kCompensated<T>) part.lo[o] = lo[r][c];
+                size_t o = size_t(c) * ldn + size_t(i - ioff);
+                ph[o] = hi[r][c];
+                if constexpr (kCompensated<T>) pl[o] = lo[r][c];
             }
         }
     }
+}
+
+template <typename T, class OP, int R>
+__global__ void __launch_bounds__(kBlock)
+k_pair(const typename OP::template Args<T> args, int n, int m, int chunk, Partials<T> part) {
+    __shared__ __attribute__((aligned(16))) T lds[kTile * OP::REC];
+    int j0, j1;
+    OP::Chunks::range(args, blockIdx.y, chunk, m, j0, j1);
+    const size_t off = size_t(blockIdx.y) * OP::NCH * size_t(n);
+    pair_span<T, OP, R>(args, blockIdx.x * (kBlock * R), n, j0, j1, part.hi + off,
+                        kCompensated<T> ? part.lo + off : nullptr, size_t(n), 0, lds);
 }
 
 // Sum of chunk partials for (i, c) in float64 (two-sum in float32 first).

@@ -201,30 +201,49 @@ int lddmm_profile_read(double* ms_each, int64_t cap, int64_t* launches) {
     return lddmm::profile_read(ms_each, cap, launches);
 }
 
-int lddmm_surface_adjacency(const int32_t* faces, int64_t nf, int64_t nv, int32_t* adj_ptr,
-                            int32_t* adj_idx) {
-    LDDMM_REQUIRE(nf >= 0 && nv >= 0, "negative sizes");
-    LDDMM_REQUIRE(3 * nf < (int64_t(1) << 31), "too many faces for int32 adjacency");
-    LDDMM_REQUIRE(nf == 0 || (faces && adj_idx), "null faces / adj_idx");
-    LDDMM_REQUIRE(adj_ptr != nullptr, "null adj_ptr");
+}  // extern "C"
+
+namespace lddmm {
+namespace {
+// CSR vertex -> (element*K + corner) adjacency for elements of K vertices.
+// Elements are visited in increasing order, so each vertex's list is sorted
+// by (element, corner): the gathers sum in the same order on every call.
+int build_adjacency(const int32_t* el, int64_t ne, int K, int64_t nv, int32_t* adj_ptr, int32_t* adj_idx,
+                    const char* what) {
+    LDDMM_REQUIRE(ne >= 0 && nv >= 0, "%s: negative sizes", what);
+    LDDMM_REQUIRE(K * ne < (int64_t(1) << 31), "%s: too many elements for int32 adjacency", what);
+    LDDMM_REQUIRE(ne == 0 || (el && adj_idx), "%s: null elements / adj_idx", what);
+    LDDMM_REQUIRE(adj_ptr != nullptr, "%s: null adj_ptr", what);
     std::vector<int32_t> cnt(size_t(nv) + 1, 0);
-    for (int64_t f = 0; f < nf; ++f)
-        for (int c = 0; c < 3; ++c) {
-            int32_t v = faces[3 * f + c];
-            LDDMM_REQUIRE(v >= 0 && v < nv, "face %lld references vertex %d outside [0, %lld)",
+    for (int64_t f = 0; f < ne; ++f)
+        for (int c = 0; c < K; ++c) {
+            int32_t v = el[K * f + c];
+            LDDMM_REQUIRE(v >= 0 && v < nv, "%s: element %lld references vertex %d outside [0, %lld)", what,
                           (long long)f, v, (long long)nv);
             cnt[size_t(v) + 1]++;
         }
     for (int64_t v = 0; v < nv; ++v) cnt[size_t(v) + 1] += cnt[size_t(v)];
     for (int64_t v = 0; v <= nv; ++v) adj_ptr[v] = cnt[size_t(v)];
-    // faces are visited in increasing order, so each vertex's list is sorted
-    // by (face, corner): the gather sums in the same order on every call.
-    for (int64_t f = 0; f < nf; ++f)
-        for (int c = 0; c < 3; ++c) {
-            int32_t v = faces[3 * f + c];
-            adj_idx[cnt[size_t(v)]++] = int32_t(3 * f + c);
+    for (int64_t f = 0; f < ne; ++f)
+        for (int c = 0; c < K; ++c) {
+            int32_t v = el[K * f + c];
+            adj_idx[cnt[size_t(v)]++] = int32_t(K * f + c);
         }
     return LDDMM_OK;
+}
+}  // namespace
+}  // namespace lddmm
+
+extern "C" {
+
+int lddmm_surface_adjacency(const int32_t* faces, int64_t nf, int64_t nv, int32_t* adj_ptr,
+                            int32_t* adj_idx) {
+    return lddmm::build_adjacency(faces, nf, 3, nv, adj_ptr, adj_idx, "surface_adjacency");
+}
+
+int lddmm_curve_adjacency(const int32_t* segments, int64_t ns, int64_t nv, int32_t* adj_ptr,
+                          int32_t* adj_idx) {
+    return lddmm::build_adjacency(segments, ns, 2, nv, adj_ptr, adj_idx, "curve_adjacency");
 }
 
 }  // extern "C"

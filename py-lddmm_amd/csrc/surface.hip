@@ -1,14 +1,12 @@
 // Surface data terms on the GPU: face features, the orientation-free
 // varifold energy and its fused gradient, and the deterministic face ->
 // vertex gather (py-lddmm base/surfaceDistances.py:286-468, :87-139).
-#include "pair.hpp"
+#include "varifold.hpp"
 
 #include <algorithm>
+#include <cmath>
 
 namespace lddmm {
-
-// Per-face record in float64: c (3), a*w (area times face weight), n (3), w.
-struct FaceRec { double c0, c1, c2, aw, n0, n1, n2, fw; };
 
 // Face features (base/surfaces.py:326-335, base/surfaceDistances.py:340-343):
 // c = (x1+x2+x3)/3, S = (x2-x1)x(x3-x1)/2, a = sqrt(|S|^2 + 1e-10), n = S/a,
@@ -44,161 +42,6 @@ __global__ void k_face_features(const double* __restrict__ v, const int32_t* __r
     if (fweights) fweights[i] = w;
 }
 
-// Sources are the faces of surface 1 followed by those of surface 2; the
-// weight of a surface-2 face is multiplied by `w2`.  In gradient mode chunks
-// never straddle the two surfaces (chunks [0, c1) cover surface 1).
-struct VarifoldArgsBase {
-    const FaceRec* f1;
-    const FaceRec* f2;
-    int n1, n2;
-    double w2;
-    int c1;            // number of surface-1 chunks (gradient mode)
-    const double* c0;
-};
-
-struct TwoSurfaceChunks {
-    template <class A>
-    __device__ static void range(const A& a, int chunk_id, int chunk, int m, int& j0, int& j1) {
-        if (chunk_id < a.c1) {
-            j0 = chunk_id * chunk;
-            j1 = min(a.n1, j0 + chunk);
-        } else {
-            j0 = a.n1 + (chunk_id - a.c1) * chunk;
-            j1 = min(a.n1 + a.n2, j0 + chunk);
-        }
-    }
-};
-
-struct VarifoldRaw { double c0, c1, c2, a, n0, n1, n2; };
-
-template <typename T>
-__device__ __forceinline__ void vf_load_raw(const VarifoldArgsBase& A, int j, VarifoldRaw& r) {
-    if (j < 0) {
-        r = VarifoldRaw{A.c0[0], A.c0[1], A.c0[2], 0.0, 0.0, 0.0, 0.0};
-        return;
-    }
-    const FaceRec& f = j < A.n1 ? A.f1[j] : A.f2[j - A.n1];
-    double w = j < A.n1 ? 1.0 : A.w2;
-    r = VarifoldRaw{f.c0, f.c1, f.c2, f.aw * w, f.n0, f.n1, f.n2};
-}
-
-template <typename T>
-__device__ __forceinline__ void vf_store(const VarifoldArgsBase& A, double cs, const VarifoldRaw& r, T* rec) {
-    rec[0] = T((r.c0 - A.c0[0]) * cs);
-    rec[1] = T((r.c1 - A.c0[1]) * cs);
-    rec[2] = T((r.c2 - A.c0[2]) * cs);
-    rec[3] = T(r.a);
-    rec[4] = T(r.n0);
-    rec[5] = T(r.n1);
-    rec[6] = T(r.n2);
-    rec[7] = T(0);
-}
-
-template <typename T>
-struct VfTgt { T u0, u1, u2, n0, n1, n2; };
-
-template <typename T>
-__device__ __forceinline__ void vf_target(const VarifoldArgsBase& A, double cs, int i, VfTgt<T>& t) {
-    const FaceRec& f = A.f1[i];
-    t.u0 = T((f.c0 - A.c0[0]) * cs);
-    t.u1 = T((f.c1 - A.c0[1]) * cs);
-    t.u2 = T((f.c2 - A.c0[2]) * cs);
-    t.n0 = T(f.n0);
-    t.n1 = T(f.n1);
-    t.n2 = T(f.n2);
-}
-
-template <typename T>
-__device__ __forceinline__ T fma_(T a, T b, T c) { return fma(a, b, c); }
-
-// Energy rows: row_i = sum_j g(r2) a_j (1 + (n_i.n_j)^2)
-template <bool MS>
-struct OpVarifoldE {
-    static constexpr int NCH = 1;
-    static constexpr int REC = 8;
-    using Chunks = ContiguousChunks;
-    template <typename T>
-    struct Args : VarifoldArgsBase { ScaleSet<T> sc; };
-    template <typename T> using Tgt = VfTgt<T>;
-    using Raw = VarifoldRaw;
-    template <typename T>
-    __device__ static void load_target(const Args<T>& A, int i, Tgt<T>& t) { vf_target(A, A.sc.cs, i, t); }
-    template <typename T>
-    __device__ static void load_raw(const Args<T>& A, int j, Raw& r) { vf_load_raw<T>(A, j, r); }
-    template <typename T>
-    __device__ static void store_rec(const Args<T>& A, const Raw& r, T* rec) { vf_store(A, A.sc.cs, r, rec); }
-    template <typename T>
-    __device__ static void pair(const Args<T>& A, const Tgt<T>& t, const T* rec, T* acc) {
-        Vec4<T> p = lds4(rec);
-        T dx = t.u0 - p.x, dy = t.u1 - p.y, dz = t.u2 - p.z;
-        T r2 = fma_(dz, dz, fma_(dy, dy, dx * dx));
-        T K = gK<T, MS>(A.sc, r2);
-        Vec4<T> nj = lds4(rec + 4);
-        T c = fma_(t.n2, nj.z, fma_(t.n1, nj.y, t.n0 * nj.x));
-        acc[0] = fma_(K * p.w, fma_(c, c, T(1)), acc[0]);
-    }
-};
-
-// Gradient sums: S1 = sum g a_j c n_j, A = sum g a_j, B = sum g a_j c^2,
-// D = sum g' a_j (1 + c^2) (u_i - u_j)
-template <bool MS>
-struct OpVarifoldG {
-    static constexpr int NCH = 8;
-    static constexpr int REC = 8;
-    using Chunks = TwoSurfaceChunks;
-    template <typename T>
-    struct Args : VarifoldArgsBase { ScaleSet<T> sc; };
-    template <typename T> using Tgt = VfTgt<T>;
-    using Raw = VarifoldRaw;
-    template <typename T>
-    __device__ static void load_target(const Args<T>& A, int i, Tgt<T>& t) { vf_target(A, A.sc.cs, i, t); }
-    template <typename T>
-    __device__ static void load_raw(const Args<T>& A, int j, Raw& r) { vf_load_raw<T>(A, j, r); }
-    template <typename T>
-    __device__ static void store_rec(const Args<T>& A, const Raw& r, T* rec) { vf_store(A, A.sc.cs, r, rec); }
-    template <typename T>
-    __device__ static void pair(const Args<T>& A, const Tgt<T>& t, const T* rec, T* acc) {
-        Vec4<T> p = lds4(rec);
-        T dx = t.u0 - p.x, dy = t.u1 - p.y, dz = t.u2 - p.z;
-        T r2 = fma_(dz, dz, fma_(dy, dy, dx * dx));
-        T gk, gd;
-        gKD<T, MS>(A.sc, r2, gk, gd);
-        Vec4<T> nj = lds4(rec + 4);
-        T c = fma_(t.n2, nj.z, fma_(t.n1, nj.y, t.n0 * nj.x));
-        T Ka = gk * p.w;
-        T Kac = Ka * c;
-        acc[0] = fma_(Kac, nj.x, acc[0]);
-        acc[1] = fma_(Kac, nj.y, acc[1]);
-        acc[2] = fma_(Kac, nj.z, acc[2]);
-        T Kac2 = Kac * c;
-        acc[3] += Ka;
-        acc[4] += Kac2;
-        T w;
-        if constexpr (!MS) w = Ka + Kac2;
-        else { T Kd = gd * p.w; w = fma_(Kd * c, c, Kd); }
-        acc[5] = fma_(w, dx, acc[5]);
-        acc[6] = fma_(w, dy, acc[6]);
-        acc[7] = fma_(w, dz, acc[7]);
-    }
-};
-
-template <typename T>
-__device__ __forceinline__ double sum_range(const Partials<T>& p, int k0, int k1, int nch, int i, int c) {
-    if constexpr (kCompensated<T>) {
-        float h = 0.f, l = 0.f;
-        for (int k = k0; k < k1; ++k) {
-            size_t o = (size_t(k) * nch + c) * size_t(p.n) + i;
-            two_sum_acc(h, l, p.hi[o]);
-            l += p.lo[o];
-        }
-        return double(h) + double(l);
-    } else {
-        double s = 0.0;
-        for (int k = k0; k < k1; ++k) s += p.hi[(size_t(k) * nch + c) * size_t(p.n) + i];
-        return s;
-    }
-}
-
 // Energy: E = fK * sum_i aw_i row_i, one block, fixed order (deterministic).
 template <typename T>
 __global__ void __launch_bounds__(1024) k_vf_energy(Partials<T> part, int nchunk, const FaceRec* f1, int n1,
@@ -216,9 +59,14 @@ __global__ void __launch_bounds__(1024) k_vf_energy(Partials<T> part, int nchunk
 }
 
 // Gradient epilogue per face: z1, dz1 and the energy row.
+// Surfaces (base/surfaceDistances.py:394-468): zf = 1, sd = 1, dzf = 2/3 (centre
+// = mean of 3 vertices).  Curves (base/curveDistances.py:280-336): zf = 2,
+// sd = sqrt(d), dzf = 1 (no 1/3 factor; the midpoint's 1/2 cancels the 2 of
+// the symmetric term).  z1 = fw zf (2 sd S + (A - B) n) is the weighted z1.
 template <typename T>
 __global__ void k_vf_grad_epi(Partials<T> part, int c1, int nchunk, const FaceRec* f1, int n1, double fK,
-                              double fD, double* z1, double* dz1, double* erow) {
+                              double fD, double zf, double sd, double dzf, double* z1, double* dz1,
+                              double* erow) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n1) return;
     double S[3], D[3];
@@ -232,8 +80,8 @@ __global__ void k_vf_grad_epi(Partials<T> part, int c1, int nchunk, const FaceRe
     double AmB = (A1 + A2) - (B1 + B2);
     double n[3] = {f.n0, f.n1, f.n2};
     for (int c = 0; c < 3; ++c) {
-        z1[3 * size_t(i) + c] = f.fw * fK * (2.0 * S[c] + AmB * n[c]);
-        dz1[3 * size_t(i) + c] = (2.0 / 3.0) * fD * f.aw * D[c];
+        z1[3 * size_t(i) + c] = f.fw * fK * zf * (2.0 * sd * S[c] + AmB * n[c]);
+        dz1[3 * size_t(i) + c] = dzf * fD * f.aw * D[c];
     }
     // surface-2 sources carry weight -a_j here; the cross term of the energy is -2<S1,S2>
     erow[i] = f.aw * fK * ((A1 + B1) + 2.0 * (A2 + B2));
@@ -256,9 +104,11 @@ __global__ void __launch_bounds__(1024) k_sum_rows(const double* rows, int n, do
 __global__ void k_face_gather(const double* __restrict__ x, const int32_t* __restrict__ f,
                               const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx, int nv,
                               const double* __restrict__ dz1, const double* __restrict__ z1,
-                              double* __restrict__ px) {
+                              double* __restrict__ px, const int32_t* __restrict__ vpair,
+                              const uint8_t* __restrict__ active) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
+    if (vpair && !active[vpair[v]]) return;
     double acc[3] = {0.0, 0.0, 0.0};
     for (int e = ptr[v]; e < ptr[v + 1]; ++e) {
         int code = idx[e];
@@ -328,26 +178,34 @@ __global__ void __launch_bounds__(1024) k_center_faces(const FaceRec* a, int na,
     }
 }
 
-template <typename T, bool MS>
-int varifold_t(const lddmm_kernel* kd, const lddmm_surface* s1, const lddmm_surface* s2, double* energy,
-               double* grad, hipStream_t st) {
+// Shape of the varifold for one geometry (surface faces or curve segments).
+struct VfShape {
+    double sd;    // sqrt(d) of f(u) = 1 + d u^2
+    double zf;    // factor of z1
+    double dzf;   // factor of dz1 (centre derivative)
+};
+
+// Shared driver: `feat(r1, r2)` fills the records of both shapes, `gather(dz1, z1)`
+// scatters the per-element gradient to the vertices.
+template <typename T, bool MS, class Feat, class Gather>
+int varifold_core(const lddmm_kernel* kd, int n1, int n2, bool has_tgt, VfShape shp, Feat&& feat,
+                  Gather&& gather, double* energy, bool want_grad, hipStream_t st) {
     SingleScaleConsts ss;
     ScaleSet<T> sc = make_scales<T>(kd, &ss);
-    const int n1 = int(s1->nf), n2 = s2 ? int(s2->nf) : 0;
     const double fK = 1.0;
     const double fD = MS ? -1.0 : ss.diff;
     const int R1 = n1 >= 8192 && !MS ? 2 : 1;
 
     PairPlan plan;
     size_t need = 256 + size_t(n1 + n2) * sizeof(FaceRec) + 256;
-    if (grad) {
+    if (want_grad) {
         DeviceInfo di;
         LDDMM_TRY(device_info(&di));
         int occ = blocks_per_cu(R1 == 2 ? reinterpret_cast<const void*>(&k_pair<T, OpVarifoldG<MS>, 2>)
                                         : reinterpret_cast<const void*>(&k_pair<T, OpVarifoldG<MS>, 1>),
                                 kBlock, 0);
         plan = plan_pair(n1, n1 + n2, R1, di.num_cu, occ);
-        // chunks aligned to the surface boundary
+        // chunks aligned to the boundary between the two shapes
         int c1 = (n1 + plan.chunk - 1) / plan.chunk, c2 = (n2 + plan.chunk - 1) / plan.chunk;
         plan.nchunk = c1 + c2;
         need += partial_bytes<T>(plan, 8, n1) + size_t(n1) * (7 * sizeof(double)) + 1024;
@@ -363,14 +221,13 @@ int varifold_t(const lddmm_kernel* kd, const lddmm_surface* s1, const lddmm_surf
     double* c0 = reinterpret_cast<double*>(base + cv.take(32));
     FaceRec* r1 = reinterpret_cast<FaceRec*>(base + cv.take(size_t(n1) * sizeof(FaceRec)));
     FaceRec* r2 = reinterpret_cast<FaceRec*>(base + cv.take(size_t(std::max(n2, 1)) * sizeof(FaceRec)));
-    LDDMM_TRY(launch_features(s1, r1, nullptr, nullptr, nullptr, st));
-    if (s2) LDDMM_TRY(launch_features(s2, r2, nullptr, nullptr, nullptr, st));
+    LDDMM_TRY(feat(r1, has_tgt ? r2 : nullptr));
     hipLaunchKernelGGL(k_center_faces, dim3(1), dim3(1024), 0, st, r1, n1, r2, n2, c0);
     LDDMM_CHECK_LAUNCH();
 
-    if (!grad) {
+    if (!want_grad) {
         typename OpVarifoldE<MS>::template Args<T> A;
-        A.f1 = r1; A.f2 = r2; A.n1 = n1; A.n2 = n2; A.w2 = -2.0; A.c1 = 0; A.c0 = c0; A.sc = sc;
+        A.f1 = r1; A.f2 = r2; A.n1 = n1; A.n2 = n2; A.w2 = -2.0; A.c1 = 0; A.c0 = c0; A.sd = shp.sd; A.sc = sc;
         Partials<T> part = carve_partials<T>(base, cv, plan, 1, n1);
         if (R1 == 2) LDDMM_TRY((run_pair<T, OpVarifoldE<MS>, 2>(A, n1, n1 + n2, plan, part, st)));
         else LDDMM_TRY((run_pair<T, OpVarifoldE<MS>, 1>(A, n1, n1 + n2, plan, part, st)));
@@ -381,7 +238,7 @@ int varifold_t(const lddmm_kernel* kd, const lddmm_surface* s1, const lddmm_surf
 
     typename OpVarifoldG<MS>::template Args<T> A;
     const int c1 = (n1 + plan.chunk - 1) / plan.chunk;
-    A.f1 = r1; A.f2 = r2; A.n1 = n1; A.n2 = n2; A.w2 = -1.0; A.c1 = c1; A.c0 = c0; A.sc = sc;
+    A.f1 = r1; A.f2 = r2; A.n1 = n1; A.n2 = n2; A.w2 = -1.0; A.c1 = c1; A.c0 = c0; A.sd = shp.sd; A.sc = sc;
     Partials<T> part = carve_partials<T>(base, cv, plan, 8, n1);
     double* z1 = reinterpret_cast<double*>(base + cv.take(size_t(n1) * 3 * sizeof(double)));
     double* dz1 = reinterpret_cast<double*>(base + cv.take(size_t(n1) * 3 * sizeof(double)));
@@ -389,18 +246,130 @@ int varifold_t(const lddmm_kernel* kd, const lddmm_surface* s1, const lddmm_surf
     if (R1 == 2) LDDMM_TRY((run_pair<T, OpVarifoldG<MS>, 2>(A, n1, n1 + n2, plan, part, st)));
     else LDDMM_TRY((run_pair<T, OpVarifoldG<MS>, 1>(A, n1, n1 + n2, plan, part, st)));
     hipLaunchKernelGGL((k_vf_grad_epi<T>), dim3(unsigned((n1 + 255) / 256)), dim3(256), 0, st, part, c1,
-                       plan.nchunk, r1, n1, fK, fD, z1, dz1, erow);
+                       plan.nchunk, r1, n1, fK, fD, shp.zf, shp.sd, shp.dzf, z1, dz1, erow);
     LDDMM_CHECK_LAUNCH();
     if (energy) {
         hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(1024), 0, st, erow, n1, energy);
         LDDMM_CHECK_LAUNCH();
     }
-    if (s1->nv > 0) {
-        hipLaunchKernelGGL(k_face_gather, dim3(unsigned((s1->nv + 255) / 256)), dim3(256), 0, st, s1->vertices,
-                           s1->faces, s1->adj_ptr, s1->adj_idx, int(s1->nv), dz1, z1, grad);
-        LDDMM_CHECK_LAUNCH();
+    return gather(dz1, z1);
+}
+
+template <typename T, bool MS>
+int varifold_t(const lddmm_kernel* kd, const lddmm_surface* s1, const lddmm_surface* s2, double* energy,
+               double* grad, hipStream_t st) {
+    auto feat = [&](FaceRec* r1, FaceRec* r2) -> int {
+        LDDMM_TRY(launch_features(s1, r1, nullptr, nullptr, nullptr, st));
+        if (r2) LDDMM_TRY(launch_features(s2, r2, nullptr, nullptr, nullptr, st));
+        return LDDMM_OK;
+    };
+    auto gather = [&](const double* dz1, const double* z1) -> int {
+        if (s1->nv > 0) {
+            hipLaunchKernelGGL(k_face_gather, dim3(unsigned((s1->nv + 255) / 256)), dim3(256), 0, st,
+                               s1->vertices, s1->faces, s1->adj_ptr, s1->adj_idx, int(s1->nv), dz1, z1, grad,
+                               nullptr, nullptr);
+            LDDMM_CHECK_LAUNCH();
+        }
+        return LDDMM_OK;
+    };
+    return varifold_core<T, MS>(kd, int(s1->nf), s2 ? int(s2->nf) : 0, s2 != nullptr,
+                                VfShape{1.0, 1.0, 2.0 / 3.0}, feat, gather, energy, grad != nullptr, st);
+}
+
+// ---------------------------------------------------------------- curves
+
+// Segment features (base/curves.py:117-125, base/curveDistances.py:186-336):
+// c = (x1+x2)/2, l = x2-x1, w = mean of the two vertex weights.  Energy form
+// (varifoldNorm0/Def) folds w into l before the norm: a = sqrt(|w l|^2 + 1e-10);
+// gradient form: a = sqrt(|l|^2 + 1e-10), record weight a*w.  2-D curves are
+// padded with a zero third coordinate.
+__global__ void k_seg_features(const double* __restrict__ v, int dim, const int32_t* __restrict__ sg, int ns,
+                               const double* __restrict__ vw, int energy_form, FaceRec* __restrict__ rec,
+                               double* __restrict__ centers, double* __restrict__ linel,
+                               double* __restrict__ sweights) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    int i0 = sg[2 * i], i1 = sg[2 * i + 1];
+    double x1[3] = {0.0, 0.0, 0.0}, x2[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < dim; ++d) {
+        x1[d] = v[size_t(dim) * i0 + d];
+        x2[d] = v[size_t(dim) * i1 + d];
     }
+    double w = vw ? (vw[i0] + vw[i1]) / 2.0 : 1.0;
+    double l[3] = {x2[0] - x1[0], x2[1] - x1[1], x2[2] - x1[2]};
+    double c[3] = {(x1[0] + x2[0]) / 2.0, (x1[1] + x2[1]) / 2.0, (x1[2] + x2[2]) / 2.0};
+    if (rec) {
+        double ll = l[0] * l[0] + l[1] * l[1] + l[2] * l[2];
+        if (energy_form) {
+            double a = sqrt(w * w * ll + 1e-10);
+            rec[i] = FaceRec{c[0], c[1], c[2], a, w * l[0] / a, w * l[1] / a, w * l[2] / a, w};
+        } else {
+            double a = sqrt(ll + 1e-10);
+            rec[i] = FaceRec{c[0], c[1], c[2], a * w, l[0] / a, l[1] / a, l[2] / a, w};
+        }
+    }
+    for (int d = 0; d < dim; ++d) {
+        if (centers) centers[size_t(dim) * i + d] = c[d];
+        if (linel) linel[size_t(dim) * i + d] = l[d];
+    }
+    if (sweights) sweights[i] = w;
+}
+
+// px[v] = sum over incident (segment, end): dz1_s - wz1_s (end 0) or dz1_s + wz1_s (end 1)
+// (base/curveDistances.py:97-103, :324-326).  dz1 / wz1 rows have stride ld.
+__global__ void k_seg_gather(const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx, int nv, int dim,
+                             const double* __restrict__ dz1, const double* __restrict__ wz1, int ld,
+                             double* __restrict__ px) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int e = ptr[v]; e < ptr[v + 1]; ++e) {
+        int code = idx[e];
+        int si = code >> 1;
+        double sg = (code & 1) ? 1.0 : -1.0;
+        for (int k = 0; k < dim; ++k) acc[k] += dz1[size_t(ld) * si + k] + sg * wz1[size_t(ld) * si + k];
+    }
+    for (int k = 0; k < dim; ++k) px[size_t(dim) * v + k] = acc[k];
+}
+
+int check_curve(const lddmm_curve* c, bool need_adj, const char* what) {
+    LDDMM_REQUIRE(c != nullptr, "%s: null curve", what);
+    LDDMM_REQUIRE(c->nv >= 0 && c->ns >= 0, "%s: negative sizes", what);
+    LDDMM_REQUIRE(c->dim == 2 || c->dim == 3, "%s: dim must be 2 or 3 (got %d)", what, int(c->dim));
+    LDDMM_REQUIRE(c->ns < (int64_t(1) << 30) && c->nv < (int64_t(1) << 31), "%s: sizes beyond int32", what);
+    LDDMM_REQUIRE(c->ns == 0 || (c->vertices && c->segments), "%s: null vertices / segments", what);
+    if (need_adj) LDDMM_REQUIRE(c->nv == 0 || (c->adj_ptr && c->adj_idx), "%s: adjacency required", what);
     return LDDMM_OK;
+}
+
+int launch_seg_features(const lddmm_curve* c, int energy_form, FaceRec* rec, double* ctr, double* ll, double* sw,
+                        hipStream_t st) {
+    if (c->ns == 0) return LDDMM_OK;
+    hipLaunchKernelGGL(k_seg_features, dim3(unsigned((c->ns + 255) / 256)), dim3(256), 0, st, c->vertices,
+                       int(c->dim), c->segments, int(c->ns), c->vweights, energy_form, rec, ctr, ll, sw);
+    LDDMM_CHECK_LAUNCH();
+    return LDDMM_OK;
+}
+
+template <typename T, bool MS>
+int curve_varifold_t(const lddmm_kernel* kd, const lddmm_curve* c1, const lddmm_curve* c2, double d,
+                     double* energy, double* grad, hipStream_t st) {
+    const int eform = grad ? 0 : 1;
+    auto feat = [&](FaceRec* r1, FaceRec* r2) -> int {
+        LDDMM_TRY(launch_seg_features(c1, eform, r1, nullptr, nullptr, nullptr, st));
+        if (r2) LDDMM_TRY(launch_seg_features(c2, eform, r2, nullptr, nullptr, nullptr, st));
+        return LDDMM_OK;
+    };
+    auto gather = [&](const double* dz1, const double* z1) -> int {
+        if (c1->nv > 0) {
+            hipLaunchKernelGGL(k_seg_gather, dim3(unsigned((c1->nv + 255) / 256)), dim3(256), 0, st, c1->adj_ptr,
+                               c1->adj_idx, int(c1->nv), int(c1->dim), dz1, z1, 3, grad);
+            LDDMM_CHECK_LAUNCH();
+        }
+        return LDDMM_OK;
+    };
+    return varifold_core<T, MS>(kd, int(c1->ns), c2 ? int(c2->ns) : 0, c2 != nullptr,
+                                VfShape{std::sqrt(d), 2.0, 1.0}, feat, gather, energy, grad != nullptr, st);
 }
 
 }  // namespace
@@ -421,7 +390,7 @@ extern "C" int lddmm_surface_face_gather(const lddmm_surface* s, const double* d
     LDDMM_REQUIRE(px && (s->nf == 0 || (dz1 && z1)), "surface_face_gather: null arrays");
     hipLaunchKernelGGL(k_face_gather, dim3(unsigned((s->nv + 255) / 256)), dim3(256), 0,
                        static_cast<hipStream_t>(stream), s->vertices, s->faces, s->adj_ptr, s->adj_idx,
-                       int(s->nv), dz1, z1, px);
+                       int(s->nv), dz1, z1, px, nullptr, nullptr);
     LDDMM_CHECK_LAUNCH();
     return LDDMM_OK;
 }
@@ -444,4 +413,47 @@ extern "C" int lddmm_surface_varifold(const lddmm_kernel* kd, const lddmm_surfac
                   : varifold_t<float, false>(kd, s1, tgt, energy, grad, st);
     return ms ? varifold_t<double, true>(kd, s1, tgt, energy, grad, st)
               : varifold_t<double, false>(kd, s1, tgt, energy, grad, st);
+}
+
+extern "C" int lddmm_curve_features(const lddmm_curve* c, double* centers, double* linel, double* seg_weights,
+                                    void* stream) {
+    LDDMM_TRY(check_curve(c, false, "curve_features"));
+    return launch_seg_features(c, 0, nullptr, centers, linel, seg_weights, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int lddmm_curve_seg_gather(const lddmm_curve* c, const double* dz1, const double* wz1, double* px,
+                                      void* stream) {
+    LDDMM_TRY(check_curve(c, true, "curve_seg_gather"));
+    if (c->nv == 0) return LDDMM_OK;
+    LDDMM_REQUIRE(px && (c->ns == 0 || (dz1 && wz1)), "curve_seg_gather: null arrays");
+    hipLaunchKernelGGL(k_seg_gather, dim3(unsigned((c->nv + 255) / 256)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), c->adj_ptr, c->adj_idx, int(c->nv), int(c->dim), dz1, wz1,
+                       int(c->dim), px);
+    LDDMM_CHECK_LAUNCH();
+    return LDDMM_OK;
+}
+
+extern "C" int lddmm_curve_varifold(const lddmm_kernel* kd, const lddmm_curve* c1, const lddmm_curve* tgt,
+                                    double weight, double* energy, double* grad, void* stream) {
+    LDDMM_TRY(validate_kernel(kd));
+    LDDMM_TRY(check_curve(c1, grad != nullptr, "curve_varifold"));
+    if (tgt) {
+        LDDMM_TRY(check_curve(tgt, false, "curve_varifold (target)"));
+        LDDMM_REQUIRE(tgt->dim == c1->dim, "curve_varifold: dimension mismatch (%d vs %d)", int(c1->dim),
+                      int(tgt->dim));
+    }
+    LDDMM_REQUIRE(energy || grad, "curve_varifold: nothing requested");
+    LDDMM_REQUIRE(weight >= 0.0 && std::isfinite(weight), "curve_varifold: weight must be finite and >= 0");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (c1->ns == 0) {
+        if (energy) LDDMM_TRY(launch_fill(energy, 1, 0.0, st));
+        if (grad) LDDMM_TRY(launch_fill(grad, size_t(c1->nv) * size_t(c1->dim), 0.0, st));
+        return LDDMM_OK;
+    }
+    const bool ms = kd->nscale > 1;
+    if (kd->prec == LDDMM_PREC_F32C)
+        return ms ? curve_varifold_t<float, true>(kd, c1, tgt, weight, energy, grad, st)
+                  : curve_varifold_t<float, false>(kd, c1, tgt, weight, energy, grad, st);
+    return ms ? curve_varifold_t<double, true>(kd, c1, tgt, weight, energy, grad, st)
+              : curve_varifold_t<double, false>(kd, c1, tgt, weight, energy, grad, st);
 }

@@ -37,6 +37,14 @@ class LddmmSurface(ctypes.Structure):
                 ("adj_ptr", ctypes.c_void_p), ("adj_idx", ctypes.c_void_p)]
 
 
+class LddmmCurve(ctypes.Structure):
+    _fields_ = [("vertices", ctypes.c_void_p), ("nv", ctypes.c_int64),
+                ("dim", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("segments", ctypes.c_void_p), ("ns", ctypes.c_int64),
+                ("vweights", ctypes.c_void_p),
+                ("adj_ptr", ctypes.c_void_p), ("adj_idx", ctypes.c_void_p)]
+
+
 class LddmmError(RuntimeError):
     pass
 
@@ -47,6 +55,7 @@ _I32 = ctypes.c_int32
 _D = ctypes.c_double
 _KP = ctypes.POINTER(LddmmKernel)
 _SP = ctypes.POINTER(LddmmSurface)
+_CP = ctypes.POINTER(LddmmCurve)
 
 # name -> argtypes (restype is int for every compute entry point)
 SIGNATURES = {
@@ -61,6 +70,10 @@ SIGNATURES = {
     "lddmm_surface_varifold": [_KP, _SP, _SP, _P, _P, _P],
     "lddmm_surface_features": [_SP, _P, _P, _P, _P],
     "lddmm_surface_face_gather": [_SP, _P, _P, _P, _P],
+    "lddmm_curve_adjacency": [_P, _I64, _I64, _P, _P],
+    "lddmm_curve_varifold": [_KP, _CP, _CP, _D, _P, _P, _P],
+    "lddmm_curve_features": [_CP, _P, _P, _P, _P],
+    "lddmm_curve_seg_gather": [_CP, _P, _P, _P, _P],
     "lddmm_shoot_forward": [_KP, _P, _I32, _I64, _I32, _P, _D, _P, _P],
     "lddmm_shoot_adjoint": [_KP, _P, _P, _I32, _I64, _I32, _P, _D, _I32, _P, _P, _P],
     "lddmm_epdiff_forward": [_KP, _I32, _I64, _I32, _P, _P, _P],

@@ -108,7 +108,7 @@ def warm_control(sm, net, cg_iters=20):
     return at, a0
 
 
-def register(seed, net=None, target_data=None, cfg=DEFAULT, max_iter=None, a0=None, cg_iters=20):
+def register(seed, net=None, target_data=None, cfg=DEFAULT, max_iter=None, a0=None, cg_iters=20, at=None):
     """Registration (plain, warm-started by `net`, or warm-started from a given
     initial momentum `a0`) until the data term reaches target_data (1e-4 rel).
 
@@ -122,8 +122,10 @@ def register(seed, net=None, target_data=None, cfg=DEFAULT, max_iter=None, a0=No
     t0 = time.perf_counter()
     sm = make_matching(seed, c)
     used_warm = False
-    if net is not None or a0 is not None:
-        if a0 is None:
+    if net is not None or a0 is not None or at is not None:
+        if at is not None:
+            at = B.to_dev(at)
+        elif a0 is None:
             at, _ = warm_control(sm, net, cg_iters=cg_iters)
         else:
             _, at = evol.landmarkEPDiff(sm.Tsize, sm.x0_dev, B.to_dev(a0), sm.options['KparDiff'])

@@ -113,3 +113,30 @@ def plain_matching_pair(seed=0, subdiv=4, radius=10.0, nbumps=20, width=3.0, amp
 
 def uniform_cloud(n, lo=0.0, hi=50.0, seed=0, dim=3):
     return np.random.default_rng(seed).uniform(lo, hi, size=(n, dim))
+
+
+def closed_curve(n, dim=2, radius=5.0, nmodes=4, amp=0.15, seed=0):
+    """Seeded closed curve of n vertices: a circle of the given radius (in the
+    xy plane) with random low-frequency radial modes; in 3-D the z coordinate
+    carries its own modes.  Returns (vertices [n, dim], segments [n, 2])."""
+    rng = np.random.default_rng(seed)
+    t = np.linspace(0, 2 * np.pi, n, endpoint=False)
+    r = np.ones(n)
+    for k in range(1, nmodes + 1):
+        r += amp / k * (rng.normal() * np.cos(k * t) + rng.normal() * np.sin(k * t))
+    v = np.stack([radius * r * np.cos(t), radius * r * np.sin(t)], 1)
+    if dim == 3:
+        z = np.zeros(n)
+        for k in range(1, nmodes + 1):
+            z += amp * radius / k * (rng.normal() * np.cos(k * t) + rng.normal() * np.sin(k * t))
+        v = np.concatenate([v, z[:, None]], 1)
+    f = np.stack([np.arange(n), (np.arange(n) + 1) % n], 1)
+    return v, f
+
+
+def curve_matching_pair(seed=0, n=400, dim=2, radius=5.0):
+    """Template = closed_curve(seed), target = closed_curve(seed + 1) with the
+    same vertex count and segment list."""
+    v, f = closed_curve(n, dim, radius, seed=seed)
+    v2, _ = closed_curve(n, dim, radius, seed=seed + 1)
+    return (v, f), (v2, f)

@@ -80,6 +80,30 @@ def test_adjacency_builder(lib):
     assert B.LIB.lddmm_surface_adjacency(bad.ctypes.data, 1, 4, ptr.ctypes.data, idx.ctypes.data) == B.EINVAL
 
 
+def test_curve_adjacency_and_validation(lib):
+    B = lib
+    s = np.array([[0, 1], [1, 2], [2, 0], [2, 3]], dtype=np.int32)
+    ptr = np.zeros(5, dtype=np.int32)
+    idx = np.zeros(8, dtype=np.int32)
+    assert B.LIB.lddmm_curve_adjacency(s.ctypes.data, 4, 4, ptr.ctypes.data, idx.ctypes.data) == B.OK
+    for v in range(4):
+        got = sorted((e // 2, e % 2) for e in idx[ptr[v]:ptr[v + 1]])
+        assert got == sorted((si, c) for si in range(4) for c in range(2) if s[si, c] == v)
+    bad = np.array([[0, 4]], dtype=np.int32)
+    assert B.LIB.lddmm_curve_adjacency(bad.ctypes.data, 1, 4, ptr.ctypes.data, idx.ctypes.data) == B.EINVAL
+    # argument checks return before any device work
+    kd = B.kernel_desc([1.0])
+    c = B.LddmmCurve()
+    c.nv, c.ns, c.dim = 4, 4, 4
+    e = ctypes.c_void_p(1)
+    assert B.LIB.lddmm_curve_varifold(kd, ctypes.byref(c), None, 1.0, e, None, None) == B.EINVAL   # dim
+    c.dim = 2
+    assert B.LIB.lddmm_curve_varifold(kd, ctypes.byref(c), None, 1.0, e, None, None) == B.EINVAL   # nulls
+    c.vertices, c.segments = 8, 8        # never dereferenced: the weight check comes first
+    assert B.LIB.lddmm_curve_varifold(kd, ctypes.byref(c), None, -1.0, e, None, None) == B.EINVAL
+    assert b"weight" in B.LIB.lddmm_last_error()
+
+
 def test_kernel_desc_mirrors_kernelspec(lib):
     kd = lib.kernel_desc([2.5, 5.0], kp=-1, prec='float32')
     assert kd.nscale == 2 and kd.sigma[1] == 5.0 and kd.prec == lib.PREC_F32C

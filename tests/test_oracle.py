@@ -273,6 +273,42 @@ def test_curve_varifold(curves):
                                rtol=1e-12)
 
 
+def test_curve_varifold_weight_and_3d():
+    """Orientation weight d: FD gradient for d = 0.5 on a weighted 3-D curve; d = 0
+    reduces to the length measure sum_ij K(c_i, c_j) |l_i| |l_j| (brute force)."""
+    rng = np.random.default_rng(4)
+    t = np.linspace(0, 2 * np.pi, 31)[:-1]
+    v1 = np.stack([np.cos(t), np.sin(t), 0.3 * np.sin(2 * t)], 1) * 2
+    v2 = v1 * 1.1 + 0.05 * rng.normal(size=v1.shape)
+    f = np.stack([np.arange(30), (np.arange(30) + 1) % 30], 1)
+    w = rng.uniform(0.5, 1.5, size=30)
+    A, Bc = (v1, f, w), (v2, f)
+    dv = rnd(*v1.shape)
+    fd = _fd(lambda vv: O.curveVarifoldNormDef((vv, f, w), Bc, [0.7], weight=0.5), v1, dv)
+    np.testing.assert_allclose((O.curveVarifoldNormGradient(A, Bc, [0.7], weight=0.5) * dv).sum(), fd,
+                               rtol=1e-6)
+    c = (v1[f[:, 0]] + v1[f[:, 1]]) / 2
+    lw = (w[f[:, 0]] + w[f[:, 1]]) / 2
+    a = np.sqrt(((v1[f[:, 1]] - v1[f[:, 0]]) ** 2).sum(1) * lw ** 2 + 1e-10)
+    d2 = ((c[:, None] - c[None]) ** 2).sum(-1)
+    brute = (np.exp(-d2 / (2 * 0.7 ** 2)) * a[:, None] * a[None]).sum()
+    np.testing.assert_allclose(O.curveVarifoldNorm0(A, [0.7], weight=0.), brute, rtol=1e-12)
+
+
+def test_curve_matching_gradient_fd(curves):
+    """Curve-matching gradient as BFGS sees it, K(x_t)(2 r a_t - p_{t+1}) / T,
+    is the exact derivative of the discrete objective (FD), for both data terms."""
+    A, Bc = curves
+    v, f = A
+    at = 0.05 * rnd(5, *v.shape)
+    dat = rnd(*at.shape)
+    for kind in ('curve_varifold', 'curve_current'):
+        g = O.matching_gradient(v, f, at, Bc, [1.5], [0.5], kind=kind, regweight=0.7)
+        fd = _fd(lambda a_: O.matching_objective(v, f, a_, Bc, [1.5], [0.5], kind=kind, regweight=0.7)[0],
+                 at, dat)
+        np.testing.assert_allclose((g * dat).sum(), fd, rtol=1e-6)
+
+
 def test_curve_current(curves):
     A, Bc = curves
     v, f = A
