@@ -123,6 +123,83 @@ def registration_cfg1(torch):
     return out
 
 
+def population_cfg4(torch, npairs=256, T=10):
+    """Batched population workload: 256 template-target pairs, vertex counts
+    log-uniform in [500, 20000] (Fibonacci sphere / ellipsoid hulls with bumps),
+    sigma_def 3, varifold sigma 1, T = 10, FP32.  Times one batched forward
+    shoot, one Euclidean adjoint and one varifold gradient of all pairs (one
+    persistent launch per Gaussian sum, shooting loops replayed from hipGraphs),
+    and the same forward shoot as 256 single-pair calls."""
+    import numpy as np
+    from lddmm_amd import synth, _backend as B, pointEvolution as evol
+    from lddmm_amd.population import Population
+    from lddmm_amd.surfaces import Surface
+    from lddmm_amd.kernelFunctions import Kernel
+    t0 = time.perf_counter()
+    prs = synth.population_pairs(npairs, seed=4)
+    tm = [Surface((f, v)) for (v, f), _ in prs]
+    tg = [Surface((f2, v2)) for _, (v2, f2) in prs]
+    pop = Population(tm, tg)
+    gen_s = time.perf_counter() - t0
+    n = np.diff(pop.voff).astype(np.float64)
+    nf = np.diff(pop.foff).astype(np.float64)
+    ntf = np.diff(pop.tfoff).astype(np.float64)
+    kd = B.kernel_desc([3.0], -1, "float32")
+    kv = B.kernel_desc([1.0], -1, "float32")
+    rng = np.random.default_rng(0)
+    at = torch.tensor(rng.normal(size=(T, pop.V, 3)) * 0.05, device="cuda")
+    xt = pop.x0.expand(T + 1, -1, -1).clone()
+    en = torch.zeros(pop.P, dtype=torch.float64, device="cuda")
+    px1 = torch.zeros((pop.V, 3), dtype=torch.float64, device="cuda")
+    g = torch.zeros_like(at)
+    e = torch.zeros_like(en)
+
+    def timed(fn, reps=3):
+        fn()                                  # tuning / capture
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    fwd = timed(lambda: pop.shoot(kd, at, xt, 1.0, en))
+    pop.varifold(kv, xt[-1], energies=e, grad=px1)
+    adj = timed(lambda: pop.adjoint(kd, at, xt, px1, g, 1.0, euclidean=True))
+    vf = timed(lambda: pop.varifold(kv, xt[-1], energies=e, grad=px1))
+    # the same forward shoot, one pair per call
+    K = Kernel(name="gauss", sigma=3.0)
+    K.pk_dtype = "float32"
+    offs = [(int(pop.voff[p]), int(pop.voff[p + 1])) for p in range(pop.P)]
+    ats = [at[:, r0:r1].contiguous() for r0, r1 in offs]
+    x0s = [pop.x0[r0:r1].contiguous() for r0, r1 in offs]
+    for p in range(min(4, pop.P)):
+        evol.shoot(x0s[p], ats[p], K, energy=True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for p in range(pop.P):
+        evol.shoot(x0s[p], ats[p], K, energy=True)
+    torch.cuda.synchronize()
+    seq = time.perf_counter() - t
+    pk = float(T * (n ** 2).sum())
+    out = {
+        "pairs": int(pop.P), "vertices": int(pop.V), "faces": int(pop.foff[-1]),
+        "vertex_count_range": [int(n.min()), int(n.max())], "generation_s": gen_s,
+        "pair_evals_per_forward": pk,
+        "forward_ms": fwd * 1e3, "forward_pair_evals_per_s": pk / fwd,
+        "adjoint_ms": adj * 1e3, "adjoint_pair_evals_per_s": 2 * pk / adj,
+        "varifold_gradient_ms": vf * 1e3,
+        "varifold_face_pairs_per_s": float((nf * (nf + ntf)).sum()) / vf,
+        "sequential_single_pair_forward_ms": seq * 1e3, "batched_speedup_forward": seq / fwd,
+        "tuning": pop.tuning(),
+    }
+    log(f"population cfg4: {pop.P} pairs, V={pop.V}: forward {fwd * 1e3:.1f} ms "
+        f"({pk / fwd:.3g} pair-evals/s), adjoint {adj * 1e3:.1f} ms, varifold grad {vf * 1e3:.1f} ms; "
+        f"256 single-pair forwards {seq * 1e3:.1f} ms")
+    pop.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -294,6 +371,7 @@ def main():
     if not args.no_extras:
         extra["varifold_100k"] = varifold_microbench(B, torch, synth, np)
         extra["registration_plain_cfg1"] = registration_cfg1(torch)
+        extra["population_cfg4"] = population_cfg4(torch)
 
     if not args.no_cpu:
         log("cpu baseline ...")

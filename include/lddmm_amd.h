@@ -309,6 +309,79 @@ int lddmm_nn_adam(float* w, uint16_t* wbf, const float* g, float* m, float* v, i
                   float lr, float b1, float b2, float eps, int32_t step, float wd, void* stream);
 int lddmm_nn_to_bf16(const float* w, uint16_t* wbf, int64_t count, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Population registration: P template-target surface pairs of uneven sizes
+ * computed together.  No counterpart in the reference (it registers one pair
+ * per SurfaceMatching); per pair, every result equals the single-pair entry
+ * point's (lddmm_shoot_forward / lddmm_shoot_adjoint / lddmm_surface_varifold)
+ * up to floating-point summation order.  Each Gaussian sum over all active
+ * pairs is ONE persistent launch driven by a segment table of (pair, target
+ * tile, source chunk) work items, scheduled longest-first over the
+ * workgroups on the host; tile and chunk shapes come from an autotuner that
+ * times the candidates on the first call.  The T-step shooting loops are
+ * captured once into hipGraphs and replayed.
+ * Template vertices of all pairs are concatenated ([V,3], pair p owns rows
+ * [voff[p], voff[p+1])), and so are the template faces (global vertex
+ * indices), the target vertices and the target faces.
+ * ---------------------------------------------------------------------- */
+typedef struct lddmm_population_desc {
+    int32_t        npairs;
+    int32_t        reserved;
+    const int64_t* voff;        /* host [npairs+1] template vertex offsets */
+    const int64_t* foff;        /* host [npairs+1] template face offsets */
+    const int32_t* faces;       /* device [F,3] */
+    const double*  vweights;    /* device [V] or NULL (ones) */
+    const int32_t* adj_ptr;     /* device CSR vertex->(face,corner) over all */
+    const int32_t* adj_idx;     /*   template vertices (lddmm_surface_adjacency) */
+    const int64_t* tfoff;       /* host [npairs+1] target face offsets */
+    const double*  tvertices;   /* device target vertices of all pairs */
+    const int32_t* tfaces;      /* device [TF,3] */
+    const double*  tvweights;   /* device or NULL */
+} lddmm_population_desc;
+
+typedef struct lddmm_population lddmm_population;   /* opaque; one device */
+
+int lddmm_population_create(const lddmm_population_desc* desc, lddmm_population** out);
+int lddmm_population_destroy(lddmm_population* pop);
+/* Pairs computed by the following calls (active_host[p] != 0; NULL = all).
+ * Rows of inactive pairs in every output are left untouched. */
+int lddmm_population_set_active(lddmm_population* pop, const uint8_t* active_host);
+/* 0 = eager launches, 1 = hipGraph capture/replay of the shooting loops (default). */
+int lddmm_population_set_graphs(lddmm_population* pop, int32_t on);
+/* Forward Euler flow of every active pair: at [T,V,3], xt [T+1,V,3] (xt[0]
+ * read); energies (device f64[P], may be NULL) receive
+ * sum_t r dt a_t.K(x_t)a_t per pair. */
+int lddmm_population_shoot_forward(const lddmm_kernel* kd, lddmm_population* pop, const double* at,
+                                   int32_t T, double* xt, double regweight, double* energies,
+                                   void* stream);
+/* Exact discrete adjoint from px1 [V,3]: grad [T,V,3] (Euclidean K(x_t)
+ * correction when euclidean != 0), as lddmm_shoot_adjoint. */
+int lddmm_population_shoot_adjoint(const lddmm_kernel* kd, lddmm_population* pop, const double* at,
+                                   const double* xt, int32_t T, const double* px1, double regweight,
+                                   int32_t euclidean, double* grad, void* stream);
+/* Varifold data term of the deformed templates (vertices [V,3]) against the
+ * targets: energies (device f64[P] or NULL) = |S_p|^2 - 2 <S_p, T_p>,
+ * grad (device [V,3] or NULL) = d/dx |S_p - T_p|^2. */
+int lddmm_population_varifold(const lddmm_kernel* kd, lddmm_population* pop, const double* vertices,
+                              double* energies, double* grad, void* stream);
+/* out[p] = sum over t < T and the rows of pair p of a.b ([T,V,3] arrays). */
+int lddmm_population_seg_dot(lddmm_population* pop, const double* a, const double* b, int32_t T,
+                             double* out, void* stream);
+/* y[t,i,:] += alpha[pair(i)] * x[t,i,:] for the rows of active pairs (alpha device [P]). */
+int lddmm_population_seg_axpy(lddmm_population* pop, double* y, const double* alpha, const double* x,
+                              int32_t T, void* stream);
+/* Host-only self-test of the segment-table scheduler: builds the work items
+ * of one family (0 = vertex sums, 1 = varifold faces) for the given sizes,
+ * active set and launch shape (R targets per lane, f10 = items per workgroup
+ * x 10, G workgroups) and checks every item against the buffer bounds the
+ * kernels use.  No device call. */
+int lddmm_population_plan_check(int32_t npairs, const int64_t* voff, const int64_t* foff,
+                                const int64_t* tfoff, const uint8_t* active, int32_t family,
+                                int32_t R, int32_t f10, int32_t G, int64_t* nitems);
+/* Autotuned launch shapes: up to cap records of 4 int32 (op, precision, R,
+ * items per workgroup x 10); returns the number of records in *count. */
+int lddmm_population_tuning(const lddmm_population* pop, int32_t* out, int32_t cap, int32_t* count);
+
 #ifdef __cplusplus
 }
 #endif

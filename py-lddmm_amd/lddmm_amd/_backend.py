@@ -45,6 +45,15 @@ class LddmmCurve(ctypes.Structure):
                 ("adj_ptr", ctypes.c_void_p), ("adj_idx", ctypes.c_void_p)]
 
 
+class LddmmPopulationDesc(ctypes.Structure):
+    _fields_ = [("npairs", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("voff", ctypes.c_void_p), ("foff", ctypes.c_void_p),
+                ("faces", ctypes.c_void_p), ("vweights", ctypes.c_void_p),
+                ("adj_ptr", ctypes.c_void_p), ("adj_idx", ctypes.c_void_p),
+                ("tfoff", ctypes.c_void_p), ("tvertices", ctypes.c_void_p),
+                ("tfaces", ctypes.c_void_p), ("tvweights", ctypes.c_void_p)]
+
+
 class LddmmError(RuntimeError):
     pass
 
@@ -92,6 +101,17 @@ SIGNATURES = {
     "lddmm_nn_adam": [_P, _P, _P, _P, _P, _I64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                       ctypes.c_float, _I32, ctypes.c_float, _P],
     "lddmm_nn_to_bf16": [_P, _P, _I64, _P],
+    "lddmm_population_create": [ctypes.POINTER(LddmmPopulationDesc), ctypes.POINTER(ctypes.c_void_p)],
+    "lddmm_population_destroy": [_P],
+    "lddmm_population_set_active": [_P, _P],
+    "lddmm_population_set_graphs": [_P, _I32],
+    "lddmm_population_shoot_forward": [_KP, _P, _P, _I32, _P, _D, _P, _P],
+    "lddmm_population_shoot_adjoint": [_KP, _P, _P, _P, _I32, _P, _D, _I32, _P, _P],
+    "lddmm_population_varifold": [_KP, _P, _P, _P, _P, _P],
+    "lddmm_population_seg_dot": [_P, _P, _P, _I32, _P, _P],
+    "lddmm_population_seg_axpy": [_P, _P, _P, _P, _I32, _P],
+    "lddmm_population_tuning": [_P, _P, _I32, _P],
+    "lddmm_population_plan_check": [_I32, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _P],
 }
 
 

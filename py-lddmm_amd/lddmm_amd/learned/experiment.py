@@ -4,8 +4,10 @@
         --out momentum_net.npz --report report.json
 
 1. plain zero-momentum registrations of seeded deformed spheres (training
-   seeds 10000.., held-out seeds 900000..) record (features, a0 = at[0]);
-2. MomentumNet is trained on them with the package's HIP kernels;
+   seeds 10000.., held-out seeds 900000..) record the features and targets;
+2. MomentumNet is trained on them with the package's HIP kernels, one
+   network per feature:target variant (default target abar = time-mean of
+   the plain control, used as a time-constant initial control);
 3. on held-out pairs, the plain run's final varifold data term E* is the
    target; iterations / seconds to reach E* (1e-4 relative) are compared for
    plain (from its own per-iteration trace) and warm-started registrations.
@@ -29,7 +31,7 @@ def log(msg):
 
 def plain_with_trace(seed, cfg):
     sm = W.make_matching(seed, cfg)
-    feats = W.features(sm)
+    feats = {f"feat_{k}": W.features(sm, k).cpu().numpy() for k in W.FEATURES}
     sm.setTargetDataEnergy(None)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -59,8 +61,9 @@ def main(argv=None):
     ap.add_argument("--cg-iters", type=int, default=20)
     ap.add_argument("--oracle-check", type=int, default=5,
                     help="held-out pairs also warm-started from the plain run's own a0")
-    ap.add_argument("--targets", default="momentum,velocity",
-                    help="comma-separated network targets: momentum (a0) and/or velocity (K a0)")
+    ap.add_argument("--variants", default="force:abar,meanshift:abar,kforce:abar",
+                    help="comma-separated feature:target pairs (features force/meanshift/kforce; "
+                         "targets abar/momentum/velocity)")
     a = ap.parse_args(argv)
     logging.getLogger().setLevel(logging.WARNING)
     cfg = dict(W.DEFAULT)
@@ -71,9 +74,9 @@ def main(argv=None):
     for k in range(a.train):
         seed = 10000 + k
         sm, feats, sec = plain_with_trace(seed, cfg)
-        a0 = sm.control['at'][0]
-        samples.append({"features": feats.cpu().numpy(), "a0": a0.cpu().numpy(),
-                        "v0": W.velocity(sm, a0).cpu().numpy()})
+        smp = dict(feats)
+        smp.update(W.targets_of(sm, sm.control['at']))
+        samples.append(smp)
         if k % 100 == 0:
             log(f"train registration {k}: {sec:.2f} s, data {sm.varifold_data():.4f}")
         if time.perf_counter() - t0 > a.time_budget:
@@ -83,9 +86,12 @@ def main(argv=None):
     rep["generation_seconds"] = time.perf_counter() - t0
 
     nets = {}
-    for target in a.targets.split(","):
+    for variant in a.variants.split(","):
+        feat_kind, target = variant.split(":")
         t1 = time.perf_counter()
-        net, losses = W.train(samples, epochs=a.epochs, batch=a.batch, lr=a.lr, log_every=0, target=target)
+        net, losses = W.train(samples, epochs=a.epochs, batch=a.batch, lr=a.lr, log_every=0, target=target,
+                              features=feat_kind)
+        target = variant
         torch.cuda.synchronize()
         rep[f"train_seconds_{target}"] = time.perf_counter() - t1
         rep[f"loss_first_last_{target}"] = [losses[0], losses[-1]]
@@ -110,16 +116,15 @@ def main(argv=None):
             row.update({f"it_warm_{target}": r["iterations"], f"sec_warm_{target}": r["seconds"],
                         f"E_warm_{target}": r["data"], f"E0_warm_{target}": r["data0"],
                         f"reached_{target}": r["reached"], f"used_warm_{target}": r["used_warm"]})
-            if target == 'velocity':
-                feats = W.features(W.make_matching(seed, cfg))
-                a0_true = sm.control['at'][0]
-                v_true = W.velocity(sm, a0_true)
-                v_pred = net.predict(feats, [feats.shape[0]]).to(v_true.dtype)
-                row["v_rel_err"] = float((v_pred - v_true).norm() / v_true.norm())
+            tg = W.targets_of(sm, sm.control['at'])[{"momentum": "a0", "velocity": "v0", "abar": "abar"}[net.target]]
+            fx = W.features(W.make_matching(seed, cfg), net.features)
+            pred = net.predict(fx, [fx.shape[0]]).double().cpu().numpy()
+            row[f"pred_rel_err_{target}"] = float(np.linalg.norm(pred - tg) / np.linalg.norm(tg))
         if k < a.oracle_check:
-            # upper bound: warm start from the plain run's own a0 (perfect predictor)
-            r = W.register(seed, a0=sm.control['at'][0], target_data=E, cfg=cfg)
-            row.update({"it_warm_true_a0": r["iterations"], "E0_true_a0": r["data0"]})
+            # upper bound: warm start from the plain run's own time-mean control (perfect predictor)
+            abar = sm.control['at'].mean(0)
+            r = W.register(seed, at=abar[None].expand(sm.Tsize, -1, -1).contiguous(), target_data=E, cfg=cfg)
+            row.update({"it_warm_true_abar": r["iterations"], "E0_true_abar": r["data0"]})
         rows.append(row)
         if k % 10 == 0 or k < a.oracle_check:
             log(f"test {k}: plain {it_plain} it ({sec_plain:.2f} s) -> warm "
@@ -137,13 +142,12 @@ def main(argv=None):
             "warm_used_fraction": float(np.mean([r[f"used_warm_{target}"] for r in rows])),
             "mean_E0_over_E": float(np.mean([r[f"E0_warm_{target}"] / r["E_plain"] for r in rows])),
         }
-    if "v_rel_err" in rows[0]:
-        rep["test"]["mean_v_rel_err"] = float(np.mean([r["v_rel_err"] for r in rows]))
-    ora = [r for r in rows if "it_warm_true_a0" in r]
+        rep["test"][target]["mean_pred_rel_err"] = float(np.mean([r[f"pred_rel_err_{target}"] for r in rows]))
+    ora = [r for r in rows if "it_warm_true_abar" in r]
     if ora:
-        rep["test"]["true_a0"] = {"pairs": len(ora),
-                                  "mean_it": float(np.mean([r["it_warm_true_a0"] for r in ora])),
-                                  "mean_E0_over_E": float(np.mean([r["E0_true_a0"] / r["E_plain"] for r in ora]))}
+        rep["test"]["true_abar"] = {"pairs": len(ora),
+                                  "mean_it": float(np.mean([r["it_warm_true_abar"] for r in ora])),
+                                  "mean_E0_over_E": float(np.mean([r["E0_true_abar"] / r["E_plain"] for r in ora]))}
     rep["rows"] = rows
     log(json.dumps(rep["test"]))
     if a.report:

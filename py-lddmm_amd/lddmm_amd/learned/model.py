@@ -69,8 +69,12 @@ class Segments:
 
 
 class MomentumNet:
-    def __init__(self, params=None, seed=0, feat_scale=1.0, out_scale=1.0, target='momentum'):
-        self.target = target          # 'momentum' (a0) or 'velocity' (v0 = K(x0) a0)
+    def __init__(self, params=None, seed=0, feat_scale=1.0, out_scale=1.0, target='abar', features='force'):
+        # target: 'abar' (time-mean of the control, used as a time-constant control),
+        # 'momentum' (a0 = at[0], shot with EPDiff) or 'velocity' (K(x0) a0)
+        self.target = target
+        self.features = features      # input channels: 'force', 'meanshift' or 'kforce'
+
         self.dev = B.device()
         self.lay, self.total = layout()
         p = init_params(seed) if params is None else np.asarray(params, dtype=np.float32)
@@ -203,11 +207,12 @@ class MomentumNet:
 
     def save(self, path):
         np.savez(path, params=self.params_numpy(), feat_scale=self.feat_scale, out_scale=self.out_scale,
-                 target=np.array(self.target))
+                 target=np.array(self.target), features=np.array(self.features))
 
     @classmethod
     def load(cls, path):
         d = np.load(path)
         target = str(d["target"]) if "target" in d.files else 'momentum'
+        features = str(d["features"]) if "features" in d.files else 'force'
         return cls(params=d["params"], feat_scale=float(d["feat_scale"]), out_scale=float(d["out_scale"]),
-                   target=target)
+                   target=target, features=features)

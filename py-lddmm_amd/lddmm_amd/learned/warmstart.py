@@ -37,9 +37,28 @@ def make_matching(seed, cfg=DEFAULT, at=None):
     return SurfaceMatching(Template=Surface((f, v)), Target=Surface((f, v2)), options=opts)
 
 
-def features(sm):
-    """Initial data-term force at the template vertices, [N, 3] float64 device tensor."""
-    return -sm.endPointGradient_device(sm.x0_dev)
+FEATURES = ('force', 'meanshift', 'kforce')
+TARGETS = ('abar', 'momentum', 'velocity')
+MEANSHIFT_SIGMA = 1.5
+
+
+def features(sm, kind='force'):
+    """Per-vertex input channels [N, 3] (float64 device tensor), no correspondence assumed:
+    force     - initial data-term force -grad D(x0) / sigma_err^2;
+    meanshift - Gaussian-weighted mean of the target's vertices around each
+                template vertex minus the vertex (width MEANSHIFT_SIGMA);
+    kforce    - the force smoothed by the deformation kernel, K(x0) force."""
+    if kind == 'force':
+        return -sm.endPointGradient_device(sm.x0_dev)
+    if kind == 'kforce':
+        return sm.options['KparDiff'].applyK(sm.x0_dev, -sm.endPointGradient_device(sm.x0_dev))
+    if kind == 'meanshift':
+        from .. import kernelFunctions_util as ku
+        y = B.to_dev(sm.fv1.vertices)
+        w = torch.cat([y, torch.ones_like(y[:, :1])], 1)
+        s = ku.applyK(sm.x0_dev, y, w, 'gauss', [MEANSHIFT_SIGMA], 10)
+        return s[:, :3] / s[:, 3:4].clamp_min(1e-300) - sm.x0_dev
+    raise ValueError(f"unknown feature kind {kind!r}")
 
 
 def velocity(sm, a0):
@@ -47,31 +66,41 @@ def velocity(sm, a0):
     return sm.options['KparDiff'].applyK(sm.x0_dev, B.to_dev(a0))
 
 
+def targets_of(sm, at):
+    """Training targets of a finished plain registration with control `at` [T, N, 3]."""
+    a0 = at[0]
+    return {"a0": a0.cpu().numpy(), "abar": at.mean(0).cpu().numpy(), "v0": velocity(sm, a0).cpu().numpy()}
+
+
 def plain_registration(seed, cfg=DEFAULT):
-    """Zero-momentum plain registration: returns dict with features, a0 target,
-    final data term, iterations, seconds."""
+    """Zero-momentum plain registration: every feature kind, the targets
+    (abar, a0, v0), final data term, iterations, seconds."""
     sm = make_matching(seed, cfg)
-    feats = features(sm)
+    feats = {f"feat_{k}": features(sm, k).cpu().numpy() for k in FEATURES}
     t0 = time.perf_counter()
     sm.optimizeMatching()
     torch.cuda.synchronize()
-    return {"seed": seed, "features": feats.cpu().numpy(), "a0": sm.control['at'][0].cpu().numpy(),
-            "data": sm.varifold_data(), "iterations": int(sm.bfgs_iterations),
-            "seconds": time.perf_counter() - t0, "history": list(sm.bfgs_history)}
+    out = {"seed": seed, "data": sm.varifold_data(), "iterations": int(sm.bfgs_iterations),
+           "seconds": time.perf_counter() - t0, "history": list(sm.bfgs_history)}
+    out.update(feats)
+    out.update(targets_of(sm, sm.control['at']))
+    return out
 
 
-def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50, target='momentum'):
-    """Fit MomentumNet on (features -> a0, or -> v0 = K(x0) a0) pairs with the package's HIP kernels."""
+def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50, target='abar', features='force'):
+    """Fit MomentumNet on (features -> target) pairs with the package's HIP kernels.
+    Each sample holds every feature kind under "feat_<kind>" and the targets
+    "abar", "a0" and "v0"."""
     rng = np.random.default_rng(seed)
-    key = "a0" if target == 'momentum' else "v0"
-    F = np.concatenate([s["features"] for s in samples])
+    key = {"momentum": "a0", "velocity": "v0", "abar": "abar"}[target]
+    fkey = "feat_" + features
+    F = np.concatenate([s[fkey] for s in samples])
     A = np.concatenate([s[key] for s in samples])
     feat_scale = float(np.sqrt((F ** 2).mean())) or 1.0
     out_scale = float(np.sqrt((A ** 2).mean())) or 1.0
-    net = MomentumNet(seed=seed, feat_scale=feat_scale, out_scale=out_scale)
-    net.target = target
+    net = MomentumNet(seed=seed, feat_scale=feat_scale, out_scale=out_scale, target=target, features=features)
     dev = net.dev
-    Xs = [torch.tensor(s["features"] / feat_scale, dtype=torch.float32, device=dev) for s in samples]
+    Xs = [torch.tensor(s[fkey] / feat_scale, dtype=torch.float32, device=dev) for s in samples]
     Ys = [torch.tensor(s[key] / out_scale, dtype=torch.float32, device=dev) for s in samples]
     losses = []
     for ep in range(epochs):
@@ -92,18 +121,22 @@ def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50, target='m
 
 
 def warm_control(sm, net, cg_iters=20):
-    """EPDiff control from the predicted initial momentum.
+    """Initial control [T, N, 3] from the network's prediction.
 
-    A net trained on velocities (net.target == 'velocity') predicts v0 = K(x0) a0,
-    which is local in the data force; a0 then comes from a conjugate-gradient
-    solve of K(x0) a0 = v0 (Kernel.solve), truncated at cg_iters products."""
-    feats = features(sm)
+    target 'abar': the prediction is the time-mean of the plain optimiser's
+    control, used as a time-constant control; 'momentum': a0, shot with
+    EPDiff; 'velocity': K(x0) a0, turned into a0 by a conjugate-gradient solve
+    (Kernel.solve, truncated at cg_iters products) and shot with EPDiff."""
+    feats = features(sm, getattr(net, 'features', 'force'))
     pred = net.predict(feats, [feats.shape[0]])
-    if getattr(net, 'target', 'momentum') == 'velocity':
+    T = sm.Tsize
+    target = getattr(net, 'target', 'momentum')
+    if target == 'abar':
+        return pred[None].expand(T, -1, -1).contiguous(), pred
+    if target == 'velocity':
         a0 = sm.options['KparDiff'].solve(sm.x0_dev, pred, iterMax=cg_iters)
     else:
         a0 = pred
-    T = sm.Tsize
     _, at = evol.landmarkEPDiff(T, sm.x0_dev, a0, sm.options['KparDiff'])
     return at, a0
 

@@ -140,3 +140,27 @@ def curve_matching_pair(seed=0, n=400, dim=2, radius=5.0):
     v, f = closed_curve(n, dim, radius, seed=seed)
     v2, _ = closed_curve(n, dim, radius, seed=seed + 1)
     return (v, f), (v2, f)
+
+
+def population_pairs(npairs=256, seed=0, nmin=500, nmax=20000):
+    """Seeded template-target pairs of the batched population workload.
+
+    Per pair: a vertex count N log-uniform in [nmin, nmax]; the template is a
+    Fibonacci-lattice sphere hull (radius U[5, 20]) or an ellipsoid hull with
+    axes U[5, 20]^3 (one in two each); the target is the template with
+    U{10..50} Gaussian bumps of width 0.3 R and amplitude N(0, (0.1 R)^2)
+    along the normals, R the template's mean radius (the plain workload's
+    width 3 / amplitude 1 at R = 10).  Returns [((v, f), (v2, f)), ...]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(npairs):
+        n = int(round(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))))
+        if rng.random() < 0.5:
+            v, f = sphere_hull(n, radius=rng.uniform(5.0, 20.0), fibonacci=True)
+        else:
+            v, f = ellipsoid(n, rng.uniform(5.0, 20.0, size=3))
+        R = np.linalg.norm(v - v.mean(0), axis=1).mean()
+        nb = int(rng.integers(10, 51))
+        v2 = bump_deform(v, f, nb, 0.3 * R, 0.1 * R, int(rng.integers(1 << 30)))
+        out.append(((v, f), (v2, f)))
+    return out

@@ -104,6 +104,24 @@ def test_curve_adjacency_and_validation(lib):
     assert b"weight" in B.LIB.lddmm_last_error()
 
 
+def test_population_create_validation(lib):
+    """Descriptor checks run before any device call."""
+    B = lib
+    h = ctypes.c_void_p()
+    d = B.LddmmPopulationDesc()
+    d.npairs = 0
+    assert B.LIB.lddmm_population_create(ctypes.byref(d), ctypes.byref(h)) == B.EINVAL
+    assert b"npairs" in B.LIB.lddmm_last_error()
+    voff = np.array([0, 5, 3], dtype=np.int64)
+    foff = np.array([0, 6, 8], dtype=np.int64)
+    d.npairs = 2
+    d.voff, d.foff, d.tfoff = voff.ctypes.data, foff.ctypes.data, foff.ctypes.data
+    assert B.LIB.lddmm_population_create(ctypes.byref(d), ctypes.byref(h)) == B.EINVAL
+    assert b"non-decreasing" in B.LIB.lddmm_last_error()
+    assert B.LIB.lddmm_population_create(ctypes.byref(d), None) == B.EINVAL
+    assert B.LIB.lddmm_population_destroy(None) == B.OK
+
+
 def test_kernel_desc_mirrors_kernelspec(lib):
     kd = lib.kernel_desc([2.5, 5.0], kp=-1, prec='float32')
     assert kd.nscale == 2 and kd.sigma[1] == 5.0 and kd.prec == lib.PREC_F32C
@@ -111,3 +129,32 @@ def test_kernel_desc_mirrors_kernelspec(lib):
         lib.kernel_desc([1.0], prec='float16')
     with pytest.raises(NotImplementedError):
         lib.kernel_kind('laplacian')
+
+
+def test_population_schedule_covers_every_tile_chunk_within_bounds(lib):
+    """Host-only: for random populations, active subsets and launch shapes the
+    segment table covers every (tile, chunk) of every active pair exactly once
+    and every item stays inside its pair's rows and sources (checked by
+    lddmm_population_plan_check against the bounds the kernels index)."""
+    B = lib
+    rng = np.random.default_rng(5)
+    for trial in range(40):
+        P = int(rng.integers(1, 40))
+        nv = rng.integers(0, 3000, size=P)
+        nv[rng.random(P) < 0.1] = 0
+        nf = np.where(nv > 0, 2 * nv - 4, 0).clip(min=0)
+        ntf = np.where(rng.random(P) < 0.1, 0, rng.integers(1, 6000, size=P))
+        voff = np.concatenate([[0], np.cumsum(nv)]).astype(np.int64)
+        foff = np.concatenate([[0], np.cumsum(nf)]).astype(np.int64)
+        tfoff = np.concatenate([[0], np.cumsum(ntf)]).astype(np.int64)
+        act = (rng.random(P) < 0.7).astype(np.uint8)
+        for fam in (0, 1):
+            for R in (1, 2):
+                for f10 in (10, 20, 40):
+                    G = int(rng.choice([1, 7, 256, 1024]))
+                    n = ctypes.c_int64()
+                    rc = B.LIB.lddmm_population_plan_check(P, voff.ctypes.data, foff.ctypes.data,
+                                                           tfoff.ctypes.data, act.ctypes.data, fam, R, f10,
+                                                           G, ctypes.byref(n))
+                    assert rc == B.OK, (trial, fam, R, f10, G, B.LIB.lddmm_last_error())
+                    assert n.value >= 0
