@@ -116,15 +116,27 @@ def main(argv=None):
             row.update({f"it_warm_{target}": r["iterations"], f"sec_warm_{target}": r["seconds"],
                         f"E_warm_{target}": r["data"], f"E0_warm_{target}": r["data0"],
                         f"reached_{target}": r["reached"], f"used_warm_{target}": r["used_warm"]})
-            tg = W.targets_of(sm, sm.control['at'])[{"momentum": "a0", "velocity": "v0", "abar": "abar"}[net.target]]
+            tg = W.targets_of(sm, sm.control['at'])[W.TARGET_KEY[net.target]]
             fx = W.features(W.make_matching(seed, cfg), net.features)
             pred = net.predict(fx, [fx.shape[0]]).double().cpu().numpy()
             row[f"pred_rel_err_{target}"] = float(np.linalg.norm(pred - tg) / np.linalg.norm(tg))
         if k < a.oracle_check:
-            # upper bound: warm start from the plain run's own time-mean control (perfect predictor)
+            # upper bounds: the plain run's own time-mean control (perfect predictor),
+            # and its velocity K abar turned back into abar by truncated CG, exact
+            # and with 10% / 30% relative white noise added to the velocity
             abar = sm.control['at'].mean(0)
-            r = W.register(seed, at=abar[None].expand(sm.Tsize, -1, -1).contiguous(), target_data=E, cfg=cfg)
+            T_ = sm.Tsize
+            r = W.register(seed, at=abar[None].expand(T_, -1, -1).contiguous(), target_data=E, cfg=cfg)
             row.update({"it_warm_true_abar": r["iterations"], "E0_true_abar": r["data0"]})
+            vb = W.velocity(sm, abar)
+            gen = torch.Generator(device=vb.device).manual_seed(seed)
+            for lvl in (0.0, 0.1, 0.3):
+                noise = torch.randn(vb.shape, generator=gen, device=vb.device, dtype=vb.dtype)
+                vn = vb + lvl * vb.norm() / noise.norm() * noise
+                for cg in (20, 100):
+                    ah = sm.options['KparDiff'].solve(sm.x0_dev, vn, iterMax=cg)
+                    r = W.register(seed, at=ah[None].expand(T_, -1, -1).contiguous(), target_data=E, cfg=cfg)
+                    row[f"it_vbar_noise{lvl}_cg{cg}"] = r["iterations"]
         rows.append(row)
         if k % 10 == 0 or k < a.oracle_check:
             log(f"test {k}: plain {it_plain} it ({sec_plain:.2f} s) -> warm "
@@ -144,6 +156,9 @@ def main(argv=None):
         }
         rep["test"][target]["mean_pred_rel_err"] = float(np.mean([r[f"pred_rel_err_{target}"] for r in rows]))
     ora = [r for r in rows if "it_warm_true_abar" in r]
+    if ora:
+        rep["test"]["vbar_bounds"] = {k: float(np.mean([r[k] for r in ora])) for k in ora[0]
+                                      if k.startswith("it_vbar_")}
     if ora:
         rep["test"]["true_abar"] = {"pairs": len(ora),
                                   "mean_it": float(np.mean([r["it_warm_true_abar"] for r in ora])),

@@ -38,7 +38,8 @@ def make_matching(seed, cfg=DEFAULT, at=None):
 
 
 FEATURES = ('force', 'meanshift', 'kforce')
-TARGETS = ('abar', 'momentum', 'velocity')
+TARGETS = ('abar', 'vbar', 'momentum', 'velocity')
+TARGET_KEY = {"abar": "abar", "vbar": "vbar", "momentum": "a0", "velocity": "v0"}
 MEANSHIFT_SIGMA = 1.5
 
 
@@ -69,7 +70,9 @@ def velocity(sm, a0):
 def targets_of(sm, at):
     """Training targets of a finished plain registration with control `at` [T, N, 3]."""
     a0 = at[0]
-    return {"a0": a0.cpu().numpy(), "abar": at.mean(0).cpu().numpy(), "v0": velocity(sm, a0).cpu().numpy()}
+    abar = at.mean(0)
+    return {"a0": a0.cpu().numpy(), "abar": abar.cpu().numpy(), "v0": velocity(sm, a0).cpu().numpy(),
+            "vbar": velocity(sm, abar).cpu().numpy()}
 
 
 def plain_registration(seed, cfg=DEFAULT):
@@ -92,7 +95,7 @@ def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50, target='a
     Each sample holds every feature kind under "feat_<kind>" and the targets
     "abar", "a0" and "v0"."""
     rng = np.random.default_rng(seed)
-    key = {"momentum": "a0", "velocity": "v0", "abar": "abar"}[target]
+    key = TARGET_KEY[target]
     fkey = "feat_" + features
     F = np.concatenate([s[fkey] for s in samples])
     A = np.concatenate([s[key] for s in samples])
@@ -124,15 +127,20 @@ def warm_control(sm, net, cg_iters=20):
     """Initial control [T, N, 3] from the network's prediction.
 
     target 'abar': the prediction is the time-mean of the plain optimiser's
-    control, used as a time-constant control; 'momentum': a0, shot with
-    EPDiff; 'velocity': K(x0) a0, turned into a0 by a conjugate-gradient solve
-    (Kernel.solve, truncated at cg_iters products) and shot with EPDiff."""
+    control, used as a time-constant control; 'vbar': the prediction is
+    K(x0) abar, turned into abar by a conjugate-gradient solve (Kernel.solve,
+    truncated at cg_iters products) and used as a time-constant control;
+    'momentum': a0, shot with EPDiff; 'velocity': K(x0) a0, solved for a0 the
+    same way and shot with EPDiff."""
     feats = features(sm, getattr(net, 'features', 'force'))
     pred = net.predict(feats, [feats.shape[0]])
     T = sm.Tsize
     target = getattr(net, 'target', 'momentum')
     if target == 'abar':
         return pred[None].expand(T, -1, -1).contiguous(), pred
+    if target == 'vbar':
+        a = sm.options['KparDiff'].solve(sm.x0_dev, pred, iterMax=cg_iters)
+        return a[None].expand(T, -1, -1).contiguous(), a
     if target == 'velocity':
         a0 = sm.options['KparDiff'].solve(sm.x0_dev, pred, iterMax=cg_iters)
     else:
