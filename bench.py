@@ -123,6 +123,64 @@ def registration_cfg1(torch):
     return out
 
 
+def precision_cfg3(torch):
+    """Large single registration with the precision form: icosphere subdivision 6
+    (40,962 vertices, 81,920 faces, R = 50), target with 100 bumps (width 8,
+    amplitude N(0, 2^2)), sigma_def 10, varifold sigma 2, T = 20.  The control is
+    the one after 5 FP64 BFGS iterations; at that control the FP32 compensated
+    objective and gradient are compared with FP64, and one objective + gradient
+    evaluation is timed in each precision."""
+    import logging
+    import numpy as np
+    from lddmm_amd import synth
+    from lddmm_amd.surfaces import Surface
+    from lddmm_amd.kernelFunctions import Kernel
+    from lddmm_amd.surfaceMatching import SurfaceMatching
+    logging.getLogger().setLevel(logging.WARNING)
+    v, f = synth.icosphere(6, 50.0)
+    v2 = synth.bump_deform(v, f, 100, 8.0, 2.0, seed=11)
+
+    def make(pr, iters):
+        o = {'KparDiff': Kernel(name='gauss', sigma=10.0), 'KparDist': Kernel(name='gauss', sigma=2.0),
+             'errorType': 'varifold', 'timeStep': 0.05, 'pk_dtype': pr, 'maxIter': iters, 'mode': 'quiet'}
+        return SurfaceMatching(Template=Surface((f, v)), Target=Surface((f, v2)), options=o)
+
+    sm64 = make('float64', 5)
+    t0 = time.perf_counter()
+    sm64.optimizeMatching()
+    torch.cuda.synchronize()
+    t_reg5 = time.perf_counter() - t0
+    at = sm64.control['at'].clone()
+    res = {"vertices": int(v.shape[0]), "faces": int(f.shape[0]), "T": 20,
+           "fp64_5_iterations_s": t_reg5}
+    vals = {}
+    for pr in ("float64", "float32"):
+        sm = make(pr, 1)
+        sm.setInitialControl(at)
+        sm.obj = None
+        sm.objectiveFun()
+        sm.getGradient()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        sm.obj = None
+        obj = sm.objectiveFun()
+        g = sm.getGradient()['at']
+        torch.cuda.synchronize()
+        res[f"{pr}_objective_plus_gradient_s"] = time.perf_counter() - t
+        vals[pr] = (obj, g, sm.varifold_data())
+    o64, g64, d64 = vals["float64"]
+    o32, g32, d32 = vals["float32"]
+    res["objective_fp64"] = o64
+    res["objective_rel_err_fp32"] = abs(o32 - o64) / abs(o64)
+    res["data_term_rel_err_fp32"] = abs(d32 - d64) / abs(d64)
+    res["gradient_rel_err_fp32"] = float((g32 - g64).norm() / g64.norm())
+    log(f"precision cfg3: objective rel err {res['objective_rel_err_fp32']:.2e}, gradient rel err "
+        f"{res['gradient_rel_err_fp32']:.2e}; objective+gradient FP64 "
+        f"{res['float64_objective_plus_gradient_s'] * 1e3:.0f} ms, FP32 "
+        f"{res['float32_objective_plus_gradient_s'] * 1e3:.0f} ms")
+    return res
+
+
 def population_cfg4(torch, npairs=256, T=10):
     """Batched population workload: 256 template-target pairs, vertex counts
     log-uniform in [500, 20000] (Fibonacci sphere / ellipsoid hulls with bumps),
@@ -210,6 +268,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference pass")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the varifold and registration measurements reported under `extra`")
+    ap.add_argument("--probe", action="store_true",
+                    help="only launch the dominant kernel (FP32 applyDiffKT plain, sigma 2.5) 3 times and a "
+                         "256 MiB device copy 3 times, for rocprofv3 counter runs")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -253,6 +314,17 @@ def main():
         for s in SIGMAS:
             applyK(s, pr, outK)
             applyDiffKT(s, pr, outD)
+
+    if args.probe:
+        for _ in range(3):
+            applyDiffKT(SIGMAS[0], "float32", outD)
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)     # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("probe done: 3 applyDiffKT launches (N=M=%d), 3 copies of 256 MiB" % N)
+        return
 
     log(f"warmup {args.warmup} steps, N=M={N}")
     for _ in range(args.warmup):
@@ -372,6 +444,7 @@ def main():
         extra["varifold_100k"] = varifold_microbench(B, torch, synth, np)
         extra["registration_plain_cfg1"] = registration_cfg1(torch)
         extra["population_cfg4"] = population_cfg4(torch)
+        extra["precision_cfg3"] = precision_cfg3(torch)
 
     if not args.no_cpu:
         log("cpu baseline ...")

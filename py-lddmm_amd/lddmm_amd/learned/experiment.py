@@ -89,8 +89,8 @@ def main(argv=None):
     for variant in a.variants.split(","):
         feat_kind, target = variant.split(":")
         t1 = time.perf_counter()
-        net, losses = W.train(samples, epochs=a.epochs, batch=a.batch, lr=a.lr, log_every=0, target=target,
-                              features=feat_kind)
+        net, losses = W.train(samples, epochs=a.epochs, batch=a.batch, lr=a.lr, log_every=10, target=target,
+                              features=feat_kind, progress=lambda m: log(f"[{variant}] {m}"))
         target = variant
         torch.cuda.synchronize()
         rep[f"train_seconds_{target}"] = time.perf_counter() - t1

@@ -90,7 +90,8 @@ def plain_registration(seed, cfg=DEFAULT):
     return out
 
 
-def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50, target='abar', features='force'):
+def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50, target='abar', features='force',
+          progress=None):
     """Fit MomentumNet on (features -> target) pairs with the package's HIP kernels.
     Each sample holds every feature kind under "feat_<kind>" and the targets
     "abar", "a0" and "v0"."""
@@ -119,7 +120,10 @@ def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50, target='a
             tot += float(loss) * len(idx)
         losses.append(tot / len(order))
         if log_every and (ep % log_every == 0 or ep == epochs - 1):
-            logging.info(f"warm-start training epoch {ep}: loss {losses[-1]:.5f}")
+            msg = f"warm-start training epoch {ep}: loss {losses[-1]:.5f}"
+            logging.info(msg)
+            if progress is not None:
+                progress(msg)
     return net, losses
 
 

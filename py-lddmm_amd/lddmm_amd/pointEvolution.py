@@ -131,6 +131,28 @@ def landmarkEPDiff(T, x0, a0, KparDiff, affine=None, withJacobian=False, withNor
     return _ret(xt, tt), _ret(at[:T], tt)
 
 
+def geodesicHamiltonian(xt, at, KparDiff):
+    """H_t = 1/2 a_t . K(x_t) a_t along a shooting trajectory, in float64.
+
+    The geodesic (EPDiff) flow conserves H up to the O(dt) drift of the Euler
+    scheme; the check runs the kernel sums in float64 (and the reductions on
+    the device in float64) whatever KparDiff.pk_dtype the trajectory was shot
+    with.  xt, at: [T(+1), N, d] arrays (the first min(len) steps are used).
+    Returns (H [T] numpy, max_t |H_t - H_0| / |H_0|).  No reference counterpart
+    (the reference shoots without checking H)."""
+    from .kernelFunctions_util import applyK
+    xt_, at_ = B.to_dev(xt), B.to_dev(at)
+    T = min(xt_.shape[0], at_.shape[0])
+    H = torch.empty(T, dtype=torch.float64, device=xt_.device)
+    for t in range(T):
+        Ka = applyK(xt_[t], xt_[t], at_[t], KparDiff.name, KparDiff.sigma, KparDiff.order, dtype='float64',
+                    KP=KparDiff.ms_exponent)
+        H[t] = 0.5 * (at_[t] * Ka).sum()
+    h = H.cpu().numpy()
+    drift = float(np.max(np.abs(h - h[0])) / max(abs(h[0]), 1e-300)) if T else 0.0
+    return h, drift
+
+
 def secondOrderEvolution(x0, a0, KparDiff, timeStep, withSpline=None, affine=None, options=None):
     """base/pointEvolution.py:385-472 (no spline, no affine)."""
     if withSpline is not None:

@@ -144,3 +144,24 @@ def test_epdiff_forward_adjoint(mods, dtype, n, T, sig):
     H = pe.secondOrderHamiltonian(x0, a0, px1, pa1, K)
     Hr = O.secondOrderHamiltonian(x0, a0, px1, pa1, sig)
     assert abs(H - Hr) / abs(Hr) < (1e-11 if dtype == 'float64' else 1e-5)
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_geodesic_hamiltonian_conservation(mods, dtype):
+    """H = 1/2 a.K(x)a along EPDiff, evaluated in float64 on the device: equals
+    the oracle's H of the same trajectory, and its drift shrinks like dt."""
+    sd, pe, synth, Surface, Kernel = mods
+    rng = np.random.default_rng(17)
+    x0 = rng.normal(size=(500, 3)) * 3
+    a0 = rng.normal(size=(500, 3)) * 0.3
+    K = Kernel(name='gauss', sigma=1.5)
+    K.pk_dtype = dtype
+    drifts = []
+    for T in (10, 40):
+        xt, at = pe.landmarkEPDiff(T, x0, a0, K)
+        H, drift = pe.geodesicHamiltonian(xt, at, K)
+        Hr = np.array([0.5 * (at[t] * O.applyK(xt[t], xt[t], at[t], [1.5])).sum() for t in range(T)])
+        assert np.max(np.abs(H - Hr) / np.abs(Hr)) < 1e-12
+        drifts.append(drift)
+    assert drifts[1] < 0.5 * drifts[0]
+    assert drifts[0] < 0.05
