@@ -409,6 +409,94 @@ def currentMagnitudeGradient(fvDef, sigma, KP=-1):
                                             dz1 - np.cross(x2 - x1, wz1)))
 
 
+# Measures ------------------------------------------------------------------
+
+def _measure_features(S, fw):
+    a = np.sqrt((S ** 2).sum(axis=1) + 1e-10)
+    return a, S / a[:, None], a * fw
+
+
+def measureNorm0(fv1, sigma, KP=-1):
+    """base/surfaceDistances.py:208-211."""
+    _, _, c2, S2, fw2 = _surf(fv1)
+    _, _, cr2 = _measure_features(S2, fw2)
+    return (cr2 * applyK(c2, c2, cr2[:, None], sigma, KP)[:, 0]).sum()
+
+
+def measureNormDef(fvDef, fv1, sigma, KP=-1):
+    """base/surfaceDistances.py:215-223."""
+    _, _, c1, S1, fw1 = _surf(fvDef)
+    _, _, c2, S2, fw2 = _surf(fv1)
+    cr1 = _measure_features(S1, fw1)[2][:, None]
+    cr2 = _measure_features(S2, fw2)[2][:, None]
+    return (cr1 * applyK(c1, c1, cr1, sigma, KP)).sum() - 2 * (cr1 * applyK(c1, c2, cr2, sigma, KP)).sum()
+
+
+def measureNormGradient(fvDef, fv1, sigma, KP=-1):
+    """base/surfaceDistances.py:231-278 (with_weights=False)."""
+    xDef, f, c1, S1, w1 = _surf(fvDef)
+    _, _, c2, S2, w2 = _surf(fv1)
+    a1, cr1, a1w = _measure_features(S1, w1)
+    _, _, a2w = _measure_features(S2, w2)
+    z1 = applyK(c1, c1, a1w[:, None], sigma, KP) - applyK(c1, c2, a2w[:, None], sigma, KP)
+    wz1 = w1[:, None] * z1 * cr1
+    dz1 = (2 / 3) * (applyDiffKT(c1, c1, a1w[:, None], a1w[:, None], sigma, KP=KP) -
+                     applyDiffKT(c1, c2, a1w[:, None], a2w[:, None], sigma, KP=KP))
+    x1, x2, x3 = xDef[f[:, 0]], xDef[f[:, 1]], xDef[f[:, 2]]
+    return _scatter_faces(xDef.shape[0], f, (dz1 - np.cross(x3 - x2, wz1),
+                                            dz1 - np.cross(x1 - x3, wz1),
+                                            dz1 - np.cross(x2 - x1, wz1)))
+
+
+def measureNormPS0(points, weights, sigma, KP=-1):
+    """Point-set target measure |mu_1|^2 (base/surfaceDistances.py:141-144)."""
+    p, w = _f64(points), _f64(weights)[:, None]
+    return (w * applyK(p, p, w, sigma, KP)).sum()
+
+
+def measureNormPSDef(fvDef, points, weights, sigma, KP=-1):
+    """Surface vs point set (base/surfaceDistances.py:148-155)."""
+    _, _, c1, S1, fw1 = _surf(fvDef)
+    cr1 = _measure_features(S1, fw1)[2][:, None]
+    p, w = _f64(points), _f64(weights)[:, None]
+    return (cr1 * applyK(c1, c1, cr1, sigma, KP)).sum() - 2 * (cr1 * applyK(c1, p, w, sigma, KP)).sum()
+
+
+def measureNormPSGradient(fvDef, points, weights, sigma, KP=-1):
+    """base/surfaceDistances.py:163-200: the surface side uses the unweighted
+    face measure a = |S| (the reference ignores face weights here)."""
+    xDef, f, c1, S1, _ = _surf(fvDef)
+    a1 = np.sqrt((S1 ** 2).sum(axis=1) + 1e-10)
+    cr1 = S1 / a1[:, None]
+    p, w = _f64(points), _f64(weights)
+    z1 = applyK(c1, c1, a1[:, None], sigma, KP) - applyK(c1, p, w[:, None], sigma, KP)
+    z1 = z1 * cr1
+    dz1 = (2 / 3) * (applyDiffKT(c1, c1, a1[:, None], a1[:, None], sigma, KP=KP) -
+                     applyDiffKT(c1, p, a1[:, None], w[:, None], sigma, KP=KP))
+    x1, x2, x3 = xDef[f[:, 0]], xDef[f[:, 1]], xDef[f[:, 2]]
+    return _scatter_faces(xDef.shape[0], f, (dz1 - np.cross(x3 - x2, z1),
+                                            dz1 - np.cross(x1 - x3, z1),
+                                            dz1 - np.cross(x2 - x1, z1)))
+
+
+def pointMeasureNorm0(x1, w1, sigma, KP=-1):
+    """Point sets (base/pointset_distances.py:14-16)."""
+    x1, w = _f64(x1), _f64(w1)[:, None]
+    return (w * applyK(x1, x1, w, sigma, KP)).sum()
+
+
+def pointMeasureNormDef(xDef, wDef, x1, w1, sigma, KP=-1):
+    """base/pointset_distances.py:20-25."""
+    xd, wd, x1, w = _f64(xDef), _f64(wDef)[:, None], _f64(x1), _f64(w1)[:, None]
+    return (wd * applyK(xd, xd, wd, sigma, KP)).sum() - 2 * (wd * applyK(xd, x1, w, sigma, KP)).sum()
+
+
+def pointMeasureNormGradient(xDef, wDef, x1, w1, sigma, KP=-1):
+    """base/pointset_distances.py:34-41."""
+    xd, wd, x1, w = _f64(xDef), _f64(wDef)[:, None], _f64(x1), _f64(w1)[:, None]
+    return 2 * (applyDiffKT(xd, xd, wd, wd, sigma, KP=KP) - applyDiffKT(xd, x1, wd, w, sigma, KP=KP))
+
+
 # Curves --------------------------------------------------------------------
 
 def curve_geometry(vertices, faces, weights=None):
@@ -556,6 +644,11 @@ def _data_term(kind, fvDef, fv1, sigma_dist):
         return varifoldNormDef(fvDef, fv1, sigma_dist), varifoldNorm0(fv1, sigma_dist)
     if kind == 'current':
         return currentNormDef(fvDef, fv1, sigma_dist), currentNorm0(fv1, sigma_dist)
+    if kind == 'measure':
+        return measureNormDef(fvDef, fv1, sigma_dist), measureNorm0(fv1, sigma_dist)
+    if kind == 'point_measure':             # fvDef = (x, w) (weights in place of faces), fv1 = (y, w1)
+        return (pointMeasureNormDef(fvDef[0], fvDef[1], fv1[0], fv1[1], sigma_dist),
+                pointMeasureNorm0(fv1[0], fv1[1], sigma_dist))
     if kind == 'curve_varifold':
         return curveVarifoldNormDef(fvDef, fv1, sigma_dist), curveVarifoldNorm0(fv1, sigma_dist)
     if kind == 'curve_current':
@@ -568,6 +661,10 @@ def _data_grad(kind, fvDef, fv1, sigma_dist):
         return varifoldNormGradient(fvDef, fv1, sigma_dist)
     if kind == 'current':
         return currentNormGradient(fvDef, fv1, sigma_dist)
+    if kind == 'measure':
+        return measureNormGradient(fvDef, fv1, sigma_dist)
+    if kind == 'point_measure':
+        return pointMeasureNormGradient(fvDef[0], fvDef[1], fv1[0], fv1[1], sigma_dist)
     if kind == 'curve_varifold':
         return curveVarifoldNormGradient(fvDef, fv1, sigma_dist)
     if kind == 'curve_current':
@@ -578,8 +675,10 @@ def _data_grad(kind, fvDef, fv1, sigma_dist):
 def matching_objective(x0, faces, at, target, sigma_diff, sigma_dist, kind='varifold',
                        regweight=1., sigma_error=1.):
     """Surface- or curve-matching objective at control `at` (plain, no affine, ds = 1).
-    kind: 'varifold' / 'current' (surfaces), 'curve_varifold' / 'curve_current'
-    (curves, base/curveMatching.py:272-313 has the same structure).
+    kind: 'varifold' / 'current' / 'measure' (surfaces), 'curve_varifold' /
+    'curve_current' (curves, base/curveMatching.py:272-313 has the same
+    structure), 'point_measure' (point sets, base/pointSetMatching.py; `faces`
+    then holds the template's point weights and `target` is (points, weights)).
 
     sum_t r dt a_t.K(x_t)a_t  +  (|def|^2 - 2<def, tgt> + |tgt|^2) / sigma_err^2.
     Follows base/surfaceMatching.py:468-565 (objectiveFunDef) and :638-675

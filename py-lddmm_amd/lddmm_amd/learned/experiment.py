@@ -62,8 +62,8 @@ def main(argv=None):
     ap.add_argument("--oracle-check", type=int, default=5,
                     help="held-out pairs also warm-started from the plain run's own a0")
     ap.add_argument("--variants", default="force:abar,meanshift:abar,kforce:abar",
-                    help="comma-separated feature:target pairs (features force/meanshift/kforce; "
-                         "targets abar/momentum/velocity)")
+                    help="comma-separated feature:target[:loss] (features force/meanshift/kforce; "
+                         "targets abar/vbar/momentum/velocity; loss mse (default) or kvel)")
     a = ap.parse_args(argv)
     logging.getLogger().setLevel(logging.WARNING)
     cfg = dict(W.DEFAULT)
@@ -87,17 +87,19 @@ def main(argv=None):
 
     nets = {}
     for variant in a.variants.split(","):
-        feat_kind, target = variant.split(":")
+        parts = variant.split(":")
+        feat_kind, target = parts[0], parts[1]
+        loss_kind = parts[2] if len(parts) > 2 else 'mse'
         t1 = time.perf_counter()
         net, losses = W.train(samples, epochs=a.epochs, batch=a.batch, lr=a.lr, log_every=10, target=target,
-                              features=feat_kind, progress=lambda m: log(f"[{variant}] {m}"))
+                              features=feat_kind, progress=lambda m: log(f"[{variant}] {m}"), loss=loss_kind)
         target = variant
         torch.cuda.synchronize()
         rep[f"train_seconds_{target}"] = time.perf_counter() - t1
         rep[f"loss_first_last_{target}"] = [losses[0], losses[-1]]
         log(f"[{target}] trained on {len(samples)} pairs in {rep[f'train_seconds_{target}']:.1f} s, "
             f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
-        net.save(a.out.replace(".npz", f"_{target}.npz"))
+        net.save(a.out.replace(".npz", "_" + variant.replace(":", "_") + ".npz"))
         nets[target] = net
 
     rows = []

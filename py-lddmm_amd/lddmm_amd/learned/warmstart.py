@@ -72,7 +72,7 @@ def targets_of(sm, at):
     a0 = at[0]
     abar = at.mean(0)
     return {"a0": a0.cpu().numpy(), "abar": abar.cpu().numpy(), "v0": velocity(sm, a0).cpu().numpy(),
-            "vbar": velocity(sm, abar).cpu().numpy()}
+            "vbar": velocity(sm, abar).cpu().numpy(), "x0": sm.x0_dev.cpu().numpy()}
 
 
 def plain_registration(seed, cfg=DEFAULT):
@@ -91,10 +91,17 @@ def plain_registration(seed, cfg=DEFAULT):
 
 
 def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50, target='abar', features='force',
-          progress=None):
+          progress=None, loss='mse', sigma_def=None):
     """Fit MomentumNet on (features -> target) pairs with the package's HIP kernels.
     Each sample holds every feature kind under "feat_<kind>" and the targets
-    "abar", "a0" and "v0"."""
+    "abar", "a0", "v0" and "vbar" (and the template vertices "x0" for loss='kvel').
+
+    loss 'mse': mean squared error of the output.  loss 'kvel' (momentum
+    targets): the error is measured after the deformation kernel,
+    |K(x0) (out - target)|^2, i.e. on the velocity field the momentum induces;
+    the momentum's high-frequency detail, which barely moves the shape, no
+    longer dominates the loss.  Its gradient 2 K(x0)^T K(x0) (out - target) takes
+    two kernel sums per pair (float32 compensated)."""
     rng = np.random.default_rng(seed)
     key = TARGET_KEY[target]
     fkey = "feat_" + features
@@ -107,6 +114,14 @@ def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50, target='a
     Xs = [torch.tensor(s[fkey] / feat_scale, dtype=torch.float32, device=dev) for s in samples]
     Ys = [torch.tensor(s[key] / out_scale, dtype=torch.float32, device=dev) for s in samples]
     losses = []
+    if loss == 'kvel':
+        from ..kernelFunctions import Kernel
+        Kk = Kernel(name='gauss', sigma=sigma_def if sigma_def is not None else DEFAULT['sigma_def'])
+        Kk.pk_dtype = 'float32'
+        X0 = [B.to_dev(s["x0"]) for s in samples]
+        # normalise by the targets' mean squared velocity
+        vn = np.mean([float((Kk.applyK(X0[i], Ys[i].double()) ** 2).mean()) for i in range(len(samples))])
+        kscale = 1.0 / max(vn, 1e-300)
     for ep in range(epochs):
         order = rng.permutation(len(samples))
         tot = 0.0
@@ -114,10 +129,25 @@ def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50, target='a
             idx = order[b0:b0 + batch]
             X = torch.cat([Xs[i] for i in idx])
             Y = torch.cat([Ys[i] for i in idx])
-            segs = Segments([Xs[i].shape[0] for i in idx], dev)
-            loss = net.loss_and_grad(X, segs, Y)
+            counts = [Xs[i].shape[0] for i in idx]
+            segs = Segments(counts, dev)
+            if loss == 'kvel':
+                out = net.forward(X, segs)
+                diff = (out - Y).double()
+                grads, lsum, o = [], 0.0, 0
+                ntot = diff.numel()
+                for i, c in zip(idx, counts):
+                    d = diff[o:o + c]
+                    u = Kk.applyK(X0[i], d)
+                    lsum = lsum + (u * u).sum()
+                    grads.append(Kk.applyK(X0[i], u))
+                    o += c
+                lval = lsum * kscale / ntot
+                net.backward((torch.cat(grads) * (2.0 * kscale / ntot)).float())
+            else:
+                lval = net.loss_and_grad(X, segs, Y)
             net.adam(lr=lr)
-            tot += float(loss) * len(idx)
+            tot += float(lval) * len(idx)
         losses.append(tot / len(order))
         if log_every and (ep % log_every == 0 or ep == epochs - 1):
             msg = f"warm-start training epoch {ep}: loss {losses[-1]:.5f}"

@@ -1,14 +1,16 @@
 """Surface data terms with the reference's signatures (py-lddmm
 base/surfaceDistances.py): varifoldNorm0 / varifoldNormDef / varifoldNorm /
-varifoldNormGradient (orientation-free varifold, f0(u) = 1 + u^2) and
+varifoldNormGradient (orientation-free varifold, f0(u) = 1 + u^2),
 currentNorm0 / currentNormDef / currentNorm / currentNormGradient /
-currentMagnitude / currentMagnitudeGradient.
+currentMagnitude / currentMagnitudeGradient, measureNorm0 / measureNormDef /
+measureNorm / measureNormGradient and the point-set-target forms
+measureNormPS0 / measureNormPSDef / measureNormPS / measureNormPSGradient.
 
 The varifold energy and gradient are single fused GPU passes
 (lddmm_surface_varifold: face features, one pair kernel over both surfaces,
 face -> vertex gather); the energy total is reduced in float64 on the GPU.
-The current terms compose applyK / applyDiffKT on face centres with the GPU
-gather.  Precision follows KparDist.pk_dtype, as in the reference's pykeops
+The current and measure terms compose applyK / applyDiffKT on face centres
+with the GPU gather.  Precision follows KparDist.pk_dtype, as in the reference's pykeops
 branch; `cpu` is accepted and ignored (single device path).
 """
 import ctypes
@@ -112,6 +114,111 @@ def _gather(dk, dz1, z1):
 def _ap(KparDist):
     return dict(name=KparDist.name, scale=KparDist.sigma, order=KparDist.order,
                 dtype=KparDist.pk_dtype, KP=KparDist.ms_exponent)
+
+
+# ------------------------------------------------------------------ measures
+def _measure_data(fv, vertices=None):
+    """(centers [F,3], unit surfel S/a [F,3], a [F] with a = sqrt(|S|^2 + 1e-10),
+    face weights [F], gather handle) on the device; S is the unweighted surfel."""
+    d, keep = fv.device(vertices=vertices)
+    dev = B.device()
+    c = torch.empty((d.nf, 3), dtype=torch.float64, device=dev)
+    S = torch.empty((d.nf, 3), dtype=torch.float64, device=dev)
+    w = torch.empty((d.nf,), dtype=torch.float64, device=dev)
+    B.check(B.LIB.lddmm_surface_features(ctypes.byref(d), None, None, B.ptr(w), B.stream_handle()),
+            "surface_features")
+    vw = d.vweights
+    d.vweights = None                   # unweighted surfel
+    B.check(B.LIB.lddmm_surface_features(ctypes.byref(d), B.ptr(c), B.ptr(S), None, B.stream_handle()),
+            "surface_features")
+    d.vweights = vw
+    a = torch.sqrt((S * S).sum(1) + 1e-10)
+    return c, S / a[:, None], a, w, (d, keep)
+
+
+def measureNorm0(fv1, KparDist):
+    """|mu_1|^2 with mu = sum_f a_f w_f delta_{c_f} (base/surfaceDistances.py:208-211)."""
+    c2, _, a2, w2, _ = _measure_data(fv1)
+    cr2 = (a2 * w2)[:, None]
+    return float((cr2 * ku.applyK(c2, c2, cr2, **_ap(KparDist))).sum().item())
+
+
+def measureNormDef_device(fvDef, fv1, KparDist, vertices=None):
+    c1, _, a1, w1, _ = _measure_data(fvDef, vertices)
+    c2, _, a2, w2, _ = _measure_data(fv1)
+    cr1, cr2 = (a1 * w1)[:, None], (a2 * w2)[:, None]
+    kw = _ap(KparDist)
+    return (cr1 * ku.applyK(c1, c1, cr1, **kw)).sum() - 2 * (cr1 * ku.applyK(c1, c2, cr2, **kw)).sum()
+
+
+def measureNormDef(fvDef, fv1, KparDist):
+    """|mu_def|^2 - 2 <mu_def, mu_1> (base/surfaceDistances.py:215-223)."""
+    return float(measureNormDef_device(fvDef, fv1, KparDist).item())
+
+
+def measureNorm(fvDef, fv1, KparDist):
+    """base/surfaceDistances.py:226-227"""
+    return measureNormDef(fvDef, fv1, KparDist) + measureNorm0(fv1, KparDist)
+
+
+def measureNormGradient_device(fvDef, fv1, KparDist, vertices=None):
+    """z1 = K(c1,c1) a1w - K(c1,c2) a2w, w z1 n gathered with the cross-product
+    chain rule, dz1 = (2/3)(DiffKT terms) (base/surfaceDistances.py:231-271)."""
+    c1, n1, a1, w1, dk = _measure_data(fvDef, vertices)
+    c2, _, a2, w2, _ = _measure_data(fv1)
+    a1w, a2w = (a1 * w1)[:, None], (a2 * w2)[:, None]
+    kw = _ap(KparDist)
+    z1 = ku.applyK(c1, c1, a1w, **kw) - ku.applyK(c1, c2, a2w, **kw)
+    wz1 = w1[:, None] * z1 * n1
+    dz1 = (2 / 3) * (ku.applyDiffKT(c1, c1, a1w, a1w, **kw) - ku.applyDiffKT(c1, c2, a1w, a2w, **kw))
+    return _gather(dk, dz1, wz1)
+
+
+def measureNormGradient(fvDef, fv1, KparDist, with_weights=False):
+    """base/surfaceDistances.py:231-278"""
+    if with_weights:
+        raise NotImplementedError("measure gradient wrt vertex weights is not built")
+    return measureNormGradient_device(fvDef, fv1, KparDist).cpu().numpy()
+
+
+def _ps(fv1):
+    """Points and weights of a point-set target (`points` or `vertices`, `weights`)."""
+    pts = getattr(fv1, 'points', None)
+    if pts is None:
+        pts = fv1.vertices
+    return B.to_dev(pts), B.to_dev(np.asarray(fv1.weights, dtype=np.float64))[:, None]
+
+
+def measureNormPS0(fv1, KparDist):
+    """Point-set target |mu_1|^2 (base/surfaceDistances.py:141-144)."""
+    p, w = _ps(fv1)
+    return float((w * ku.applyK(p, p, w, **_ap(KparDist))).sum().item())
+
+
+def measureNormPSDef(fvDef, fv1, KparDist):
+    """Surface vs point set (base/surfaceDistances.py:148-155)."""
+    c1, _, a1, w1, _ = _measure_data(fvDef)
+    p, w = _ps(fv1)
+    cr1 = (a1 * w1)[:, None]
+    kw = _ap(KparDist)
+    return float(((cr1 * ku.applyK(c1, c1, cr1, **kw)).sum() - 2 * (cr1 * ku.applyK(c1, p, w, **kw)).sum()).item())
+
+
+def measureNormPS(fvDef, fv1, KparDist):
+    """base/surfaceDistances.py:158-159"""
+    return measureNormPSDef(fvDef, fv1, KparDist) + measureNormPS0(fv1, KparDist)
+
+
+def measureNormPSGradient(fvDef, fv1, KparDist):
+    """base/surfaceDistances.py:163-200; the surface side uses the unweighted
+    face measure, as the reference does."""
+    c1, n1, a1, _, dk = _measure_data(fvDef)
+    p, w = _ps(fv1)
+    kw = _ap(KparDist)
+    a1c = a1[:, None]
+    z1 = (ku.applyK(c1, c1, a1c, **kw) - ku.applyK(c1, p, w, **kw)) * n1
+    dz1 = (2 / 3) * (ku.applyDiffKT(c1, c1, a1c, a1c, **kw) - ku.applyDiffKT(c1, p, a1c, w, **kw))
+    return _gather(dk, dz1, z1).cpu().numpy()
 
 
 def currentMagnitude(fv1, KparDist):

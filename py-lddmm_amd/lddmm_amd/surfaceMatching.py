@@ -2,7 +2,7 @@
 restating py-lddmm base/surfaceMatching.py + base/pointSetMatching.py +
 base/basicMatching.py for the configuration the hot path serves:
 `SurfaceMatching(Template, Target, options).optimizeMatching()` with
-errorType 'varifold' or 'current', algorithm 'bfgs' (weak-Wolfe line search),
+errorType 'varifold', 'current' or 'measure', algorithm 'bfgs' (weak-Wolfe line search),
 affine None, no internal cost, not symmetric, not unreduced.
 
 The control `at` [T, N, 3], the trajectories and the BFGS memory live on
@@ -101,8 +101,9 @@ class SurfaceMatching:
             raise NotImplementedError("internal costs are not built in lddmm_amd")
         if o['Landmarks'] is not None or o['passenger'] is not None:
             raise NotImplementedError("landmark / passenger tracking is not built in lddmm_amd")
-        if o['errorType'] not in ('varifold', 'current'):
-            raise NotImplementedError(f"errorType {o['errorType']!r}: lddmm_amd builds 'varifold' and 'current'")
+        if o['errorType'] not in ('varifold', 'current', 'measure'):
+            raise NotImplementedError(f"errorType {o['errorType']!r}: lddmm_amd builds 'varifold', 'current' "
+                                      "and 'measure'")
         if o['vfun'] is not None:
             raise NotImplementedError("custom varifold functions are not built")
 
@@ -196,6 +197,8 @@ class SurfaceMatching:
         K = self.options['KparDist']
         if self.options['errorType'] == 'varifold':
             return sd.varifoldNorm0(fv1, K)
+        if self.options['errorType'] == 'measure':
+            return sd.measureNorm0(fv1, K)
         return sd.currentNorm0(fv1, K)
 
     def dataTerm_device(self, xT):
@@ -204,6 +207,8 @@ class SurfaceMatching:
         if self.options['errorType'] == 'varifold':
             e, _ = sd.varifold_device(self._dist_kd(), self.fv0, self.fv1, energy=True, grad=False, v1=xT)
             e = e[0]
+        elif self.options['errorType'] == 'measure':
+            e = sd.measureNormDef_device(self.fv0, self.fv1, K, vertices=xT)
         else:
             c1, cr1, _, _ = sd._face_data(self.fv0, xT)
             c2, cr2, _, _ = sd._face_data(self.fv1)
@@ -219,6 +224,8 @@ class SurfaceMatching:
         K = self.options['KparDist']
         if self.options['errorType'] == 'varifold':
             _, g = sd.varifold_device(self._dist_kd(), self.fv0, self.fv1, energy=False, grad=True, v1=xT)
+        elif self.options['errorType'] == 'measure':
+            g = sd.measureNormGradient_device(self.fv0, self.fv1, K, vertices=xT)
         else:
             g = sd.currentNormGradient_device(self.fv0, self.fv1, K, vertices=xT)
         return g / self.options['sigmaError'] ** 2

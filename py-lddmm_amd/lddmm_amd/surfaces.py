@@ -182,8 +182,9 @@ class Surface:
         from . import _backend as B
         faces, ptr, idx = self.topology()
         if vertices is None:
-            if self._dev is None:
-                self._dev = (B.to_dev(self.vertices), B.to_dev(np.asarray(self.weights, dtype=np.float64)))
+            if self._dev is None or self._dev[0] is None:
+                w_ = self._dev[1] if self._dev is not None else B.to_dev(np.asarray(self.weights, dtype=np.float64))
+                self._dev = (B.to_dev(self.vertices), w_)
             v, w = self._dev
         else:
             v = B.to_dev(vertices)

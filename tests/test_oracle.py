@@ -181,6 +181,52 @@ def spheres():
     return (v, f), (v2, f)
 
 
+def test_measure_terms(spheres):
+    """Surface measure: |mu - mu| = 0, face-flip invariance, sigma -> inf limit
+    (total area)^2, FD gradient; the point-set form with the target's face
+    centres / areas equals the surface form; point-set measure FD gradient."""
+    A, Bs = spheres
+    v, f = A
+    v2, f2 = Bs
+    assert abs(O.measureNormDef(A, A, [0.8]) + O.measureNorm0(A, [0.8])) < 1e-9 * O.measureNorm0(A, [0.8])
+    flip = (v2, f2[:, [0, 2, 1]])
+    np.testing.assert_allclose(O.measureNormDef(A, flip, [0.8]), O.measureNormDef(A, Bs, [0.8]), rtol=1e-12)
+    area = np.linalg.norm(np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]]), axis=1).sum() / 2
+    np.testing.assert_allclose(O.measureNorm0(A, [1e6]), area ** 2, rtol=1e-8)
+    dv = rnd(*v.shape)
+    fd = _fd(lambda vv: O.measureNormDef((vv, f), Bs, [0.8]), v, dv)
+    np.testing.assert_allclose((O.measureNormGradient(A, Bs, [0.8]) * dv).sum(), fd, rtol=1e-6)
+    c2, S2, _ = O.surface_geometry(v2, f2)
+    a2 = np.sqrt((S2 ** 2).sum(1) + 1e-10)
+    np.testing.assert_allclose(O.measureNormPSDef(A, c2, a2, [0.8]), O.measureNormDef(A, Bs, [0.8]), rtol=1e-12)
+    np.testing.assert_allclose(O.measureNormPS0(c2, a2, [0.8]), O.measureNorm0(Bs, [0.8]), rtol=1e-12)
+    fd = _fd(lambda vv: O.measureNormPSDef((vv, f), c2, a2, [0.8]), v, dv)
+    np.testing.assert_allclose((O.measureNormPSGradient(A, c2, a2, [0.8]) * dv).sum(), fd, rtol=1e-6)
+    # weighted template: surface-form gradient stays the FD derivative
+    w = np.random.default_rng(2).uniform(0.5, 1.5, size=v.shape[0])
+    fd = _fd(lambda vv: O.measureNormDef((vv, f, w), Bs, [0.8]), v, dv)
+    np.testing.assert_allclose((O.measureNormGradient((v, f, w), Bs, [0.8]) * dv).sum(), fd, rtol=1e-6)
+    # point sets
+    x, y = rnd(30, 3), rnd(25, 3)
+    wx, wy = np.abs(rnd(30)) + 0.1, np.abs(rnd(25)) + 0.1
+    np.testing.assert_allclose(O.pointMeasureNorm0(x, wx, [1e6]), wx.sum() ** 2, rtol=1e-8)
+    dx = rnd(30, 3)
+    fd = _fd(lambda xx: O.pointMeasureNormDef(xx, wx, y, wy, [1.3]), x, dx)
+    np.testing.assert_allclose((O.pointMeasureNormGradient(x, wx, y, wy, [1.3]) * dx).sum(), fd, rtol=1e-6)
+
+
+def test_point_set_matching_gradient_fd():
+    """Point-set measure matching: Euclidean gradient = FD of the objective."""
+    x0 = rnd(40, 3) * 2
+    w0 = np.abs(rnd(40)) + 0.2
+    y, wy = rnd(35, 3) * 2 + 0.3, np.abs(rnd(35)) + 0.2
+    at = 0.05 * rnd(4, 40, 3)
+    d = rnd(*at.shape)
+    g = O.matching_gradient(x0, w0, at, (y, wy), [1.5], [0.9], kind='point_measure')
+    fd = _fd(lambda a_: O.matching_objective(x0, w0, a_, (y, wy), [1.5], [0.9], kind='point_measure')[0], at, d)
+    np.testing.assert_allclose((g * d).sum(), fd, rtol=1e-6)
+
+
 def test_icosphere_closed_forms():
     v, f = S.icosphere(3, radius=2.0)
     assert v.shape == (642, 3) and f.shape == (1280, 3)
