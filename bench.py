@@ -33,6 +33,22 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_EXP_PER_S = 256 * 4 * 16 / 2 * 2.4e9   # v_exp_f32 issues at half the FMA rate (profiles/gfx950_rates.md)
 
 
+def measured_traffic(n):
+    """HBM bytes per launch of the dominant kernel (applyDiffKT at N = M = 50k),
+    from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised, with their
+    commands, in profiles/k_pair_kernel_sums_50k.md; None for other sizes."""
+    if n != 50_000:
+        return None
+    try:
+        with open(os.path.join(ROOT, "profiles", "k_pair_kernel_sums_50k.md")) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return int(line.split("=")[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -434,7 +450,8 @@ def main():
             "engine": "FP32 VALU + v_exp_f32 (compute-bound; packed FP32 disabled)",
             "kernel": "k_pair<float, OpDiffKT<3,false,false>, 2> (applyDiffKT plain)",
             "achieved": achievedD, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "frac": achievedD / PEAK_FP32_TFLOPS, "traffic": None,
+            "frac": achievedD / PEAK_FP32_TFLOPS, "traffic": measured_traffic(N),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "flop_per_pair": FLOP_PER_PAIR["applyDiffKT"], "pairs_per_launch": N * M,
             "avg_launch_ms": avgD,
         },
