@@ -60,3 +60,31 @@ def test_adam_step(net_mod):
     assert rel(w1, expect) < 1e-5
     bf = net.wbf.cpu().numpy().view(np.uint16).astype(np.uint32) << 16
     np.testing.assert_array_equal(bf.view(np.float32), model.bf16_round(w1))
+
+
+def test_trained_network_warm_start_gain(net_mod):
+    """The committed network (mean-shift features -> time-mean momentum, trained
+    on 693 plain registrations) on held-out pairs of the plain workload: the
+    warm-started registration reaches the plain run's final varifold energy
+    (1e-4 relative) in no more iterations than the plain run, and saves
+    iterations on average.  Measured on 12 held-out pairs: 44.1 vs 49.9
+    iterations (1.13x), below the 3x target."""
+    model, torch = net_mod
+    import os
+    from lddmm_amd.learned import warmstart as W
+    from lddmm_amd.learned.experiment import plain_with_trace, iterations_to
+    path = os.path.join(os.path.dirname(W.__file__), "momentum_net.npz")
+    net = model.MomentumNet.load(path)
+    assert net.target == 'abar' and net.features == 'meanshift'
+    its_p, its_w = [], []
+    for seed in (900001, 900003):
+        sm, _, _ = plain_with_trace(seed, W.DEFAULT)
+        E = sm.varifold_data()
+        it_plain = iterations_to(sm.data_trace, E)
+        r = W.register(seed, net=net, target_data=E)
+        assert r["reached"]
+        assert r["used_warm"]
+        assert r["iterations"] <= it_plain
+        its_p.append(it_plain)
+        its_w.append(r["iterations"])
+    assert sum(its_w) < sum(its_p)
