@@ -831,12 +831,12 @@ int pop_forward_t(const lddmm_kernel* kd, lddmm_population* pop, const double* a
     auto enqueue = [&](hipStream_t s) -> int {
         hipLaunchKernelGGL(k_pop_zero, dim3(nblk(V)), dim3(256), 0, s, int(V), pop->d_vpair, pop->d_active, rowacc);
         LDDMM_CHECK_LAUNCH();
+        // one centre per pair (from x_0) for the whole flow
+        hipLaunchKernelGGL(k_pop_center, dim3(unsigned(pop->P)), dim3(256), 0, s, xt, pop->d_voff, pop->d_active, c0);
+        LDDMM_CHECK_LAUNCH();
         for (int t = 0; t < T_; ++t) {
             const double* x = xt + size_t(t) * stride;
             const double* a = at + size_t(t) * stride;
-            hipLaunchKernelGGL(k_pop_center, dim3(unsigned(pop->P)), dim3(256), 0, s, x, pop->d_voff, pop->d_active,
-                               c0);
-            LDDMM_CHECK_LAUNCH();
             typename OPK::template Args<T> args{x, x, a, nullptr, 3, 3, 0, sc};
             LDDMM_TRY((launch_pop<T, OPK>(args, L, c0, pp.phi, pp.plo, s)));
             hipLaunchKernelGGL((k_pop_shoot_epi<T>), dim3(nblk(V)), dim3(256), 0, s, pp.phi, pp.plo,
@@ -902,14 +902,13 @@ int pop_adjoint_t(const lddmm_kernel* kd, lddmm_population* pop, const double* a
         hipLaunchKernelGGL(k_pop_axpby, dim3(nblk(V)), dim3(256), 0, s, int(V), pop->d_vpair, pop->d_active, 1.0,
                            px1, 0.0, nullptr, pb[T_ & 1]);
         LDDMM_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_pop_center, dim3(unsigned(pop->P)), dim3(256), 0, s, xt, pop->d_voff, pop->d_active, c0);
+        LDDMM_CHECK_LAUNCH();
         for (int t = T_ - 1; t >= 0; --t) {
             const double* x = xt + size_t(t) * stride;
             const double* a = at + size_t(t) * stride;
             double* pn = pb[(t + 1) & 1];
             double* pc = pb[t & 1];
-            hipLaunchKernelGGL(k_pop_center, dim3(unsigned(pop->P)), dim3(256), 0, s, x, pop->d_voff, pop->d_active,
-                               c0);
-            LDDMM_CHECK_LAUNCH();
             typename OPD::template Args<T> args{x, x, pn, a, nullptr, 3, 3, 0, 2.0 * r, sc};
             LDDMM_TRY((launch_pop<T, OPD>(args, LD, c0, pp.phi, pp.plo, s)));
             // dat = 2 r a_t - p_{t+1}  (before p_t overwrites the other buffer)

@@ -105,10 +105,12 @@ int shoot_forward_t(const lddmm_kernel* kd, const double* at, int T_, int n, dou
     Partials<T> part = carve_partials<T>(base, cv, jk.plan, KW, n);
     const double dt = 1.0 / T_;
     const size_t stride = size_t(n) * KW;
+    // one centre for the whole flow: any central point serves the centred
+    // float32 coordinates, and the shape's centre moves little over one shoot
+    LDDMM_TRY(launch_center(xt, n, xt, 0, KW, c0, st));
     for (int t = 0; t < T_; ++t) {
         const double* x = xt + t * stride;
         const double* a = at + t * stride;
-        LDDMM_TRY(launch_center(x, n, x, 0, KW, c0, st));
         jk.args = {x, x, a, c0, KW, KW, 0, sc};
         LDDMM_TRY(run_job(jk, n, n, part, st));
         hipLaunchKernelGGL((k_shoot_epi<T, KW>), dim3(unsigned(nb)), dim3(256), 0, st, part, jk.plan.nchunk, n, x,
@@ -152,12 +154,12 @@ int shoot_adjoint_t(const lddmm_kernel* kd, const double* at, const double* xt, 
     auto P = [&](int t) -> double* { return pxt ? pxt + size_t(t) * stride : pbuf[t & 1]; };
     LDDMM_HIP(hipMemcpyAsync(P(T_), px1, stride * sizeof(double), hipMemcpyDeviceToDevice, st));
     const double dt = 1.0 / T_;
+    LDDMM_TRY(launch_center(xt, n, xt, 0, KW, c0, st));       // one centre (x_0) for all steps
     for (int t = T_ - 1; t >= 0; --t) {
         const double* x = xt + size_t(t) * stride;
         const double* a = at + size_t(t) * stride;
         double* pn = P(t + 1);
         double* pc = P(t);
-        LDDMM_TRY(launch_center(x, n, x, 0, KW, c0, st));
         Carve c2; c2.off = poff;
         Partials<T> partD = carve_partials<T>(base, c2, jd.plan, 3, n);
         jd.args = {x, x, pn, a, c0, KW, KW, 0, 2.0 * r, sc};
@@ -200,10 +202,10 @@ int epdiff_forward_t(const lddmm_kernel* kd, int T_, int n, double* xt, double* 
     size_t poff = cv.off;
     const size_t stride = size_t(n) * KW;
     const double dt = 1.0 / T_;
+    LDDMM_TRY(launch_center(xt, n, xt, 0, KW, c0, st));       // one centre (x_0) for all steps
     for (int k = 0; k < T_; ++k) {
         const double* x = xt + size_t(k) * stride;
         const double* a = at + size_t(k) * stride;
-        LDDMM_TRY(launch_center(x, n, x, 0, KW, c0, st));
         Carve c1; c1.off = poff;
         Partials<T> pk = carve_partials<T>(base, c1, jk.plan, KW, n);
         jk.args = {x, x, a, c0, KW, KW, 0, sc};
@@ -256,6 +258,7 @@ int epdiff_adjoint_t(const lddmm_kernel* kd, int T_, int n, const double* xt, co
     const double fdiff = MS ? -1.0 : ss.diff;
     const double fdd = MS ? 1.0 : ss.ddiff;
     const double fd1 = ms1 ? -1.0 : ss1.diff;
+    LDDMM_TRY(launch_center(xt, n, xt, 0, KW, c0, st));       // one centre (x_0) for all steps
     for (int t = T_ - 1; t >= 0; --t) {
         const double* x = xt + size_t(t) * stride;
         const double* a = at + size_t(t) * stride;
@@ -263,7 +266,6 @@ int epdiff_adjoint_t(const lddmm_kernel* kd, int T_, int n, const double* xt, co
         const double* pa = pat + size_t(t + 1) * stride;
         double* pxo = pxt + size_t(t) * stride;
         double* pao = pat + size_t(t) * stride;
-        LDDMM_TRY(launch_center(x, n, x, 0, KW, c0, st));
         Carve c; c.off = poff;
         Partials<T> part = carve_partials<T>(base, c, j1.plan, 3, n);
         // zpx = DiffKT_sym(x, px, a) - DDiffK11and12(x, a, a, pa)
@@ -289,7 +291,6 @@ int epdiff_adjoint_t(const lddmm_kernel* kd, int T_, int n, const double* xt, co
     }
     // da0 = K(x0) a0 - pa_0
     LDDMM_TRY(axpby(stride, -1.0, pat, 0.0, nullptr, da0, st));
-    LDDMM_TRY(launch_center(xt, n, xt, 0, KW, c0, st));
     Carve c; c.off = poff;
     Partials<T> part = carve_partials<T>(base, c, j3.plan, KW, n);
     j3.args = {xt, xt, at, c0, KW, KW, 0, sc};

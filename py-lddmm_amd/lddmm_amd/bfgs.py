@@ -4,7 +4,8 @@ Same protocol as the reference: the optimisable object provides
 getVariable, objectiveFun, updateTry, acceptVarTry, getGradient and
 optionally startOfIteration / endOfIteration / endOfProcedure,
 dotProduct_euclidean, addProd / prod / copyDir, gradEps, gradCoeff, epsMax,
-burnIn, reset.  Two-loop recursion with memory `memory`, t_init = 1/|q| on
+burnIn, reset, twoLoop (the two-loop recursion computed by the object, e.g.
+with its scalars kept on the device).  Two-loop recursion with memory `memory`, t_init = 1/|q| on
 the first (and reset) iterations, weak-Wolfe search (c1 = 1e-4, c2 = 0.9,
 at most 10 gradients), stop on |obj - obj_old| < 1e-7 after burn-in or on
 |g| < gradEps.  Directions may hold numpy arrays or device tensors; dot
@@ -116,23 +117,27 @@ def bfgs(opt, verb=True, maxIter=1000, TestGradient=False, epsInit=0.01, memory=
             storedGrad.append([diffVar, addProd(grd, grdOld, -1)])
             if len(storedGrad) > memory:
                 storedGrad.pop(0)
-            q = copyDir(grd)
-            rho = []
-            alpha = []
-            for m in reversed(storedGrad):
-                rho.append(1 / dotProduct(m[1], [m[0]])[0])
-                alpha.append(rho[-1] * dotProduct(m[0], [q])[0])
-                q = addProd(q, m[1], -alpha[-1])
-            rho.reverse()
-            alpha.reverse()
-            m = storedGrad[-1]
-            c = dotProduct(m[0], [m[1]])[0] / dotProduct(m[1], [m[1]])[0]
-            if c < 1e-10:
-                c = 1
-            q = prod(q, c)
-            for k, m in enumerate(storedGrad):
-                beta = rho[k] * dotProduct(m[1], [q])[0]
-                q = addProd(q, m[0], alpha[k] - beta)
+            if hasattr(opt, 'twoLoop'):
+                # same recursion with the scalars kept where the vectors live
+                q = opt.twoLoop(grd, storedGrad)
+            else:
+                q = copyDir(grd)
+                rho = []
+                alpha = []
+                for m in reversed(storedGrad):
+                    rho.append(1 / dotProduct(m[1], [m[0]])[0])
+                    alpha.append(rho[-1] * dotProduct(m[0], [q])[0])
+                    q = addProd(q, m[1], -alpha[-1])
+                rho.reverse()
+                alpha.reverse()
+                m = storedGrad[-1]
+                c = dotProduct(m[0], [m[1]])[0] / dotProduct(m[1], [m[1]])[0]
+                if c < 1e-10:
+                    c = 1
+                q = prod(q, c)
+                for k, m in enumerate(storedGrad):
+                    beta = rho[k] * dotProduct(m[1], [q])[0]
+                    q = addProd(q, m[0], alpha[k] - beta)
         else:
             storedGrad = []
             q = copyDir(grd)

@@ -312,6 +312,34 @@ class SurfaceMatching:
                     res[ll] += float((g1[k] * gr[k]).sum())
         return res
 
+    def twoLoop(self, grd, storedGrad):
+        """L-BFGS two-loop recursion of bfgs.bfgs (base/bfgs.py:188-207) with the
+        step scalars kept on the device as 0-d tensors: the same operations in
+        the same order, without a host synchronisation per dot product.
+        storedGrad: [[s_k, y_k], ...] oldest first (Control dicts)."""
+        def dot(a, b):
+            return torch.dot(a.reshape(-1), b.reshape(-1))
+        q = grd['at'].clone()
+        rho, alpha = [], []
+        for sk, yk in reversed(storedGrad):
+            r = 1.0 / dot(yk['at'], sk['at'])
+            a = r * dot(sk['at'], q)
+            q.addcmul_(a, yk['at'], value=-1.0)
+            rho.append(r)
+            alpha.append(a)
+        rho.reverse()
+        alpha.reverse()
+        sk, yk = storedGrad[-1]
+        c = dot(sk['at'], yk['at']) / dot(yk['at'], yk['at'])
+        c = torch.where(c < 1e-10, torch.ones_like(c), c)
+        q.mul_(c)
+        for k, (sk, yk) in enumerate(storedGrad):
+            beta = rho[k] * dot(yk['at'], q)
+            q.addcmul_(alpha[k] - beta, sk['at'])
+        out = Control()
+        out['at'] = q
+        return out
+
     def addProd(self, dir1, dir2, beta):
         dr = Control()
         for k in dir1.keys():
