@@ -310,6 +310,21 @@ int lddmm_nn_adam(float* w, uint16_t* wbf, const float* g, float* m, float* v, i
 int lddmm_nn_to_bf16(const float* w, uint16_t* wbf, int64_t count, void* stream);
 
 /* ------------------------------------------------------------------------
+ * L-BFGS direction on the device (the two-loop recursion of base/bfgs.py:188-207
+ * with the step scalars kept in device memory; no host synchronisation).
+ * lddmm_lbfgs_pair_scalars: for a stored pair (s, y) of n doubles, writes the
+ * device scalars rho = 1/<y,s> and c = <s,y>/<y,y> (1 if below 1e-10).
+ * lddmm_lbfgs_two_loop: q = H g for the m stored pairs (oldest first; host
+ * arrays of device pointers; rho[k] device scalars; c_newest = c of pair m-1);
+ * q may equal g.  Dot products are reduced in a fixed order (deterministic).
+ * ---------------------------------------------------------------------- */
+int lddmm_lbfgs_pair_scalars(const double* s, const double* y, int64_t n, double* rho, double* c,
+                             void* stream);
+int lddmm_lbfgs_two_loop(const double* g, int64_t n, int32_t m, const double* const* s,
+                         const double* const* y, const double* const* rho, const double* c_newest,
+                         double* q, void* stream);
+
+/* ------------------------------------------------------------------------
  * Population registration: P template-target surface pairs of uneven sizes
  * computed together.  No counterpart in the reference (it registers one pair
  * per SurfaceMatching); per pair, every result equals the single-pair entry

@@ -313,29 +313,30 @@ class SurfaceMatching:
         return res
 
     def twoLoop(self, grd, storedGrad):
-        """L-BFGS two-loop recursion of bfgs.bfgs (base/bfgs.py:188-207) with the
-        step scalars kept on the device as 0-d tensors: the same operations in
-        the same order, without a host synchronisation per dot product.
-        storedGrad: [[s_k, y_k], ...] oldest first (Control dicts)."""
-        def dot(a, b):
-            return torch.dot(a.reshape(-1), b.reshape(-1))
-        q = grd['at'].clone()
-        rho, alpha = [], []
-        for sk, yk in reversed(storedGrad):
-            r = 1.0 / dot(yk['at'], sk['at'])
-            a = r * dot(sk['at'], q)
-            q.addcmul_(a, yk['at'], value=-1.0)
-            rho.append(r)
-            alpha.append(a)
-        rho.reverse()
-        alpha.reverse()
-        sk, yk = storedGrad[-1]
-        c = dot(sk['at'], yk['at']) / dot(yk['at'], yk['at'])
-        c = torch.where(c < 1e-10, torch.ones_like(c), c)
-        q.mul_(c)
-        for k, (sk, yk) in enumerate(storedGrad):
-            beta = rho[k] * dot(yk['at'], q)
-            q.addcmul_(alpha[k] - beta, sk['at'])
+        """L-BFGS two-loop recursion of bfgs.bfgs (base/bfgs.py:188-207) on the
+        device (lddmm_lbfgs_two_loop): the same operations in the same order,
+        with rho and alpha / beta kept in device memory, so no host
+        synchronisation per dot product.  storedGrad: [[s_k, y_k], ...] oldest
+        first (Control dicts); the pair scalars (rho, c) are computed once per
+        stored pair and kept with it."""
+        import ctypes
+        for entry in storedGrad:
+            if len(entry) == 2:
+                sk, yk = entry[0]['at'].contiguous(), entry[1]['at'].contiguous()
+                sc = torch.empty(2, dtype=torch.float64, device=sk.device)
+                B.check(B.LIB.lddmm_lbfgs_pair_scalars(B.ptr(sk), B.ptr(yk), sk.numel(), B.ptr(sc[0:1]),
+                                                       B.ptr(sc[1:2]), B.stream_handle()), "lbfgs_pair_scalars")
+                entry.append((sk, yk, sc))
+        g = grd['at'].contiguous()
+        q = torch.empty_like(g)
+        m = len(storedGrad)
+        P = ctypes.c_void_p * m
+        sp = P(*[e[2][0].data_ptr() for e in storedGrad])
+        yp = P(*[e[2][1].data_ptr() for e in storedGrad])
+        rp = P(*[e[2][2].data_ptr() for e in storedGrad])
+        cn = storedGrad[-1][2][2][1:2]
+        B.check(B.LIB.lddmm_lbfgs_two_loop(B.ptr(g), g.numel(), m, sp, yp, rp, B.ptr(cn), B.ptr(q),
+                                           B.stream_handle()), "lbfgs_two_loop")
         out = Control()
         out['at'] = q
         return out

@@ -133,42 +133,3 @@ def test_mlp_oracle_gradient_fd():
         e[i] = 1e-6
         fd = (mlp.loss_and_grad(w + e, w + e, X, counts, T)[0] - mlp.loss_and_grad(w - e, w - e, X, counts, T)[0]) / 2e-6
         assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(fd)
-
-
-def test_device_two_loop_equals_reference_recursion():
-    """SurfaceMatching.twoLoop (scalars as 0-d tensors, no host syncs) gives the
-    direction of the reference two-loop recursion (bfgs.bfgs inline loop)."""
-    import torch
-    from lddmm_amd.surfaceMatching import SurfaceMatching, Control
-    rng = np.random.default_rng(3)
-
-    def ctl(a):
-        c = Control()
-        c['at'] = torch.tensor(a)
-        return c
-    for m in (1, 2, 7, 50):
-        stored = []
-        for _ in range(m):
-            s = rng.normal(size=(4, 30, 3))
-            y = s * rng.uniform(0.5, 2.0) + 0.1 * rng.normal(size=s.shape)     # positive curvature
-            stored.append([ctl(s), ctl(y)])
-        g = ctl(rng.normal(size=(4, 30, 3)))
-        q = SurfaceMatching.twoLoop(None, g, stored)['at'].numpy()
-        # the reference loop on floats
-        dot = lambda a, b: float((a['at'] * b['at']).sum())
-        qr = g['at'].numpy().copy()
-        rho, alpha = [], []
-        for sk, yk in reversed(stored):
-            rho.append(1 / dot(yk, sk))
-            alpha.append(rho[-1] * float((sk['at'].numpy() * qr).sum()))
-            qr = qr - alpha[-1] * yk['at'].numpy()
-        rho.reverse()
-        alpha.reverse()
-        sk, yk = stored[-1]
-        c = dot(sk, yk) / dot(yk, yk)
-        c = 1 if c < 1e-10 else c
-        qr = c * qr
-        for k, (sk, yk) in enumerate(stored):
-            beta = rho[k] * float((yk['at'].numpy() * qr).sum())
-            qr = qr + (alpha[k] - beta) * sk['at'].numpy()
-        assert np.max(np.abs(q - qr)) <= 1e-12 * np.max(np.abs(qr))
