@@ -35,7 +35,10 @@ extern "C" {
 
 #define LDDMM_MAX_SCALES 8
 
-#define LDDMM_KERNEL_GAUSS 0
+#define LDDMM_KERNEL_GAUSS     0
+/* Laplacian (Matern) family: K = P_order(r) exp(-r), r = |x - y| / sigma,
+ * order 0..4 (base/kernelFunctions_util.py:20-52, :92-95). */
+#define LDDMM_KERNEL_LAPLACIAN 1
 
 #define LDDMM_PREC_F64  0
 #define LDDMM_PREC_F32C 1
@@ -60,15 +63,18 @@ extern "C" {
 #define LDDMM_DDIFFK_12     1
 #define LDDMM_DDIFFK_11AND12 2
 
-/* Kernel description: mirrors KernelSpec (base/kernelFunctions.py:339-362):
- * name -> kind, sigma -> sigma[0..nscale), ms_exponent -> kp, pk_dtype -> prec. */
+/* Kernel description: mirrors KernelSpec (base/kernelFunctions.py:339-369):
+ * name -> kind, sigma -> sigma[0..nscale), ms_exponent -> kp, pk_dtype -> prec,
+ * order -> order (Laplacian only; the reference resets orders above 4 to 3,
+ * this library rejects them).  The fused data terms (surface and curve
+ * varifold, population varifold) take LDDMM_KERNEL_GAUSS only. */
 typedef struct lddmm_kernel {
-    int32_t kind;                      /* LDDMM_KERNEL_GAUSS */
+    int32_t kind;                      /* LDDMM_KERNEL_GAUSS / LDDMM_KERNEL_LAPLACIAN */
     int32_t nscale;                    /* 1 .. LDDMM_MAX_SCALES */
     double  sigma[LDDMM_MAX_SCALES];
     double  kp;                        /* multi-scale exponent, reference default -1 */
     int32_t prec;                      /* LDDMM_PREC_F64 / LDDMM_PREC_F32C */
-    int32_t reserved;
+    int32_t order;                     /* Laplacian polynomial order 0..4; ignored for gauss */
 } lddmm_kernel;
 
 const char* lddmm_last_error(void);

@@ -28,6 +28,15 @@ How each function is pinned (tests/test_oracle.py):
                   objective through the discrete adjoint
   EPDiff          trajectory identity with landmarkEPDiff, Hamiltonian drift
                   O(dt), finite differences of the second-order gradient
+  laplacian       (every kernel sum above takes kernel=('laplacian', order))
+                  profile = the Matern nu = order + 1/2 closed form written
+                  with factorials; first-derivative sums = finite differences
+                  of applyK for orders 1-4, second-derivative sums = finite
+                  differences of applyDiffKT for orders 2-4; the reference's
+                  table rows that are not derivatives (Q_0 = 0; R_0 = R_1 = 0)
+                  are pinned by the zero they produce (order 0) and by the
+                  difference from the finite derivative being exactly the
+                  dropped R term (order 1)
 """
 import numpy as np
 
@@ -72,10 +81,45 @@ def _as2d(a):
 
 
 # ---------------------------------------------------------------------------
-# L1 kernel sums (Gaussian)
+# Radial profiles.  Gaussian: K = exp(-rho^2/2) and both derivative factors
+# equal K.  Laplacian (Matern) of order k (base/kernelFunctions_util.py:20-52,
+# 92-95, 197-201, 238-243, 309-314): with rho = |d| (scaled distance),
+# K = P_k(rho) e^-rho, first-derivative factor Q_k(rho) e^-rho, second
+# R_k(rho) e^-rho, where Q = (P - P')/rho and R = (Q - Q')/rho for the orders
+# where these are polynomials (the reference's tables set Q_0 = 0, R_0 = R_1 = 0).
+# ---------------------------------------------------------------------------
+LAP_P = [[1, 0, 0, 0, 0], [1, 1, 0, 0, 0], [1, 1, 1 / 3, 0, 0], [1, 1, 0.4, 1 / 15, 0],
+         [1, 1, 3 / 7, 2 / 21, 1 / 105]]
+LAP_Q = [[0, 0, 0, 0], [1, 0, 0, 0], [1 / 3, 1 / 3, 0, 0], [1 / 5, 1 / 5, 1 / 15, 0],
+         [1 / 7, 1 / 7, 2 / 35, 1 / 105]]
+LAP_R = [[0, 0, 0], [0, 0, 0], [1 / 3, 0, 0], [1 / 15, 1 / 15, 0], [1 / 35, 1 / 35, 1 / 105]]
+
+
+def _poly(c, u):
+    return sum(ci * u ** i for i, ci in enumerate(c))
+
+
+def _profile(d, kernel, which):
+    """which 0: K, 1: first-derivative factor, 2: second-derivative factor, for
+    scaled differences d [..., dim]; kernel = ('gauss', _) or ('laplacian', order)."""
+    name, order = kernel
+    if 'gauss' in name:
+        return np.exp(-(d ** 2).sum(-1) / 2)
+    if 'lap' in name:
+        r = np.sqrt((d ** 2).sum(-1))
+        c = (LAP_P, LAP_Q, LAP_R)[which][order]
+        return _poly(c, r) * np.exp(-r)
+    raise ValueError(name)
+
+
+GAUSS = ('gauss', 0)
+
+
+# ---------------------------------------------------------------------------
+# L1 kernel sums
 # ---------------------------------------------------------------------------
 
-def applyK(y, x, a, scale, KP=-1):
+def applyK(y, x, a, scale, KP=-1, kernel=GAUSS):
     """out[i] = sum_s w_s sum_j exp(-|y_i/s - x_j/s|^2 / 2) a[j].
 
     Follows base/kernelFunctions_util.py:451-466 (applyK_numba) with the
@@ -88,12 +132,12 @@ def applyK(y, x, a, scale, KP=-1):
         ys, xs = y / sc, x / sc
         for i0, i1 in _row_blocks(y.shape[0], x.shape[0]):
             d = ys[i0:i1, None, :] - xs[None, :, :]
-            K = np.exp(-(d ** 2).sum(-1) / 2)
+            K = _profile(d, kernel, 0)
             res[i0:i1] += (K @ a) * sKP[si]
     return res / W
 
 
-def applyDiffKT(y, x, p, a, scale, regweight=1., sym=False, lddmm=False, KP=-1):
+def applyDiffKT(y, x, p, a, scale, regweight=1., sym=False, lddmm=False, KP=-1, kernel=GAUSS):
     """out[i] = sum_s s^(KP-1)/W sum_j -(y_i - x_j)/s * K_s(y_i, x_j) * c_ij.
 
     c_ij = p_i.a_j (plain), + a_i.p_j (sym), + a_i.p_j - 2 r a_i.a_j (lddmm).
@@ -108,7 +152,7 @@ def applyDiffKT(y, x, p, a, scale, regweight=1., sym=False, lddmm=False, KP=-1):
         ys, xs = y / sc, x / sc
         for i0, i1 in _row_blocks(y.shape[0], x.shape[0]):
             d = ys[i0:i1, None, :] - xs[None, :, :]
-            K = np.exp(-(d ** 2).sum(-1) / 2)
+            K = _profile(d, kernel, 1)
             c = p[i0:i1] @ a.T
             if lddmm:
                 c = c + a[i0:i1] @ p.T - 2 * regweight * (a[i0:i1] @ a.T)
@@ -118,7 +162,7 @@ def applyDiffKT(y, x, p, a, scale, regweight=1., sym=False, lddmm=False, KP=-1):
     return res / W
 
 
-def applyktensor(y, x, ay, ax, betay, betax, scale, KP=-1):
+def applyktensor(y, x, ay, ax, betay, betax, scale, KP=-1, kernel=GAUSS):
     """out[i] = sum_s w_s sum_j K_s(y_i, x_j) (ay_i ax_j + (betay_i . betax_j)^2).
 
     Follows base/kernelFunctions_util.py:1475-1506 (applyktensor_numba, gauss
@@ -132,13 +176,13 @@ def applyktensor(y, x, ay, ax, betay, betax, scale, KP=-1):
         ys, xs = y / sc, x / sc
         for i0, i1 in _row_blocks(y.shape[0], x.shape[0]):
             d = ys[i0:i1, None, :] - xs[None, :, :]
-            K = np.exp(-(d ** 2).sum(-1) / 2)
+            K = _profile(d, kernel, 0)
             g = by[i0:i1] @ bx.T
             res[i0:i1] += (K * (ay[i0:i1, None] * ax[None, :] + g * g)).sum(1) * sKP[si]
     return res / W
 
 
-def applykmat(y, x, beta, scale, KP=-1):
+def applykmat(y, x, beta, scale, KP=-1, kernel=GAUSS):
     """out[i] = sum_s w_s sum_j K_s(y_i, x_j) beta[i, j, :] (dense matrix weights).
 
     Follows base/kernelFunctions_util.py:1509-1540 (applykmat, gauss branch).
@@ -152,12 +196,12 @@ def applykmat(y, x, beta, scale, KP=-1):
         ys, xs = y / sc, x / sc
         for i0, i1 in _row_blocks(y.shape[0], x.shape[0]):
             d = ys[i0:i1, None, :] - xs[None, :, :]
-            K = np.exp(-(d ** 2).sum(-1) / 2)
+            K = _profile(d, kernel, 0)
             res[i0:i1] += (K[..., None] * beta[i0:i1]).sum(1) * sKP[si]
     return res / W
 
 
-def applykdiffmat(y, x, beta, scale, KP=-1):
+def applykdiffmat(y, x, beta, scale, KP=-1, kernel=GAUSS):
     """out[i] = -sum_s s^(KP-1)/W sum_j K_s(y_i, x_j) (y_i - x_j)/s beta[i, j].
 
     Follows base/kernelFunctions_util.py:1638-1669 (applykdiffmat_numba).
@@ -170,12 +214,12 @@ def applykdiffmat(y, x, beta, scale, KP=-1):
         ys, xs = y / sc, x / sc
         for i0, i1 in _row_blocks(y.shape[0], x.shape[0]):
             d = ys[i0:i1, None, :] - xs[None, :, :]
-            K = np.exp(-(d ** 2).sum(-1) / 2)
+            K = _profile(d, kernel, 1)
             res[i0:i1] -= (d * (K * beta[i0:i1])[..., None]).sum(1) * sKP1[si]
     return res / W
 
 
-def applyDiffK(x, a1, a2, scale, KP=-1, option='1and2'):
+def applyDiffK(x, a1, a2, scale, KP=-1, option='1and2', kernel=GAUSS):
     """First derivative sums on one point set (y = x).
 
     option '1':     out[k] = sum_l -K (d_kl . a1_k) a2_l
@@ -195,7 +239,7 @@ def applyDiffK(x, a1, a2, scale, KP=-1, option='1and2'):
         xs = x / sc
         for i0, i1 in _row_blocks(x.shape[0], x.shape[0]):
             d = xs[i0:i1, None, :] - xs[None, :, :]
-            K = np.exp(-(d ** 2).sum(-1) / 2)
+            K = _profile(d, kernel, 1)
             if option == '1':
                 da = (d * a1[i0:i1, None, :]).sum(-1)
                 res[i0:i1] += ((-K * da) @ a2) * sKP1[si]
@@ -210,7 +254,7 @@ def applyDiffK(x, a1, a2, scale, KP=-1, option='1and2'):
     return res / W
 
 
-def applyDDiffK(x, a1, a2, p, scale, KP=-1, option='11and12'):
+def applyDDiffK(x, a1, a2, p, scale, KP=-1, option='11and12', kernel=GAUSS):
     """Second derivative sums on one point set.
 
     out[k] = sum_s s^(KP-2)/W sum_l ((d.q) d - q) K_s(x_k, x_l) (a1_k . a2_l)
@@ -226,7 +270,8 @@ def applyDDiffK(x, a1, a2, p, scale, KP=-1, option='11and12'):
         KS = sc ** (KP - 2)
         for i0, i1 in _row_blocks(x.shape[0], x.shape[0]):
             d = xs[i0:i1, None, :] - xs[None, :, :]
-            K = np.exp(-(d ** 2).sum(-1) / 2)
+            K = _profile(d, kernel, 1)
+            KDD = _profile(d, kernel, 2)
             aa = a1[i0:i1] @ a2.T
             if option == '11':
                 q = np.broadcast_to(p[i0:i1, None, :], d.shape)
@@ -240,7 +285,7 @@ def applyDDiffK(x, a1, a2, p, scale, KP=-1, option='11and12'):
             else:
                 raise ValueError(option)
             dq = (d * q).sum(-1)
-            t = (dq[..., None] * d - q) * (K * aa)[..., None]
+            t = ((dq * KDD)[..., None] * d - K[..., None] * q) * aa[..., None]
             res[i0:i1] += sign * t.sum(1) * KS
     return res / W
 
@@ -603,7 +648,7 @@ def curveCurrentNormGradient(fvDef, fv1, sigma, KP=-1):
 # Landmark shooting (first order) and its discrete adjoint
 # ---------------------------------------------------------------------------
 
-def landmarkDirectEvolutionEuler(x0, at, sigma, KP=-1):
+def landmarkDirectEvolutionEuler(x0, at, sigma, KP=-1, kernel=GAUSS):
     """x_{t+1} = x_t + (1/T) K(x_t, x_t) a_t; base/pointEvolution.py:88-164 (no affine)."""
     x0, at = _f64(x0), _f64(at)
     T = at.shape[0]
@@ -611,13 +656,13 @@ def landmarkDirectEvolutionEuler(x0, at, sigma, KP=-1):
     xt = np.zeros((T + 1,) + x0.shape)
     xt[0] = x0
     for t in range(T):
-        xt[t + 1] = xt[t] + dt * applyK(xt[t], xt[t], at[t], sigma, KP)
+        xt[t + 1] = xt[t] + dt * applyK(xt[t], xt[t], at[t], sigma, KP, kernel=kernel)
     return xt
 
 
-def landmarkHamiltonianCovector(x0, at, px1, sigma, regweight, KP=-1):
+def landmarkHamiltonianCovector(x0, at, px1, sigma, regweight, KP=-1, kernel=GAUSS):
     """base/pointEvolution.py:167-201 (no affine, no extra term)."""
-    xt = landmarkDirectEvolutionEuler(x0, at, sigma, KP)
+    xt = landmarkDirectEvolutionEuler(x0, at, sigma, KP, kernel=kernel)
     T = at.shape[0]
     dt = 1.0 / T
     pxt = np.zeros(xt.shape)
@@ -626,14 +671,14 @@ def landmarkHamiltonianCovector(x0, at, px1, sigma, regweight, KP=-1):
         px = pxt[T - t]
         z = xt[T - t - 1]
         a = at[T - t - 1]
-        zpx = applyDiffKT(z, z, px, a, sigma, regweight=regweight, lddmm=True, KP=KP)
+        zpx = applyDiffKT(z, z, px, a, sigma, regweight=regweight, lddmm=True, KP=KP, kernel=kernel)
         pxt[T - t - 1] = px + dt * zpx
     return pxt, xt
 
 
-def landmarkHamiltonianGradient(x0, at, px1, sigma, regweight, KP=-1):
+def landmarkHamiltonianGradient(x0, at, px1, sigma, regweight, KP=-1, kernel=GAUSS):
     """dat_t = 2 r a_t - p_{t+1}; base/pointEvolution.py:205-236 (no affine)."""
-    pxt, xt = landmarkHamiltonianCovector(x0, at, px1, sigma, regweight, KP)
+    pxt, xt = landmarkHamiltonianCovector(x0, at, px1, sigma, regweight, KP, kernel=kernel)
     at = _f64(at)
     dat = 2 * regweight * at - pxt[1:]
     return dat, xt, pxt
@@ -673,7 +718,7 @@ def _data_grad(kind, fvDef, fv1, sigma_dist):
 
 
 def matching_objective(x0, faces, at, target, sigma_diff, sigma_dist, kind='varifold',
-                       regweight=1., sigma_error=1.):
+                       regweight=1., sigma_error=1., kernel_diff=GAUSS):
     """Surface- or curve-matching objective at control `at` (plain, no affine, ds = 1).
     kind: 'varifold' / 'current' / 'measure' (surfaces), 'curve_varifold' /
     'curve_current' (curves, base/curveMatching.py:272-313 has the same
@@ -687,17 +732,17 @@ def matching_objective(x0, faces, at, target, sigma_diff, sigma_dist, kind='vari
     at = _f64(at)
     T = at.shape[0]
     dt = 1.0 / T
-    xt = landmarkDirectEvolutionEuler(x0, at, sigma_diff)
+    xt = landmarkDirectEvolutionEuler(x0, at, sigma_diff, kernel=kernel_diff)
     obj = 0.
     for t in range(T):
-        ra = applyK(xt[t], xt[t], at[t], sigma_diff)
+        ra = applyK(xt[t], xt[t], at[t], sigma_diff, kernel=kernel_diff)
         obj += regweight * dt * (at[t] * ra).sum()
     dterm, obj0 = _data_term(kind, (xt[-1], faces), target, sigma_dist)
     return obj + (obj0 + dterm) / sigma_error ** 2, xt
 
 
 def matching_gradient(x0, faces, at, target, sigma_diff, sigma_dist, kind='varifold',
-                      regweight=1., sigma_error=1., euclidean=True):
+                      regweight=1., sigma_error=1., euclidean=True, kernel_diff=GAUSS):
     """grad['at'] of the plain surface-matching objective as BFGS sees it.
 
     px1 = -endPointGradient/sigma_err^2 (base/surfaceMatching.py:702-721,
@@ -706,12 +751,12 @@ def matching_gradient(x0, faces, at, target, sigma_diff, sigma_dist, kind='varif
     """
     at = _f64(at)
     T = at.shape[0]
-    xt = landmarkDirectEvolutionEuler(x0, at, sigma_diff)
+    xt = landmarkDirectEvolutionEuler(x0, at, sigma_diff, kernel=kernel_diff)
     px1 = -_data_grad(kind, (xt[-1], faces), target, sigma_dist) / sigma_error ** 2
-    dat, xt, pxt = landmarkHamiltonianGradient(x0, at, px1, sigma_diff, regweight)
+    dat, xt, pxt = landmarkHamiltonianGradient(x0, at, px1, sigma_diff, regweight, kernel=kernel_diff)
     if euclidean:
         for t in range(T):
-            dat[t] = applyK(xt[t], xt[t], dat[t], sigma_diff)
+            dat[t] = applyK(xt[t], xt[t], dat[t], sigma_diff, kernel=kernel_diff)
     return dat / T
 
 
@@ -719,7 +764,7 @@ def matching_gradient(x0, faces, at, target, sigma_diff, sigma_dist, kind='varif
 # Geodesic shooting (second order) and its adjoint
 # ---------------------------------------------------------------------------
 
-def landmarkEPDiff(T, x0, a0, sigma, KP=-1):
+def landmarkEPDiff(T, x0, a0, sigma, KP=-1, kernel=GAUSS):
     """base/pointEvolution.py:320-379 (simple output)."""
     x0, a0 = _f64(x0), _f64(a0)
     dt = 1.0 / T
@@ -729,13 +774,13 @@ def landmarkEPDiff(T, x0, a0, sigma, KP=-1):
     at[0] = a0
     for k in range(T):
         z, a = xt[k], at[k]
-        xt[k + 1] = z + dt * applyK(z, z, a, sigma, KP)
+        xt[k + 1] = z + dt * applyK(z, z, a, sigma, KP, kernel=kernel)
         if k < T - 1:
-            at[k + 1] = a - dt * applyDiffKT(z, z, a, a, sigma, KP=KP)
+            at[k + 1] = a - dt * applyDiffKT(z, z, a, a, sigma, KP=KP, kernel=kernel)
     return xt, at
 
 
-def secondOrderEvolution(x0, a0, sigma, timeStep, KP=-1):
+def secondOrderEvolution(x0, a0, sigma, timeStep, KP=-1, kernel=GAUSS):
     """base/pointEvolution.py:385-472 (no spline, no affine, no point set)."""
     x0, a0 = _f64(x0), _f64(a0)
     T = int(np.round(1 / timeStep))
@@ -744,40 +789,42 @@ def secondOrderEvolution(x0, a0, sigma, timeStep, KP=-1):
     xt[0], at[0] = x0, a0
     for k in range(T):
         x, a = xt[k], at[k]
-        zx = applyK(x, x, a, sigma, KP)
-        za = -applyDiffKT(x, x, a, a, sigma, KP=KP)
+        zx = applyK(x, x, a, sigma, KP, kernel=kernel)
+        za = -applyDiffKT(x, x, a, a, sigma, KP=KP, kernel=kernel)
         xt[k + 1] = x + timeStep * zx
         at[k + 1] = a + timeStep * za
     return xt, at
 
 
-def secondOrderHamiltonian(x, a, px, pa, sigma, KP=-1):
+def secondOrderHamiltonian(x, a, px, pa, sigma, KP=-1, kernel=GAUSS):
     """base/pointEvolution.py:475-486 (no spline, no affine)."""
-    Ht = (px * applyK(x, x, a, sigma, KP)).sum()
-    Ht += (pa * (-applyDiffKT(x, x, a, a, sigma, KP=KP))).sum()
+    Ht = (px * applyK(x, x, a, sigma, KP, kernel=kernel)).sum()
+    Ht += (pa * (-applyDiffKT(x, x, a, a, sigma, KP=KP, kernel=kernel))).sum()
     return Ht
 
 
-def secondOrderCovector(x0, a0, px1, pa1, sigma, timeStep, KP=-1):
+def secondOrderCovector(x0, a0, px1, pa1, sigma, timeStep, KP=-1, kernel=GAUSS):
     """base/pointEvolution.py:489-552 (single target, no affine)."""
     T = int(np.round(1 / timeStep))
-    xt, at = secondOrderEvolution(x0, a0, sigma, timeStep, KP)
+    xt, at = secondOrderEvolution(x0, a0, sigma, timeStep, KP, kernel=kernel)
     pxt = np.zeros(xt.shape)
     pat = np.zeros(xt.shape)
     pxt[T], pat[T] = px1, pa1
     for t in range(T):
         px, pa = pxt[T - t], pat[T - t]
         x, a = xt[T - t - 1], at[T - t - 1]
-        zpx = applyDiffKT(x, x, px, a, sigma, sym=True, KP=KP) - applyDDiffK(x, a, a, pa, sigma, KP=KP)
+        zpx = (applyDiffKT(x, x, px, a, sigma, sym=True, KP=KP, kernel=kernel)
+               - applyDDiffK(x, a, a, pa, sigma, KP=KP, kernel=kernel))
         # Kernel.applyDiffK1and2 -> applykdiff1and2, which forces KP = -1
-        zpa = applyK(x, x, px, sigma, KP) - applyDiffK(x, pa, a, sigma, KP=-1, option='1and2')
+        zpa = (applyK(x, x, px, sigma, KP, kernel=kernel)
+               - applyDiffK(x, pa, a, sigma, KP=-1, option='1and2', kernel=kernel))
         pxt[T - t - 1] = px + timeStep * zpx
         pat[T - t - 1] = pa + timeStep * zpa
     return pxt, pat, xt, at
 
 
-def secondOrderGradient(x0, a0, px1, pa1, sigma, timeStep, KP=-1):
+def secondOrderGradient(x0, a0, px1, pa1, sigma, timeStep, KP=-1, kernel=GAUSS):
     """da0 = K(x0) a0 - pa_0; base/pointEvolution.py:555-618 (no affine)."""
-    pxt, pat, xt, at = secondOrderCovector(x0, a0, px1, pa1, sigma, timeStep, KP)
-    da0 = applyK(x0, x0, a0, sigma, KP) - pat[0]
+    pxt, pat, xt, at = secondOrderCovector(x0, a0, px1, pa1, sigma, timeStep, KP, kernel=kernel)
+    da0 = applyK(x0, x0, a0, sigma, KP, kernel=kernel) - pat[0]
     return da0, xt, at, pxt, pat

@@ -72,17 +72,51 @@ int blocks_per_cu(const void* func, int block, size_t dyn_lds);
 // ---------------------------------------------------------------------------
 constexpr double kLog2e = 1.4426950408889634;
 
+// Radial profiles of the general path.  With r = |x - y| / s:
+//   gauss:     K = D = DD = exp(-r^2 / 2)
+//   laplacian: K = P(r) e^-r, D = Q(r) e^-r, DD = R(r) e^-r
+// where the derivative sums use D (first) and DD (second derivative), and
+// Q = (P - P') / r, R = (Q - Q') / r for the order's polynomial P
+// (base/kernelFunctions_util.py:20-52 tables c_, c1_, c2_).
 template <typename T>
 struct ScaleSet {
     int ns;                         // number of scales
-    T beta[LDDMM_MAX_SCALES];       // exp2 coefficient on |x - y|^2 (centred units)
+    int kind;                       // LDDMM_KERNEL_GAUSS / LDDMM_KERNEL_LAPLACIAN
+    T beta[LDDMM_MAX_SCALES];       // gauss: exp2 coefficient on |x - y|^2 (centred units)
+    T rinv[LDDMM_MAX_SCALES];       // laplacian: 1 / s
     T wK[LDDMM_MAX_SCALES];         // s^KP / W
     T wD[LDDMM_MAX_SCALES];         // s^(KP-2) / W
     T wDD[LDDMM_MAX_SCALES];        // s^(KP-4) / W
-    double cs;                      // coordinate factor (single scale: K = exp2(-|u_i - u_j|^2), else 1)
+    T pc[5], qc[4], rc[3];          // laplacian P, Q, R coefficients (ascending powers of r)
+    double cs;                      // coordinate factor (single-scale gauss: K = exp2(-|u_i - u_j|^2), else 1)
 };
 
-int validate_kernel(const lddmm_kernel* kd);
+// Checks a kernel description.  `data_term` entry points (the fused varifold
+// kernels) accept only the Gaussian kind.
+int validate_kernel(const lddmm_kernel* kd, bool data_term = false);
+
+// The single-scale Gaussian instantiation (scaled coordinates, exp2 of r^2)
+// serves one Gaussian scale; every other kernel takes the general one.
+inline bool general_path(const lddmm_kernel* kd) {
+    return kd->nscale > 1 || kd->kind != LDDMM_KERNEL_GAUSS;
+}
+
+// Laplacian polynomial coefficients by order (base/kernelFunctions_util.py:20-37).
+constexpr double kLapP[5][5] = {{1, 0, 0, 0, 0},
+                                {1, 1, 0, 0, 0},
+                                {1, 1, 1.0 / 3, 0, 0},
+                                {1, 1, 0.4, 1.0 / 15, 0},
+                                {1, 1, 3.0 / 7, 2.0 / 21, 1.0 / 105}};
+constexpr double kLapQ[5][4] = {{0, 0, 0, 0},
+                                {1, 0, 0, 0},
+                                {1.0 / 3, 1.0 / 3, 0, 0},
+                                {1.0 / 5, 1.0 / 5, 1.0 / 15, 0},
+                                {1.0 / 7, 1.0 / 7, 2.0 / 35, 1.0 / 105}};
+constexpr double kLapR[5][3] = {{0, 0, 0},
+                                {0, 0, 0},
+                                {1.0 / 3, 0, 0},
+                                {1.0 / 15, 1.0 / 15, 0},
+                                {1.0 / 35, 1.0 / 35, 1.0 / 105}};
 
 // Constant factors a single-scale op applies at the end, in float64.
 struct SingleScaleConsts {
@@ -96,18 +130,26 @@ template <typename T>
 ScaleSet<T> make_scales(const lddmm_kernel* kd, SingleScaleConsts* single) {
     ScaleSet<T> s{};
     s.ns = kd->nscale;
+    s.kind = kd->kind;
     double W = 0;
     for (int i = 0; i < kd->nscale; ++i) W += std::pow(kd->sigma[i], kd->kp);
     for (int i = 0; i < kd->nscale; ++i) {
         double sg = kd->sigma[i];
         s.beta[i] = T(kLog2e / (2.0 * sg * sg));
+        s.rinv[i] = T(1.0 / sg);
         s.wK[i] = T(std::pow(sg, kd->kp) / W);
         s.wD[i] = T(std::pow(sg, kd->kp - 2) / W);
         s.wDD[i] = T(std::pow(sg, kd->kp - 4) / W);
     }
+    if (kd->kind == LDDMM_KERNEL_LAPLACIAN) {
+        const int o = kd->order;
+        for (int c = 0; c < 5; ++c) s.pc[c] = T(kLapP[o][c]);
+        for (int c = 0; c < 4; ++c) s.qc[c] = T(kLapQ[o][c]);
+        for (int c = 0; c < 3; ++c) s.rc[c] = T(kLapR[o][c]);
+    }
     double sg = kd->sigma[0];
     double cs = std::sqrt(kLog2e / 2.0) / sg;
-    s.cs = kd->nscale == 1 ? cs : 1.0;
+    s.cs = general_path(kd) ? 1.0 : cs;
     if (single) {
         single->sigma = sg;
         single->cs = cs;

@@ -359,13 +359,11 @@ k_dense(const double* y, int n, const double* x, int m, int dim, const double* b
                 r2 += dd[d] * dd[d];
             }
             if constexpr (DIFF) {
-                double g = 0.0;
-                for (int q = 0; q < sc.ns; ++q) g += sc.wD[q] * exp2(-sc.beta[q] * r2);
+                double g = gD<double, true>(sc, r2);
                 double b = beta[size_t(i) * m + j] * g;
                 for (int c = 0; c < 4 && c0 + c < dim; ++c) acc[c] -= b * dd[c0 + c];
             } else {
-                double g = 0.0;
-                for (int q = 0; q < sc.ns; ++q) g += sc.wK[q] * exp2(-sc.beta[q] * r2);
+                double g = gK<double, true>(sc, r2);
                 const double* bp = beta + (size_t(i) * m + j) * kb + c0;
                 for (int c = 0; c < 4 && c0 + c < kb; ++c) acc[c] += g * bp[c];
             }
@@ -394,8 +392,8 @@ extern "C" int lddmm_applyK(const lddmm_kernel* kd, const double* y, int64_t n, 
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (m == 0) return launch_fill(out, size_t(n) * k, 0.0, st);
     LDDMM_REQUIRE(x && a, "null x/a");
-    const bool ms = kd->nscale > 1;
-    if (kd->prec == LDDMM_PREC_F32C && pair_engine() == LDDMM_ENGINE_MFMA)
+    const bool ms = general_path(kd);
+    if (kd->prec == LDDMM_PREC_F32C && kd->kind == LDDMM_KERNEL_GAUSS && pair_engine() == LDDMM_ENGINE_MFMA)
         return mfma_applyK(kd, y, int(n), x, int(m), dim, a, k, out, st);
     if (kd->prec == LDDMM_PREC_F32C)
         return ms ? applyK_t<float, true>(kd, y, n, x, m, dim, a, k, out, st)
@@ -417,10 +415,11 @@ extern "C" int lddmm_applyDiffKT(const lddmm_kernel* kd, const double* y, int64_
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (m == 0) return launch_fill(out, size_t(n) * dim, 0.0, st);
     LDDMM_REQUIRE(x && a, "null x/a");
-    const bool ms = kd->nscale > 1;
+    const bool ms = general_path(kd);
     const bool sym = mode != LDDMM_DIFFKT_PLAIN;
     const double r2w = mode == LDDMM_DIFFKT_LDDMM ? 2.0 * regweight : 0.0;
-    if (kd->prec == LDDMM_PREC_F32C && !sym && pair_engine() == LDDMM_ENGINE_MFMA)
+    if (kd->prec == LDDMM_PREC_F32C && !sym && kd->kind == LDDMM_KERNEL_GAUSS &&
+        pair_engine() == LDDMM_ENGINE_MFMA)
         return mfma_diffKT_plain(kd, y, int(n), x, int(m), dim, p, a, k, out, st);
 #define LDDMM_DKT(T, S, M) diffKT_t<T, S, M>(kd, y, n, x, m, dim, p, a, k, r2w, out, st)
     if (kd->prec == LDDMM_PREC_F32C) {
@@ -446,7 +445,7 @@ extern "C" int lddmm_applyktensor(const lddmm_kernel* kd, const double* y, int64
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (m == 0) return launch_fill(out, size_t(n), 0.0, st);
     LDDMM_REQUIRE(x && ax && betax, "null source arrays");
-    const bool ms = kd->nscale > 1;
+    const bool ms = general_path(kd);
     if (kd->prec == LDDMM_PREC_F32C)
         return ms ? ktensor_t<float, true>(kd, y, n, x, m, dim, ay, ax, betay, betax, kb, out, st)
                   : ktensor_t<float, false>(kd, y, n, x, m, dim, ay, ax, betay, betax, kb, out, st);
@@ -463,7 +462,7 @@ extern "C" int lddmm_applyDiffK(const lddmm_kernel* kd, const double* x, int64_t
     if (n == 0) return LDDMM_OK;
     LDDMM_REQUIRE(x && a1 && a2 && out, "null pointer");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const bool ms = kd->nscale > 1;
+    const bool ms = general_path(kd);
     const bool f32 = kd->prec == LDDMM_PREC_F32C;
 #define LDDMM_DK(T, O, M) diffK_t<T, O, M>(kd, x, n, dim, a1, a2, k, out, st)
 #define LDDMM_DK_OPT(T, M)                                                     \
@@ -485,7 +484,7 @@ extern "C" int lddmm_applyDDiffK(const lddmm_kernel* kd, const double* x, int64_
     if (n == 0) return LDDMM_OK;
     LDDMM_REQUIRE(x && a1 && a2 && p && out, "null pointer");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const bool ms = kd->nscale > 1;
+    const bool ms = general_path(kd);
     const bool f32 = kd->prec == LDDMM_PREC_F32C;
 #define LDDMM_DDK(T, O, M) ddiffK_t<T, O, M>(kd, x, n, dim, a1, a2, k, p, out, st)
 #define LDDMM_DDK_OPT(T, M)                                                    \

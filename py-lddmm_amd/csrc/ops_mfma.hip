@@ -142,14 +142,14 @@ int mfma_diffKT_t(const lddmm_kernel* kd, const double* y, int n, const double* 
 
 int mfma_applyK(const lddmm_kernel* kd, const double* y, int n, const double* x, int m, int dim, const double* a, int k,
                 double* out, hipStream_t st) {
-    return kd->nscale > 1 ? mfma_applyK_t<true>(kd, y, n, x, m, dim, a, k, out, st)
-                          : mfma_applyK_t<false>(kd, y, n, x, m, dim, a, k, out, st);
+    return general_path(kd) ? mfma_applyK_t<true>(kd, y, n, x, m, dim, a, k, out, st)
+                              : mfma_applyK_t<false>(kd, y, n, x, m, dim, a, k, out, st);
 }
 
 int mfma_diffKT_plain(const lddmm_kernel* kd, const double* y, int n, const double* x, int m, int dim,
                       const double* p, const double* a, int k, double* out, hipStream_t st) {
-    return kd->nscale > 1 ? mfma_diffKT_t<true>(kd, y, n, x, m, dim, p, a, k, out, st)
-                          : mfma_diffKT_t<false>(kd, y, n, x, m, dim, p, a, k, out, st);
+    return general_path(kd) ? mfma_diffKT_t<true>(kd, y, n, x, m, dim, p, a, k, out, st)
+                              : mfma_diffKT_t<false>(kd, y, n, x, m, dim, p, a, k, out, st);
 }
 
 }  // namespace lddmm

@@ -234,16 +234,39 @@ int run_reduce(const Partials<T>& part, int nchunk, int nch, int nout, int n, co
 }
 
 // ---------------------------------------------------------------------------
-// Gaussian factors for one pair given r2 (centred units; single scale: scaled
-// so that K = exp2(-r2)).
+// Radial factors for one pair given r2 = |x - y|^2 (centred units; single
+// scale: scaled so that K = exp2(-r2)).  The general path evaluates, per
+// scale q, the kernel's profile (see ScaleSet) weighted by wK (kernel sums),
+// wD (first-derivative sums) or wDD (second-derivative sums).
 // ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T lap_poly5(const T* c, T u) {
+    return c[0] + u * (c[1] + u * (c[2] + u * (c[3] + u * c[4])));
+}
+template <typename T>
+__device__ __forceinline__ T lap_poly4(const T* c, T u) {
+    return c[0] + u * (c[1] + u * (c[2] + u * c[3]));
+}
+template <typename T>
+__device__ __forceinline__ T lap_poly3(const T* c, T u) {
+    return c[0] + u * (c[1] + u * c[2]);
+}
+
 template <typename T, bool MS>
 __device__ __forceinline__ T gK(const ScaleSet<T>& s, T r2) {
     if constexpr (!MS) {
         return fast_exp2(-r2);
     } else {
         T g = T(0);
-        for (int q = 0; q < s.ns; ++q) g += s.wK[q] * fast_exp2(-s.beta[q] * r2);
+        if (s.kind == LDDMM_KERNEL_GAUSS) {
+            for (int q = 0; q < s.ns; ++q) g += s.wK[q] * fast_exp2(-s.beta[q] * r2);
+        } else {
+            const T r = sqrt(r2);
+            for (int q = 0; q < s.ns; ++q) {
+                const T u = r * s.rinv[q];
+                g += s.wK[q] * lap_poly5(s.pc, u) * fast_exp2(-T(kLog2e) * u);
+            }
+        }
         return g;
     }
 }
@@ -254,36 +277,65 @@ __device__ __forceinline__ T gD(const ScaleSet<T>& s, T r2) {
         return fast_exp2(-r2);
     } else {
         T g = T(0);
-        for (int q = 0; q < s.ns; ++q) g += s.wD[q] * fast_exp2(-s.beta[q] * r2);
+        if (s.kind == LDDMM_KERNEL_GAUSS) {
+            for (int q = 0; q < s.ns; ++q) g += s.wD[q] * fast_exp2(-s.beta[q] * r2);
+        } else {
+            const T r = sqrt(r2);
+            for (int q = 0; q < s.ns; ++q) {
+                const T u = r * s.rinv[q];
+                g += s.wD[q] * lap_poly4(s.qc, u) * fast_exp2(-T(kLog2e) * u);
+            }
+        }
         return g;
     }
 }
 
-// Both weightings from one set of exponentials.
+// Kernel and first-derivative weightings from one set of exponentials.
 template <typename T, bool MS>
 __device__ __forceinline__ void gKD(const ScaleSet<T>& s, T r2, T& k, T& d) {
     if constexpr (!MS) {
         k = d = fast_exp2(-r2);
     } else {
         k = d = T(0);
-        for (int q = 0; q < s.ns; ++q) {
-            T e = fast_exp2(-s.beta[q] * r2);
-            k += s.wK[q] * e;
-            d += s.wD[q] * e;
+        if (s.kind == LDDMM_KERNEL_GAUSS) {
+            for (int q = 0; q < s.ns; ++q) {
+                T e = fast_exp2(-s.beta[q] * r2);
+                k += s.wK[q] * e;
+                d += s.wD[q] * e;
+            }
+        } else {
+            const T r = sqrt(r2);
+            for (int q = 0; q < s.ns; ++q) {
+                const T u = r * s.rinv[q];
+                const T e = fast_exp2(-T(kLog2e) * u);
+                k += s.wK[q] * lap_poly5(s.pc, u) * e;
+                d += s.wD[q] * lap_poly4(s.qc, u) * e;
+            }
         }
     }
 }
 
+// Second- and first-derivative weightings from one set of exponentials.
 template <typename T, bool MS>
 __device__ __forceinline__ void gDDD(const ScaleSet<T>& s, T r2, T& dd, T& d) {
     if constexpr (!MS) {
         dd = d = fast_exp2(-r2);
     } else {
         dd = d = T(0);
-        for (int q = 0; q < s.ns; ++q) {
-            T e = fast_exp2(-s.beta[q] * r2);
-            dd += s.wDD[q] * e;
-            d += s.wD[q] * e;
+        if (s.kind == LDDMM_KERNEL_GAUSS) {
+            for (int q = 0; q < s.ns; ++q) {
+                T e = fast_exp2(-s.beta[q] * r2);
+                dd += s.wDD[q] * e;
+                d += s.wD[q] * e;
+            }
+        } else {
+            const T r = sqrt(r2);
+            for (int q = 0; q < s.ns; ++q) {
+                const T u = r * s.rinv[q];
+                const T e = fast_exp2(-T(kLog2e) * u);
+                dd += s.wDD[q] * lap_poly3(s.rc, u) * e;
+                d += s.wD[q] * lap_poly4(s.qc, u) * e;
+            }
         }
     }
 }

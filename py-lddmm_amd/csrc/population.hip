@@ -789,6 +789,7 @@ void key_kernel(std::vector<uint64_t>& k, const lddmm_kernel* kd) {
     for (int i = 0; i < kd->nscale; ++i) key_add(k, kd->sigma[i]);
     key_add(k, kd->kp);
     key_add(k, kd->prec);
+    key_add(k, kd->order);
 }
 
 void key_ptr(std::vector<uint64_t>& k, const void* p) { k.push_back(uint64_t(reinterpret_cast<uintptr_t>(p))); }
@@ -1062,7 +1063,7 @@ using namespace lddmm;
 
 #define POP_DISPATCH(FN, ...)                                                                  \
     do {                                                                                       \
-        const bool ms_ = kd->nscale > 1;                                                       \
+        const bool ms_ = general_path(kd);                                                       \
         if (kd->prec == LDDMM_PREC_F32C)                                                       \
             return ms_ ? FN<float, true>(__VA_ARGS__) : FN<float, false>(__VA_ARGS__);         \
         return ms_ ? FN<double, true>(__VA_ARGS__) : FN<double, false>(__VA_ARGS__);           \
@@ -1187,7 +1188,7 @@ extern "C" int lddmm_population_shoot_adjoint(const lddmm_kernel* kd, lddmm_popu
 
 extern "C" int lddmm_population_varifold(const lddmm_kernel* kd, lddmm_population* pop, const double* vertices,
                                          double* energies, double* grad, void* stream) {
-    LDDMM_TRY(validate_kernel(kd));
+    LDDMM_TRY(validate_kernel(kd, true));
     LDDMM_TRY(check_pop(pop));
     LDDMM_REQUIRE(energies || grad, "population_varifold: nothing requested");
     LDDMM_REQUIRE(pop->V == 0 || vertices, "population_varifold: null vertices");

@@ -162,10 +162,16 @@ int profile_read(double* ms_each, int64_t cap, int64_t* launches) {
     return LDDMM_OK;
 }
 
-int validate_kernel(const lddmm_kernel* kd) {
+int validate_kernel(const lddmm_kernel* kd, bool data_term) {
     LDDMM_REQUIRE(kd != nullptr, "null kernel description");
-    if (kd->kind != LDDMM_KERNEL_GAUSS) {
-        set_error("kernel kind %d not supported (only gauss)", kd->kind);
+    if (kd->kind == LDDMM_KERNEL_LAPLACIAN) {
+        if (data_term) {
+            set_error("the fused varifold data terms take the gauss kernel only (got laplacian)");
+            return LDDMM_EUNSUPPORTED;
+        }
+        LDDMM_REQUIRE(kd->order >= 0 && kd->order <= 4, "laplacian order %d outside [0, 4]", kd->order);
+    } else if (kd->kind != LDDMM_KERNEL_GAUSS) {
+        set_error("kernel kind %d not supported (gauss = 0, laplacian = 1)", kd->kind);
         return LDDMM_EUNSUPPORTED;
     }
     LDDMM_REQUIRE(kd->nscale >= 1 && kd->nscale <= LDDMM_MAX_SCALES,

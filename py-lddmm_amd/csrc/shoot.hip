@@ -306,7 +306,7 @@ using namespace lddmm;
 
 #define LDDMM_DISPATCH(FN, ...)                                                          \
     do {                                                                                 \
-        const bool ms_ = kd->nscale > 1;                                                 \
+        const bool ms_ = general_path(kd);                                                 \
         const bool f32_ = kd->prec == LDDMM_PREC_F32C;                                   \
         if (dim == 3) {                                                                  \
             if (f32_) return ms_ ? FN<float, true, 3>(__VA_ARGS__) : FN<float, false, 3>(__VA_ARGS__);    \

@@ -397,7 +397,7 @@ extern "C" int lddmm_surface_face_gather(const lddmm_surface* s, const double* d
 
 extern "C" int lddmm_surface_varifold(const lddmm_kernel* kd, const lddmm_surface* s1,
                                       const lddmm_surface* tgt, double* energy, double* grad, void* stream) {
-    LDDMM_TRY(validate_kernel(kd));
+    LDDMM_TRY(validate_kernel(kd, true));
     LDDMM_TRY(check_surface(s1, grad != nullptr, "surface_varifold"));
     if (tgt) LDDMM_TRY(check_surface(tgt, false, "surface_varifold (target)"));
     LDDMM_REQUIRE(energy || grad, "surface_varifold: nothing requested");
@@ -407,7 +407,7 @@ extern "C" int lddmm_surface_varifold(const lddmm_kernel* kd, const lddmm_surfac
         if (grad) LDDMM_TRY(launch_fill(grad, size_t(s1->nv) * 3, 0.0, st));
         return LDDMM_OK;
     }
-    const bool ms = kd->nscale > 1;
+    const bool ms = general_path(kd);
     if (kd->prec == LDDMM_PREC_F32C)
         return ms ? varifold_t<float, true>(kd, s1, tgt, energy, grad, st)
                   : varifold_t<float, false>(kd, s1, tgt, energy, grad, st);
@@ -435,7 +435,7 @@ extern "C" int lddmm_curve_seg_gather(const lddmm_curve* c, const double* dz1, c
 
 extern "C" int lddmm_curve_varifold(const lddmm_kernel* kd, const lddmm_curve* c1, const lddmm_curve* tgt,
                                     double weight, double* energy, double* grad, void* stream) {
-    LDDMM_TRY(validate_kernel(kd));
+    LDDMM_TRY(validate_kernel(kd, true));
     LDDMM_TRY(check_curve(c1, grad != nullptr, "curve_varifold"));
     if (tgt) {
         LDDMM_TRY(check_curve(tgt, false, "curve_varifold (target)"));
@@ -450,7 +450,7 @@ extern "C" int lddmm_curve_varifold(const lddmm_kernel* kd, const lddmm_curve* c
         if (grad) LDDMM_TRY(launch_fill(grad, size_t(c1->nv) * size_t(c1->dim), 0.0, st));
         return LDDMM_OK;
     }
-    const bool ms = kd->nscale > 1;
+    const bool ms = general_path(kd);
     if (kd->prec == LDDMM_PREC_F32C)
         return ms ? curve_varifold_t<float, true>(kd, c1, tgt, weight, energy, grad, st)
                   : curve_varifold_t<float, false>(kd, c1, tgt, weight, energy, grad, st);

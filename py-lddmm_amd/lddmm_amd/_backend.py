@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "liblddmm_amd.so")
 
 MAX_SCALES = 8
 KERNEL_GAUSS = 0
+KERNEL_LAPLACIAN = 1
 PREC_F64 = 0
 PREC_F32C = 1
 OK, EINVAL, EUNSUPPORTED, EHIP = 0, -1, -2, -3
@@ -27,7 +28,7 @@ DDIFFK_11, DDIFFK_12, DDIFFK_11AND12 = 0, 1, 2
 class LddmmKernel(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("nscale", ctypes.c_int32),
                 ("sigma", ctypes.c_double * MAX_SCALES), ("kp", ctypes.c_double),
-                ("prec", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("prec", ctypes.c_int32), ("order", ctypes.c_int32)]
 
 
 class LddmmSurface(ctypes.Structure):
@@ -191,7 +192,7 @@ def profile_read(cap=4096):
     return [buf[i] for i in range(min(n.value, cap))]
 
 
-def kernel_desc(sigma, kp=-1.0, prec='float64', kind=KERNEL_GAUSS):
+def kernel_desc(sigma, kp=-1.0, prec='float64', kind=KERNEL_GAUSS, order=3):
     s = np.atleast_1d(np.asarray(sigma, dtype=np.float64)).ravel()
     if s.size < 1 or s.size > MAX_SCALES:
         raise ValueError(f"lddmm_amd supports 1..{MAX_SCALES} scales (got {s.size})")
@@ -202,7 +203,17 @@ def kernel_desc(sigma, kp=-1.0, prec='float64', kind=KERNEL_GAUSS):
         kd.sigma[i] = float(v)
     kd.kp = float(kp)
     kd.prec = precision_code(prec)
+    if kind == KERNEL_LAPLACIAN:
+        if int(order) != order or not 0 <= order <= 4:
+            raise ValueError(f"laplacian kernel order must be an integer in 0..4 (got {order!r})")
+        kd.order = int(order)
     return kd
+
+
+def kernel_for(name, sigma, order, kp=-1.0, prec='float64'):
+    """Descriptor of the reference kernel `name` ('gauss'/'gaussian' or
+    'laplacian', matched as base/kernelFunctions_util.py:395-400 does)."""
+    return kernel_desc(sigma, kp, prec, kernel_kind(name), order)
 
 
 def precision_code(prec):
@@ -214,7 +225,9 @@ def precision_code(prec):
 
 
 def kernel_kind(name):
-    if name is None or 'gauss' not in str(name):
-        raise NotImplementedError(
-            f"lddmm_amd builds the Gaussian kernel family only (got name={name!r})")
-    return KERNEL_GAUSS
+    if name is not None and 'gauss' in str(name):
+        return KERNEL_GAUSS
+    if name is not None and 'lap' in str(name):
+        return KERNEL_LAPLACIAN
+    raise NotImplementedError(
+        f"lddmm_amd builds the gauss and laplacian kernel families (got name={name!r})")

@@ -1,4 +1,4 @@
-"""Module-level Gaussian kernel sums with the reference's signatures.
+"""Module-level kernel sums (gauss and laplacian) with the reference's signatures.
 
 Drop-in for the dispatch functions of py-lddmm's base/kernelFunctions_util.py
 (applyK :443, applyDiffKT :765, applyktensor :1424, applykmat :1510,
@@ -44,7 +44,6 @@ def _w(a, n):
 
 def applyK(y, x, a, name, scale, order, cpu=False, dtype='float64', KP=-1):
     """out[i,:] = sum_s w_s sum_j exp(-|y_i - x_j|^2 / 2 s^2) a[j,:]; shape [N, k]."""
-    B.kernel_kind(name)
     tt = _is_torch(y, x, a)
     y_, x_ = _pts(y), _pts(x)
     n, m, dim = y_.shape[0], x_.shape[0], y_.shape[1]
@@ -53,7 +52,7 @@ def applyK(y, x, a, name, scale, order, cpu=False, dtype='float64', KP=-1):
     a_ = _w(a, m)
     k = a_.shape[1]
     out = torch.empty((n, k), dtype=torch.float64, device=y_.device)
-    kd = B.kernel_desc(scale, KP, dtype)
+    kd = B.kernel_for(name, scale, order, KP, dtype)
     B.check(B.LIB.lddmm_applyK(kd, B.ptr(y_), n, B.ptr(x_), m, dim, B.ptr(a_), k, B.ptr(out),
                                B.stream_handle()), "applyK")
     return _out(out, tt)
@@ -62,7 +61,6 @@ def applyK(y, x, a, name, scale, order, cpu=False, dtype='float64', KP=-1):
 def applyDiffKT(y, x, p, a, name, scale, order, regweight=1., sym=False, lddmm=False, cpu=False,
                 dtype='float64', KP=-1):
     """out[i] = sum_s s^(KP-1)/W sum_j -(y_i - x_j)/s K_s(y_i, x_j) c_ij; shape [N, d]."""
-    B.kernel_kind(name)
     tt = _is_torch(y, x, p, a)
     y_, x_ = _pts(y), _pts(x)
     n, m, dim = y_.shape[0], x_.shape[0], y_.shape[1]
@@ -75,7 +73,7 @@ def applyDiffKT(y, x, p, a, name, scale, order, regweight=1., sym=False, lddmm=F
     if mode != B.DIFFKT_PLAIN and n != m:
         raise ValueError("sym / lddmm modes need as many targets as sources")
     out = torch.empty((n, dim), dtype=torch.float64, device=y_.device)
-    kd = B.kernel_desc(scale, KP, dtype)
+    kd = B.kernel_for(name, scale, order, KP, dtype)
     B.check(B.LIB.lddmm_applyDiffKT(kd, B.ptr(y_), n, B.ptr(x_), m, dim, B.ptr(p_), B.ptr(a_), k,
                                     mode, float(regweight), B.ptr(out), B.stream_handle()),
             "applyDiffKT")
@@ -84,7 +82,6 @@ def applyDiffKT(y, x, p, a, name, scale, order, regweight=1., sym=False, lddmm=F
 
 def applyktensor(y, x, ay, ax, betay, betax, name, scale, order, cpu=False, dtype='float64', KP=-1):
     """out[i] = sum_s w_s sum_j K_s(y_i, x_j)(ay_i ax_j + (betay_i . betax_j)^2); shape [N]."""
-    B.kernel_kind(name)
     tt = _is_torch(y, x, ay, ax, betay, betax)
     y_, x_ = _pts(y), _pts(x)
     n, m, dim = y_.shape[0], x_.shape[0], y_.shape[1]
@@ -94,7 +91,7 @@ def applyktensor(y, x, ay, ax, betay, betax, name, scale, order, cpu=False, dtyp
     bx_ = _w(betax, m)
     kb = by_.shape[1]
     out = torch.empty((n,), dtype=torch.float64, device=y_.device)
-    kd = B.kernel_desc(scale, KP, dtype)
+    kd = B.kernel_for(name, scale, order, KP, dtype)
     B.check(B.LIB.lddmm_applyktensor(kd, B.ptr(y_), n, B.ptr(x_), m, dim, B.ptr(ay_), B.ptr(ax_),
                                      B.ptr(by_), B.ptr(bx_), kb, B.ptr(out), B.stream_handle()),
             "applyktensor")
@@ -103,7 +100,6 @@ def applyktensor(y, x, ay, ax, betay, betax, name, scale, order, cpu=False, dtyp
 
 def applykmat(y, x, beta, name, scale, order, KP=-1):
     """out[i,:] = sum_s w_s sum_j K_s(y_i, x_j) beta[i,j,:] (dense beta [N, M, k])."""
-    B.kernel_kind(name)
     tt = _is_torch(y, x, beta)
     y_, x_ = _pts(y), _pts(x)
     n, m, dim = y_.shape[0], x_.shape[0], y_.shape[1]
@@ -113,7 +109,7 @@ def applykmat(y, x, beta, name, scale, order, KP=-1):
     b_ = b_.reshape(n, m, -1).contiguous()
     kb = b_.shape[2]
     out = torch.empty((n, kb), dtype=torch.float64, device=y_.device)
-    kd = B.kernel_desc(scale, KP, 'float64')
+    kd = B.kernel_for(name, scale, order, KP, 'float64')
     B.check(B.LIB.lddmm_applykmat(kd, B.ptr(y_), n, B.ptr(x_), m, dim, B.ptr(b_), kb, B.ptr(out),
                                   B.stream_handle()), "applykmat")
     return _out(out, tt)
@@ -121,13 +117,12 @@ def applykmat(y, x, beta, name, scale, order, KP=-1):
 
 def applykdiffmat(y, x, beta, name, scale, order, cpu=False, dtype='float64', KP=-1):
     """out[i] = -sum_s s^(KP-1)/W sum_j K_s(y_i, x_j)(y_i - x_j)/s beta[i,j] (dense [N, M])."""
-    B.kernel_kind(name)
     tt = _is_torch(y, x, beta)
     y_, x_ = _pts(y), _pts(x)
     n, m, dim = y_.shape[0], x_.shape[0], y_.shape[1]
     b_ = B.to_dev(beta).reshape(n, m).contiguous()
     out = torch.empty((n, dim), dtype=torch.float64, device=y_.device)
-    kd = B.kernel_desc(scale, KP, 'float64')
+    kd = B.kernel_for(name, scale, order, KP, 'float64')
     B.check(B.LIB.lddmm_applykdiffmat(kd, B.ptr(y_), n, B.ptr(x_), m, dim, B.ptr(b_), B.ptr(out),
                                       B.stream_handle()), "applykdiffmat")
     return _out(out, tt)
@@ -139,7 +134,6 @@ _DDIFFK_OPT = {'11': B.DDIFFK_11, '12': B.DDIFFK_12, '11and12': B.DDIFFK_11AND12
 
 def applyDiffK(x, a1, a2, name, scale, order, cpu=False, dtype='float64', KP=-1, option='1'):
     """First-derivative sums on one point set (options '1', '2', '1and2')."""
-    B.kernel_kind(name)
     if option not in _DIFFK_OPT:
         raise ValueError(f"DiffK: unknown option {option!r}")
     tt = _is_torch(x, a1, a2)
@@ -151,7 +145,7 @@ def applyDiffK(x, a1, a2, name, scale, order, cpu=False, dtype='float64', KP=-1,
     a2_ = _w(a2, n)
     k = a2_.shape[1]
     out = torch.empty((n, k), dtype=torch.float64, device=x_.device)
-    kd = B.kernel_desc(scale, KP, dtype)
+    kd = B.kernel_for(name, scale, order, KP, dtype)
     B.check(B.LIB.lddmm_applyDiffK(kd, B.ptr(x_), n, dim, B.ptr(a1_), B.ptr(a2_), k,
                                    _DIFFK_OPT[option], B.ptr(out), B.stream_handle()), "applyDiffK")
     return _out(out, tt)
@@ -159,7 +153,6 @@ def applyDiffK(x, a1, a2, name, scale, order, cpu=False, dtype='float64', KP=-1,
 
 def applyDDiffK(x, a1, a2, p, name, scale, order, cpu=False, dtype='float64', KP=-1, option='11'):
     """Second-derivative sums on one point set (options '11', '12', '11and12')."""
-    B.kernel_kind(name)
     if option not in _DDIFFK_OPT:
         raise ValueError(f"DDiffK: unknown option {option!r}")
     tt = _is_torch(x, a1, a2, p)
@@ -173,7 +166,7 @@ def applyDDiffK(x, a1, a2, p, name, scale, order, cpu=False, dtype='float64', KP
     if p_.shape[1] != dim:
         raise ValueError("p must have the point dimension")
     out = torch.empty((n, dim), dtype=torch.float64, device=x_.device)
-    kd = B.kernel_desc(scale, KP, dtype)
+    kd = B.kernel_for(name, scale, order, KP, dtype)
     B.check(B.LIB.lddmm_applyDDiffK(kd, B.ptr(x_), n, dim, B.ptr(a1_), B.ptr(a2_), a1_.shape[1],
                                     B.ptr(p_), _DDIFFK_OPT[option], B.ptr(out), B.stream_handle()),
             "applyDDiffK")

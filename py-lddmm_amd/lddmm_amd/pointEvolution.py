@@ -21,7 +21,7 @@ def _kd(K):
     B.kernel_kind(K.name)
     if getattr(K, 'affine', 'none') not in ('none', None):
         raise NotImplementedError("device shooting supports affine='none' kernels only")
-    return B.kernel_desc(K.sigma, K.ms_exponent, K.pk_dtype)
+    return B.kernel_for(K.name, K.sigma, K.order, K.ms_exponent, K.pk_dtype)
 
 
 def _is_torch(*a):

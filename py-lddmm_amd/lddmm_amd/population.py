@@ -255,7 +255,7 @@ class PopulationMatching:
         s2 = o['sigmaError'] ** 2
         self.obj0_by = {}
         for dt in {self.dtype0, 'float64'}:
-            k = Kernel(name='gauss', sigma=kv_.sigma)
+            k = Kernel(name=kv_.name, sigma=kv_.sigma, order=kv_.order)
             k.ms_exponent = kv_.ms_exponent
             k.pk_dtype = dt
             self.obj0_by[dt] = np.array([sd.varifoldNorm0(t, k) / s2 for t in self.fv1])
@@ -265,7 +265,7 @@ class PopulationMatching:
 
     # ---------------------------------------------------------- evaluations
     def _kd(self, K, dtype):
-        return B.kernel_desc(K.sigma, K.ms_exponent, dtype)
+        return B.kernel_for(K.name, K.sigma, K.order, K.ms_exponent, dtype)
 
     def _groups(self, mask):
         """Active masks per precision for the pairs in mask."""

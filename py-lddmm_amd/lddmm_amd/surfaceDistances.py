@@ -36,10 +36,10 @@ def _check_fun(fun):
 
 
 def _kd(KparDist, dtype=None):
-    B.kernel_kind(KparDist.name)
     if getattr(KparDist, 'affine', 'none') not in ('none', None):
         raise NotImplementedError("affine terms do not apply to data-term kernels")
-    return B.kernel_desc(KparDist.sigma, KparDist.ms_exponent, dtype or KparDist.pk_dtype)
+    return B.kernel_for(KparDist.name, KparDist.sigma, KparDist.order, KparDist.ms_exponent,
+                        dtype or KparDist.pk_dtype)
 
 
 def varifold_device(kd, s1, s2=None, energy=True, grad=False, v1=None):

@@ -191,7 +191,7 @@ class SurfaceMatching:
     # ----------------------------------------------------------- data term
     def _dist_kd(self):
         K = self.options['KparDist']
-        return B.kernel_desc(K.sigma, K.ms_exponent, K.pk_dtype)
+        return B.kernel_for(K.name, K.sigma, K.order, K.ms_exponent, K.pk_dtype)
 
     def fun_obj0(self, fv1):
         K = self.options['KparDist']

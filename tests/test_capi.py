@@ -127,8 +127,34 @@ def test_kernel_desc_mirrors_kernelspec(lib):
     assert kd.nscale == 2 and kd.sigma[1] == 5.0 and kd.prec == lib.PREC_F32C
     with pytest.raises(ValueError):
         lib.kernel_desc([1.0], prec='float16')
+    assert lib.kernel_kind('laplacian') == lib.KERNEL_LAPLACIAN
+    assert lib.kernel_kind('gaussian') == lib.KERNEL_GAUSS
     with pytest.raises(NotImplementedError):
-        lib.kernel_kind('laplacian')
+        lib.kernel_kind('poly')
+    kd = lib.kernel_for('laplacian', [1.0], 4)
+    assert kd.kind == lib.KERNEL_LAPLACIAN and kd.order == 4
+    with pytest.raises(ValueError):
+        lib.kernel_for('laplacian', [1.0], 5)
+
+
+def test_laplacian_descriptor_checks(lib):
+    """Order range and the gauss-only data terms are checked before any device call."""
+    B = lib
+    null = ctypes.c_void_p(0)
+    kd = B.kernel_for('laplacian', [1.0], 3)
+    kd.order = 7
+    assert B.LIB.lddmm_applyK(kd, null, 1, null, 1, 3, null, 3, null, null) == B.EINVAL
+    assert b"order" in B.LIB.lddmm_last_error()
+    kd.order = -1
+    assert B.LIB.lddmm_applyK(kd, null, 1, null, 1, 3, null, 3, null, null) == B.EINVAL
+    kd = B.kernel_for('laplacian', [1.0], 3)
+    assert B.LIB.lddmm_applyK(kd, null, 0, null, 0, 3, null, 3, null, null) == B.OK
+    s = B.LddmmSurface()
+    e = ctypes.c_void_p(1)
+    assert B.LIB.lddmm_surface_varifold(kd, ctypes.byref(s), None, e, None, None) == B.EUNSUPPORTED
+    assert b"gauss" in B.LIB.lddmm_last_error()
+    c = B.LddmmCurve()
+    assert B.LIB.lddmm_curve_varifold(kd, ctypes.byref(c), None, 1.0, e, None, None) == B.EUNSUPPORTED
 
 
 def test_population_schedule_covers_every_tile_chunk_within_bounds(lib):

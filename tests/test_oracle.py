@@ -431,3 +431,148 @@ def test_second_order_gradient_fd():
     da0 = O.secondOrderGradient(x0, a0, px1, np.zeros_like(a0), [1.2], 1 / T)[0]
     d = rnd(10, 3)
     np.testing.assert_allclose((da0 * d).sum(), _fd(J, a0, d), rtol=1e-6)
+
+
+# ------------------------------------------------------ laplacian family
+def _matern(order, r):
+    """Matern nu = order + 1/2 without its e^-r: order!/(2 order)! sum_i
+    (order + i)! / (i! (order - i)!) (2 r)^(order - i)."""
+    from math import factorial as f
+    k = order
+    return f(k) / f(2 * k) * sum(f(k + i) / (f(i) * f(k - i)) * (2 * r) ** (k - i) for i in range(k + 1))
+
+
+@pytest.mark.parametrize("order", [0, 1, 2, 3, 4])
+def test_laplacian_profile_is_matern(order):
+    kern = ('laplacian', order)
+    y = np.zeros((1, 3))
+    u = rnd(1, 3)
+    u /= np.linalg.norm(u)
+    a = np.array([[1.5, -0.5]])
+    for r in (0.0, 0.3, 1.7, 4.2):
+        x = u * r * 0.8
+        got = O.applyK(y, x, a, [0.8], kernel=kern)
+        np.testing.assert_allclose(got, _matern(order, r) * np.exp(-r) * a, rtol=1e-13, atol=1e-300)
+    # two scales: weights s^-1 / sum s^-1
+    x = u * 1.1
+    w = np.array([1 / 0.8, 1 / 2.0]) / (1 / 0.8 + 1 / 2.0)
+    expect = sum(wi * _matern(order, 1.1 / s) * np.exp(-1.1 / s) for wi, s in zip(w, (0.8, 2.0)))
+    np.testing.assert_allclose(O.applyK(y, x, a, [0.8, 2.0], kernel=kern), expect * a, rtol=1e-13)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+@pytest.mark.parametrize("sig", [[1.2], [0.9, 2.1]])
+def test_laplacian_diffKT_is_gradient_wrt_targets(order, sig):
+    kern = ('laplacian', order)
+    y, x, p, a = rnd(15, 3), rnd(20, 3), rnd(15, 3), rnd(20, 3)
+    dy = rnd(15, 3)
+    fd = _fd(lambda yy: (p * O.applyK(yy, x, a, sig, kernel=kern)).sum(), y, dy)
+    g = O.applyDiffKT(y, x, p, a, sig, kernel=kern)
+    np.testing.assert_allclose((g * dy).sum(), fd, rtol=1e-7)
+    r = 0.6
+    a = a[:15]
+    f = lambda xx: ((p * O.applyK(xx, xx, a, sig, kernel=kern)).sum()
+                    - r * (a * O.applyK(xx, xx, a, sig, kernel=kern)).sum())
+    g = O.applyDiffKT(y, y, p, a, sig, regweight=r, lddmm=True, kernel=kern)
+    np.testing.assert_allclose((g * dy).sum(), _fd(f, y, dy), rtol=1e-7)
+
+
+def test_laplacian_order0_derivative_sums_vanish():
+    """The reference's first-derivative row for order 0 is zero
+    (base/kernelFunctions_util.py:27), so every derivative sum is zero."""
+    kern = ('laplacian', 0)
+    x, p, a = rnd(10, 3), rnd(10, 3), rnd(10, 3)
+    assert np.all(O.applyDiffKT(x, x, p, a, [1.0], kernel=kern) == 0)
+    assert np.all(O.applyDiffK(x, p, a, [1.0], option='1and2', kernel=kern) == 0)
+    assert np.all(O.applyDDiffK(x, p, a, p, [1.0], kernel=kern) == 0)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_laplacian_diffK_1and2_is_directional_derivative(order):
+    kern = ('laplacian', order)
+    x, a1, a2 = rnd(16, 3), rnd(16, 3), rnd(16, 2)
+    fd = _fd(lambda t: O.applyK(x + t * a1, x + t * a1, a2, [1.1], kernel=kern), 0.0, 1.0)
+    np.testing.assert_allclose(O.applyDiffK(x, a1, a2, [1.1], option='1and2', kernel=kern), fd,
+                               rtol=1e-6, atol=1e-9)
+
+
+@pytest.mark.parametrize("order", [2, 3, 4])
+@pytest.mark.parametrize("sig", [[1.3], [0.9, 2.2]])
+def test_laplacian_ddiffK_is_derivative_of_diffKT(order, sig):
+    kern = ('laplacian', order)
+    x, a1, a2, p = rnd(14, 3), rnd(14, 3), rnd(14, 3), rnd(14, 3)
+    fd = _fd(lambda t: O.applyDiffKT(x + t * p, x + t * p, a1, a2, sig, kernel=kern), 0.0, 1.0)
+    np.testing.assert_allclose(O.applyDDiffK(x, a1, a2, p, sig, option='11and12', kernel=kern), fd,
+                               rtol=1e-6, atol=1e-9)
+
+
+def test_laplacian_order1_ddiffK_drops_the_second_derivative_term():
+    """Order 1: Q = 1, so the exact second-derivative factor is 1/r, which the
+    reference's table replaces by 0 (base/kernelFunctions_util.py:34).  The
+    finite derivative minus the oracle's sum is then exactly the dropped
+    term sum_l (d.q) d / r e^-r (a1_i . a2_l) s^(KP-2) / W (d scaled by 1/s,
+    s^(KP-2) / W = s^-2 for one scale)."""
+    kern = ('laplacian', 1)
+    x, a1, a2, p = rnd(9, 3), rnd(9, 3), rnd(9, 3), rnd(9, 3)
+    s = 1.3
+    fd = _fd(lambda t: O.applyDiffKT(x + t * p, x + t * p, a1, a2, [s], kernel=kern), 0.0, 1.0)
+    got = O.applyDDiffK(x, a1, a2, p, [s], option='11and12', kernel=kern)
+    dropped = np.zeros_like(x)
+    for i in range(9):
+        for l in range(9):
+            if i == l:
+                continue
+            d = (x[i] - x[l]) / s
+            q = p[i] - p[l]
+            r = np.linalg.norm(d)
+            dropped[i] += (d @ q) * d / r * np.exp(-r) * (a1[i] @ a2[l]) * s ** (-2)
+    np.testing.assert_allclose(fd - got, dropped, rtol=1e-6, atol=1e-9)
+
+
+@pytest.mark.parametrize("order", [0, 3])
+def test_laplacian_dense_helpers_and_tensor(order):
+    kern = ('laplacian', order)
+    y, x, a, p = rnd(12, 3), rnd(17, 3), rnd(17, 3), rnd(12, 3)
+    beta = np.broadcast_to(a[None], (12, 17, 3))
+    np.testing.assert_allclose(O.applykmat(y, x, beta, [1.3, 2.0], kernel=kern),
+                               O.applyK(y, x, a, [1.3, 2.0], kernel=kern), rtol=1e-13)
+    np.testing.assert_allclose(O.applykdiffmat(y, x, p @ a.T, [1.3], kernel=kern),
+                               O.applyDiffKT(y, x, p, a, [1.3], kernel=kern), rtol=1e-12, atol=1e-300)
+    y1, x1 = np.zeros((1, 3)), np.array([[0.3, 0.2, -0.1]])
+    by, bx = np.array([[1.0, 2.0, 0.5]]), np.array([[-0.5, 1.0, 2.0]])
+    r = np.linalg.norm(x1) / 0.8
+    expect = _matern(order, r) * np.exp(-r) * (1.5 * 0.7 + (by * bx).sum() ** 2)
+    np.testing.assert_allclose(O.applyktensor(y1, x1, [1.5], [0.7], by, bx, [0.8], kernel=kern), [expect],
+                               rtol=1e-13)
+
+
+def test_laplacian_matching_gradient_fd(spheres):
+    """Surface matching with a laplacian deformation kernel (the reference's
+    examples pass Kernel(name='laplacian') as KparDiff): the Euclidean
+    gradient through the discrete adjoint matches finite differences."""
+    A, Bs = spheres
+    v, f = A
+    kern = ('laplacian', 3)
+    at = rnd(3, v.shape[0], 3, scale=0.05)
+    d = rnd(*at.shape)
+    obj = lambda aa: O.matching_objective(v, f, aa, Bs, [1.5], [0.8], regweight=0.5, kernel_diff=kern)[0]
+    g = O.matching_gradient(v, f, at, Bs, [1.5], [0.8], regweight=0.5, euclidean=True, kernel_diff=kern)
+    np.testing.assert_allclose((g * d).sum(), _fd(obj, at, d), rtol=1e-6)
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_laplacian_second_order_gradient_fd(order):
+    """Geodesic (EPDiff) shooting with a laplacian kernel (orders >= 2, where
+    every table row is the true derivative): J(a0) = 1/2 a0.K a0 + |x_T - tgt|^2."""
+    kern = ('laplacian', order)
+    x0, a0, tgt = rnd(10, 3), rnd(10, 3, scale=0.3), rnd(10, 3)
+    T = 4
+
+    def J(a):
+        xt, _ = O.secondOrderEvolution(x0, a, [1.4], 1 / T, kernel=kern)
+        return 0.5 * (a * O.applyK(x0, x0, a, [1.4], kernel=kern)).sum() + ((xt[-1] - tgt) ** 2).sum()
+    xt, _ = O.secondOrderEvolution(x0, a0, [1.4], 1 / T, kernel=kern)
+    da0 = O.secondOrderGradient(x0, a0, -2 * (xt[-1] - tgt), np.zeros_like(a0), [1.4], 1 / T,
+                                kernel=kern)[0]
+    d = rnd(10, 3)
+    np.testing.assert_allclose((da0 * d).sum(), _fd(J, a0, d), rtol=1e-6)
