@@ -63,11 +63,24 @@ extern "C" {
 #define LDDMM_DDIFFK_12     1
 #define LDDMM_DDIFFK_11AND12 2
 
-/* Kernel description: mirrors KernelSpec (base/kernelFunctions.py:339-369):
+/* Affine part of a kernel (KernelSpec.setAffine, base/kernelFunctions.py:393-412):
+ * 'affine' adds w1 (y - c)(x - c)^T a + w2 sum(a) to K(y, x) a, 'euclidean'
+ * keeps the rotation part of the linear term (an orthonormal basis of
+ * antisymmetric matrices) and the translation. */
+#define LDDMM_AFFINE_NONE      0
+#define LDDMM_AFFINE_FULL      1   /* affine = 'affine' */
+#define LDDMM_AFFINE_EUCLIDEAN 2   /* affine = 'euclidean' */
+
+/* Kernel description: mirrors KernelSpec (base/kernelFunctions.py:339-412):
  * name -> kind, sigma -> sigma[0..nscale), ms_exponent -> kp, pk_dtype -> prec,
  * order -> order (Laplacian only; the reference resets orders above 4 to 3,
- * this library rejects them).  The fused data terms (surface and curve
- * varifold, population varifold) take LDDMM_KERNEL_GAUSS only. */
+ * this library rejects them), affine / w1 / w2 / center -> the fields of the
+ * same names.  The fused data terms (surface and curve varifold, population
+ * varifold) take LDDMM_KERNEL_GAUSS only.  The affine part is applied by the
+ * shooting entry points (lddmm_shoot_forward / lddmm_shoot_adjoint, where the
+ * reference's Kernel.applyK / applyDiffKT add it); every other entry point
+ * restates a kernelFunctions_util function, which has none, and rejects a
+ * description with affine != LDDMM_AFFINE_NONE. */
 typedef struct lddmm_kernel {
     int32_t kind;                      /* LDDMM_KERNEL_GAUSS / LDDMM_KERNEL_LAPLACIAN */
     int32_t nscale;                    /* 1 .. LDDMM_MAX_SCALES */
@@ -75,6 +88,10 @@ typedef struct lddmm_kernel {
     double  kp;                        /* multi-scale exponent, reference default -1 */
     int32_t prec;                      /* LDDMM_PREC_F64 / LDDMM_PREC_F32C */
     int32_t order;                     /* Laplacian polynomial order 0..4; ignored for gauss */
+    int32_t affine;                    /* LDDMM_AFFINE_* */
+    int32_t reserved;
+    double  w1, w2;                    /* linear and translation weights of the affine part */
+    double  center[3];                 /* origin c of the linear part */
 } lddmm_kernel;
 
 const char* lddmm_last_error(void);
@@ -252,6 +269,7 @@ int lddmm_curve_seg_gather(const lddmm_curve* c, const double* dz1, const double
 /* Forward Euler landmark flow x_{t+1} = x_t + (1/T) K(x_t,x_t) a_t, t < T.
  * at [T,n,dim], xt [T+1,n,dim] (xt[0] is read, xt[1..T] written).
  * If energy != NULL (device f64[1]) it receives sum_t r dt a_t.K(x_t)a_t.
+ * K includes the kernel's affine part (kd->affine).
  * Replaces base/pointEvolution.py:88-164 and the deformation energy of
  * base/surfaceMatching.py:533-548. */
 int lddmm_shoot_forward(const lddmm_kernel* kd, const double* at, int32_t T, int64_t n,
@@ -261,6 +279,8 @@ int lddmm_shoot_forward(const lddmm_kernel* kd, const double* at, int32_t T, int
 /* Discrete adjoint: p_T = px1; p_t = p_{t+1} + dt applyDiffKT(x_t,p_{t+1},a_t,lddmm,r);
  * grad_t = K(x_t)(2 r a_t - p_{t+1}) / T when euclidean != 0, else
  * (2 r a_t - p_{t+1}) / T.  pxt [T+1,n,dim] (may be NULL), grad [T,n,dim].
+ * applyDiffKT and K include the kernel's affine part (Kernel.applyDiffKT,
+ * base/kernelFunctions.py:601-625; Kernel.applyK, :476-505).
  * Replaces base/surfaceMatching.py:728-908 (hamiltonianCovector/Gradient)
  * and base/pointEvolution.py:167-236, with the /T of getGradient (:1086). */
 int lddmm_shoot_adjoint(const lddmm_kernel* kd, const double* at, const double* xt,

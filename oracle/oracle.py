@@ -28,6 +28,10 @@ How each function is pinned (tests/test_oracle.py):
                   objective through the discrete adjoint
   EPDiff          trajectory identity with landmarkEPDiff, Hamiltonian drift
                   O(dt), finite differences of the second-order gradient
+  affine parts    finite differences of <p, affine K(y, x) a> for the plain,
+                  sym and lddmm covectors (both kinds); the Euclidean part is
+                  a rotation field (antisymmetric linear map) plus a
+                  translation; shooting with them: FD of the matching objective
   laplacian       (every kernel sum above takes kernel=('laplacian', order))
                   profile = the Matern nu = order + 1/2 closed form written
                   with factorials; first-derivative sums = finite differences
@@ -42,7 +46,7 @@ import numpy as np
 
 __all__ = [
     "applyK", "applyDiffKT", "applyktensor", "applykmat", "applykdiffmat",
-    "applyDiffK", "applyDDiffK",
+    "applyDiffK", "applyDDiffK", "affineApplyK", "affineApplyDiffKT",
     "surface_geometry", "surface_volume",
     "varifoldNorm0", "varifoldNormDef", "varifoldNorm", "varifoldNormGradient",
     "currentNorm0", "currentNormDef", "currentNorm", "currentNormGradient",
@@ -288,6 +292,58 @@ def applyDDiffK(x, a1, a2, p, scale, KP=-1, option='11and12', kernel=GAUSS):
             t = ((dq * KDD)[..., None] * d - K[..., None] * q) * aa[..., None]
             res[i0:i1] += sign * t.sum(1) * KS
     return res / W
+
+
+# ---------------------------------------------------------------------------
+# Affine part of a kernel (KernelSpec.setAffine, base/kernelFunctions.py:393-412;
+# Kernel.applyK :476-505; Kernel.applyDiffKT :601-625).  kaff = (kind, w1, w2,
+# center), kind 'affine' or 'euclidean'; the Euclidean basis is the
+# reference's list of three (3-D) or one (2-D) antisymmetric matrices / sqrt 2.
+# ---------------------------------------------------------------------------
+
+def _euclid_basis(dim):
+    s2 = np.sqrt(2.0)
+    if dim == 3:
+        return [np.array([[0, 1, 0], [-1, 0, 0], [0, 0, 0]]) / s2,
+                np.array([[0, 0, 1], [0, 0, 0], [-1, 0, 0]]) / s2,
+                np.array([[0, 0, 0], [0, 0, 1], [0, -1, 0]]) / s2]
+    return [np.array([[0, 1], [-1, 0]]) / s2]
+
+
+def affineApplyK(y, x, a, kaff):
+    """Affine part of Kernel.applyK(x, a, firstVar=y) (base/kernelFunctions.py:476-505)."""
+    kind, w1, w2, c = kaff
+    y, x, a = _f64(y), _f64(x), _as2d(a)
+    xx, yy = x - c, y - c
+    if kind == 'affine':
+        return w1 * (yy @ (xx.T @ a)) + w2 * a.sum(axis=0)
+    z = np.zeros((y.shape[0], a.shape[1])) + w2 * a.sum(axis=0)
+    for E in _euclid_basis(x.shape[1]):
+        z += w1 * ((xx @ E.T) * a).sum() * (yy @ E.T)
+    return z
+
+
+def affineApplyDiffKT(x, p, a, kaff, regweight=1., sym=False, lddmm=False):
+    """Affine part of Kernel.applyDiffKT(x, p, a) (base/kernelFunctions.py:601-625)."""
+    kind, w1, w2, c = kaff
+    x, p, a = _f64(x), _as2d(p), _as2d(a)
+    xx = x - c
+    if kind == 'affine':
+        z = w1 * (p @ (a.T @ xx))
+        if lddmm:
+            z += w1 * (a @ (p.T @ xx) - 2 * regweight * a @ (a.T @ xx))
+        elif sym:
+            z += w1 * (a @ (p.T @ xx))
+        return z
+    z = np.zeros(x.shape)
+    for E in _euclid_basis(x.shape[1]):
+        ETp, ETa = p @ E, a @ E
+        z += w1 * (xx * ETa).sum() * ETp
+        if lddmm:
+            z += w1 * ((xx * ETp).sum() * ETa - 2 * regweight * (xx * ETa).sum() * ETa)
+        elif sym:
+            z += w1 * (xx * ETp).sum() * ETa
+    return z
 
 
 # ---------------------------------------------------------------------------
@@ -648,21 +704,28 @@ def curveCurrentNormGradient(fvDef, fv1, sigma, KP=-1):
 # Landmark shooting (first order) and its discrete adjoint
 # ---------------------------------------------------------------------------
 
-def landmarkDirectEvolutionEuler(x0, at, sigma, KP=-1, kernel=GAUSS):
-    """x_{t+1} = x_t + (1/T) K(x_t, x_t) a_t; base/pointEvolution.py:88-164 (no affine)."""
+def _kv(x, a, sigma, KP, kernel, kaff):
+    """Kernel.applyK(x, a): the kernel sum plus the affine part, if any."""
+    v = applyK(x, x, a, sigma, KP, kernel=kernel)
+    return v if kaff is None else v + affineApplyK(x, x, a, kaff)
+
+
+def landmarkDirectEvolutionEuler(x0, at, sigma, KP=-1, kernel=GAUSS, kaff=None):
+    """x_{t+1} = x_t + (1/T) K(x_t, x_t) a_t; base/pointEvolution.py:88-164 (no
+    affine control; kaff = the kernel's affine part)."""
     x0, at = _f64(x0), _f64(at)
     T = at.shape[0]
     dt = 1.0 / T
     xt = np.zeros((T + 1,) + x0.shape)
     xt[0] = x0
     for t in range(T):
-        xt[t + 1] = xt[t] + dt * applyK(xt[t], xt[t], at[t], sigma, KP, kernel=kernel)
+        xt[t + 1] = xt[t] + dt * _kv(xt[t], at[t], sigma, KP, kernel, kaff)
     return xt
 
 
-def landmarkHamiltonianCovector(x0, at, px1, sigma, regweight, KP=-1, kernel=GAUSS):
-    """base/pointEvolution.py:167-201 (no affine, no extra term)."""
-    xt = landmarkDirectEvolutionEuler(x0, at, sigma, KP, kernel=kernel)
+def landmarkHamiltonianCovector(x0, at, px1, sigma, regweight, KP=-1, kernel=GAUSS, kaff=None):
+    """base/pointEvolution.py:167-201 (no affine control, no extra term)."""
+    xt = landmarkDirectEvolutionEuler(x0, at, sigma, KP, kernel=kernel, kaff=kaff)
     T = at.shape[0]
     dt = 1.0 / T
     pxt = np.zeros(xt.shape)
@@ -672,13 +735,15 @@ def landmarkHamiltonianCovector(x0, at, px1, sigma, regweight, KP=-1, kernel=GAU
         z = xt[T - t - 1]
         a = at[T - t - 1]
         zpx = applyDiffKT(z, z, px, a, sigma, regweight=regweight, lddmm=True, KP=KP, kernel=kernel)
+        if kaff is not None:
+            zpx += affineApplyDiffKT(z, px, a, kaff, regweight=regweight, lddmm=True)
         pxt[T - t - 1] = px + dt * zpx
     return pxt, xt
 
 
-def landmarkHamiltonianGradient(x0, at, px1, sigma, regweight, KP=-1, kernel=GAUSS):
-    """dat_t = 2 r a_t - p_{t+1}; base/pointEvolution.py:205-236 (no affine)."""
-    pxt, xt = landmarkHamiltonianCovector(x0, at, px1, sigma, regweight, KP, kernel=kernel)
+def landmarkHamiltonianGradient(x0, at, px1, sigma, regweight, KP=-1, kernel=GAUSS, kaff=None):
+    """dat_t = 2 r a_t - p_{t+1}; base/pointEvolution.py:205-236 (no affine control)."""
+    pxt, xt = landmarkHamiltonianCovector(x0, at, px1, sigma, regweight, KP, kernel=kernel, kaff=kaff)
     at = _f64(at)
     dat = 2 * regweight * at - pxt[1:]
     return dat, xt, pxt
@@ -718,7 +783,7 @@ def _data_grad(kind, fvDef, fv1, sigma_dist):
 
 
 def matching_objective(x0, faces, at, target, sigma_diff, sigma_dist, kind='varifold',
-                       regweight=1., sigma_error=1., kernel_diff=GAUSS):
+                       regweight=1., sigma_error=1., kernel_diff=GAUSS, kaff=None):
     """Surface- or curve-matching objective at control `at` (plain, no affine, ds = 1).
     kind: 'varifold' / 'current' / 'measure' (surfaces), 'curve_varifold' /
     'curve_current' (curves, base/curveMatching.py:272-313 has the same
@@ -732,17 +797,17 @@ def matching_objective(x0, faces, at, target, sigma_diff, sigma_dist, kind='vari
     at = _f64(at)
     T = at.shape[0]
     dt = 1.0 / T
-    xt = landmarkDirectEvolutionEuler(x0, at, sigma_diff, kernel=kernel_diff)
+    xt = landmarkDirectEvolutionEuler(x0, at, sigma_diff, kernel=kernel_diff, kaff=kaff)
     obj = 0.
     for t in range(T):
-        ra = applyK(xt[t], xt[t], at[t], sigma_diff, kernel=kernel_diff)
+        ra = _kv(xt[t], at[t], sigma_diff, -1, kernel_diff, kaff)
         obj += regweight * dt * (at[t] * ra).sum()
     dterm, obj0 = _data_term(kind, (xt[-1], faces), target, sigma_dist)
     return obj + (obj0 + dterm) / sigma_error ** 2, xt
 
 
 def matching_gradient(x0, faces, at, target, sigma_diff, sigma_dist, kind='varifold',
-                      regweight=1., sigma_error=1., euclidean=True, kernel_diff=GAUSS):
+                      regweight=1., sigma_error=1., euclidean=True, kernel_diff=GAUSS, kaff=None):
     """grad['at'] of the plain surface-matching objective as BFGS sees it.
 
     px1 = -endPointGradient/sigma_err^2 (base/surfaceMatching.py:702-721,
@@ -751,12 +816,13 @@ def matching_gradient(x0, faces, at, target, sigma_diff, sigma_dist, kind='varif
     """
     at = _f64(at)
     T = at.shape[0]
-    xt = landmarkDirectEvolutionEuler(x0, at, sigma_diff, kernel=kernel_diff)
+    xt = landmarkDirectEvolutionEuler(x0, at, sigma_diff, kernel=kernel_diff, kaff=kaff)
     px1 = -_data_grad(kind, (xt[-1], faces), target, sigma_dist) / sigma_error ** 2
-    dat, xt, pxt = landmarkHamiltonianGradient(x0, at, px1, sigma_diff, regweight, kernel=kernel_diff)
+    dat, xt, pxt = landmarkHamiltonianGradient(x0, at, px1, sigma_diff, regweight, kernel=kernel_diff,
+                                               kaff=kaff)
     if euclidean:
         for t in range(T):
-            dat[t] = applyK(xt[t], xt[t], dat[t], sigma_diff, kernel=kernel_diff)
+            dat[t] = _kv(xt[t], dat[t], sigma_diff, -1, kernel_diff, kaff)
     return dat / T
 
 

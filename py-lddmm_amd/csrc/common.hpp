@@ -91,9 +91,11 @@ struct ScaleSet {
     double cs;                      // coordinate factor (single-scale gauss: K = exp2(-|u_i - u_j|^2), else 1)
 };
 
-// Checks a kernel description.  `data_term` entry points (the fused varifold
-// kernels) accept only the Gaussian kind.
-int validate_kernel(const lddmm_kernel* kd, bool data_term = false);
+// Checks a kernel description.  Flags: kValDataTerm for the fused varifold
+// kernels (Gaussian kind only), kValAffine for the entry points that apply
+// the kernel's affine part (the others reject affine != LDDMM_AFFINE_NONE).
+enum : unsigned { kValDataTerm = 1u, kValAffine = 2u };
+int validate_kernel(const lddmm_kernel* kd, unsigned flags = 0u);
 
 // The single-scale Gaussian instantiation (scaled coordinates, exp2 of r^2)
 // serves one Gaussian scale; every other kernel takes the general one.

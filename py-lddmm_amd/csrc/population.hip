@@ -1188,7 +1188,7 @@ extern "C" int lddmm_population_shoot_adjoint(const lddmm_kernel* kd, lddmm_popu
 
 extern "C" int lddmm_population_varifold(const lddmm_kernel* kd, lddmm_population* pop, const double* vertices,
                                          double* energies, double* grad, void* stream) {
-    LDDMM_TRY(validate_kernel(kd, true));
+    LDDMM_TRY(validate_kernel(kd, kValDataTerm));
     LDDMM_TRY(check_pop(pop));
     LDDMM_REQUIRE(energies || grad, "population_varifold: nothing requested");
     LDDMM_REQUIRE(pop->V == 0 || vertices, "population_varifold: null vertices");

@@ -162,10 +162,22 @@ int profile_read(double* ms_each, int64_t cap, int64_t* launches) {
     return LDDMM_OK;
 }
 
-int validate_kernel(const lddmm_kernel* kd, bool data_term) {
+int validate_kernel(const lddmm_kernel* kd, unsigned flags) {
     LDDMM_REQUIRE(kd != nullptr, "null kernel description");
+    if (kd->affine != LDDMM_AFFINE_NONE) {
+        if (!(flags & kValAffine)) {
+            set_error("affine kernel terms are applied by the shooting entry points only (got affine = %d)",
+                      kd->affine);
+            return LDDMM_EUNSUPPORTED;
+        }
+        LDDMM_REQUIRE(kd->affine == LDDMM_AFFINE_FULL || kd->affine == LDDMM_AFFINE_EUCLIDEAN,
+                      "bad affine kind %d", kd->affine);
+        LDDMM_REQUIRE(std::isfinite(kd->w1) && std::isfinite(kd->w2), "affine weights must be finite");
+        for (int i = 0; i < 3; ++i)
+            LDDMM_REQUIRE(std::isfinite(kd->center[i]), "affine center must be finite");
+    }
     if (kd->kind == LDDMM_KERNEL_LAPLACIAN) {
-        if (data_term) {
+        if (flags & kValDataTerm) {
             set_error("the fused varifold data terms take the gauss kernel only (got laplacian)");
             return LDDMM_EUNSUPPORTED;
         }

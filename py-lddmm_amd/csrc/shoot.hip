@@ -57,6 +57,155 @@ __global__ void __launch_bounds__(1024) k_sum_doubles(const double* rows, int n,
     if (threadIdx.x == 0) *out = red[0];
 }
 
+// ---------------------------------------------------------------------------
+// Affine part of the kernel (Kernel.applyK, base/kernelFunctions.py:476-505;
+// Kernel.applyDiffKT, :601-625).  With xx = x - c, M(u) = sum_j xx_j u_j^T
+// (d x d, M[p][q] = sum_j xx_j[p] u_j[q]) and S(u) = sum_j u_j:
+//   velocity:  z_i += (x_i - c) w1 f(M(a)) + w2 S(a)
+//   lddmm covector: zpx_i += p_i w1 f(M(a))^T + a_i w1 (f(M(p))^T - 2 r f(M(a))^T)
+// f = identity ('affine') or the antisymmetric part (M - M^T)/2
+// ('euclidean': the reference sums <E, X> E over an orthonormal basis of
+// antisymmetric matrices, which is that projection).  The energy term
+// sum_i a_i.z_i(affine) = w1 <f(M(a)), M(a)> + w2 |S(a)|^2.
+// Moments are reduced deterministically: a fixed grid for a given n writes
+// per-workgroup partials, one workgroup adds them in order.
+// ---------------------------------------------------------------------------
+constexpr int kAffGrid = 256;
+constexpr int kAffVals = 24;   // two weight arrays x (9 moments + 3 sums)
+
+struct AffineArgs {
+    int kind;                  // LDDMM_AFFINE_FULL / LDDMM_AFFINE_EUCLIDEAN
+    double w1, w2, c[3];
+};
+
+AffineArgs affine_args(const lddmm_kernel* kd) {
+    return {kd->affine, kd->w1, kd->w2, {kd->center[0], kd->center[1], kd->center[2]}};
+}
+
+__global__ void __launch_bounds__(256)
+k_aff_partial(int n, int dim, const double* __restrict__ x, AffineArgs A, const double* __restrict__ u0,
+              const double* __restrict__ u1, double* __restrict__ part) {
+    __shared__ double red[kAffVals][256];
+    double acc[kAffVals];
+#pragma unroll
+    for (int v = 0; v < kAffVals; ++v) acc[v] = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        double xx[3], uu[3];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) xx[p] = p < dim ? x[size_t(i) * dim + p] - A.c[p] : 0.0;
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+            const double* u = w ? u1 : u0;
+            if (!u) continue;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) uu[q] = q < dim ? u[size_t(i) * dim + q] : 0.0;
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) acc[w * 12 + p * 3 + q] = fma(xx[p], uu[q], acc[w * 12 + p * 3 + q]);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) acc[w * 12 + 9 + q] += uu[q];
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < kAffVals; ++v) red[v][threadIdx.x] = acc[v];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int v = 0; v < kAffVals; ++v) red[v][threadIdx.x] += red[v][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < kAffVals) part[size_t(blockIdx.x) * kAffVals + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// f(M) (or its transpose) into B[9].
+__device__ __forceinline__ void aff_f(const double* M, int kind, bool transpose, double* B) {
+    for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) {
+            double v = kind == LDDMM_AFFINE_EUCLIDEAN ? 0.5 * (M[p * 3 + q] - M[q * 3 + p]) : M[p * 3 + q];
+            B[transpose ? q * 3 + p : p * 3 + q] = v;
+        }
+}
+
+// mom[24] = ordered sum of the partials; *energy_out (if set) = escale *
+// (w1 <f(M0), M0> + w2 |S0|^2).
+__global__ void __launch_bounds__(64)
+k_aff_final(const double* __restrict__ part, int nblocks, AffineArgs A, double* __restrict__ mom,
+            double* __restrict__ energy_out, double escale) {
+    __shared__ double sh[kAffVals];
+    const int v = threadIdx.x;
+    if (v < kAffVals) {
+        double s = 0.0;
+        for (int b = 0; b < nblocks; ++b) s += part[size_t(b) * kAffVals + v];
+        sh[v] = s;
+        mom[v] = s;
+    }
+    __syncthreads();
+    if (v == 0 && energy_out) {
+        double B[9];
+        aff_f(sh, A.kind, false, B);
+        double e = 0.0;
+        for (int k = 0; k < 9; ++k) e = fma(B[k], sh[k], e);
+        double t = 0.0;
+        for (int q = 0; q < 3; ++q) t = fma(sh[9 + q], sh[9 + q], t);
+        *energy_out = escale * (A.w1 * e + A.w2 * t);
+    }
+}
+
+// out_i += scale * (r1_i B1 + r2_i B2 + t): mode 0 (velocity): r1 = x_i - c,
+// B1 = w1 f(M0), t = w2 S0; mode 1 (lddmm covector): r1 = p_i, B1 = w1 f(M0)^T,
+// r2 = a_i, B2 = w1 (f(M1)^T - 2 r f(M0)^T).
+__global__ void __launch_bounds__(256)
+k_aff_apply(int n, int dim, int mode, AffineArgs A, double r, const double* __restrict__ mom,
+            const double* __restrict__ x, const double* __restrict__ r1, const double* __restrict__ r2,
+            double scale, double* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double B1[9], B2[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, t[3] = {0, 0, 0}, u1[3] = {0, 0, 0}, u2[3] = {0, 0, 0};
+    if (mode == 0) {
+        aff_f(mom, A.kind, false, B1);
+        for (int q = 0; q < 3; ++q) t[q] = A.w2 * mom[9 + q];
+        for (int p = 0; p < dim; ++p) u1[p] = x[size_t(i) * dim + p] - A.c[p];
+    } else {
+        double F1[9];
+        aff_f(mom, A.kind, true, B1);
+        aff_f(mom + 12, A.kind, true, F1);
+        for (int k = 0; k < 9; ++k) B2[k] = A.w1 * (F1[k] - 2.0 * r * B1[k]);
+        for (int p = 0; p < dim; ++p) {
+            u1[p] = r1[size_t(i) * dim + p];
+            u2[p] = r2[size_t(i) * dim + p];
+        }
+    }
+    for (int k = 0; k < 9; ++k) B1[k] *= A.w1;
+    for (int q = 0; q < dim; ++q) {
+        double z = t[q];
+        for (int p = 0; p < 3; ++p) z = fma(u1[p], B1[p * 3 + q], fma(u2[p], B2[p * 3 + q], z));
+        out[size_t(i) * dim + q] += scale * z;
+    }
+}
+
+// Workspace of the affine part: [kAffGrid * kAffVals partials][kAffVals moments].
+constexpr size_t kAffBytes = size_t(kAffGrid + 1) * kAffVals * sizeof(double);
+
+int aff_moments(const AffineArgs& A, int n, int dim, const double* x, const double* u0, const double* u1,
+                double* ws, double* energy_out, double escale, hipStream_t st) {
+    const int grid = std::max(1, std::min(kAffGrid, (n + 255) / 256));
+    hipLaunchKernelGGL(k_aff_partial, dim3(unsigned(grid)), dim3(256), 0, st, n, dim, x, A, u0, u1, ws);
+    LDDMM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_aff_final, dim3(1), dim3(64), 0, st, ws, grid, A, ws + size_t(kAffGrid) * kAffVals,
+                       energy_out, escale);
+    LDDMM_CHECK_LAUNCH();
+    return LDDMM_OK;
+}
+
+int aff_apply(const AffineArgs& A, int n, int dim, int mode, double r, const double* ws, const double* x,
+              const double* r1, const double* r2, double scale, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_aff_apply, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, n, dim, mode, A, r,
+                       ws + size_t(kAffGrid) * kAffVals, x, r1, r2, scale, out);
+    LDDMM_CHECK_LAUNCH();
+    return LDDMM_OK;
+}
+
 int axpby(size_t count, double alpha, const double* a, double beta, const double* b, double* out,
           hipStream_t st) {
     if (count == 0) return LDDMM_OK;
@@ -66,8 +215,8 @@ int axpby(size_t count, double alpha, const double* a, double beta, const double
     return LDDMM_OK;
 }
 
-int check_shoot(const lddmm_kernel* kd, int T, int64_t n, int dim) {
-    LDDMM_TRY(validate_kernel(kd));
+int check_shoot(const lddmm_kernel* kd, int T, int64_t n, int dim, unsigned flags) {
+    LDDMM_TRY(validate_kernel(kd, flags));
     LDDMM_REQUIRE(T >= 1, "T must be >= 1 (got %d)", T);
     LDDMM_REQUIRE(n >= 0 && n < (int64_t(1) << 31), "bad n");
     LDDMM_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3 (got %d)", dim);
@@ -97,11 +246,16 @@ int shoot_forward_t(const lddmm_kernel* kd, const double* at, int T_, int n, dou
     LDDMM_TRY(plan_job(jk, n, n, !MS));
     const int nb = (n + 255) / 256;
     void* ws;
-    LDDMM_TRY(workspace(256 + partial_bytes<T>(jk.plan, KW, n) + size_t(T_) * nb * sizeof(double) + 512, st, &ws));
+    const bool aff = kd->affine != LDDMM_AFFINE_NONE;
+    const AffineArgs AA = affine_args(kd);
+    LDDMM_TRY(workspace(256 + partial_bytes<T>(jk.plan, KW, n) + size_t(T_) * (nb + 1) * sizeof(double) +
+                            (aff ? kAffBytes + 256 : 0) + 512,
+                        st, &ws));
     char* base = static_cast<char*>(ws);
     Carve cv;
     double* c0 = reinterpret_cast<double*>(base + cv.take(32));
-    double* dots = reinterpret_cast<double*>(base + cv.take(size_t(T_) * nb * sizeof(double)));
+    double* dots = reinterpret_cast<double*>(base + cv.take(size_t(T_) * (nb + 1) * sizeof(double)));
+    double* aws = aff ? reinterpret_cast<double*>(base + cv.take(kAffBytes)) : nullptr;
     Partials<T> part = carve_partials<T>(base, cv, jk.plan, KW, n);
     const double dt = 1.0 / T_;
     const size_t stride = size_t(n) * KW;
@@ -116,9 +270,13 @@ int shoot_forward_t(const lddmm_kernel* kd, const double* at, int T_, int n, dou
         hipLaunchKernelGGL((k_shoot_epi<T, KW>), dim3(unsigned(nb)), dim3(256), 0, st, part, jk.plan.nchunk, n, x,
                            a, dt, xt + (t + 1) * stride, dots + size_t(t) * nb, r * dt);
         LDDMM_CHECK_LAUNCH();
+        if (aff) {
+            LDDMM_TRY(aff_moments(AA, n, KW, x, a, nullptr, aws, dots + size_t(T_) * nb + t, r * dt, st));
+            LDDMM_TRY(aff_apply(AA, n, KW, 0, 0.0, aws, x, nullptr, nullptr, dt, xt + (t + 1) * stride, st));
+        }
     }
     if (energy) {
-        hipLaunchKernelGGL(k_sum_doubles, dim3(1), dim3(1024), 0, st, dots, T_ * nb, energy);
+        hipLaunchKernelGGL(k_sum_doubles, dim3(1), dim3(1024), 0, st, dots, T_ * (aff ? nb + 1 : nb), energy);
         LDDMM_CHECK_LAUNCH();
     }
     return LDDMM_OK;
@@ -137,13 +295,16 @@ int shoot_adjoint_t(const lddmm_kernel* kd, const double* at, const double* xt, 
     LDDMM_TRY(plan_job(jd, n, n, !MS));
     LDDMM_TRY(plan_job(jk, n, n, !MS));
     const size_t stride = size_t(n) * KW;
+    const bool aff = kd->affine != LDDMM_AFFINE_NONE;
+    const AffineArgs AA = affine_args(kd);
     size_t need = 256 + std::max(partial_bytes<T>(jd.plan, 3, n), partial_bytes<T>(jk.plan, KW, n)) +
-                  stride * sizeof(double) * (pxt ? 1 : 3) + 1024;
+                  stride * sizeof(double) * (pxt ? 1 : 3) + (aff ? kAffBytes + 256 : 0) + 1024;
     void* ws;
     LDDMM_TRY(workspace(need, st, &ws));
     char* base = static_cast<char*>(ws);
     Carve cv;
     double* c0 = reinterpret_cast<double*>(base + cv.take(32));
+    double* aws = aff ? reinterpret_cast<double*>(base + cv.take(kAffBytes)) : nullptr;
     double* dat = reinterpret_cast<double*>(base + cv.take(stride * sizeof(double)));
     double* pbuf[2] = {nullptr, nullptr};
     if (!pxt) {
@@ -167,6 +328,10 @@ int shoot_adjoint_t(const lddmm_kernel* kd, const double* at, const double* xt, 
         // dat = 2 r a_t - p_{t+1}  (read before p_t overwrites the other buffer)
         LDDMM_TRY(axpby(stride, 2.0 * r, a, -1.0, pn, dat, st));
         LDDMM_TRY(reduce_job(jd, partD, n, KW, fdiff, pc, 0, pn, dt, st));
+        if (aff) {
+            LDDMM_TRY(aff_moments(AA, n, KW, x, a, pn, aws, nullptr, 0.0, st));
+            LDDMM_TRY(aff_apply(AA, n, KW, 1, r, aws, x, pn, a, dt, pc, st));
+        }
         double* g = grad + size_t(t) * stride;
         if (euclidean) {
             Carve c3; c3.off = poff;
@@ -174,6 +339,10 @@ int shoot_adjoint_t(const lddmm_kernel* kd, const double* at, const double* xt, 
             jk.args = {x, x, dat, c0, KW, KW, 0, sc};
             LDDMM_TRY(run_job(jk, n, n, partK, st));
             LDDMM_TRY(reduce_job(jk, partK, n, KW, 1.0, g, 0, nullptr, dt, st));
+            if (aff) {
+                LDDMM_TRY(aff_moments(AA, n, KW, x, dat, nullptr, aws, nullptr, 0.0, st));
+                LDDMM_TRY(aff_apply(AA, n, KW, 0, 0.0, aws, x, nullptr, nullptr, dt, g, st));
+            }
         } else {
             LDDMM_TRY(axpby(stride, dt, dat, 0.0, nullptr, g, st));
         }
@@ -318,7 +487,7 @@ using namespace lddmm;
 
 extern "C" int lddmm_shoot_forward(const lddmm_kernel* kd, const double* at, int32_t T, int64_t n, int32_t dim,
                                    double* xt, double regweight, double* energy, void* stream) {
-    LDDMM_TRY(check_shoot(kd, T, n, dim));
+    LDDMM_TRY(check_shoot(kd, T, n, dim, kValAffine));
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (n == 0) return energy ? launch_fill(energy, 1, 0.0, st) : LDDMM_OK;
     LDDMM_REQUIRE(at && xt, "null at / xt");
@@ -328,7 +497,7 @@ extern "C" int lddmm_shoot_forward(const lddmm_kernel* kd, const double* at, int
 extern "C" int lddmm_shoot_adjoint(const lddmm_kernel* kd, const double* at, const double* xt, int32_t T,
                                    int64_t n, int32_t dim, const double* px1, double regweight, int32_t euclidean,
                                    double* pxt, double* grad, void* stream) {
-    LDDMM_TRY(check_shoot(kd, T, n, dim));
+    LDDMM_TRY(check_shoot(kd, T, n, dim, kValAffine));
     if (n == 0) return LDDMM_OK;
     LDDMM_REQUIRE(at && xt && px1 && grad, "null pointer");
     hipStream_t st = static_cast<hipStream_t>(stream);
@@ -337,7 +506,7 @@ extern "C" int lddmm_shoot_adjoint(const lddmm_kernel* kd, const double* at, con
 
 extern "C" int lddmm_epdiff_forward(const lddmm_kernel* kd, int32_t T, int64_t n, int32_t dim, double* xt,
                                     double* at, void* stream) {
-    LDDMM_TRY(check_shoot(kd, T, n, dim));
+    LDDMM_TRY(check_shoot(kd, T, n, dim, 0u));
     if (n == 0) return LDDMM_OK;
     LDDMM_REQUIRE(xt && at, "null pointer");
     hipStream_t st = static_cast<hipStream_t>(stream);
@@ -347,7 +516,7 @@ extern "C" int lddmm_epdiff_forward(const lddmm_kernel* kd, int32_t T, int64_t n
 extern "C" int lddmm_epdiff_adjoint(const lddmm_kernel* kd, int32_t T, int64_t n, int32_t dim, const double* xt,
                                     const double* at, const double* px1, const double* pa1, double* pxt,
                                     double* pat, double* da0, void* stream) {
-    LDDMM_TRY(check_shoot(kd, T, n, dim));
+    LDDMM_TRY(check_shoot(kd, T, n, dim, 0u));
     if (n == 0) return LDDMM_OK;
     LDDMM_REQUIRE(xt && at && px1 && pa1 && pxt && pat && da0, "null pointer");
     hipStream_t st = static_cast<hipStream_t>(stream);

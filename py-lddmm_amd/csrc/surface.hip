@@ -397,7 +397,7 @@ extern "C" int lddmm_surface_face_gather(const lddmm_surface* s, const double* d
 
 extern "C" int lddmm_surface_varifold(const lddmm_kernel* kd, const lddmm_surface* s1,
                                       const lddmm_surface* tgt, double* energy, double* grad, void* stream) {
-    LDDMM_TRY(validate_kernel(kd, true));
+    LDDMM_TRY(validate_kernel(kd, kValDataTerm));
     LDDMM_TRY(check_surface(s1, grad != nullptr, "surface_varifold"));
     if (tgt) LDDMM_TRY(check_surface(tgt, false, "surface_varifold (target)"));
     LDDMM_REQUIRE(energy || grad, "surface_varifold: nothing requested");
@@ -435,7 +435,7 @@ extern "C" int lddmm_curve_seg_gather(const lddmm_curve* c, const double* dz1, c
 
 extern "C" int lddmm_curve_varifold(const lddmm_kernel* kd, const lddmm_curve* c1, const lddmm_curve* tgt,
                                     double weight, double* energy, double* grad, void* stream) {
-    LDDMM_TRY(validate_kernel(kd, true));
+    LDDMM_TRY(validate_kernel(kd, kValDataTerm));
     LDDMM_TRY(check_curve(c1, grad != nullptr, "curve_varifold"));
     if (tgt) {
         LDDMM_TRY(check_curve(tgt, false, "curve_varifold (target)"));

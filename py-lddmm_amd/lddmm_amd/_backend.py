@@ -20,6 +20,7 @@ PREC_F64 = 0
 PREC_F32C = 1
 OK, EINVAL, EUNSUPPORTED, EHIP = 0, -1, -2, -3
 
+AFFINE_NONE, AFFINE_FULL, AFFINE_EUCLIDEAN = 0, 1, 2
 DIFFKT_PLAIN, DIFFKT_SYM, DIFFKT_LDDMM = 0, 1, 2
 DIFFK_1, DIFFK_2, DIFFK_1AND2 = 0, 1, 2
 DDIFFK_11, DDIFFK_12, DDIFFK_11AND12 = 0, 1, 2
@@ -28,7 +29,9 @@ DDIFFK_11, DDIFFK_12, DDIFFK_11AND12 = 0, 1, 2
 class LddmmKernel(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("nscale", ctypes.c_int32),
                 ("sigma", ctypes.c_double * MAX_SCALES), ("kp", ctypes.c_double),
-                ("prec", ctypes.c_int32), ("order", ctypes.c_int32)]
+                ("prec", ctypes.c_int32), ("order", ctypes.c_int32),
+                ("affine", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("w1", ctypes.c_double), ("w2", ctypes.c_double), ("center", ctypes.c_double * 3)]
 
 
 class LddmmSurface(ctypes.Structure):
@@ -214,6 +217,29 @@ def kernel_for(name, sigma, order, kp=-1.0, prec='float64'):
     """Descriptor of the reference kernel `name` ('gauss'/'gaussian' or
     'laplacian', matched as base/kernelFunctions_util.py:395-400 does)."""
     return kernel_desc(sigma, kp, prec, kernel_kind(name), order)
+
+
+def kernel_of(K, prec=None, with_affine=False):
+    """Descriptor of a Kernel object; with_affine carries its affine part
+    (KernelSpec.setAffine: affine, w1, w2, center) for the shooting entry
+    points, the only ones that apply it."""
+    kd = kernel_for(K.name, K.sigma, K.order, K.ms_exponent, prec or K.pk_dtype)
+    aff = getattr(K, 'affine', 'none')
+    if aff in (None, 'none'):
+        return kd
+    if not with_affine:
+        raise NotImplementedError(f"affine={aff!r} kernel terms are applied in shooting only")
+    kinds = {'affine': AFFINE_FULL, 'euclidean': AFFINE_EUCLIDEAN}
+    if aff not in kinds:
+        raise NotImplementedError(f"affine={aff!r}: kernels take 'affine' or 'euclidean'")
+    kd.affine = kinds[aff]
+    kd.w1, kd.w2 = float(K.w1), float(K.w2)
+    c = np.zeros(3)
+    cc = np.asarray(K.center, dtype=np.float64).ravel()
+    c[:cc.size] = cc[:3]
+    for i in range(3):
+        kd.center[i] = c[i]
+    return kd
 
 
 def precision_code(prec):

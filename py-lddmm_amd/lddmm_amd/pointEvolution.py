@@ -6,9 +6,11 @@ secondOrderGradient.
 
 The T-step loops run on the GPU (lddmm_shoot_* / lddmm_epdiff_*): the
 trajectory stays in HBM and only the requested outputs come back.  numpy
-inputs return numpy arrays; CUDA tensors return CUDA tensors.  Affine
-controls, Jacobians, normals, passenger point sets and splines are not built
-(NotImplementedError), as in the configurations this build targets.
+inputs return numpy arrays; CUDA tensors return CUDA tensors.  A kernel's
+affine part (Kernel(affine='affine'|'euclidean')) is applied in first-order
+shooting and its adjoint.  Affine controls, Jacobians, normals, passenger
+point sets and splines are not built (NotImplementedError), as in the
+configurations this build targets.
 """
 import numpy as np
 import torch
@@ -18,10 +20,9 @@ from . import kernelFunctions_util as ku
 
 
 def _kd(K):
-    B.kernel_kind(K.name)
-    if getattr(K, 'affine', 'none') not in ('none', None):
-        raise NotImplementedError("device shooting supports affine='none' kernels only")
-    return B.kernel_for(K.name, K.sigma, K.order, K.ms_exponent, K.pk_dtype)
+    """Kernel descriptor, with the kernel's affine part (applied by the
+    first-order shooting loops; the EPDiff entry points reject it)."""
+    return B.kernel_of(K, with_affine=True)
 
 
 def _is_torch(*a):

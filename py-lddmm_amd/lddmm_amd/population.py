@@ -265,7 +265,7 @@ class PopulationMatching:
 
     # ---------------------------------------------------------- evaluations
     def _kd(self, K, dtype):
-        return B.kernel_for(K.name, K.sigma, K.order, K.ms_exponent, dtype)
+        return B.kernel_of(K, dtype)
 
     def _groups(self, mask):
         """Active masks per precision for the pairs in mask."""

@@ -3,7 +3,9 @@ restating py-lddmm base/surfaceMatching.py + base/pointSetMatching.py +
 base/basicMatching.py for the configuration the hot path serves:
 `SurfaceMatching(Template, Target, options).optimizeMatching()` with
 errorType 'varifold', 'current' or 'measure', algorithm 'bfgs' (weak-Wolfe line search),
-affine None, no internal cost, not symmetric, not unreduced.
+affine None (or affineKernel=True with affine 'euclidean' / 'affine', which
+become terms of the deformation kernel), no internal cost, not symmetric,
+not unreduced.
 
 The control `at` [T, N, 3], the trajectories and the BFGS memory live on
 the GPU (float64 tensors); every objective is one device forward shoot with
@@ -92,9 +94,11 @@ class SurfaceMatching:
             for k in options.keys():
                 self.options[k] = options[k]
         o = self.options
-        if o['affine'] not in (None, 'none'):
+        if o['affineKernel'] and o['affine'] in ('euclidean', 'affine'):
+            pass        # turned into kernel terms in set_parameters
+        elif o['affine'] not in (None, 'none'):
             raise NotImplementedError("affine registration is not built in lddmm_amd")
-        for key in ('symmetric', 'unreduced', 'affineKernel', 'pplot'):
+        for key in ('symmetric', 'unreduced', 'pplot'):
             if o[key]:
                 raise NotImplementedError(f"option {key}=True is not built in lddmm_amd")
         if o['internalCost'] is not None:
@@ -151,20 +155,36 @@ class SurfaceMatching:
         self.gradEps = o['gradTol']
         self.iter = 0
         self.pkBuffer = 0
-        typeKernel, sigmaKernel = 'gauss', 6.5
-        typeKDist, sigmaDist = 'gauss', 2.5
+        # (name, sigma[, order]) tuples as in base/pointSetMatching.py:159-185
+        typeKernel, sigmaKernel, orderKernel = 'gauss', 6.5, 3
+        typeKDist, sigmaDist, orderKDist = 'gauss', 2.5, 3
         if isinstance(o['KparDiff'], (list, tuple)):
             typeKernel, sigmaKernel = o['KparDiff'][0], o['KparDiff'][1]
+            if typeKernel == 'laplacian' and len(o['KparDiff']) > 2:
+                orderKernel = o['KparDiff'][2]
             o['KparDiff'] = None
         if o['KparDiff'] is None:
-            o['KparDiff'] = Kernel(name=typeKernel, sigma=sigmaKernel)
+            o['KparDiff'] = Kernel(name=typeKernel, sigma=sigmaKernel, order=orderKernel)
         if isinstance(o['KparDist'], (list, tuple)):
             typeKDist, sigmaDist = o['KparDist'][0], o['KparDist'][1]
+            if typeKDist == 'laplacian' and len(o['KparDist']) > 2:
+                orderKDist = o['KparDist'][2]
             o['KparDist'] = None
         if o['KparDist'] is None:
-            o['KparDist'] = Kernel(name=typeKDist, sigma=sigmaDist)
+            o['KparDist'] = Kernel(name=typeKDist, sigma=sigmaDist, order=orderKDist)
         o['KparDiff'].pk_dtype = o['pk_dtype']
         o['KparDist'].pk_dtype = o['pk_dtype']
+        if o['affineKernel']:
+            # base/pointSetMatching.py:193-210: the affine / rigid part moves
+            # into the deformation kernel with weights 1/w, centred on the template
+            if o['affine'] == 'euclidean' and o['rotWeight'] is not None:
+                w1 = o['rotWeight']
+            else:
+                w1 = o['affineWeight']
+            w2 = o['transWeight'] if o['transWeight'] is not None else o['affineWeight']
+            o['KparDiff'].setAffine(o['affine'], w1=1 / w1, w2=1 / w2,
+                                    center=self.fv0.vertices.mean(axis=0))
+            o['affine'] = 'none'
         self.options['epsInit'] /= self.fv0.vertices.shape[0]
 
     def set_fun(self, errorType):
@@ -191,7 +211,7 @@ class SurfaceMatching:
     # ----------------------------------------------------------- data term
     def _dist_kd(self):
         K = self.options['KparDist']
-        return B.kernel_for(K.name, K.sigma, K.order, K.ms_exponent, K.pk_dtype)
+        return B.kernel_of(K)
 
     def fun_obj0(self, fv1):
         K = self.options['KparDist']

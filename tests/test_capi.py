@@ -184,3 +184,29 @@ def test_population_schedule_covers_every_tile_chunk_within_bounds(lib):
                                                            G, ctypes.byref(n))
                     assert rc == B.OK, (trial, fam, R, f10, G, B.LIB.lddmm_last_error())
                     assert n.value >= 0
+
+
+def test_affine_descriptor_checks(lib):
+    """The affine part is accepted by the first-order shooting entry points
+    only; kinds and weights are checked before any device call."""
+    B = lib
+    null = ctypes.c_void_p(0)
+    kd = B.kernel_desc([1.0])
+    kd.affine = B.AFFINE_EUCLIDEAN
+    kd.w1, kd.w2 = 0.5, 0.1
+    assert B.LIB.lddmm_applyK(kd, null, 1, null, 1, 3, null, 3, null, null) == B.EUNSUPPORTED
+    assert b"affine" in B.LIB.lddmm_last_error()
+    assert B.LIB.lddmm_epdiff_forward(kd, 3, 4, 3, null, null, null) == B.EUNSUPPORTED
+    assert B.LIB.lddmm_shoot_forward(kd, null, 3, 0, 3, null, 1.0, None, null) == B.OK      # n == 0
+    kd.affine = 5
+    assert B.LIB.lddmm_shoot_forward(kd, null, 3, 4, 3, null, 1.0, None, null) == B.EINVAL
+    kd.affine = B.AFFINE_FULL
+    kd.w1 = float('nan')
+    assert B.LIB.lddmm_shoot_forward(kd, null, 3, 4, 3, null, 1.0, None, null) == B.EINVAL
+    from lddmm_amd.kernelFunctions import Kernel
+    K = Kernel(name='gauss', sigma=1.0)
+    K.setAffine('euclidean', w1=2.0, w2=0.5, center=[1.0, 2.0, 3.0])
+    kd = B.kernel_of(K, with_affine=True)
+    assert kd.affine == B.AFFINE_EUCLIDEAN and kd.w1 == 2.0 and list(kd.center) == [1.0, 2.0, 3.0]
+    with pytest.raises(NotImplementedError):
+        B.kernel_of(K)

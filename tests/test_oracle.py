@@ -576,3 +576,59 @@ def test_laplacian_second_order_gradient_fd(order):
                                 kernel=kern)[0]
     d = rnd(10, 3)
     np.testing.assert_allclose((da0 * d).sum(), _fd(J, a0, d), rtol=1e-6)
+
+
+# ------------------------------------------------------- affine kernel part
+KAFF = [('affine', 0.7, 0.4, np.array([0.3, -0.2, 0.5])), ('euclidean', 1.3, 0.6, np.array([0.1, 0.4, -0.3]))]
+
+
+@pytest.mark.parametrize("kaff", KAFF, ids=['affine', 'euclidean'])
+def test_affine_diffKT_modes_are_gradients(kaff):
+    """plain: d/dy <p, A(y, x) a>; sym: d/dx <p, A(x, x) a>; lddmm: d/dx of
+    <p, A(x, x) a> - r <a, A(x, x) a>, A the affine part (all by finite
+    differences)."""
+    x, p, a, dx = rnd(14, 3), rnd(14, 3), rnd(14, 3), rnd(14, 3)
+    y = rnd(14, 3)
+    fd = _fd(lambda yy: (p * O.affineApplyK(yy, x, a, kaff)).sum(), y, dx)
+    # plain mode: the reference evaluates xx at the sources only, so the
+    # gradient wrt targets is p_i (a^T xx) (independent of y)
+    np.testing.assert_allclose((O.affineApplyDiffKT(x, p, a, kaff) * dx).sum(), fd, rtol=1e-7)
+    fd = _fd(lambda xx: (p * O.affineApplyK(xx, xx, a, kaff)).sum(), x, dx)
+    np.testing.assert_allclose((O.affineApplyDiffKT(x, p, a, kaff, sym=True) * dx).sum(), fd, rtol=1e-7)
+    r = 0.35
+    f = lambda xx: (p * O.affineApplyK(xx, xx, a, kaff)).sum() - r * (a * O.affineApplyK(xx, xx, a, kaff)).sum()
+    np.testing.assert_allclose((O.affineApplyDiffKT(x, p, a, kaff, regweight=r, lddmm=True) * dx).sum(),
+                               _fd(f, x, dx), rtol=1e-7)
+
+
+def test_affine_parts_closed_forms():
+    """'affine' is linear in y - c plus the column sum; 'euclidean' is an
+    infinitesimal rotation (antisymmetric linear map) plus the column sum."""
+    x, a, y = rnd(20, 3), rnd(20, 3), rnd(9, 3)
+    c = np.array([0.2, -0.1, 0.3])
+    za = O.affineApplyK(y, x, a, ('affine', 1.0, 0.0, c))
+    Mf = (x - c).T @ a
+    np.testing.assert_allclose(za, (y - c) @ Mf, rtol=1e-13)
+    ze = O.affineApplyK(y, x, a, ('euclidean', 1.0, 0.0, c))
+    L = np.linalg.lstsq(y - c, ze, rcond=None)[0]       # ze = (y - c) L
+    np.testing.assert_allclose(L, -L.T, atol=1e-12)
+    np.testing.assert_allclose(L, (Mf - Mf.T) / 2, atol=1e-12)
+    zt = O.affineApplyK(y, x, a, ('euclidean', 0.0, 2.5, c))
+    np.testing.assert_allclose(zt, np.broadcast_to(2.5 * a.sum(0), zt.shape), rtol=1e-14)
+    # 2-D Euclidean basis: one rotation generator
+    x2, a2, y2 = rnd(10, 2), rnd(10, 2), rnd(5, 2)
+    z2 = O.affineApplyK(y2, x2, a2, ('euclidean', 1.0, 0.0, np.zeros(2)))
+    M2 = x2.T @ a2
+    np.testing.assert_allclose(z2, y2 @ ((M2 - M2.T) / 2), rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("kaff", KAFF, ids=['affine', 'euclidean'])
+def test_affine_kernel_matching_gradient_fd(spheres, kaff):
+    A, Bs = spheres
+    v, f = A
+    at = rnd(3, v.shape[0], 3, scale=0.05)
+    d = rnd(*at.shape)
+    kaff = (kaff[0], kaff[1] * 1e-3, kaff[2] * 1e-3, kaff[3])
+    obj = lambda aa: O.matching_objective(v, f, aa, Bs, [1.5], [0.8], regweight=0.5, kaff=kaff)[0]
+    g = O.matching_gradient(v, f, at, Bs, [1.5], [0.8], regweight=0.5, euclidean=True, kaff=kaff)
+    np.testing.assert_allclose((g * d).sum(), _fd(obj, at, d), rtol=1e-6)
