@@ -201,6 +201,31 @@ typedef struct lddmm_surface {
 int lddmm_surface_adjacency(const int32_t* faces_host, int64_t nf, int64_t nv,
                             int32_t* adj_ptr_host, int32_t* adj_idx_host);
 
+/* Internal (regularity) costs of a velocity field on a surface
+ * (base/surfaceDistances.py:631-872): h1 = normGrad (discrete Dirichlet
+ * energy), elastic = elasticNorm, displacement = normDisplacement; a cost
+ * list holds up to LDDMM_MAX_ICOSTS (kind, weight) entries, as
+ * SurfaceMatching's internalCost option (base/surfaceMatching.py:184-204). */
+#define LDDMM_IC_H1           0
+#define LDDMM_IC_ELASTIC      1
+#define LDDMM_IC_DISPLACEMENT 2
+#define LDDMM_MAX_ICOSTS      4
+typedef struct lddmm_internal_cost {
+    int32_t ncost;                       /* 1 .. LDDMM_MAX_ICOSTS */
+    int32_t kinds[LDDMM_MAX_ICOSTS];     /* LDDMM_IC_* */
+    double  weights[LDDMM_MAX_ICOSTS];
+} lddmm_internal_cost;
+
+/* sum_k w_k C_k(s, phi) for the field phi [nv,3] at s's vertices:
+ * energy (device f64[1], may be NULL), grad_phi / grad_x (device [nv,3], may
+ * be NULL) receive sum_k w_k dC_k/dphi and dC_k/dx (the reference's
+ * diffNormGrad / diffElasticNorm / diffNormDisplacement 'phi' and 'x'
+ * entries); gradients need s's adjacency.  Replaces
+ * SurfaceMatching.internalCost__ / internalCostGrad__
+ * (base/surfaceMatching.py:156-173). */
+int lddmm_surface_internal_cost(const lddmm_surface* s, const double* phi, const lddmm_internal_cost* ic,
+                                double* energy, double* grad_phi, double* grad_x, void* stream);
+
 /* Varifold data term between surfaces (orientation-free, f0(u) = 1 + u^2):
  *   tgt == NULL : *energy = |S1|^2                       (varifoldNorm0)
  *   tgt != NULL : *energy = |S1|^2 - 2 <S1, tgt>           (varifoldNormDef)
@@ -287,6 +312,21 @@ int lddmm_shoot_adjoint(const lddmm_kernel* kd, const double* at, const double* 
                         int32_t T, int64_t n, int32_t dim, const double* px1,
                         double regweight, int32_t euclidean, double* pxt, double* grad,
                         void* stream);
+
+/* The same two loops with internal costs on the template's triangulation
+ * (faces and adjacency of `s`; its vertices are not read; dim 3, n = s->nv):
+ * forward: energy += sum_t dt C(x_t, v_t), v_t = K(x_t) a_t
+ *   (base/surfaceMatching.py:549-553);
+ * adjoint: with Lv, DLv = dC/dphi, dC/dx at (x_t, v_t),
+ *   p_t = p_{t+1} + dt (applyDiffKT(x_t, p_{t+1} - Lv, a_t, lddmm, r) - DLv),
+ *   grad_t = K(x_t)(2 r a_t - p_{t+1} + Lv) / T (euclidean) or without K
+ *   (base/surfaceMatching.py:787-804, :860-891). */
+int lddmm_shoot_forward_ic(const lddmm_kernel* kd, const lddmm_surface* s, const lddmm_internal_cost* ic,
+                           const double* at, int32_t T, double* xt, double regweight, double* energy,
+                           void* stream);
+int lddmm_shoot_adjoint_ic(const lddmm_kernel* kd, const lddmm_surface* s, const lddmm_internal_cost* ic,
+                           const double* at, const double* xt, int32_t T, const double* px1, double regweight,
+                           int32_t euclidean, double* pxt, double* grad, void* stream);
 
 /* Geodesic (EPDiff) shooting: x_{k+1} = x_k + dt K a_k, a_{k+1} = a_k - dt applyDiffKT(x_k,a_k,a_k)
  * for k < T; xt, at [T+1,n,dim] with xt[0], at[0] read.

@@ -32,6 +32,12 @@ How each function is pinned (tests/test_oracle.py):
                   sym and lddmm covectors (both kinds); the Euclidean part is
                   a rotation field (antisymmetric linear map) plus a
                   translation; shooting with them: FD of the matching objective
+  internal costs  finite differences of each cost in phi and in the vertices
+                  (h1, elastic, displacement); h1 of a linear field on a
+                  triangle = area * |gradient|^2; elastic vanishes on rigid
+                  (translation + in-plane rotation) fields; displacement of a
+                  constant field = |c|^2 * area; FD of the matching objective
+                  with internal costs through the extra-term adjoint
   laplacian       (every kernel sum above takes kernel=('laplacian', order))
                   profile = the Matern nu = order + 1/2 closed form written
                   with factorials; first-derivative sums = finite differences
@@ -46,7 +52,7 @@ import numpy as np
 
 __all__ = [
     "applyK", "applyDiffKT", "applyktensor", "applykmat", "applykdiffmat",
-    "applyDiffK", "applyDDiffK", "affineApplyK", "affineApplyDiffKT",
+    "applyDiffK", "applyDDiffK", "affineApplyK", "affineApplyDiffKT", "internal_cost", "internal_costs",
     "surface_geometry", "surface_volume",
     "varifoldNorm0", "varifoldNormDef", "varifoldNorm", "varifoldNormGradient",
     "currentNorm0", "currentNormDef", "currentNorm", "currentNormGradient",
@@ -749,6 +755,92 @@ def landmarkHamiltonianGradient(x0, at, px1, sigma, regweight, KP=-1, kernel=GAU
     return dat, xt, pxt
 
 
+# ---------------------------------------------------------------------------
+# Internal (regularity) costs of a velocity field phi on a triangulated
+# surface (base/surfaceDistances.py:631-872): 'h1' (normGrad / diffNormGrad),
+# 'elastic' (elasticNorm / diffElasticNorm), 'displacement'
+# (normDisplacement / diffNormDisplacement).  Each returns the cost, the
+# derivative in phi and the derivative in the vertices, in the reference's
+# formulas.
+# ---------------------------------------------------------------------------
+
+def _ic_face_terms(x, f, phi):
+    v1, v2, v3 = x[f[:, 0]], x[f[:, 1]], x[f[:, 2]]
+    p1, p2, p3 = phi[f[:, 0]], phi[f[:, 1]], phi[f[:, 2]]
+    S = np.cross(v2 - v1, v3 - v1) / 2
+    A = np.sqrt((S ** 2).sum(1))
+    l1 = ((v2 - v3) ** 2).sum(1)
+    l2 = ((v1 - v3) ** 2).sum(1)
+    l3 = ((v1 - v2) ** 2).sum(1)
+    return v1, v2, v3, p1, p2, p3, S, A, l1, l2, l3
+
+
+def _dot(a, b):
+    return (a * b).sum(1)
+
+
+def internal_cost(kind, vertices, faces, phi):
+    """(cost, d cost / d phi, d cost / d vertices) for kind in 'h1', 'elastic',
+    'displacement' (base/surfaceDistances.py:631-645, :680-720, :731-777,
+    :779-872)."""
+    x, f, phi = _f64(vertices), np.asarray(faces), _f64(phi)
+    v1, v2, v3, p1, p2, p3, S, A, l1, l2, l3 = _ic_face_terms(x, f, phi)
+    nv = x.shape[0]
+    if kind == 'displacement':
+        pc = (p1 + p2 + p3) / 3
+        cost = ((pc ** 2).sum(1) * A).sum()
+        g = pc * A[:, None] * (2 / 3)
+        gphi = _scatter_faces(nv, f, (g, g, g))
+        nphi = ((pc ** 2).sum(1) / A)[:, None]
+        gx = _scatter_faces(nv, f, (np.cross(v2 - v3, S) * nphi, np.cross(v3 - v1, S) * nphi,
+                                    np.cross(v1 - v2, S) * nphi)) / 2
+        return cost, gphi, gx
+    if kind not in ('h1', 'elastic'):
+        raise ValueError(kind)
+    q1 = _dot(p2 - p1, p3 - p1)          # (phi2 - phi1).(phi3 - phi1)
+    q2 = _dot(p3 - p2, p1 - p2)
+    q3 = _dot(p1 - p3, p2 - p3)
+    u = l1 * q1 + l2 * q2 + l3 * q3
+    a = 2 * A
+    a2 = 2 * a
+    elastic = kind == 'elastic'
+    corr = _dot(v1 - v3, p1 - p2) - _dot(v1 - v2, p1 - p3) if elastic else np.zeros_like(u)
+    ue = u - 0.5 * corr ** 2
+    cost = (ue / (4 * A)).sum()
+    r1 = (l1[:, None] * (p2 + p3 - 2 * p1) + (l2 - l3)[:, None] * (p2 - p3)) / a2[:, None]
+    r2 = (l2[:, None] * (p1 + p3 - 2 * p2) + (l1 - l3)[:, None] * (p1 - p3)) / a2[:, None]
+    r3 = (l3[:, None] * (p1 + p2 - 2 * p3) + (l2 - l1)[:, None] * (p2 - p1)) / a2[:, None]
+    if elastic:
+        c_a = (corr / a)[:, None]
+        r1 = r1 + 0.5 * (v2 - v3) * c_a
+        r2 = r2 + 0.5 * (v3 - v1) * c_a
+        r3 = r3 + 0.5 * (v1 - v2) * c_a
+    gphi = -_scatter_faces(nv, f, (r1, r2, r3))
+    uu = (ue / a ** 3)[:, None]
+    x1 = -uu * np.cross(v2 - v3, S) + 2 * ((v1 - v3) * q2[:, None] + (v1 - v2) * q3[:, None]) / a2[:, None]
+    x2 = -uu * np.cross(v3 - v1, S) + 2 * ((v2 - v1) * q3[:, None] + (v2 - v3) * q1[:, None]) / a2[:, None]
+    x3 = -uu * np.cross(v1 - v2, S) + 2 * ((v3 - v2) * q1[:, None] + (v3 - v1) * q2[:, None]) / a2[:, None]
+    if elastic:
+        c_a = (corr / a)[:, None]
+        x1 = x1 - 0.5 * (p3 - p2) * c_a
+        x2 = x2 - 0.5 * (p1 - p3) * c_a
+        x3 = x3 - 0.5 * (p2 - p1) * c_a
+    gx = _scatter_faces(nv, f, (x1, x2, x3))
+    return cost, gphi, gx
+
+
+def internal_costs(icost, vertices, faces, phi):
+    """sum_k w_k C_k over icost = [(kind, w_k), ...] (SurfaceMatching.internalCost__ /
+    internalCostGrad__, base/surfaceMatching.py:156-173)."""
+    c, gp, gx = 0.0, np.zeros(np.shape(phi)), np.zeros(np.shape(vertices))
+    for kind, w in icost:
+        ck, gpk, gxk = internal_cost(kind, vertices, faces, phi)
+        c += w * ck
+        gp += w * gpk
+        gx += w * gxk
+    return c, gp, gx
+
+
 def _data_term(kind, fvDef, fv1, sigma_dist):
     if kind == 'varifold':
         return varifoldNormDef(fvDef, fv1, sigma_dist), varifoldNorm0(fv1, sigma_dist)
@@ -782,8 +874,31 @@ def _data_grad(kind, fvDef, fv1, sigma_dist):
     raise ValueError(kind)
 
 
+def _internal_cost_gradient(xt, at, px1, faces, sigma, regweight, kernel, kaff, icost):
+    """Covector and Hamiltonian gradient with internal costs: with
+    v = K(x_t) a_t, Lv / DLv the cost's derivatives in phi / x at (x_t, v),
+    p_t = p_{t+1} + dt (applyDiffKT(x_t, p_{t+1} - Lv, a_t, lddmm) - DLv) and
+    dat_t = 2 r a_t - p_{t+1} + Lv (base/surfaceMatching.py:787-804, :860-885)."""
+    T = at.shape[0]
+    dt = 1.0 / T
+    pxt = np.zeros(xt.shape)
+    pxt[T] = px1
+    dat = np.zeros(at.shape)
+    for t in range(T - 1, -1, -1):
+        z, a, px = xt[t], at[t], pxt[t + 1]
+        v = _kv(z, a, sigma, -1, kernel, kaff)
+        _, Lv, DLv = internal_costs(icost, z, faces, v)
+        pe = px - Lv
+        zpx = applyDiffKT(z, z, pe, a, sigma, regweight=regweight, lddmm=True, kernel=kernel)
+        if kaff is not None:
+            zpx += affineApplyDiffKT(z, pe, a, kaff, regweight=regweight, lddmm=True)
+        pxt[t] = px + dt * (zpx - DLv)
+        dat[t] = 2 * regweight * a - px + Lv
+    return dat, pxt
+
+
 def matching_objective(x0, faces, at, target, sigma_diff, sigma_dist, kind='varifold',
-                       regweight=1., sigma_error=1., kernel_diff=GAUSS, kaff=None):
+                       regweight=1., sigma_error=1., kernel_diff=GAUSS, kaff=None, icost=None):
     """Surface- or curve-matching objective at control `at` (plain, no affine, ds = 1).
     kind: 'varifold' / 'current' / 'measure' (surfaces), 'curve_varifold' /
     'curve_current' (curves, base/curveMatching.py:272-313 has the same
@@ -802,12 +917,15 @@ def matching_objective(x0, faces, at, target, sigma_diff, sigma_dist, kind='vari
     for t in range(T):
         ra = _kv(xt[t], at[t], sigma_diff, -1, kernel_diff, kaff)
         obj += regweight * dt * (at[t] * ra).sum()
+        if icost:                              # base/surfaceMatching.py:551-553
+            obj += internal_costs(icost, xt[t], faces, ra)[0] * dt
     dterm, obj0 = _data_term(kind, (xt[-1], faces), target, sigma_dist)
     return obj + (obj0 + dterm) / sigma_error ** 2, xt
 
 
 def matching_gradient(x0, faces, at, target, sigma_diff, sigma_dist, kind='varifold',
-                      regweight=1., sigma_error=1., euclidean=True, kernel_diff=GAUSS, kaff=None):
+                      regweight=1., sigma_error=1., euclidean=True, kernel_diff=GAUSS, kaff=None,
+                      icost=None):
     """grad['at'] of the plain surface-matching objective as BFGS sees it.
 
     px1 = -endPointGradient/sigma_err^2 (base/surfaceMatching.py:702-721,
@@ -818,8 +936,11 @@ def matching_gradient(x0, faces, at, target, sigma_diff, sigma_dist, kind='varif
     T = at.shape[0]
     xt = landmarkDirectEvolutionEuler(x0, at, sigma_diff, kernel=kernel_diff, kaff=kaff)
     px1 = -_data_grad(kind, (xt[-1], faces), target, sigma_dist) / sigma_error ** 2
-    dat, xt, pxt = landmarkHamiltonianGradient(x0, at, px1, sigma_diff, regweight, kernel=kernel_diff,
-                                               kaff=kaff)
+    if icost:
+        dat, pxt = _internal_cost_gradient(xt, at, px1, faces, sigma_diff, regweight, kernel_diff, kaff, icost)
+    else:
+        dat, xt, pxt = landmarkHamiltonianGradient(x0, at, px1, sigma_diff, regweight, kernel=kernel_diff,
+                                                   kaff=kaff)
     if euclidean:
         for t in range(T):
             dat[t] = _kv(xt[t], dat[t], sigma_diff, -1, kernel_diff, kaff)

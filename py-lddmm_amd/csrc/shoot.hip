@@ -6,6 +6,7 @@
 //   adjoint : base/surfaceMatching.py:728-908, base/pointEvolution.py:167-236
 //   EPDiff  : base/pointEvolution.py:385-472 (forward), :489-618 (adjoint)
 #include "ops.hpp"
+#include "icost.hpp"
 
 #include <algorithm>
 
@@ -236,26 +237,62 @@ int reduce_job(const Job<T, OP>& j, Partials<T> part, int n, int dim, double sca
                       alpha, st);
 }
 
+// Internal costs on the template's triangulation (shooting with internalCost).
+struct IcCtx {
+    IcList L;
+    const int32_t* faces;
+    int nf;
+    const int32_t* adj_ptr;
+    const int32_t* adj_idx;
+};
+
+// x_next = x + dt v; dots[block] = dscale * sum_i a_i . v_i   (v given)
+__global__ void __launch_bounds__(256)
+k_shoot_epi_v(int64_t count, const double* __restrict__ x, const double* __restrict__ a,
+              const double* __restrict__ v, double dt, double* __restrict__ xn, double* __restrict__ dots,
+              double dscale) {
+    __shared__ double red[256];
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    double d = 0.0;
+    if (i < count) {
+        xn[i] = x[i] + dt * v[i];
+        d = a[i] * v[i];
+    }
+    red[threadIdx.x] = d;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dots[blockIdx.x] = dscale * red[0];
+}
+
 // ---------------------------------------------------------------------------
 template <typename T, bool MS, int KW>
 int shoot_forward_t(const lddmm_kernel* kd, const double* at, int T_, int n, double* xt, double r,
-                    double* energy, hipStream_t st) {
+                    double* energy, const IcCtx* ic, hipStream_t st) {
     ScaleSet<T> sc = make_scales<T>(kd, nullptr);
     using OPK = OpApplyK<KW, MS>;
     Job<T, OPK> jk;
     LDDMM_TRY(plan_job(jk, n, n, !MS));
     const int nb = (n + 255) / 256;
+    const int nbv = int((int64_t(n) * KW + 255) / 256);       // blocks of k_shoot_epi_v
+    const int ndot = ic ? nbv + 1 : nb + 1;                     // dot slots per step
     void* ws;
     const bool aff = kd->affine != LDDMM_AFFINE_NONE;
     const AffineArgs AA = affine_args(kd);
-    LDDMM_TRY(workspace(256 + partial_bytes<T>(jk.plan, KW, n) + size_t(T_) * (nb + 1) * sizeof(double) +
-                            (aff ? kAffBytes + 256 : 0) + 512,
+    const size_t icb = ic ? (size_t(n) * KW + ic_scratch_doubles(ic->nf, false)) * sizeof(double) + 512 : 0;
+    LDDMM_TRY(workspace(256 + partial_bytes<T>(jk.plan, KW, n) + size_t(T_) * ndot * sizeof(double) +
+                            (aff ? kAffBytes + 256 : 0) + icb + 512,
                         st, &ws));
     char* base = static_cast<char*>(ws);
     Carve cv;
     double* c0 = reinterpret_cast<double*>(base + cv.take(32));
-    double* dots = reinterpret_cast<double*>(base + cv.take(size_t(T_) * (nb + 1) * sizeof(double)));
+    double* dots = reinterpret_cast<double*>(base + cv.take(size_t(T_) * ndot * sizeof(double)));
     double* aws = aff ? reinterpret_cast<double*>(base + cv.take(kAffBytes)) : nullptr;
+    double* vbuf = ic ? reinterpret_cast<double*>(base + cv.take(size_t(n) * KW * sizeof(double))) : nullptr;
+    double* icws = ic ? reinterpret_cast<double*>(base + cv.take(ic_scratch_doubles(ic->nf, false) *
+                                                                 sizeof(double))) : nullptr;
     Partials<T> part = carve_partials<T>(base, cv, jk.plan, KW, n);
     const double dt = 1.0 / T_;
     const size_t stride = size_t(n) * KW;
@@ -267,16 +304,31 @@ int shoot_forward_t(const lddmm_kernel* kd, const double* at, int T_, int n, dou
         const double* a = at + t * stride;
         jk.args = {x, x, a, c0, KW, KW, 0, sc};
         LDDMM_TRY(run_job(jk, n, n, part, st));
-        hipLaunchKernelGGL((k_shoot_epi<T, KW>), dim3(unsigned(nb)), dim3(256), 0, st, part, jk.plan.nchunk, n, x,
-                           a, dt, xt + (t + 1) * stride, dots + size_t(t) * nb, r * dt);
-        LDDMM_CHECK_LAUNCH();
-        if (aff) {
-            LDDMM_TRY(aff_moments(AA, n, KW, x, a, nullptr, aws, dots + size_t(T_) * nb + t, r * dt, st));
-            LDDMM_TRY(aff_apply(AA, n, KW, 0, 0.0, aws, x, nullptr, nullptr, dt, xt + (t + 1) * stride, st));
+        if (!ic) {
+            hipLaunchKernelGGL((k_shoot_epi<T, KW>), dim3(unsigned(nb)), dim3(256), 0, st, part, jk.plan.nchunk, n,
+                               x, a, dt, xt + (t + 1) * stride, dots + size_t(t) * nb, r * dt);
+            LDDMM_CHECK_LAUNCH();
+            if (aff) {
+                LDDMM_TRY(aff_moments(AA, n, KW, x, a, nullptr, aws, dots + size_t(T_) * nb + t, r * dt, st));
+                LDDMM_TRY(aff_apply(AA, n, KW, 0, 0.0, aws, x, nullptr, nullptr, dt, xt + (t + 1) * stride, st));
+            }
+        } else {
+            // v_t = K(x_t) a_t (with the affine part) is needed by the cost: materialise it
+            LDDMM_TRY(reduce_job(jk, part, n, KW, 1.0, vbuf, 0, nullptr, 1.0, st));
+            if (aff) {
+                LDDMM_TRY(aff_moments(AA, n, KW, x, a, nullptr, aws, nullptr, 0.0, st));
+                LDDMM_TRY(aff_apply(AA, n, KW, 0, 0.0, aws, x, nullptr, nullptr, 1.0, vbuf, st));
+            }
+            hipLaunchKernelGGL(k_shoot_epi_v, dim3(unsigned(nbv)), dim3(256), 0, st, int64_t(n) * KW, x, a, vbuf, dt,
+                               xt + (t + 1) * stride, dots + size_t(t) * nbv, r * dt);
+            LDDMM_CHECK_LAUNCH();
+            LDDMM_TRY(ic_eval(ic->L, n, ic->nf, ic->faces, ic->adj_ptr, ic->adj_idx, x, vbuf, dt,
+                              dots + size_t(T_) * nbv + t, 0.0, nullptr, nullptr, icws, st));
         }
     }
     if (energy) {
-        hipLaunchKernelGGL(k_sum_doubles, dim3(1), dim3(1024), 0, st, dots, T_ * (aff ? nb + 1 : nb), energy);
+        const int count = ic ? T_ * (nbv + 1) : T_ * (aff ? nb + 1 : nb);
+        hipLaunchKernelGGL(k_sum_doubles, dim3(1), dim3(1024), 0, st, dots, count, energy);
         LDDMM_CHECK_LAUNCH();
     }
     return LDDMM_OK;
@@ -284,7 +336,8 @@ int shoot_forward_t(const lddmm_kernel* kd, const double* at, int T_, int n, dou
 
 template <typename T, bool MS, int KW>
 int shoot_adjoint_t(const lddmm_kernel* kd, const double* at, const double* xt, int T_, int n,
-                    const double* px1, double r, int euclidean, double* pxt, double* grad, hipStream_t st) {
+                    const double* px1, double r, int euclidean, double* pxt, double* grad, const IcCtx* ic,
+                    hipStream_t st) {
     SingleScaleConsts ss;
     ScaleSet<T> sc = make_scales<T>(kd, &ss);
     const double fdiff = MS ? -1.0 : ss.diff;
@@ -297,14 +350,24 @@ int shoot_adjoint_t(const lddmm_kernel* kd, const double* at, const double* xt, 
     const size_t stride = size_t(n) * KW;
     const bool aff = kd->affine != LDDMM_AFFINE_NONE;
     const AffineArgs AA = affine_args(kd);
+    const size_t icb = ic ? (4 * stride + ic_scratch_doubles(ic->nf, true)) * sizeof(double) + 1280 : 0;
     size_t need = 256 + std::max(partial_bytes<T>(jd.plan, 3, n), partial_bytes<T>(jk.plan, KW, n)) +
-                  stride * sizeof(double) * (pxt ? 1 : 3) + (aff ? kAffBytes + 256 : 0) + 1024;
+                  stride * sizeof(double) * (pxt ? 1 : 3) + (aff ? kAffBytes + 256 : 0) + icb + 1024;
     void* ws;
     LDDMM_TRY(workspace(need, st, &ws));
     char* base = static_cast<char*>(ws);
     Carve cv;
     double* c0 = reinterpret_cast<double*>(base + cv.take(32));
     double* aws = aff ? reinterpret_cast<double*>(base + cv.take(kAffBytes)) : nullptr;
+    // internal costs: v_t, Lv, DLv, p_{t+1} - Lv and the cost's scratch
+    double *vbuf = nullptr, *lv = nullptr, *dlv = nullptr, *pe = nullptr, *icws = nullptr;
+    if (ic) {
+        vbuf = reinterpret_cast<double*>(base + cv.take(stride * sizeof(double)));
+        lv = reinterpret_cast<double*>(base + cv.take(stride * sizeof(double)));
+        dlv = reinterpret_cast<double*>(base + cv.take(stride * sizeof(double)));
+        pe = reinterpret_cast<double*>(base + cv.take(stride * sizeof(double)));
+        icws = reinterpret_cast<double*>(base + cv.take(ic_scratch_doubles(ic->nf, true) * sizeof(double)));
+    }
     double* dat = reinterpret_cast<double*>(base + cv.take(stride * sizeof(double)));
     double* pbuf[2] = {nullptr, nullptr};
     if (!pxt) {
@@ -321,16 +384,34 @@ int shoot_adjoint_t(const lddmm_kernel* kd, const double* at, const double* xt, 
         const double* a = at + size_t(t) * stride;
         double* pn = P(t + 1);
         double* pc = P(t);
+        // p used by the covector sum: p_{t+1}, or p_{t+1} - Lv with internal costs
+        const double* pu = pn;
+        if (ic) {
+            Carve c1; c1.off = poff;
+            Partials<T> partV = carve_partials<T>(base, c1, jk.plan, KW, n);
+            jk.args = {x, x, a, c0, KW, KW, 0, sc};
+            LDDMM_TRY(run_job(jk, n, n, partV, st));
+            LDDMM_TRY(reduce_job(jk, partV, n, KW, 1.0, vbuf, 0, nullptr, 1.0, st));
+            if (aff) {
+                LDDMM_TRY(aff_moments(AA, n, KW, x, a, nullptr, aws, nullptr, 0.0, st));
+                LDDMM_TRY(aff_apply(AA, n, KW, 0, 0.0, aws, x, nullptr, nullptr, 1.0, vbuf, st));
+            }
+            LDDMM_TRY(ic_eval(ic->L, n, ic->nf, ic->faces, ic->adj_ptr, ic->adj_idx, x, vbuf, 0.0, nullptr, 1.0, lv,
+                              dlv, icws, st));
+            LDDMM_TRY(axpby(stride, 1.0, pn, -1.0, lv, pe, st));
+            pu = pe;
+        }
         Carve c2; c2.off = poff;
         Partials<T> partD = carve_partials<T>(base, c2, jd.plan, 3, n);
-        jd.args = {x, x, pn, a, c0, KW, KW, 0, 2.0 * r, sc};
+        jd.args = {x, x, pu, a, c0, KW, KW, 0, 2.0 * r, sc};
         LDDMM_TRY(run_job(jd, n, n, partD, st));
-        // dat = 2 r a_t - p_{t+1}  (read before p_t overwrites the other buffer)
-        LDDMM_TRY(axpby(stride, 2.0 * r, a, -1.0, pn, dat, st));
+        // dat = 2 r a_t - p_{t+1} (+ Lv)  (read before p_t overwrites the other buffer)
+        LDDMM_TRY(axpby(stride, 2.0 * r, a, -1.0, pu, dat, st));
         LDDMM_TRY(reduce_job(jd, partD, n, KW, fdiff, pc, 0, pn, dt, st));
+        if (ic) LDDMM_TRY(axpby(stride, 1.0, pc, -dt, dlv, pc, st));
         if (aff) {
-            LDDMM_TRY(aff_moments(AA, n, KW, x, a, pn, aws, nullptr, 0.0, st));
-            LDDMM_TRY(aff_apply(AA, n, KW, 1, r, aws, x, pn, a, dt, pc, st));
+            LDDMM_TRY(aff_moments(AA, n, KW, x, a, pu, aws, nullptr, 0.0, st));
+            LDDMM_TRY(aff_apply(AA, n, KW, 1, r, aws, x, pu, a, dt, pc, st));
         }
         double* g = grad + size_t(t) * stride;
         if (euclidean) {
@@ -491,7 +572,7 @@ extern "C" int lddmm_shoot_forward(const lddmm_kernel* kd, const double* at, int
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (n == 0) return energy ? launch_fill(energy, 1, 0.0, st) : LDDMM_OK;
     LDDMM_REQUIRE(at && xt, "null at / xt");
-    LDDMM_DISPATCH(shoot_forward_t, kd, at, T, int(n), xt, regweight, energy, st);
+    LDDMM_DISPATCH(shoot_forward_t, kd, at, T, int(n), xt, regweight, energy, nullptr, st);
 }
 
 extern "C" int lddmm_shoot_adjoint(const lddmm_kernel* kd, const double* at, const double* xt, int32_t T,
@@ -501,7 +582,51 @@ extern "C" int lddmm_shoot_adjoint(const lddmm_kernel* kd, const double* at, con
     if (n == 0) return LDDMM_OK;
     LDDMM_REQUIRE(at && xt && px1 && grad, "null pointer");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    LDDMM_DISPATCH(shoot_adjoint_t, kd, at, xt, T, int(n), px1, regweight, euclidean, pxt, grad, st);
+    LDDMM_DISPATCH(shoot_adjoint_t, kd, at, xt, T, int(n), px1, regweight, euclidean, pxt, grad, nullptr, st);
+}
+
+namespace {
+int ic_ctx(const lddmm_surface* s, const lddmm_internal_cost* ic, bool need_adj, IcCtx* out) {
+    LDDMM_TRY(ic_check(ic, &out->L));
+    LDDMM_REQUIRE(s != nullptr, "internal cost: null surface");
+    LDDMM_REQUIRE(s->nv >= 0 && s->nf >= 0 && s->nf < (int64_t(1) << 30) && s->nv < (int64_t(1) << 31),
+                  "internal cost: bad surface sizes");
+    LDDMM_REQUIRE(s->nf == 0 || s->faces, "internal cost: null faces");
+    LDDMM_REQUIRE(!need_adj || s->nv == 0 || (s->adj_ptr && s->adj_idx), "internal cost: adjacency required");
+    out->faces = s->faces;
+    out->nf = int(s->nf);
+    out->adj_ptr = s->adj_ptr;
+    out->adj_idx = s->adj_idx;
+    return LDDMM_OK;
+}
+}  // namespace
+
+extern "C" int lddmm_shoot_forward_ic(const lddmm_kernel* kd, const lddmm_surface* s, const lddmm_internal_cost* ic,
+                                      const double* at, int32_t T, double* xt, double regweight, double* energy,
+                                      void* stream) {
+    IcCtx C;
+    LDDMM_TRY(ic_ctx(s, ic, false, &C));
+    const int64_t n = s->nv;
+    const int dim = 3;
+    LDDMM_TRY(check_shoot(kd, T, n, dim, kValAffine));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (n == 0) return energy ? launch_fill(energy, 1, 0.0, st) : LDDMM_OK;
+    LDDMM_REQUIRE(at && xt, "null at / xt");
+    LDDMM_DISPATCH(shoot_forward_t, kd, at, T, int(n), xt, regweight, energy, &C, st);
+}
+
+extern "C" int lddmm_shoot_adjoint_ic(const lddmm_kernel* kd, const lddmm_surface* s, const lddmm_internal_cost* ic,
+                                      const double* at, const double* xt, int32_t T, const double* px1,
+                                      double regweight, int32_t euclidean, double* pxt, double* grad, void* stream) {
+    IcCtx C;
+    LDDMM_TRY(ic_ctx(s, ic, true, &C));
+    const int64_t n = s->nv;
+    const int dim = 3;
+    LDDMM_TRY(check_shoot(kd, T, n, dim, kValAffine));
+    if (n == 0) return LDDMM_OK;
+    LDDMM_REQUIRE(at && xt && px1 && grad, "null pointer");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    LDDMM_DISPATCH(shoot_adjoint_t, kd, at, xt, T, int(n), px1, regweight, euclidean, pxt, grad, &C, st);
 }
 
 extern "C" int lddmm_epdiff_forward(const lddmm_kernel* kd, int32_t T, int64_t n, int32_t dim, double* xt,

@@ -58,6 +58,31 @@ class LddmmPopulationDesc(ctypes.Structure):
                 ("tfaces", ctypes.c_void_p), ("tvweights", ctypes.c_void_p)]
 
 
+MAX_ICOSTS = 4
+IC_KINDS = {'h1': 0, 'elastic': 1, 'displacement': 2}
+
+
+class LddmmInternalCost(ctypes.Structure):
+    _fields_ = [("ncost", ctypes.c_int32), ("kinds", ctypes.c_int32 * MAX_ICOSTS),
+                ("weights", ctypes.c_double * MAX_ICOSTS)]
+
+
+def internal_cost_desc(costs):
+    """[(name, weight), ...] with name in 'h1', 'elastic', 'displacement'
+    (SurfaceMatching's internalCost list, base/surfaceMatching.py:184-204)."""
+    costs = list(costs)
+    if not 1 <= len(costs) <= MAX_ICOSTS:
+        raise ValueError(f"1..{MAX_ICOSTS} internal costs (got {len(costs)})")
+    d = LddmmInternalCost()
+    d.ncost = len(costs)
+    for k, (name, w) in enumerate(costs):
+        if name not in IC_KINDS:
+            raise NotImplementedError(f"internal cost {name!r}: lddmm_amd builds {sorted(IC_KINDS)}")
+        d.kinds[k] = IC_KINDS[name]
+        d.weights[k] = float(w)
+    return d
+
+
 class LddmmError(RuntimeError):
     pass
 
@@ -69,6 +94,7 @@ _D = ctypes.c_double
 _KP = ctypes.POINTER(LddmmKernel)
 _SP = ctypes.POINTER(LddmmSurface)
 _CP = ctypes.POINTER(LddmmCurve)
+_ICP = ctypes.POINTER(LddmmInternalCost)
 
 # name -> argtypes (restype is int for every compute entry point)
 SIGNATURES = {
@@ -89,6 +115,9 @@ SIGNATURES = {
     "lddmm_curve_seg_gather": [_CP, _P, _P, _P, _P],
     "lddmm_shoot_forward": [_KP, _P, _I32, _I64, _I32, _P, _D, _P, _P],
     "lddmm_shoot_adjoint": [_KP, _P, _P, _I32, _I64, _I32, _P, _D, _I32, _P, _P, _P],
+    "lddmm_surface_internal_cost": [_SP, _P, _ICP, _P, _P, _P, _P],
+    "lddmm_shoot_forward_ic": [_KP, _SP, _ICP, _P, _I32, _P, _D, _P, _P],
+    "lddmm_shoot_adjoint_ic": [_KP, _SP, _ICP, _P, _P, _I32, _P, _D, _I32, _P, _P, _P],
     "lddmm_epdiff_forward": [_KP, _I32, _I64, _I32, _P, _P, _P],
     "lddmm_epdiff_adjoint": [_KP, _I32, _I64, _I32, _P, _P, _P, _P, _P, _P, _P, _P],
     "lddmm_release_workspace": [],

@@ -41,6 +41,11 @@ class CurveMatching(SurfaceMatching):
         self.options['epsInit'] = eps
         self.gradEps = -1
 
+    def set_internal_cost(self):
+        self.icost = None
+        if self.options['internalCost'] is not None:
+            raise NotImplementedError("internal costs are built for surfaces only (not curves)")
+
     def initialize_variables(self):
         super().initialize_variables()
         self.fvDef = Curve(curve=self.fvInit)

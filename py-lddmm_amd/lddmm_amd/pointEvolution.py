@@ -12,6 +12,8 @@ shooting and its adjoint.  Affine controls, Jacobians, normals, passenger
 point sets and splines are not built (NotImplementedError), as in the
 configurations this build targets.
 """
+import ctypes
+
 import numpy as np
 import torch
 
@@ -38,29 +40,56 @@ def _no_affine(affine):
         raise NotImplementedError("affine controls are not built in lddmm_amd shooting")
 
 
-def shoot(x0, at, KparDiff, regweight=1.0, energy=False):
-    """Device forward flow: returns (xt [T+1,N,d] tensor, energy tensor [1] or None)."""
+def _ic(icost):
+    """icost = (surface, [(name, weight), ...]): the template's triangulation
+    and the internal-cost list."""
+    surf, costs = icost
+    sv, keep = surf.device()
+    return sv, B.internal_cost_desc(costs), keep
+
+
+def shoot(x0, at, KparDiff, regweight=1.0, energy=False, icost=None):
+    """Device forward flow: returns (xt [T+1,N,d] tensor, energy tensor [1] or None).
+    With icost = (surface, costs) the energy includes sum_t dt C(x_t, K(x_t) a_t)."""
     at_ = B.to_dev(at)
     T, n, dim = at_.shape
     xt = torch.empty((T + 1, n, dim), dtype=torch.float64, device=at_.device)
     xt[0] = B.to_dev(x0)
     e = torch.empty(1, dtype=torch.float64, device=at_.device) if energy else None
-    B.check(B.LIB.lddmm_shoot_forward(_kd(KparDiff), B.ptr(at_), T, n, dim, B.ptr(xt), float(regweight),
-                                      B.ptr(e), B.stream_handle()), "shoot_forward")
+    if icost is None:
+        B.check(B.LIB.lddmm_shoot_forward(_kd(KparDiff), B.ptr(at_), T, n, dim, B.ptr(xt), float(regweight),
+                                          B.ptr(e), B.stream_handle()), "shoot_forward")
+    else:
+        sv, d, keep = _ic(icost)
+        if sv.nv != n or dim != 3:
+            raise ValueError("internal costs need the template's surface (same vertex count, dim 3)")
+        B.check(B.LIB.lddmm_shoot_forward_ic(_kd(KparDiff), ctypes.byref(sv), ctypes.byref(d), B.ptr(at_), T,
+                                             B.ptr(xt), float(regweight), B.ptr(e), B.stream_handle()),
+                "shoot_forward_ic")
     return xt, e
 
 
-def shoot_adjoint(xt, at, px1, KparDiff, regweight, euclidean=True, covector=False):
-    """Device adjoint: returns (grad [T,N,d] (already / T), pxt or None)."""
+def shoot_adjoint(xt, at, px1, KparDiff, regweight, euclidean=True, covector=False, icost=None):
+    """Device adjoint: returns (grad [T,N,d] (already / T), pxt or None).
+    With icost = (surface, costs) the covector carries the costs' extra term."""
     at_ = B.to_dev(at)
     T, n, dim = at_.shape
     xt_ = B.to_dev(xt)
     px1_ = B.to_dev(px1)
     g = torch.empty_like(at_)
     pxt = torch.empty((T + 1, n, dim), dtype=torch.float64, device=at_.device) if covector else None
-    B.check(B.LIB.lddmm_shoot_adjoint(_kd(KparDiff), B.ptr(at_), B.ptr(xt_), T, n, dim, B.ptr(px1_),
-                                      float(regweight), 1 if euclidean else 0, B.ptr(pxt), B.ptr(g),
-                                      B.stream_handle()), "shoot_adjoint")
+    if icost is None:
+        B.check(B.LIB.lddmm_shoot_adjoint(_kd(KparDiff), B.ptr(at_), B.ptr(xt_), T, n, dim, B.ptr(px1_),
+                                          float(regweight), 1 if euclidean else 0, B.ptr(pxt), B.ptr(g),
+                                          B.stream_handle()), "shoot_adjoint")
+    else:
+        sv, d, keep = _ic(icost)
+        if sv.nv != n or dim != 3:
+            raise ValueError("internal costs need the template's surface (same vertex count, dim 3)")
+        B.check(B.LIB.lddmm_shoot_adjoint_ic(_kd(KparDiff), ctypes.byref(sv), ctypes.byref(d), B.ptr(at_),
+                                             B.ptr(xt_), T, B.ptr(px1_), float(regweight),
+                                             1 if euclidean else 0, B.ptr(pxt), B.ptr(g), B.stream_handle()),
+                "shoot_adjoint_ic")
     return g, pxt
 
 

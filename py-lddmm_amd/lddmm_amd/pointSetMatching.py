@@ -45,6 +45,11 @@ class PointSetMatching(SurfaceMatching):
         super().set_parameters()
         self.options['epsInit'] = eps
 
+    def set_internal_cost(self):
+        self.icost = None
+        if self.options['internalCost'] is not None:
+            raise NotImplementedError("internal costs are built for surfaces only (not point sets)")
+
     def initialize_variables(self):
         super().initialize_variables()
         self.fvDef = PointSet(data=self.fvInit)

@@ -3,8 +3,11 @@ base/surfaceDistances.py): varifoldNorm0 / varifoldNormDef / varifoldNorm /
 varifoldNormGradient (orientation-free varifold, f0(u) = 1 + u^2),
 currentNorm0 / currentNormDef / currentNorm / currentNormGradient /
 currentMagnitude / currentMagnitudeGradient, measureNorm0 / measureNormDef /
-measureNorm / measureNormGradient and the point-set-target forms
-measureNormPS0 / measureNormPSDef / measureNormPS / measureNormPSGradient.
+measureNorm / measureNormGradient, the point-set-target forms
+measureNormPS0 / measureNormPSDef / measureNormPS / measureNormPSGradient,
+and the internal costs of a velocity field normGrad / diffNormGrad,
+elasticNorm / diffElasticNorm, normDisplacement / diffNormDisplacement
+(lddmm_surface_internal_cost).
 
 The varifold energy and gradient are single fused GPU passes
 (lddmm_surface_varifold: face features, one pair kernel over both surfaces,
@@ -274,3 +277,69 @@ def currentMagnitudeGradient(fvDef, KparDist, with_weights=False):
     wz1 = w1[:, None] * z1
     dz1 = (2 / 3) * ku.applyDiffKT(c1, c1, wcr1, wcr1, **kw)
     return _gather(dk, dz1, wz1).cpu().numpy()
+
+
+# --------------------------------------------------------------------------
+# Internal costs of a velocity field phi on fv (base/surfaceDistances.py:631-872)
+# --------------------------------------------------------------------------
+def internal_cost_device(fv, phi, costs, energy=True, variables='both', vertices=None):
+    """sum_k w_k C_k(fv, phi) for costs = [(name, w), ...]; returns (energy
+    tensor [1] or None, {'phi': [nv,3], 'x': [nv,3]} device tensors as
+    requested by `variables` ('both', 'phi', 'x' or None)."""
+    d = B.internal_cost_desc(costs)
+    sv, keep = fv.device(vertices=vertices)
+    phi_ = B.to_dev(phi).reshape(sv.nv, 3).contiguous()
+    dev = phi_.device
+    e = torch.empty(1, dtype=torch.float64, device=dev) if energy else None
+    want_phi = variables in ('both', 'phi')
+    want_x = variables in ('both', 'x')
+    gp = torch.empty((sv.nv, 3), dtype=torch.float64, device=dev) if want_phi else None
+    gx = torch.empty((sv.nv, 3), dtype=torch.float64, device=dev) if want_x else None
+    B.check(B.LIB.lddmm_surface_internal_cost(ctypes.byref(sv), B.ptr(phi_), ctypes.byref(d), B.ptr(e),
+                                              B.ptr(gp), B.ptr(gx), B.stream_handle()), "surface_internal_cost")
+    grad = {}
+    if want_phi:
+        grad['phi'] = gp
+    if want_x:
+        grad['x'] = gx
+    return e, grad
+
+
+def _ic_norm(name, fv, phi):
+    e, _ = internal_cost_device(fv, phi, [(name, 1.0)], variables=None)
+    return float(e.item())
+
+
+def _ic_diff(name, fv, phi, variables):
+    _, g = internal_cost_device(fv, phi, [(name, 1.0)], energy=False, variables=variables)
+    return {k: v.cpu().numpy() for k, v in g.items()}
+
+
+def normGrad(fv, phi):
+    """Discrete H1 (Dirichlet) energy of phi (base/surfaceDistances.py:631-645)."""
+    return _ic_norm('h1', fv, phi)
+
+
+def diffNormGrad(fv, phi, variables='both'):
+    """base/surfaceDistances.py:680-720"""
+    return _ic_diff('h1', fv, phi, variables)
+
+
+def elasticNorm(fv, phi):
+    """base/surfaceDistances.py:779-801"""
+    return _ic_norm('elastic', fv, phi)
+
+
+def diffElasticNorm(fv, phi, variables='both'):
+    """base/surfaceDistances.py:805-856"""
+    return _ic_diff('elastic', fv, phi, variables)
+
+
+def normDisplacement(fv, phi):
+    """base/surfaceDistances.py:731-735"""
+    return _ic_norm('displacement', fv, phi)
+
+
+def diffNormDisplacement(fv, phi, variables='both'):
+    """base/surfaceDistances.py:737-767"""
+    return _ic_diff('displacement', fv, phi, variables)

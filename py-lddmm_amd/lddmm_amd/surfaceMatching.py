@@ -4,7 +4,8 @@ base/basicMatching.py for the configuration the hot path serves:
 `SurfaceMatching(Template, Target, options).optimizeMatching()` with
 errorType 'varifold', 'current' or 'measure', algorithm 'bfgs' (weak-Wolfe line search),
 affine None (or affineKernel=True with affine 'euclidean' / 'affine', which
-become terms of the deformation kernel), no internal cost, not symmetric,
+become terms of the deformation kernel), internal costs 'h1' / 'elastic' /
+'displacement' (device kernels inside the shooting loops), not symmetric,
 not unreduced.
 
 The control `at` [T, N, 3], the trajectories and the BFGS memory live on
@@ -101,8 +102,6 @@ class SurfaceMatching:
         for key in ('symmetric', 'unreduced', 'pplot'):
             if o[key]:
                 raise NotImplementedError(f"option {key}=True is not built in lddmm_amd")
-        if o['internalCost'] is not None:
-            raise NotImplementedError("internal costs are not built in lddmm_amd")
         if o['Landmarks'] is not None or o['passenger'] is not None:
             raise NotImplementedError("landmark / passenger tracking is not built in lddmm_amd")
         if o['errorType'] not in ('varifold', 'current', 'measure'):
@@ -186,6 +185,29 @@ class SurfaceMatching:
                                     center=self.fv0.vertices.mean(axis=0))
             o['affine'] = 'none'
         self.options['epsInit'] /= self.fv0.vertices.shape[0]
+        self.set_internal_cost()
+
+    def set_internal_cost(self):
+        """internalCost option (base/surfaceMatching.py:184-207): a name or a
+        list of [name, weight] with names 'h1', 'elastic', 'displacement';
+        unknown names are skipped; the list's weights apply and internalWeight
+        becomes 1.  Costs act on the template's triangulation."""
+        o = self.options
+        self.icost = None
+        if o['internalCost'] is None:
+            return
+        ic = o['internalCost']
+        ic = [[ic, o['internalWeight']]] if isinstance(ic, str) else list(ic)
+        o['internalCost'] = ic
+        lst = []
+        for d in ic:
+            if d[0] in B.IC_KINDS:
+                lst.append((d[0], float(d[1])))
+            else:
+                logging.info(f"unknown {d[0]}")
+        if lst:
+            self.icost = (self.fvInit, lst)
+            o['internalWeight'] = 1
 
     def set_fun(self, errorType):
         self.options['errorType'] = errorType
@@ -261,7 +283,7 @@ class SurfaceMatching:
         if withJacobian:
             raise NotImplementedError("Jacobian tracking is not built")
         xt, e = evol.shoot(self.x0_dev, control['at'], self.options['KparDiff'],
-                           regweight=self.options['regWeight'], energy=True)
+                           regweight=self.options['regWeight'], energy=True, icost=self.icost)
         st = State()
         st['xt'] = xt
         obj = e[0] * self.ds ** 2
@@ -319,7 +341,7 @@ class SurfaceMatching:
                 xt, _ = evol.shoot(self.x0_dev, control['at'], self.options['KparDiff'])
         px1 = -self.endPointGradient_device(xt[-1])
         g, _ = evol.shoot_adjoint(xt, control['at'], px1, self.options['KparDiff'], self.options['regWeight'],
-                                  euclidean=self.euclideanGradient)
+                                  euclidean=self.euclideanGradient, icost=self.icost)
         grd = Control()
         grd['at'] = g / coeff
         return grd

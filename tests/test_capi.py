@@ -210,3 +210,25 @@ def test_affine_descriptor_checks(lib):
     assert kd.affine == B.AFFINE_EUCLIDEAN and kd.w1 == 2.0 and list(kd.center) == [1.0, 2.0, 3.0]
     with pytest.raises(NotImplementedError):
         B.kernel_of(K)
+
+
+def test_internal_cost_descriptor_checks(lib):
+    B = lib
+    s = B.LddmmSurface()
+    e = ctypes.c_void_p(1)
+    d = B.internal_cost_desc([('elastic', 50.0), ('h1', 1.0)])
+    assert d.ncost == 2 and list(d.kinds)[:2] == [1, 0] and d.weights[0] == 50.0
+    d.kinds[1] = 9
+    assert B.LIB.lddmm_surface_internal_cost(ctypes.byref(s), None, ctypes.byref(d), e, None, None, None) == B.EINVAL
+    assert b"kind" in B.LIB.lddmm_last_error()
+    d.ncost = 0
+    assert B.LIB.lddmm_surface_internal_cost(ctypes.byref(s), None, ctypes.byref(d), e, None, None, None) == B.EINVAL
+    d = B.internal_cost_desc([('displacement', 1.0)])
+    s.nv, s.nf = 4, 2                      # no arrays: rejected before any device work
+    assert B.LIB.lddmm_surface_internal_cost(ctypes.byref(s), None, ctypes.byref(d), e, None, None, None) == B.EINVAL
+    kd = B.kernel_desc([1.0])
+    g = ctypes.c_void_p(1)
+    assert B.LIB.lddmm_shoot_adjoint_ic(kd, ctypes.byref(s), ctypes.byref(d), g, g, 3, g, 1.0, 1, None, g,
+                                        None) == B.EINVAL            # adjacency required
+    with pytest.raises(NotImplementedError):
+        B.internal_cost_desc([('bending', 1.0)])

@@ -632,3 +632,53 @@ def test_affine_kernel_matching_gradient_fd(spheres, kaff):
     obj = lambda aa: O.matching_objective(v, f, aa, Bs, [1.5], [0.8], regweight=0.5, kaff=kaff)[0]
     g = O.matching_gradient(v, f, at, Bs, [1.5], [0.8], regweight=0.5, euclidean=True, kaff=kaff)
     np.testing.assert_allclose((g * d).sum(), _fd(obj, at, d), rtol=1e-6)
+
+
+# ------------------------------------------------------------ internal costs
+@pytest.mark.parametrize("kind", ["h1", "elastic", "displacement"])
+def test_internal_cost_derivatives_fd(spheres, kind):
+    A, _ = spheres
+    v, f = A
+    v = v + rnd(*v.shape, scale=0.05)              # irregular triangles
+    phi = rnd(*v.shape)
+    dphi, dx = rnd(*v.shape), rnd(*v.shape)
+    c, gphi, gx = O.internal_cost(kind, v, f, phi)
+    fd_phi = _fd(lambda pp: O.internal_cost(kind, v, f, pp)[0], phi, dphi)
+    fd_x = _fd(lambda xx: O.internal_cost(kind, xx, f, phi)[0], v, dx)
+    np.testing.assert_allclose((gphi * dphi).sum(), fd_phi, rtol=1e-6)
+    np.testing.assert_allclose((gx * dx).sum(), fd_x, rtol=1e-6)
+
+
+def test_internal_cost_closed_forms():
+    v = np.array([[0., 0, 0], [1, 0, 0], [0, 1, 0], [0.3, 0.2, 0.9]])
+    f = np.array([[0, 1, 2], [0, 1, 3]])
+    area = 0.5 + 0.5 * np.linalg.norm(np.cross(v[1] - v[0], v[3] - v[0]))
+    # h1 of a linear field phi = x G^T is sum over faces of area * |G P_f|^2
+    G = rnd(3, 3)
+    expect = 0.0
+    for tri in f:
+        e1, e2 = v[tri[1]] - v[tri[0]], v[tri[2]] - v[tri[0]]
+        Q, _ = np.linalg.qr(np.stack([e1, e2], 1))          # orthonormal basis of the face plane
+        P = Q @ Q.T
+        expect += 0.5 * np.linalg.norm(np.cross(e1, e2)) * np.linalg.norm(G @ P) ** 2
+    np.testing.assert_allclose(O.internal_cost('h1', v, f, v @ G.T)[0], expect, rtol=1e-12)
+    # elastic vanishes on a rigid field (translation + in-plane rotation) of a flat triangle
+    flat = f[:1]
+    W = np.array([[0., -1, 0], [1, 0, 0], [0, 0, 0]])     # rotation about z
+    rigid = v @ W.T + np.array([0.3, -0.7, 0.0])
+    assert abs(O.internal_cost('elastic', v, flat, rigid)[0]) < 1e-14
+    assert O.internal_cost('h1', v, flat, rigid)[0] > 0.1      # h1 does not vanish there
+    # displacement of a constant field is |c|^2 * area
+    cst = np.broadcast_to(np.array([1.0, 2.0, -0.5]), v.shape)
+    np.testing.assert_allclose(O.internal_cost('displacement', v, f, cst)[0], 5.25 * area, rtol=1e-13)
+
+
+def test_internal_cost_matching_gradient_fd(spheres):
+    A, Bs = spheres
+    v, f = A
+    at = rnd(3, v.shape[0], 3, scale=0.05)
+    d = rnd(*at.shape)
+    ic = [('elastic', 5.0), ('h1', 0.5), ('displacement', 0.2)]
+    obj = lambda aa: O.matching_objective(v, f, aa, Bs, [1.5], [0.8], regweight=0.5, icost=ic)[0]
+    g = O.matching_gradient(v, f, at, Bs, [1.5], [0.8], regweight=0.5, euclidean=True, icost=ic)
+    np.testing.assert_allclose((g * d).sum(), _fd(obj, at, d), rtol=1e-6)
