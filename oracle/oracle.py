@@ -38,6 +38,9 @@ How each function is pinned (tests/test_oracle.py):
                   (translation + in-plane rotation) fields; displacement of a
                   constant field = |c|^2 * area; FD of the matching objective
                   with internal costs through the extra-term adjoint
+  unreduced       FD of the unreduced objective in both controls (ct, at),
+                  with fidelity, internal costs and affine kernel parts;
+                  c_t = x_t reduces the flow to landmarkDirectEvolutionEuler
   laplacian       (every kernel sum above takes kernel=('laplacian', order))
                   profile = the Matern nu = order + 1/2 closed form written
                   with factorials; first-derivative sums = finite differences
@@ -53,6 +56,7 @@ import numpy as np
 __all__ = [
     "applyK", "applyDiffKT", "applyktensor", "applykmat", "applykdiffmat",
     "applyDiffK", "applyDDiffK", "affineApplyK", "affineApplyDiffKT", "internal_cost", "internal_costs",
+    "semiReducedEvolution", "matching_objective_unreduced", "matching_gradient_unreduced",
     "surface_geometry", "surface_volume",
     "varifoldNorm0", "varifoldNormDef", "varifoldNorm", "varifoldNormGradient",
     "currentNorm0", "currentNormDef", "currentNorm", "currentNormGradient",
@@ -872,6 +876,96 @@ def _data_grad(kind, fvDef, fv1, sigma_dist):
     if kind == 'curve_current':
         return curveCurrentNormGradient(fvDef, fv1, sigma_dist)
     raise ValueError(kind)
+
+
+# ---------------------------------------------------------------------------
+# Unreduced (semi-reduced) matching: the velocity at time t is generated by
+# control points c_t (control 'ct' beside 'at'): v_t(x) = K(x, c_t) a_t
+# (base/pointEvolutionSemiReduced.py:7-90, base/surfaceMatching.py:468-565,
+# :728-908 unreduced branches).
+# ---------------------------------------------------------------------------
+
+def _kyx(y, x, a, sigma, kernel, kaff):
+    """Kernel.applyK(x, a, firstVar=y) with the affine part."""
+    v = applyK(y, x, a, sigma, -1, kernel=kernel)
+    return v if kaff is None else v + affineApplyK(y, x, a, kaff)
+
+
+def _dkt_yx(y, x, p, a, sigma, kernel, kaff):
+    """Kernel.applyDiffKT(x, p, a, firstVar=y) (plain) with the affine part."""
+    z = applyDiffKT(y, x, p, a, sigma, kernel=kernel)
+    return z if kaff is None else z + affineApplyDiffKT(x, p, a, kaff)
+
+
+def semiReducedEvolution(x0, ct, at, sigma, fidelity_weight=0., kernel=GAUSS, kaff=None):
+    """x_{t+1} = x_t + dt (K(x_t, c_t) a_t - fw (x_t - c_t)) (the fidelity term
+    when c_t has as many rows as x_t); base/pointEvolutionSemiReduced.py:7-90."""
+    x0, ct, at = _f64(x0), _f64(ct), _f64(at)
+    T = at.shape[0]
+    dt = 1.0 / T
+    fid = ct.shape[1] == x0.shape[0]
+    xt = np.zeros((T + 1,) + x0.shape)
+    xt[0] = x0
+    for t in range(T):
+        dx = _kyx(xt[t], ct[t], at[t], sigma, kernel, kaff)
+        if fid:
+            dx = dx - fidelity_weight * (xt[t] - ct[t])
+        xt[t + 1] = xt[t] + dt * dx
+    return xt
+
+
+def matching_objective_unreduced(x0, faces, ct, at, target, sigma_diff, sigma_dist, kind='varifold',
+                                 regweight=1., sigma_error=1., unreduced_weight=1., fidelity_weight=0.,
+                                 kernel_diff=GAUSS, kaff=None, icost=None):
+    """sum_t dt [r a_t.K(c_t, c_t) a_t + uw |c_t - x_t|^2 + C(x_t, K(x_t, c_t) a_t)]
+    + data term (base/surfaceMatching.py:536-553); `unreduced_weight` is the
+    option after its 1000 / N scaling (:93)."""
+    ct, at = _f64(ct), _f64(at)
+    T = at.shape[0]
+    dt = 1.0 / T
+    xt = semiReducedEvolution(x0, ct, at, sigma_diff, fidelity_weight, kernel_diff, kaff)
+    obj = 0.
+    for t in range(T):
+        z, a, c = xt[t], at[t], ct[t]
+        obj += regweight * dt * (a * _kyx(c, c, a, sigma_diff, kernel_diff, kaff)).sum()
+        obj += unreduced_weight * dt * ((c - z) ** 2).sum()
+        if icost:
+            obj += internal_costs(icost, z, faces, _kyx(z, c, a, sigma_diff, kernel_diff, kaff))[0] * dt
+    dterm, obj0 = _data_term(kind, (xt[-1], faces), target, sigma_dist)
+    return obj + (obj0 + dterm) / sigma_error ** 2, xt
+
+
+def matching_gradient_unreduced(x0, faces, ct, at, target, sigma_diff, sigma_dist, kind='varifold',
+                                regweight=1., sigma_error=1., unreduced_weight=1., fidelity_weight=0.,
+                                kernel_diff=GAUSS, kaff=None, icost=None):
+    """(grad ct, grad at) of the unreduced objective as BFGS sees it
+    (base/surfaceMatching.py:782-813 covector, :861-887 gradient, :1082-1083 / T)."""
+    ct, at = _f64(ct), _f64(at)
+    T = at.shape[0]
+    dt = 1.0 / T
+    fid = ct.shape[1] == np.shape(x0)[0]
+    xt = semiReducedEvolution(x0, ct, at, sigma_diff, fidelity_weight, kernel_diff, kaff)
+    pxt = np.zeros(xt.shape)
+    pxt[T] = -_data_grad(kind, (xt[-1], faces), target, sigma_dist) / sigma_error ** 2
+    dat, dct = np.zeros(at.shape), np.zeros(ct.shape)
+    for t in range(T - 1, -1, -1):
+        z, a, c, px = xt[t], at[t], ct[t], pxt[t + 1]
+        pp, DLv = px, 0.
+        if icost:
+            v = _kyx(z, c, a, sigma_diff, kernel_diff, kaff)
+            _, Lv, DLv = internal_costs(icost, z, faces, v)
+            pp = px - Lv
+        zpx = _dkt_yx(z, c, pp, a, sigma_diff, kernel_diff, kaff) - DLv - 2 * unreduced_weight * (z - c)
+        if fid:
+            zpx = zpx - fidelity_weight * px
+        pxt[t] = px + dt * zpx
+        dat[t] = 2 * regweight * _kyx(c, c, a, sigma_diff, kernel_diff, kaff) - _kyx(c, z, pp, sigma_diff,
+                                                                                     kernel_diff, kaff)
+        dct[t] = (2 * regweight * _dkt_yx(c, c, a, a, sigma_diff, kernel_diff, kaff)
+                  - _dkt_yx(c, z, a, pp, sigma_diff, kernel_diff, kaff) + 2 * unreduced_weight * (c - z))
+        if fid:
+            dct[t] -= fidelity_weight * px
+    return dct / T, dat / T
 
 
 def _internal_cost_gradient(xt, at, px1, faces, sigma, regweight, kernel, kaff, icost):

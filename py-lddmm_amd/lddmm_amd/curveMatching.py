@@ -18,6 +18,8 @@ from .surfaceMatching import SurfaceMatching
 
 
 class CurveMatching(SurfaceMatching):
+    unreduced_supported = False
+
     def getDefaultOptions(self):
         o = super().getDefaultOptions()
         o['errorType'] = 'varifold'

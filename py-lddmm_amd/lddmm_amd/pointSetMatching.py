@@ -16,6 +16,8 @@ from .surfaceMatching import SurfaceMatching
 
 
 class PointSetMatching(SurfaceMatching):
+    unreduced_supported = False
+
     def getDefaultOptions(self):
         return super().getDefaultOptions()          # errorType 'measure' (base/basicMatching.py:95)
 

@@ -5,8 +5,9 @@ base/basicMatching.py for the configuration the hot path serves:
 errorType 'varifold', 'current' or 'measure', algorithm 'bfgs' (weak-Wolfe line search),
 affine None (or affineKernel=True with affine 'euclidean' / 'affine', which
 become terms of the deformation kernel), internal costs 'h1' / 'elastic' /
-'displacement' (device kernels inside the shooting loops), not symmetric,
-not unreduced.
+'displacement' (device kernels inside the shooting loops), unreduced
+(control points 'ct' beside 'at', semi-reduced flow; no control-point
+resets), not symmetric.
 
 The control `at` [T, N, 3], the trajectories and the BFGS memory live on
 the GPU (float64 tensors); every objective is one device forward shoot with
@@ -49,6 +50,9 @@ class State(dict):
 
 
 class SurfaceMatching:
+    # the reference builds the unreduced (control-point) form for surfaces only
+    unreduced_supported = True
+
     def __init__(self, Template=None, Target=None, options=None):
         self.setInitialOptions(options)
         if self.options['algorithm'] != 'bfgs':
@@ -68,6 +72,10 @@ class SurfaceMatching:
         self.Kdist_dtype = self.options['pk_dtype']
         self.set_parameters()
         self.set_fun(self.options['errorType'])
+        self.unreduced = bool(self.options['unreduced'])
+        # base/surfaceMatching.py:93: the control-point penalty scales with 1000 / N
+        self.options['unreducedWeight'] *= 1000.0 / self.fv0.vertices.shape[0]
+        self.fidelityWeight = self.options['fidelityWeight']
         self.initialize_variables()
         self.gradCoeff = 1.
         self.ds = 1.
@@ -99,7 +107,11 @@ class SurfaceMatching:
             pass        # turned into kernel terms in set_parameters
         elif o['affine'] not in (None, 'none'):
             raise NotImplementedError("affine registration is not built in lddmm_amd")
-        for key in ('symmetric', 'unreduced', 'pplot'):
+        if o['unreduced'] and not self.unreduced_supported:
+            raise NotImplementedError(f"unreduced=True is not built for {type(self).__name__}")
+        if o['unreduced'] and o['unreducedResetRate'] > 0:
+            raise NotImplementedError("unreducedResetRate > 0 (control-point resets) is not built")
+        for key in ('symmetric', 'pplot'):
             if o[key]:
                 raise NotImplementedError(f"option {key}=True is not built in lddmm_amd")
         if o['Landmarks'] is not None or o['passenger'] is not None:
@@ -222,6 +234,10 @@ class SurfaceMatching:
         self.control['at'] = torch.zeros((self.Tsize, self.npt, self.dim), dtype=torch.float64, device=dev)
         self.controlTry['at'] = torch.zeros_like(self.control['at'])
         self.state['xt'] = self.x0_dev.expand(self.Tsize + 1, -1, -1).clone()
+        if getattr(self, 'unreduced', False):
+            # control points start on the template (base/surfaceMatching.py:357-361)
+            self.control['ct'] = self.x0_dev.expand(self.Tsize, -1, -1).clone()
+            self.controlTry['ct'] = self.control['ct'].clone()
         self.fvDef = Surface(surf=self.fvInit)
 
     def resetPK(self, newType=None):
@@ -282,6 +298,11 @@ class SurfaceMatching:
         (base/surfaceMatching.py:468-565); energy fused in the device shoot."""
         if withJacobian:
             raise NotImplementedError("Jacobian tracking is not built")
+        if self.unreduced:
+            xt, e = self._unreduced_forward(control, energy=True)
+            st = State()
+            st['xt'] = xt
+            return (e, st) if withTrajectory else e
         xt, e = evol.shoot(self.x0_dev, control['at'], self.options['KparDiff'],
                            regweight=self.options['regWeight'], energy=True, icost=self.icost)
         st = State()
@@ -338,8 +359,17 @@ class SurfaceMatching:
                 for k in update[0].keys():
                     if update[0][k] is not None:
                         control[k] = self.control[k] - update[1] * update[0][k]
-                xt, _ = evol.shoot(self.x0_dev, control['at'], self.options['KparDiff'])
+                if self.unreduced:
+                    xt, _ = self._unreduced_forward(control, energy=False)
+                else:
+                    xt, _ = evol.shoot(self.x0_dev, control['at'], self.options['KparDiff'])
         px1 = -self.endPointGradient_device(xt[-1])
+        if self.unreduced:
+            dct, dat = self._unreduced_gradient(control, xt, px1)
+            grd = Control()
+            grd['ct'] = dct / coeff
+            grd['at'] = dat / coeff
+            return grd
         g, _ = evol.shoot_adjoint(xt, control['at'], px1, self.options['KparDiff'], self.options['regWeight'],
                                   euclidean=self.euclideanGradient, icost=self.icost)
         grd = Control()
@@ -362,14 +392,17 @@ class SurfaceMatching:
         first (Control dicts); the pair scalars (rho, c) are computed once per
         stored pair and kept with it."""
         import ctypes
+        keys = [k for k in ('at', 'ct') if grd.get(k) is not None]
+        flat = (lambda c: c['at'].contiguous()) if keys == ['at'] else \
+            (lambda c: torch.cat([c[k].reshape(-1) for k in keys]))
         for entry in storedGrad:
             if len(entry) == 2:
-                sk, yk = entry[0]['at'].contiguous(), entry[1]['at'].contiguous()
+                sk, yk = flat(entry[0]), flat(entry[1])
                 sc = torch.empty(2, dtype=torch.float64, device=sk.device)
                 B.check(B.LIB.lddmm_lbfgs_pair_scalars(B.ptr(sk), B.ptr(yk), sk.numel(), B.ptr(sc[0:1]),
                                                        B.ptr(sc[1:2]), B.stream_handle()), "lbfgs_pair_scalars")
                 entry.append((sk, yk, sc))
-        g = grd['at'].contiguous()
+        g = flat(grd)
         q = torch.empty_like(g)
         m = len(storedGrad)
         P = ctypes.c_void_p * m
@@ -380,7 +413,14 @@ class SurfaceMatching:
         B.check(B.LIB.lddmm_lbfgs_two_loop(B.ptr(g), g.numel(), m, sp, yp, rp, B.ptr(cn), B.ptr(q),
                                            B.stream_handle()), "lbfgs_two_loop")
         out = Control()
-        out['at'] = q
+        if keys == ['at']:
+            out['at'] = q
+        else:
+            o = 0
+            for k in keys:
+                nk = grd[k].numel()
+                out[k] = q[o:o + nk].view_as(grd[k])
+                o += nk
         return out
 
     def addProd(self, dir1, dir2, beta):
@@ -407,7 +447,77 @@ class SurfaceMatching:
     def randomDir(self):
         d = Control()
         d['at'] = torch.randn_like(self.control['at'])
+        if self.unreduced:
+            d['ct'] = torch.randn_like(self.control['ct'])
         return d
+
+    # ------------------------------------------------------ unreduced path
+    def _unreduced_forward(self, control, energy=True):
+        """Semi-reduced flow x_{t+1} = x_t + dt (K(x_t, c_t) a_t - fw (x_t - c_t))
+        and its energy sum_t dt [r a.K(c,c)a + uw |c - x|^2 + C(x_t, v_t)]
+        (base/pointEvolutionSemiReduced.py:7-90, base/surfaceMatching.py:536-553).
+        Every kernel sum runs through the device entry points (Kernel methods
+        on CUDA tensors); the loop over the T steps is here."""
+        K = self.options['KparDiff']
+        at, ct = control['at'], control['ct']
+        T = at.shape[0]
+        dt = 1.0 / T
+        r, uw, fw = self.options['regWeight'], self.options['unreducedWeight'], self.fidelityWeight
+        fid = ct.shape[1] == self.x0_dev.shape[0]
+        xt = torch.empty((T + 1,) + tuple(self.x0_dev.shape), dtype=torch.float64, device=at.device)
+        xt[0] = self.x0_dev
+        e = torch.zeros(1, dtype=torch.float64, device=at.device) if energy else None
+        for t in range(T):
+            z, a, c = xt[t], at[t], ct[t]
+            v = K.applyK(c, a, firstVar=z)
+            dx = v - fw * (z - c) if fid else v
+            xt[t + 1] = z + dt * dx
+            if energy:
+                ca = K.applyK(c, a)
+                e += r * dt * (a * ca).sum() * self.ds ** 2 + uw * dt * ((c - z) ** 2).sum()
+                if self.icost is not None:
+                    ei, _ = sd.internal_cost_device(self.icost[0], v * self.ds, self.icost[1], variables=None,
+                                                    vertices=z)
+                    e += dt * ei
+        return xt, e
+
+    def _unreduced_gradient(self, control, xt, px1):
+        """Covector p_t = p_{t+1} + dt (applyDiffKT(x_t <- c_t, p_{t+1} - Lv, a_t)
+        - DLv - 2 uw (x_t - c_t) - fw p_{t+1}) and the gradients
+        dat = 2 r K(c,c) a - K(c <- x) (p - Lv), dct = 2 r applyDiffKT(c, a, a)
+        - applyDiffKT(c <- x, a, p - Lv) + 2 uw (c - x) - fw p, both / T
+        (base/surfaceMatching.py:782-813, :861-887, :1082-1083)."""
+        K = self.options['KparDiff']
+        at, ct = control['at'], control['ct']
+        T = at.shape[0]
+        dt = 1.0 / T
+        r, uw, fw = self.options['regWeight'], self.options['unreducedWeight'], self.fidelityWeight
+        fid = ct.shape[1] == self.x0_dev.shape[0]
+        dat = torch.empty_like(at)
+        dct = torch.empty_like(ct)
+        px = px1
+        for t in range(T - 1, -1, -1):
+            z, a, c = xt[t], at[t], ct[t]
+            pp = px
+            DLv = None
+            if self.icost is not None:
+                v = K.applyK(c, a, firstVar=z) * self.ds
+                _, g = sd.internal_cost_device(self.icost[0], v, self.icost[1], energy=False, variables='both',
+                                               vertices=z)
+                pp = px - g['phi']
+                DLv = g['x']
+            zpx = K.applyDiffKT(c, pp, a, firstVar=z) - 2 * uw * (z - c)
+            if DLv is not None:
+                zpx = zpx - DLv
+            if fid:
+                zpx = zpx - fw * px
+            dat[t] = 2 * r * K.applyK(c, a) * self.ds ** 2 - K.applyK(z, pp, firstVar=c) * self.ds
+            dct[t] = (2 * r * K.applyDiffKT(c, a, a) * self.ds ** 2 - K.applyDiffKT(z, a, pp, firstVar=c) * self.ds
+                      + 2 * uw * (c - z))
+            if fid:
+                dct[t] -= fw * px
+            px = px + dt * zpx
+        return dct / T, dat / T
 
     # ----------------------------------------------------------- iteration
     def startOfIteration(self):
@@ -463,7 +573,10 @@ class SurfaceMatching:
             raise ValueError(f"control shape {tuple(at_.shape)} != {tuple(self.control['at'].shape)}")
         self.control['at'] = at_.clone()
         self.obj = None
-        self.state['xt'], _ = evol.shoot(self.x0_dev, self.control['at'], self.options['KparDiff'])
+        if self.unreduced:
+            self.state['xt'], _ = self._unreduced_forward(self.control, energy=False)
+        else:
+            self.state['xt'], _ = evol.shoot(self.x0_dev, self.control['at'], self.options['KparDiff'])
 
     def setTargetDataEnergy(self, value, rtol=1e-4):
         """Record the data term (varifold / current distance / sigmaError^2)

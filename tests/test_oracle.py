@@ -682,3 +682,31 @@ def test_internal_cost_matching_gradient_fd(spheres):
     obj = lambda aa: O.matching_objective(v, f, aa, Bs, [1.5], [0.8], regweight=0.5, icost=ic)[0]
     g = O.matching_gradient(v, f, at, Bs, [1.5], [0.8], regweight=0.5, euclidean=True, icost=ic)
     np.testing.assert_allclose((g * d).sum(), _fd(obj, at, d), rtol=1e-6)
+
+
+# ------------------------------------------------------------ unreduced
+def test_semi_reduced_flow_reduces_to_landmark_flow():
+    """With c_t equal to the trajectory itself the semi-reduced flow is the
+    reduced one (one step at a time, since c_t is given up front)."""
+    x0, at = rnd(12, 3), rnd(1, 12, 3, scale=0.3)
+    xr = O.landmarkDirectEvolutionEuler(x0, at, [1.3])
+    xs = O.semiReducedEvolution(x0, x0[None], at, [1.3], fidelity_weight=0.7)
+    np.testing.assert_allclose(xs, xr, rtol=1e-14)
+
+
+@pytest.mark.parametrize("variant", ["plain", "fidelity+costs+affine"])
+def test_unreduced_matching_gradient_fd(spheres, variant):
+    A, Bs = spheres
+    v, f = A
+    T = 3
+    ct = np.broadcast_to(v, (T,) + v.shape) + rnd(T, *v.shape, scale=0.1)
+    at = rnd(T, v.shape[0], 3, scale=0.05)
+    kw = dict(regweight=0.5, unreduced_weight=0.8)
+    if variant != "plain":
+        kw.update(fidelity_weight=0.3, icost=[('elastic', 2.0), ('displacement', 0.1)],
+                  kaff=('euclidean', 1e-3, 2e-3, v.mean(0)), kernel_diff=('laplacian', 3))
+    obj = lambda cc, aa: O.matching_objective_unreduced(v, f, cc, aa, Bs, [1.5], [0.8], **kw)[0]
+    gc, ga = O.matching_gradient_unreduced(v, f, ct, at, Bs, [1.5], [0.8], **kw)
+    dc, da = rnd(*ct.shape), rnd(*at.shape)
+    np.testing.assert_allclose((ga * da).sum(), _fd(lambda aa: obj(ct, aa), at, da), rtol=1e-6)
+    np.testing.assert_allclose((gc * dc).sum(), _fd(lambda cc: obj(cc, at), ct, dc), rtol=1e-6)
