@@ -1,7 +1,8 @@
 """Triangulated surfaces (py-lddmm base/surfaces.py:226-351, :603-605, :1645-1651).
 
-`Surface((faces, vertices))`, `Surface(surf=other)` and `Surface('file.vtk')`
-(legacy ASCII VTK POLYDATA, the format of the reference's TestData) build
+`Surface((faces, vertices))`, `Surface(surf=other)` and `Surface('file')`
+(.vtk legacy ASCII POLYDATA, the format of the reference's TestData, .byu,
+.off, .obj, .stl) build
 the same attributes as the reference: vertices, faces, weights, centers,
 surfel, face_weights, component.  `device()` returns the C-ABI view of the
 surface (float64 vertices, int32 faces, vertex -> face adjacency) on the GPU,
@@ -12,6 +13,7 @@ import os
 import numpy as np
 
 from . import vtk_io
+from .vtk_io import vtkFields  # noqa: F401  (reference name, base/surfaces.py:53-58)
 
 
 class Surface:
@@ -61,23 +63,68 @@ class Surface:
 
     # ------------------------------------------------------------------ I/O
     def read(self, filename):
+        """Dispatch on the extension (base/surfaces.py:303-323)."""
         ext = os.path.splitext(filename)[1]
-        if ext != '.vtk':
-            raise NotImplementedError(f"lddmm_amd reads legacy ASCII .vtk surfaces only ({filename})")
-        v, f, w = vtk_io.read_polydata(filename)
-        self.vertices = v
-        self.faces = f
-        self.component = np.zeros(f.shape[0], dtype=int)
-        if w is None:
-            self.weights = np.ones(v.shape[0], dtype=int)
+        if ext in ('.byu', '.g'):
+            self.readbyu(filename)
+        elif ext == '.off':
+            self.readOFF(filename)
+        elif ext == '.vtk':
+            self.readVTK(filename)
+        elif ext == '.stl':
+            self.readSTL(filename)
+        elif ext == '.obj':
+            self.readOBJ(filename)
         else:
-            self.weights = w
+            raise NameError('Unknown Surface Extension: ' + filename)
+
+    def _set_mesh(self, v, f, w=None, component=None):
+        self.vertices = np.asarray(v, dtype=np.float64)
+        self.faces = np.asarray(f, dtype=np.int64)
+        self.weights = np.ones(self.vertices.shape[0]) if w is None else np.asarray(w, dtype=np.float64)
+        self.component = np.zeros(self.faces.shape[0], dtype=int) if component is None else component
+        self._topo = None
         self.computeCentersAreas()
+
+    def readVTK(self, fileName):
+        """Legacy ASCII VTK: vertices, faces, 'weights' point scalars and
+        'labels' cell scalars (the face components), base/surfaces.py:1880-1926."""
+        v, f, w, lab = vtk_io.read_polydata(fileName, with_labels=True)
+        self._set_mesh(v, f, w, lab)
+
+    def readbyu(self, byufile):
+        """base/surfaces.py:1697-1747"""
+        self._set_mesh(*vtk_io.read_byu(byufile))
+
+    def readOFF(self, offfile):
+        """base/surfaces.py:1658-1695"""
+        self._set_mesh(*vtk_io.read_off(offfile))
+
+    def readOBJ(self, fileName):
+        """base/surfaces.py:1948-1978"""
+        self._set_mesh(*vtk_io.read_obj(fileName))
+
+    def readSTL(self, fileName):
+        """base/surfaces.py:1981-2009"""
+        self._set_mesh(*vtk_io.read_stl(fileName))
 
     def saveVTK(self, fileName, scalars=None, normals=None, tensors=None, scal_name='scalars',
                 vectors=None, vect_name='vectors'):
+        """base/surfaces.py:1794-1812"""
         vtk_io.write_polydata(fileName, self.vertices, self.faces, weights=self.weights,
-                              scalars=scalars, scal_name=scal_name, vectors=vectors, vect_name=vect_name)
+                              scalars=scalars, scal_name=scal_name, vectors=vectors, vect_name=vect_name,
+                              normals=normals, tensors=tensors, component=self.component)
+
+    def saveVTK2(self, fileName, vtkFields=None):
+        """base/surfaces.py:1815-1877"""
+        vtk_io.write_polydata2(fileName, self.vertices, self.faces, self.component, vtkFields)
+
+    def savebyu(self, byufile):
+        """base/surfaces.py:1750-1787"""
+        vtk_io.write_byu(byufile, self.vertices, self.faces)
+
+    def saveOFF(self, offfile):
+        vtk_io.write_off(offfile, self.vertices, self.faces)
 
     def concatenate(self, fvl):
         nv = 0
