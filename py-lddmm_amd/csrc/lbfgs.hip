@@ -4,7 +4,7 @@
 // step's update to q elementwise and then accumulates the next dot product
 // on the updated values; the dot product is reduced deterministically (each
 // workgroup writes its partial sum, the last workgroup to finish adds the
-// partials in workgroup order) and stored, already multiplied by its rho, in
+// partials in a fixed tree order) and stored, already multiplied by its rho, in
 // device memory, where the next launch reads it.  The host issues 2m + 1
 // launches for m stored pairs and never waits.
 #include "common.hpp"
@@ -19,6 +19,7 @@ namespace {
 constexpr int kLbBlock = 256;
 constexpr int kLbGrid = 256;        // fixed: the reduction order depends only on n
 constexpr int kLbMaxPairs = 1024;
+static_assert(kLbGrid <= kLbBlock, "the last workgroup reads one partial per lane");
 
 // Per-device state: [kLbGrid] partials, [2*kLbMaxPairs + 8] scalars, counter.
 struct LbState {
@@ -80,11 +81,18 @@ k_lb_step(int64_t n, double* __restrict__ q, const double* __restrict__ u, const
         last = ticket == gridDim.x - 1;
     }
     __syncthreads();
-    if (last && threadIdx.x == 0) {
-        __threadfence();
-        double t = 0.0;
-        for (unsigned k = 0; k < gridDim.x; ++k) t += partials[k];
-        *out = (mul_ptr ? *mul_ptr : 1.0) * t;
+    if (!last) return;
+    // last workgroup: every lane reads one partial (gridDim.x <= kLbBlock), then a
+    // fixed tree, so the order depends only on the grid
+    __threadfence();
+    red[threadIdx.x] = threadIdx.x < gridDim.x ? partials[threadIdx.x] : 0.0;
+    __syncthreads();
+    for (int k = kLbBlock / 2; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *out = (mul_ptr ? *mul_ptr : 1.0) * red[0];
         atomicExch(counter, 0u);
     }
 }

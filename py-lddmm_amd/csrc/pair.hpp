@@ -82,11 +82,23 @@ __device__ __forceinline__ void pair_span(const typename OP::template Args<T>& a
 #pragma unroll
             for (int c = 0; c < NCH; ++c) acc[r][c] = T(0);
 
+        if (j1 - jt >= kTile) {
 #pragma unroll 4
-        for (int t = 0; t < kTile; ++t) {
-            const T* rec = &lds[t * REC];
+            for (int t = 0; t < kTile; ++t) {
+                const T* rec = &lds[t * REC];
 #pragma unroll
-            for (int r = 0; r < R; ++r) OP::pair(args, tg[r], rec, acc[r]);
+                for (int r = 0; r < R; ++r) OP::pair(args, tg[r], rec, acc[r]);
+            }
+        } else {
+            // last, partial tile: records past j1 are zero-weight padding, so the
+            // count is rounded up to a multiple of 4 and the rest skipped
+            const int tn = (j1 - jt + 3) & ~3;
+#pragma unroll 4
+            for (int t = 0; t < tn; ++t) {
+                const T* rec = &lds[t * REC];
+#pragma unroll
+                for (int r = 0; r < R; ++r) OP::pair(args, tg[r], rec, acc[r]);
+            }
         }
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -123,23 +135,49 @@ k_pair(const typename OP::template Args<T> args, int n, int m, int chunk, Partia
 }
 
 // Sum of chunk partials for (i, c) in float64 (two-sum in float32 first).
+// Loads are issued in batches of kSumBatch independent reads (the sums stay in
+// chunk order, so the result is the same as one chunk at a time).
+constexpr int kSumBatch = 8;
 template <typename T>
 __device__ __forceinline__ double sum_chunks(const Partials<T>& p, int nchunk, int nch, int i, int c,
                                              const double* chunk_w) {
+    const size_t stride = size_t(nch) * size_t(p.n);
+    const size_t base = size_t(c) * size_t(p.n) + size_t(i);
     if constexpr (kCompensated<T>) {
         float h = 0.f, l = 0.f;
-        for (int k = 0; k < nchunk; ++k) {
-            size_t o = (size_t(k) * nch + c) * size_t(p.n) + i;
-            float w = chunk_w ? float(chunk_w[k]) : 1.f;
-            two_sum_acc(h, l, p.hi[o] * w);
-            l += p.lo[o] * w;
+        for (int k0 = 0; k0 < nchunk; k0 += kSumBatch) {
+            float vh[kSumBatch], vl[kSumBatch];
+#pragma unroll
+            for (int b = 0; b < kSumBatch; ++b) {
+                const int k = k0 + b;
+                vh[b] = k < nchunk ? p.hi[base + size_t(k) * stride] : 0.f;
+                vl[b] = k < nchunk ? p.lo[base + size_t(k) * stride] : 0.f;
+            }
+#pragma unroll
+            for (int b = 0; b < kSumBatch; ++b) {
+                const int k = k0 + b;
+                if (k < nchunk) {
+                    float w = chunk_w ? float(chunk_w[k]) : 1.f;
+                    two_sum_acc(h, l, vh[b] * w);
+                    l += vl[b] * w;
+                }
+            }
         }
         return double(h) + double(l);
     } else {
         double s = 0.0;
-        for (int k = 0; k < nchunk; ++k) {
-            size_t o = (size_t(k) * nch + c) * size_t(p.n) + i;
-            s += chunk_w ? p.hi[o] * chunk_w[k] : p.hi[o];
+        for (int k0 = 0; k0 < nchunk; k0 += kSumBatch) {
+            double v[kSumBatch];
+#pragma unroll
+            for (int b = 0; b < kSumBatch; ++b) {
+                const int k = k0 + b;
+                v[b] = k < nchunk ? p.hi[base + size_t(k) * stride] : 0.0;
+            }
+#pragma unroll
+            for (int b = 0; b < kSumBatch; ++b) {
+                const int k = k0 + b;
+                if (k < nchunk) s += chunk_w ? v[b] * chunk_w[k] : v[b];
+            }
         }
         return s;
     }
@@ -150,19 +188,19 @@ struct ChanScale { double s[8]; };
 // out[o] = (base ? base[o] : (accumulate ? out[o] : 0)) + alpha * scale_c * v,
 // o = i*ldo + col0 + c, v = sum over chunks of channel c; partials hold `nch`
 // channels per chunk, of which the first `nout` are written.
+// One thread per (i, c): c = blockIdx.y.
 template <typename T>
 __global__ void k_reduce(Partials<T> part, int nchunk, int nch, int nout, int n, ChanScale sc,
                          double* out, int ldo, int col0, int accumulate, const double* base,
                          double alpha) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    for (int c = 0; c < nout; ++c) {
-        double v = alpha * sum_chunks(part, nchunk, nch, i, c, nullptr) * sc.s[c];
-        size_t o = size_t(i) * ldo + col0 + c;
-        if (base) out[o] = base[o] + v;
-        else if (accumulate) out[o] += v;
-        else out[o] = v;
-    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= n || c >= nout) return;
+    double v = alpha * sum_chunks(part, nchunk, nch, i, c, nullptr) * sc.s[c];
+    size_t o = size_t(i) * ldo + col0 + c;
+    if (base) out[o] = base[o] + v;
+    else if (accumulate) out[o] += v;
+    else out[o] = v;
 }
 
 __global__ void k_fill(double* p, size_t count, double v);
@@ -227,7 +265,8 @@ int run_reduce(const Partials<T>& part, int nchunk, int nch, int nout, int n, co
                double* out, int ldo, int col0, int accumulate, const double* base, double alpha,
                hipStream_t st) {
     int nb = (n + 255) / 256;
-    hipLaunchKernelGGL((k_reduce<T>), dim3(nb), dim3(256), 0, st, part, nchunk, nch, nout, n, sc, out,
+    hipLaunchKernelGGL((k_reduce<T>), dim3(nb, std::max(1, nout)), dim3(256), 0, st, part, nchunk, nch, nout, n,
+                       sc, out,
                        ldo, col0, accumulate, base, alpha);
     LDDMM_CHECK_LAUNCH();
     return LDDMM_OK;
