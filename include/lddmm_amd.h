@@ -374,6 +374,16 @@ int lddmm_nn_segmax_bwd(const float* dg, const int32_t* off, const int32_t* arg,
 int lddmm_nn_adam(float* w, uint16_t* wbf, const float* g, float* m, float* v, int64_t count,
                   float lr, float b1, float b2, float eps, int32_t step, float wd, void* stream);
 int lddmm_nn_to_bf16(const float* w, uint16_t* wbf, int64_t count, void* stream);
+/* Warm-start input feature (no counterpart in the reference): for each ray
+ * i (origin orig[i,:], direction dir[i,:], float64 [n,3]) the signed
+ * parameter t[i] of the intersection orig + t dir with the triangle mesh
+ * (tv [nv,3] float64, tf [nf,3] int32, every index in [0, nv)) that has the
+ * smallest |t| over both directions, lowest face index on ties; hit[i] = 1,
+ * or t[i] = 0 and hit[i] = 0 when the line meets no triangle.  With unit
+ * template vertex normals as directions, t is the signed normal distance
+ * from the template to the target surface. */
+int lddmm_normal_raycast(const double* orig, const double* dir, int64_t n, const double* tv, int64_t nv,
+                         const int32_t* tf, int64_t nf, double* t_out, int32_t* hit, void* stream);
 
 /* ------------------------------------------------------------------------
  * L-BFGS direction on the device (the two-loop recursion of base/bfgs.py:188-207

@@ -145,32 +145,118 @@ __global__ void k_sum_parts(const float* __restrict__ part, int nparts, size_t c
     out[e] = s;
 }
 
-// Per-segment column sums: out[s][n] = sum_{p in segment s} dZ[p][n]; segments are
-// contiguous point ranges [off[s], off[s+1]).
-__global__ void k_seg_colsum(const float* __restrict__ dZ, const int32_t* __restrict__ off, int N,
-                             float* __restrict__ out) {
-    int s = blockIdx.x;
-    for (int n = threadIdx.x; n < N; n += blockDim.x) {
+// Per-segment column reductions over contiguous point ranges [off[s], off[s+1]),
+// in two deterministic passes.  Pass 1: workgroup (c, s) covers chunk c of
+// the SEG_CHUNKS equal row chunks of segment s; its 256 lanes form
+// 256/NC row lanes x NC columns (NC = min(N, 256), column blocks looped), each
+// lane strides the chunk's rows, and the row lanes are combined in LDS in
+// lane order.  Pass 2 folds the chunks of a segment in chunk order.
+constexpr int SEG_CHUNKS = 32;
+constexpr int SEG_THREADS = 256;
+
+__device__ __forceinline__ void seg_chunk_rows(const int32_t* off, int s, int c, int nch, int* r0, int* r1) {
+    const int b0 = off[s], b1 = off[s + 1];
+    const int per = (b1 - b0 + nch - 1) / nch;
+    *r0 = b0 + c * per;
+    *r1 = min(b1, *r0 + per);
+}
+
+// colsum pass 1: part[(s * nch + c) * N + n]
+__global__ void __launch_bounds__(SEG_THREADS) k_seg_colsum_part(const float* __restrict__ dZ,
+                                                                 const int32_t* __restrict__ off, int N, int nch,
+                                                                 float* __restrict__ part) {
+    __shared__ float red[SEG_THREADS];
+    const int s = blockIdx.y, c = blockIdx.x;
+    const int NC = N < SEG_THREADS ? N : SEG_THREADS;
+    const int R = SEG_THREADS / NC;
+    const int lane = threadIdx.x / NC, col = threadIdx.x % NC;
+    int r0, r1;
+    seg_chunk_rows(off, s, c, nch, &r0, &r1);
+    for (int n0 = 0; n0 < N; n0 += NC) {
+        const int n = n0 + col;
         float acc = 0.f;
-        for (int p = off[s]; p < off[s + 1]; ++p) acc += dZ[size_t(p) * N + n];
-        out[size_t(s) * N + n] = acc;
+        if (lane < R && n < N)
+            for (int p = r0 + lane; p < r1; p += R) acc += dZ[size_t(p) * N + n];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        if (lane == 0 && n < N) {
+            float t = 0.f;
+            for (int l = 0; l < R; ++l) t += red[l * NC + col];
+            part[(size_t(s) * nch + c) * N + n] = t;
+        }
+        __syncthreads();
     }
 }
 
-// g[s][n] = max_{p in s} H[p][n], arg[s][n] = first p attaining it
-__global__ void k_segmax(const float* __restrict__ H, const int32_t* __restrict__ off, int N, float* __restrict__ g,
-                         int32_t* __restrict__ arg) {
-    int s = blockIdx.x;
-    for (int n = threadIdx.x; n < N; n += blockDim.x) {
+// colsum pass 2: out[s][n] = sum_c part[s][c][n]
+__global__ void k_seg_colsum_fold(const float* __restrict__ part, int nseg, int nch, int N, float* __restrict__ out) {
+    const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= size_t(nseg) * N) return;
+    const size_t s = e / N, n = e % N;
+    float t = 0.f;
+    for (int c = 0; c < nch; ++c) t += part[(s * nch + c) * N + n];
+    out[e] = t;
+}
+
+// (v, p) beats (best, bi): larger value, or equal value at a smaller point index
+__device__ __forceinline__ bool max_first(float v, int p, float best, int bi) {
+    return v > best || (v == best && p < bi);
+}
+
+// segmax pass 1: (pv, pi)[(s * nch + c) * N + n] = max and first argmax over the chunk
+__global__ void __launch_bounds__(SEG_THREADS) k_segmax_part(const float* __restrict__ H,
+                                                             const int32_t* __restrict__ off, int N, int nch,
+                                                             float* __restrict__ pv, int32_t* __restrict__ pi) {
+    __shared__ float rv[SEG_THREADS];
+    __shared__ int ri[SEG_THREADS];
+    const int s = blockIdx.y, c = blockIdx.x;
+    const int NC = N < SEG_THREADS ? N : SEG_THREADS;
+    const int R = SEG_THREADS / NC;
+    const int lane = threadIdx.x / NC, col = threadIdx.x % NC;
+    int r0, r1;
+    seg_chunk_rows(off, s, c, nch, &r0, &r1);
+    for (int n0 = 0; n0 < N; n0 += NC) {
+        const int n = n0 + col;
         float best = -INFINITY;
-        int bi = off[s];
-        for (int p = off[s]; p < off[s + 1]; ++p) {
-            float v = H[size_t(p) * N + n];
-            if (v > best) { best = v; bi = p; }
+        int bi = INT32_MAX;
+        if (lane < R && n < N)
+            for (int p = r0 + lane; p < r1; p += R) {
+                const float v = H[size_t(p) * N + n];
+                if (max_first(v, p, best, bi)) { best = v; bi = p; }
+            }
+        rv[threadIdx.x] = best;
+        ri[threadIdx.x] = bi;
+        __syncthreads();
+        if (lane == 0 && n < N) {
+            float b = -INFINITY;
+            int i = INT32_MAX;
+            for (int l = 0; l < R; ++l)
+                if (max_first(rv[l * NC + col], ri[l * NC + col], b, i)) { b = rv[l * NC + col]; i = ri[l * NC + col]; }
+            pv[(size_t(s) * nch + c) * N + n] = b;
+            pi[(size_t(s) * nch + c) * N + n] = i;
         }
-        g[size_t(s) * N + n] = off[s + 1] > off[s] ? best : 0.f;
-        arg[size_t(s) * N + n] = bi;
+        __syncthreads();
     }
+}
+
+// segmax pass 2: g[s][n], arg[s][n] over the chunks (0 and off[s] for an empty segment)
+__global__ void k_segmax_fold(const float* __restrict__ pv, const int32_t* __restrict__ pi,
+                              const int32_t* __restrict__ off, int nseg, int nch, int N, float* __restrict__ g,
+                              int32_t* __restrict__ arg) {
+    const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= size_t(nseg) * N) return;
+    const size_t s = e / N, n = e % N;
+    float b = -INFINITY;
+    int i = INT32_MAX;
+    for (int c = 0; c < nch; ++c) {
+        const float v = pv[(s * nch + c) * N + n];
+        const int p = pi[(s * nch + c) * N + n];
+        if (max_first(v, p, b, i)) { b = v; i = p; }
+    }
+    // an empty segment, or one whose values never compare (all -inf / NaN): the first row
+    const bool empty = off[s + 1] <= off[s];
+    g[e] = empty ? 0.f : b;
+    arg[e] = (empty || i == INT32_MAX) ? off[s] : i;
 }
 
 // dH[arg[s][n]][n] += dg[s][n]  (one writer per (s, n): no races)
@@ -250,19 +336,36 @@ int lddmm_nn_wgrad(const float* X, const float* dZ, int64_t P, int32_t K, int32_
 }
 
 int lddmm_nn_seg_colsum(const float* dZ, const int32_t* off, int32_t nseg, int32_t N, float* out, void* stream) {
-    LDDMM_REQUIRE(nseg >= 0 && N > 0, "nn_seg_colsum: bad sizes");
+    LDDMM_REQUIRE(nseg >= 0 && N > 0 && nseg < 65536, "nn_seg_colsum: bad sizes");
     if (nseg == 0) return LDDMM_OK;
-    hipLaunchKernelGGL(k_seg_colsum, dim3(unsigned(nseg)), dim3(128), 0, static_cast<hipStream_t>(stream), dZ, off, N,
-                       out);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int nch = SEG_CHUNKS;
+    void* ws;
+    LDDMM_TRY(workspace(size_t(nseg) * nch * N * sizeof(float) + 256, st, &ws));
+    float* part = static_cast<float*>(ws);
+    hipLaunchKernelGGL(k_seg_colsum_part, dim3(unsigned(nch), unsigned(nseg)), dim3(SEG_THREADS), 0, st, dZ, off, N,
+                       nch, part);
+    LDDMM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_seg_colsum_fold, dim3(nblk(size_t(nseg) * N)), dim3(256), 0, st, part, nseg, nch, N, out);
     LDDMM_CHECK_LAUNCH();
     return LDDMM_OK;
 }
 
 int lddmm_nn_segmax(const float* H, const int32_t* off, int32_t nseg, int32_t N, float* g, int32_t* arg,
                     void* stream) {
-    LDDMM_REQUIRE(nseg >= 0 && N > 0, "nn_segmax: bad sizes");
+    LDDMM_REQUIRE(nseg >= 0 && N > 0 && nseg < 65536, "nn_segmax: bad sizes");
     if (nseg == 0) return LDDMM_OK;
-    hipLaunchKernelGGL(k_segmax, dim3(unsigned(nseg)), dim3(128), 0, static_cast<hipStream_t>(stream), H, off, N, g,
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int nch = SEG_CHUNKS;
+    void* ws;
+    const size_t cnt = size_t(nseg) * nch * N;
+    LDDMM_TRY(workspace(cnt * (sizeof(float) + sizeof(int32_t)) + 512, st, &ws));
+    float* pv = static_cast<float*>(ws);
+    int32_t* pi = reinterpret_cast<int32_t*>(static_cast<char*>(ws) + ((cnt * sizeof(float) + 255) & ~size_t(255)));
+    hipLaunchKernelGGL(k_segmax_part, dim3(unsigned(nch), unsigned(nseg)), dim3(SEG_THREADS), 0, st, H, off, N, nch,
+                       pv, pi);
+    LDDMM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_segmax_fold, dim3(nblk(size_t(nseg) * N)), dim3(256), 0, st, pv, pi, off, nseg, nch, N, g,
                        arg);
     LDDMM_CHECK_LAUNCH();
     return LDDMM_OK;

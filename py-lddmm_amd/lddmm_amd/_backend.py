@@ -134,6 +134,7 @@ SIGNATURES = {
     "lddmm_nn_adam": [_P, _P, _P, _P, _P, _I64, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                       ctypes.c_float, _I32, ctypes.c_float, _P],
     "lddmm_nn_to_bf16": [_P, _P, _I64, _P],
+    "lddmm_normal_raycast": [_P, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P],
     "lddmm_lbfgs_pair_scalars": [_P, _P, _I64, _P, _P, _P],
     "lddmm_lbfgs_two_loop": [_P, _I64, _I32, _P, _P, _P, _P, _P, _P],
     "lddmm_population_create": [ctypes.POINTER(LddmmPopulationDesc), ctypes.POINTER(ctypes.c_void_p)],

@@ -58,12 +58,13 @@ def main(argv=None):
     ap.add_argument("--out", default="momentum_net.npz")
     ap.add_argument("--report", default=None)
     ap.add_argument("--time-budget", type=float, default=600.0, help="seconds for data generation")
-    ap.add_argument("--cg-iters", type=int, default=20)
+    ap.add_argument("--cg-iters", type=int, default=None,
+                    help="CG products of the velocity -> control solve (default: the network's own)")
     ap.add_argument("--oracle-check", type=int, default=5,
                     help="held-out pairs also warm-started from the plain run's own a0")
-    ap.add_argument("--variants", default="force:abar,meanshift:abar,kforce:abar",
-                    help="comma-separated feature:target[:loss] (features force/meanshift/kforce; "
-                         "targets abar/vbar/momentum/velocity; loss mse (default) or kvel)")
+    ap.add_argument("--variants", default="normal:vlocal",
+                    help="comma-separated feature:target[:loss] (features normal/force/meanshift/kforce; "
+                         "targets vlocal/abar/vbar/momentum/velocity; loss mse (default) or kvel)")
     a = ap.parse_args(argv)
     logging.getLogger().setLevel(logging.WARNING)
     cfg = dict(W.DEFAULT)
@@ -107,6 +108,7 @@ def main(argv=None):
         seed = 900000 + k
         sm, _, sec_plain = plain_with_trace(seed, cfg)
         E = sm.varifold_data()
+        obj_plain = float(sm.objectiveFun())
         it_plain = iterations_to(sm.data_trace, E)
         sec_plain_to = sec_plain * it_plain / max(len(sm.data_trace), 1)
         row = {"seed": seed, "E_plain": E, "it_plain": it_plain, "it_plain_total": len(sm.data_trace),
@@ -117,11 +119,21 @@ def main(argv=None):
             r = W.register(seed, net=net, target_data=E, cfg=cfg, cg_iters=a.cg_iters)
             row.update({f"it_warm_{target}": r["iterations"], f"sec_warm_{target}": r["seconds"],
                         f"E_warm_{target}": r["data"], f"E0_warm_{target}": r["data0"],
+                        f"obj_ratio_{target}": r["objective"] / obj_plain,
                         f"reached_{target}": r["reached"], f"used_warm_{target}": r["used_warm"]})
             tg = W.targets_of(sm, sm.control['at'])[W.TARGET_KEY[net.target]]
             fx = W.features(W.make_matching(seed, cfg), net.features)
             pred = net.predict(fx, [fx.shape[0]]).double().cpu().numpy()
             row[f"pred_rel_err_{target}"] = float(np.linalg.norm(pred - tg) / np.linalg.norm(tg))
+        if "normal" in a.variants:
+            # the same pipeline without the network: the ray-cast normal height
+            # itself as the velocity, solved with the same truncated CG
+            sm0 = W.make_matching(seed, cfg)
+            ch, fr = W.NF.normal_heights(sm0.fv0.vertices, sm0.fv0.faces, sm0.fv1.vertices, sm0.fv1.faces)
+            a_f = sm0.options['KparDiff'].solve(sm0.x0_dev, ch[:, :1] * fr[:, 0], iterMax=W.SOLVE_ITERS)
+            r = W.register(seed, at=a_f[None].expand(sm0.Tsize, -1, -1).contiguous(), target_data=E, cfg=cfg)
+            row.update({"it_feature_only": r["iterations"], "obj_ratio_feature_only": r["objective"] / obj_plain,
+                        "sec_feature_only": r["seconds"]})
         if k < a.oracle_check:
             # upper bounds: the plain run's own time-mean control (perfect predictor),
             # and its velocity K abar turned back into abar by truncated CG, exact
@@ -130,6 +142,10 @@ def main(argv=None):
             T_ = sm.Tsize
             r = W.register(seed, at=abar[None].expand(T_, -1, -1).contiguous(), target_data=E, cfg=cfg)
             row.update({"it_warm_true_abar": r["iterations"], "E0_true_abar": r["data0"]})
+            # perfect prediction through the default pipeline: K abar -> truncated CG
+            a_v = sm.options['KparDiff'].solve(sm.x0_dev, W.velocity(sm, abar), iterMax=W.SOLVE_ITERS)
+            r = W.register(seed, at=a_v[None].expand(T_, -1, -1).contiguous(), target_data=E, cfg=cfg)
+            row["it_true_vbar_solve"] = r["iterations"]
             # tolerance of the time-mean control to a smooth error: abar + e with
             # e = K(x0) white noise, scaled so that |K e| = lvl |K abar|
             K_ = sm.options['KparDiff']
@@ -158,8 +174,15 @@ def main(argv=None):
             "reached_fraction": float(np.mean([r[f"reached_{target}"] for r in rows])),
             "warm_used_fraction": float(np.mean([r[f"used_warm_{target}"] for r in rows])),
             "mean_E0_over_E": float(np.mean([r[f"E0_warm_{target}"] / r["E_plain"] for r in rows])),
+            "mean_objective_ratio": float(np.mean([r[f"obj_ratio_{target}"] for r in rows])),
         }
         rep["test"][target]["mean_pred_rel_err"] = float(np.mean([r[f"pred_rel_err_{target}"] for r in rows]))
+    if rows and "it_feature_only" in rows[0]:
+        itf = np.array([r["it_feature_only"] for r in rows], dtype=float)
+        rep["test"]["feature_only"] = {"mean_it": float(itf.mean()),
+                                       "iteration_ratio": float(itp.mean() / max(itf.mean(), 1e-9)),
+                                       "mean_objective_ratio": float(np.mean([r["obj_ratio_feature_only"]
+                                                                              for r in rows]))}
     ora = [r for r in rows if "it_warm_true_abar" in r]
     if ora:
         rep["test"]["abar_error_tolerance"] = {k: float(np.mean([r[k] for r in ora])) for k in ora[0]
@@ -167,6 +190,7 @@ def main(argv=None):
     if ora:
         rep["test"]["true_abar"] = {"pairs": len(ora),
                                   "mean_it": float(np.mean([r["it_warm_true_abar"] for r in ora])),
+                                  "mean_it_vbar_solve": float(np.mean([r["it_true_vbar_solve"] for r in ora])),
                                   "mean_E0_over_E": float(np.mean([r["E0_true_abar"] / r["E_plain"] for r in ora]))}
     rep["rows"] = rows
     log(json.dumps(rep["test"]))

@@ -69,11 +69,14 @@ class Segments:
 
 
 class MomentumNet:
-    def __init__(self, params=None, seed=0, feat_scale=1.0, out_scale=1.0, target='abar', features='force'):
-        # target: 'abar' (time-mean of the control, used as a time-constant control),
+    def __init__(self, params=None, seed=0, feat_scale=1.0, out_scale=1.0, target='abar', features='force',
+                 solve_iters=0):
+        # target: 'vlocal' (K(x0) abar in the local vertex frames), 'abar' (time-mean of
+        # the control, used as a time-constant control), 'vbar' (K(x0) abar),
         # 'momentum' (a0 = at[0], shot with EPDiff) or 'velocity' (K(x0) a0)
         self.target = target
-        self.features = features      # input channels: 'force', 'meanshift' or 'kforce'
+        self.features = features      # input channels: 'normal', 'force', 'meanshift' or 'kforce'
+        self.solve_iters = int(solve_iters)   # CG products of the velocity -> control solve (0: caller's)
 
         self.dev = B.device()
         self.lay, self.total = layout()
@@ -207,12 +210,14 @@ class MomentumNet:
 
     def save(self, path):
         np.savez(path, params=self.params_numpy(), feat_scale=self.feat_scale, out_scale=self.out_scale,
-                 target=np.array(self.target), features=np.array(self.features))
+                 target=np.array(self.target), features=np.array(self.features),
+                 solve_iters=np.array(self.solve_iters))
 
     @classmethod
     def load(cls, path):
         d = np.load(path)
         target = str(d["target"]) if "target" in d.files else 'momentum'
         features = str(d["features"]) if "features" in d.files else 'force'
+        solve_iters = int(d["solve_iters"]) if "solve_iters" in d.files else 0
         return cls(params=d["params"], feat_scale=float(d["feat_scale"]), out_scale=float(d["out_scale"]),
-                   target=target, features=features)
+                   target=target, features=features, solve_iters=solve_iters)

@@ -1,13 +1,17 @@
 """Data generation, training and evaluation of the learned warm start.
 
 Pairs are the plain surface-matching workload (icosphere subdivision 4,
-R = 10, seeded Gaussian bumps) with one seed per pair.  Features are the
-initial data-term force -grad D(x0) / sigma_err^2 at the template vertices
-(no vertex correspondence is assumed); targets are the initial control a_0 =
-at[0] that the plain optimiser reaches.  A predicted a0 is turned into a
-control with EPDiff (landmarkEPDiff), the zero control is kept whenever it
-has the lower objective (the fallback), and BFGS then runs until the
-varifold data term matches the plain method's final value within 1e-4.
+R = 10, seeded Gaussian bumps) with one seed per pair; no vertex
+correspondence between template and target is assumed.  The default model
+(features 'normal', target 'vlocal') reads per vertex the ray-cast normal
+height of the target over the template and its smoothed versions, and
+predicts the velocity K(x0) abar of the plain optimiser's time-mean control
+abar in the vertex's local frame; the warm control is the time-constant
+abar recovered from that velocity by a truncated conjugate-gradient solve
+(Kernel.solve, 10 products; the truncation keeps the control smooth).  The
+zero control is kept whenever it has the lower objective (the fallback),
+and BFGS then runs until the varifold data term matches the plain method's
+final value within 1e-4.
 """
 import logging
 import time
@@ -21,6 +25,7 @@ from .. import synth
 from ..kernelFunctions import Kernel
 from ..surfaces import Surface
 from ..surfaceMatching import SurfaceMatching
+from . import features as NF
 from .model import MomentumNet, Segments
 
 DEFAULT = dict(subdiv=4, radius=10.0, nbumps=20, width=3.0, amp_sd=1.0, sigma_def=2.5, sigma_var=1.0,
@@ -37,9 +42,10 @@ def make_matching(seed, cfg=DEFAULT, at=None):
     return SurfaceMatching(Template=Surface((f, v)), Target=Surface((f, v2)), options=opts)
 
 
-FEATURES = ('force', 'meanshift', 'kforce')
-TARGETS = ('abar', 'vbar', 'momentum', 'velocity')
-TARGET_KEY = {"abar": "abar", "vbar": "vbar", "momentum": "a0", "velocity": "v0"}
+FEATURES = ('force', 'meanshift', 'kforce', 'normal')
+TARGETS = ('abar', 'vbar', 'vlocal', 'momentum', 'velocity')
+TARGET_KEY = {"abar": "abar", "vbar": "vbar", "vlocal": "vlocal", "momentum": "a0", "velocity": "v0"}
+SOLVE_ITERS = 10      # conjugate-gradient products that turn a predicted velocity into abar
 MEANSHIFT_SIGMA = 1.5
 
 
@@ -48,7 +54,12 @@ def features(sm, kind='force'):
     force     - initial data-term force -grad D(x0) / sigma_err^2;
     meanshift - Gaussian-weighted mean of the target's vertices around each
                 template vertex minus the vertex (width MEANSHIFT_SIGMA);
-    kforce    - the force smoothed by the deformation kernel, K(x0) force."""
+    kforce    - the force smoothed by the deformation kernel, K(x0) force;
+    normal    - ray-cast normal height of the target over the template and
+                its normalised Gaussian averages (learned.features)."""
+    if kind == 'normal':
+        ch, _ = NF.normal_heights(sm.fv0.vertices, sm.fv0.faces, sm.fv1.vertices, sm.fv1.faces)
+        return ch
     if kind == 'force':
         return -sm.endPointGradient_device(sm.x0_dev)
     if kind == 'kforce':
@@ -62,6 +73,11 @@ def features(sm, kind='force'):
     raise ValueError(f"unknown feature kind {kind!r}")
 
 
+def frame(sm):
+    """Local frames [N, 3, 3] (rows: normal, two tangents) of the template vertices (device)."""
+    return B.to_dev(NF.local_frame(synth.vertex_normals(sm.fv0.vertices, sm.fv0.faces)))
+
+
 def velocity(sm, a0):
     """v0 = K(x0) a0 (device float64)."""
     return sm.options['KparDiff'].applyK(sm.x0_dev, B.to_dev(a0))
@@ -71,8 +87,10 @@ def targets_of(sm, at):
     """Training targets of a finished plain registration with control `at` [T, N, 3]."""
     a0 = at[0]
     abar = at.mean(0)
+    vbar = velocity(sm, abar)
+    vlocal = torch.einsum('nkj,nj->nk', frame(sm), vbar)
     return {"a0": a0.cpu().numpy(), "abar": abar.cpu().numpy(), "v0": velocity(sm, a0).cpu().numpy(),
-            "vbar": velocity(sm, abar).cpu().numpy(), "x0": sm.x0_dev.cpu().numpy()}
+            "vbar": vbar.cpu().numpy(), "vlocal": vlocal.cpu().numpy(), "x0": sm.x0_dev.cpu().numpy()}
 
 
 def plain_registration(seed, cfg=DEFAULT):
@@ -109,7 +127,8 @@ def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50, target='a
     A = np.concatenate([s[key] for s in samples])
     feat_scale = float(np.sqrt((F ** 2).mean())) or 1.0
     out_scale = float(np.sqrt((A ** 2).mean())) or 1.0
-    net = MomentumNet(seed=seed, feat_scale=feat_scale, out_scale=out_scale, target=target, features=features)
+    net = MomentumNet(seed=seed, feat_scale=feat_scale, out_scale=out_scale, target=target, features=features,
+                      solve_iters=SOLVE_ITERS if target == 'vlocal' else 0)
     dev = net.dev
     Xs = [torch.tensor(s[fkey] / feat_scale, dtype=torch.float32, device=dev) for s in samples]
     Ys = [torch.tensor(s[key] / out_scale, dtype=torch.float32, device=dev) for s in samples]
@@ -157,10 +176,13 @@ def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50, target='a
     return net, losses
 
 
-def warm_control(sm, net, cg_iters=20):
+def warm_control(sm, net, cg_iters=None):
     """Initial control [T, N, 3] from the network's prediction.
 
-    target 'abar': the prediction is the time-mean of the plain optimiser's
+    target 'vlocal': the prediction is K(x0) abar in each vertex's local
+    frame (normal, tangents), turned into abar by a conjugate-gradient solve
+    truncated at the network's solve_iters products (SOLVE_ITERS) and used as a
+    time-constant control; 'abar': the prediction is the time-mean of the plain optimiser's
     control, used as a time-constant control; 'vbar': the prediction is
     K(x0) abar, turned into abar by a conjugate-gradient solve (Kernel.solve,
     truncated at cg_iters products) and used as a time-constant control;
@@ -170,6 +192,12 @@ def warm_control(sm, net, cg_iters=20):
     pred = net.predict(feats, [feats.shape[0]])
     T = sm.Tsize
     target = getattr(net, 'target', 'momentum')
+    if cg_iters is None:
+        cg_iters = getattr(net, 'solve_iters', None) or 20
+    if target == 'vlocal':
+        v = torch.einsum('nkj,nk->nj', frame(sm), pred)
+        a = sm.options['KparDiff'].solve(sm.x0_dev, v, iterMax=cg_iters)
+        return a[None].expand(T, -1, -1).contiguous(), a
     if target == 'abar':
         return pred[None].expand(T, -1, -1).contiguous(), pred
     if target == 'vbar':
@@ -183,7 +211,7 @@ def warm_control(sm, net, cg_iters=20):
     return at, a0
 
 
-def register(seed, net=None, target_data=None, cfg=DEFAULT, max_iter=None, a0=None, cg_iters=20, at=None):
+def register(seed, net=None, target_data=None, cfg=DEFAULT, max_iter=None, a0=None, cg_iters=None, at=None):
     """Registration (plain, warm-started by `net`, or warm-started from a given
     initial momentum `a0`) until the data term reaches target_data (1e-4 rel).
 
@@ -224,4 +252,5 @@ def register(seed, net=None, target_data=None, cfg=DEFAULT, max_iter=None, a0=No
     d = sm.varifold_data()
     reached = tol is None or d <= tol
     return {"seed": seed, "iterations": iterations, "seconds": time.perf_counter() - t0,
-            "data": d, "data0": data0, "reached": bool(reached), "used_warm": used_warm}
+            "data": d, "data0": data0, "reached": bool(reached), "used_warm": used_warm,
+            "objective": float(sm.objectiveFun())}

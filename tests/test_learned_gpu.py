@@ -47,6 +47,38 @@ def test_forward_backward_match_oracle(net_mod, counts):
     assert rel(g, g_r) < 1e-4
 
 
+@pytest.mark.parametrize("N", [3, 128, 300])
+def test_segment_reductions_ragged(net_mod, N):
+    """Per-segment column sums and max / first argmax over ragged segments
+    (an empty one, one shorter than the chunk count, long ones), against numpy."""
+    model, torch = net_mod
+    from lddmm_amd import _backend as B
+    rng = np.random.default_rng(N)
+    counts = [0, 5, 3000, 20000, 1, 77]
+    P = sum(counts)
+    H = rng.normal(size=(P, N)).astype(np.float32)
+    H[3000:3010] = H[3005]                       # ties: the first row must win
+    segs = model.Segments(counts, "cuda")
+    Hd = torch.tensor(H, device="cuda")
+    out = torch.empty((len(counts), N), dtype=torch.float32, device="cuda")
+    g = torch.empty_like(out)
+    arg = torch.empty((len(counts), N), dtype=torch.int32, device="cuda")
+    B.check(B.LIB.lddmm_nn_seg_colsum(B.ptr(Hd), B.ptr(segs.off), len(counts), N, B.ptr(out), B.stream_handle()))
+    B.check(B.LIB.lddmm_nn_segmax(B.ptr(Hd), B.ptr(segs.off), len(counts), N, B.ptr(g), B.ptr(arg),
+                                  B.stream_handle()))
+    off = np.concatenate([[0], np.cumsum(counts)])
+    for s in range(len(counts)):
+        blk = H[off[s]:off[s + 1]].astype(np.float64)
+        if counts[s] == 0:
+            np.testing.assert_array_equal(out[s].cpu().numpy(), 0)
+            np.testing.assert_array_equal(g[s].cpu().numpy(), 0)
+            np.testing.assert_array_equal(arg[s].cpu().numpy(), off[s])
+            continue
+        np.testing.assert_allclose(out[s].cpu().numpy(), blk.sum(0), rtol=1e-4, atol=1e-3)
+        np.testing.assert_array_equal(g[s].cpu().numpy(), blk.max(0).astype(np.float32))
+        np.testing.assert_array_equal(arg[s].cpu().numpy(), off[s] + blk.argmax(0))
+
+
 def test_adam_step(net_mod):
     model, torch = net_mod
     net = model.MomentumNet(seed=1)
@@ -63,28 +95,31 @@ def test_adam_step(net_mod):
 
 
 def test_trained_network_warm_start_gain(net_mod):
-    """The committed network (mean-shift features -> time-mean momentum, trained
-    on 693 plain registrations) on held-out pairs of the plain workload: the
-    warm-started registration reaches the plain run's final varifold energy
-    (1e-4 relative) in no more iterations than the plain run, and saves
-    iterations on average.  Measured on 12 held-out pairs: 44.1 vs 49.9
-    iterations (1.13x), below the 3x target."""
+    """The committed network (ray-cast normal heights -> the plain optimiser's
+    time-mean velocity in the local vertex frames, turned into a control by a
+    10-product conjugate-gradient solve; trained on 1,200 plain registrations)
+    on held-out pairs of the plain workload: every warm-started registration
+    reaches the plain run's final varifold energy (1e-4 relative), and it
+    takes at least 3x fewer BFGS iterations in total than the zero-momentum
+    plain runs.  Measured on 60 held-out pairs: 12.9 vs 49.9 iterations
+    (3.86x); on the six seeds below 13.5 vs 50."""
     model, torch = net_mod
     import os
     from lddmm_amd.learned import warmstart as W
     from lddmm_amd.learned.experiment import plain_with_trace, iterations_to
     path = os.path.join(os.path.dirname(W.__file__), "momentum_net.npz")
     net = model.MomentumNet.load(path)
-    assert net.target == 'abar' and net.features == 'meanshift'
+    assert net.target == 'vlocal' and net.features == 'normal' and net.solve_iters == W.SOLVE_ITERS
     its_p, its_w = [], []
-    for seed in (900001, 900003):
+    for seed in range(900000, 900006):
         sm, _, _ = plain_with_trace(seed, W.DEFAULT)
         E = sm.varifold_data()
         it_plain = iterations_to(sm.data_trace, E)
         r = W.register(seed, net=net, target_data=E)
+        print(f"[warmstart] seed {seed}: plain {it_plain} it, warm {r['iterations']} it", flush=True)
         assert r["reached"]
         assert r["used_warm"]
-        assert r["iterations"] <= it_plain
+        assert r["iterations"] < it_plain
         its_p.append(it_plain)
         its_w.append(r["iterations"])
-    assert sum(its_w) < sum(its_p)
+    assert sum(its_p) >= 3 * sum(its_w)
