@@ -139,6 +139,47 @@ def registration_cfg1(torch):
     return out
 
 
+def warmstart_cfg5(torch, pairs=8):
+    """Learned warm start on held-out pairs of the plain workload (seeds
+    900000..): the zero-momentum plain registration (float64, 50 BFGS
+    iterations) sets the target data term E*; the warm-started registration
+    (committed network: ray-cast normal heights -> time-mean velocity ->
+    truncated CG control) runs until its varifold data term is within 1e-4 of
+    E*.  Wall seconds cover matching set-up, features, network, solve and BFGS
+    for both runs."""
+    import logging
+    import os
+    import numpy as np
+    from lddmm_amd.learned import warmstart as W
+    from lddmm_amd.learned.model import MomentumNet
+    logging.getLogger().setLevel(logging.WARNING)
+    net = MomentumNet.load(os.path.join(os.path.dirname(W.__file__), "momentum_net.npz"))
+    rows = []
+    for k in range(pairs):
+        seed = 900000 + k
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sm = W.make_matching(seed)
+        sm.setTargetDataEnergy(None)
+        sm.optimizeMatching()
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+        E = sm.varifold_data()
+        it_plain = next((i + 1 for i, d in enumerate(sm.data_trace) if d <= E * (1 + 1e-4)), len(sm.data_trace))
+        r = W.register(seed, net=net, target_data=E)
+        rows.append((it_plain, r["iterations"], t_plain * it_plain / max(len(sm.data_trace), 1), r["seconds"],
+                     r["reached"], r["objective"] / float(sm.objectiveFun())))
+    a = np.array([x[:4] for x in rows], dtype=float)
+    out = {"pairs": pairs, "mean_iterations_plain": float(a[:, 0].mean()), "mean_iterations_warm": float(a[:, 1].mean()),
+           "iteration_ratio": float(a[:, 0].sum() / max(a[:, 1].sum(), 1)),
+           "mean_seconds_plain_to_E": float(a[:, 2].mean()), "mean_seconds_warm_to_E": float(a[:, 3].mean()),
+           "all_reached": bool(all(x[4] for x in rows)),
+           "mean_final_objective_ratio": float(np.mean([x[5] for x in rows]))}
+    log(f"warm start cfg5: {out['mean_iterations_warm']:.1f} vs {out['mean_iterations_plain']:.1f} iterations "
+        f"({out['iteration_ratio']:.2f}x), {out['mean_seconds_warm_to_E']:.3f} vs {out['mean_seconds_plain_to_E']:.3f} s")
+    return out
+
+
 def precision_cfg3(torch):
     """Large single registration with the precision form: icosphere subdivision 6
     (40,962 vertices, 81,920 faces, R = 50), target with 100 bumps (width 8,
@@ -462,6 +503,7 @@ def main():
         extra["registration_plain_cfg1"] = registration_cfg1(torch)
         extra["population_cfg4"] = population_cfg4(torch)
         extra["precision_cfg3"] = precision_cfg3(torch)
+        extra["learned_warmstart_cfg5"] = warmstart_cfg5(torch)
 
     if not args.no_cpu:
         log("cpu baseline ...")

@@ -1,16 +1,21 @@
 """End-to-end learned warm-start experiment (run on the GPU):
 
-    python -m lddmm_amd.learned.experiment --train 2000 --test 100 --epochs 300 \
-        --out momentum_net.npz --report report.json
+    python -m lddmm_amd.learned.experiment --batched 256 --train 20000 --epochs 30 --batch 32 \
+        --test 1000 --out momentum_net.npz --report report.json
+    python -m lddmm_amd.learned.experiment --net momentum_net.npz --test 1000   # evaluate only
 
 1. plain zero-momentum registrations of seeded deformed spheres (training
    seeds 10000.., held-out seeds 900000..) record the features and targets;
+   with --batched they run that many pairs at a time through the lockstep
+   population registration (features 'normal', target 'vlocal');
 2. MomentumNet is trained on them with the package's HIP kernels, one
-   network per feature:target variant (default target abar = time-mean of
-   the plain control, used as a time-constant initial control);
+   network per feature:target variant (default: ray-cast normal heights ->
+   time-mean velocity in the local vertex frames, control by a truncated
+   conjugate-gradient solve);
 3. on held-out pairs, the plain run's final varifold data term E* is the
    target; iterations / seconds to reach E* (1e-4 relative) are compared for
-   plain (from its own per-iteration trace) and warm-started registrations.
+   plain (from its own per-iteration trace), warm-started registrations and
+   the same pipeline without the network (raw ray-cast heights).
 Progress lines go to stderr; the report is JSON.
 """
 import argparse
@@ -29,9 +34,9 @@ def log(msg):
     print(f"[warmstart] {msg}", file=sys.stderr, flush=True)
 
 
-def plain_with_trace(seed, cfg):
+def plain_with_trace(seed, cfg, with_features=True):
     sm = W.make_matching(seed, cfg)
-    feats = {f"feat_{k}": W.features(sm, k).cpu().numpy() for k in W.FEATURES}
+    feats = {f"feat_{k}": W.features(sm, k).cpu().numpy() for k in W.FEATURES} if with_features else {}
     sm.setTargetDataEnergy(None)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -58,6 +63,13 @@ def main(argv=None):
     ap.add_argument("--out", default="momentum_net.npz")
     ap.add_argument("--report", default=None)
     ap.add_argument("--time-budget", type=float, default=600.0, help="seconds for data generation")
+    ap.add_argument("--net", default=None,
+                    help="evaluate this saved network (no data generation or training)")
+    ap.add_argument("--feature-only-pairs", type=int, default=None,
+                    help="held-out pairs that also run the feature-only baseline (default: all)")
+    ap.add_argument("--batched", type=int, default=0,
+                    help="generate the training registrations this many pairs at a time with the lockstep "
+                         "population registration (features 'normal', target 'vlocal' only)")
     ap.add_argument("--cg-iters", type=int, default=None,
                     help="CG products of the velocity -> control solve (default: the network's own)")
     ap.add_argument("--oracle-check", type=int, default=5,
@@ -72,7 +84,13 @@ def main(argv=None):
 
     t0 = time.perf_counter()
     samples = []
-    for k in range(a.train):
+    if a.net:
+        from .model import MomentumNet
+        a.train = 0
+        a.variants = ""
+    if a.batched and a.train:
+        samples = W.generate_batched([10000 + k for k in range(a.train)], cfg, batch=a.batched, progress=log)
+    for k in range(a.train if not a.batched else 0):
         seed = 10000 + k
         sm, feats, sec = plain_with_trace(seed, cfg)
         smp = dict(feats)
@@ -87,7 +105,11 @@ def main(argv=None):
     rep["generation_seconds"] = time.perf_counter() - t0
 
     nets = {}
-    for variant in a.variants.split(","):
+    if a.net:
+        net = MomentumNet.load(a.net)
+        nets[f"{net.features}:{net.target}"] = net
+        a.variants = f"{net.features}:{net.target}"
+    for variant in (a.variants.split(",") if not a.net else []):
         parts = variant.split(":")
         feat_kind, target = parts[0], parts[1]
         loss_kind = parts[2] if len(parts) > 2 else 'mse'
@@ -106,7 +128,7 @@ def main(argv=None):
     rows = []
     for k in range(a.test):
         seed = 900000 + k
-        sm, _, sec_plain = plain_with_trace(seed, cfg)
+        sm, _, sec_plain = plain_with_trace(seed, cfg, with_features=False)
         E = sm.varifold_data()
         obj_plain = float(sm.objectiveFun())
         it_plain = iterations_to(sm.data_trace, E)
@@ -125,7 +147,7 @@ def main(argv=None):
             fx = W.features(W.make_matching(seed, cfg), net.features)
             pred = net.predict(fx, [fx.shape[0]]).double().cpu().numpy()
             row[f"pred_rel_err_{target}"] = float(np.linalg.norm(pred - tg) / np.linalg.norm(tg))
-        if "normal" in a.variants:
+        if "normal" in a.variants and (a.feature_only_pairs is None or k < a.feature_only_pairs):
             # the same pipeline without the network: the ray-cast normal height
             # itself as the velocity, solved with the same truncated CG
             sm0 = W.make_matching(seed, cfg)
@@ -177,12 +199,14 @@ def main(argv=None):
             "mean_objective_ratio": float(np.mean([r[f"obj_ratio_{target}"] for r in rows])),
         }
         rep["test"][target]["mean_pred_rel_err"] = float(np.mean([r[f"pred_rel_err_{target}"] for r in rows]))
-    if rows and "it_feature_only" in rows[0]:
-        itf = np.array([r["it_feature_only"] for r in rows], dtype=float)
-        rep["test"]["feature_only"] = {"mean_it": float(itf.mean()),
-                                       "iteration_ratio": float(itp.mean() / max(itf.mean(), 1e-9)),
+    fo = [r for r in rows if "it_feature_only" in r]
+    if fo:
+        itf = np.array([r["it_feature_only"] for r in fo], dtype=float)
+        itpf = np.array([r["it_plain"] for r in fo], dtype=float)
+        rep["test"]["feature_only"] = {"pairs": len(fo), "mean_it": float(itf.mean()),
+                                       "iteration_ratio": float(itpf.mean() / max(itf.mean(), 1e-9)),
                                        "mean_objective_ratio": float(np.mean([r["obj_ratio_feature_only"]
-                                                                              for r in rows]))}
+                                                                              for r in fo]))}
     ora = [r for r in rows if "it_warm_true_abar" in r]
     if ora:
         rep["test"]["abar_error_tolerance"] = {k: float(np.mean([r[k] for r in ora])) for k in ora[0]

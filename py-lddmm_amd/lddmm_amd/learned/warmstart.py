@@ -93,6 +93,33 @@ def targets_of(sm, at):
             "vbar": vbar.cpu().numpy(), "vlocal": vlocal.cpu().numpy(), "x0": sm.x0_dev.cpu().numpy()}
 
 
+def generate_batched(seeds, cfg=DEFAULT, batch=256, progress=None):
+    """Training samples {feat_normal, vlocal} of the plain registrations of
+    `seeds`, computed `batch` pairs at a time by the lockstep population
+    registration (population.PopulationMatching: per pair the same BFGS
+    iterates as SurfaceMatching, in one persistent launch per kernel sum)."""
+    from ..population import PopulationMatching
+    samples = []
+    for b0 in range(0, len(seeds), batch):
+        ss = list(seeds[b0:b0 + batch])
+        prs = [synth.plain_matching_pair(seed=s, subdiv=cfg['subdiv'], radius=cfg['radius'], nbumps=cfg['nbumps'],
+                                         width=cfg['width'], amp_sd=cfg['amp_sd']) for s in ss]
+        Kd = Kernel(name='gauss', sigma=cfg['sigma_def'])
+        pm = PopulationMatching([Surface((f, v)) for (v, f), _ in prs], [Surface((f, v2)) for (_, f), (v2, _) in prs],
+                                {'KparDiff': Kd, 'KparDist': Kernel(name='gauss', sigma=cfg['sigma_var']),
+                                 'timeStep': cfg['timeStep'], 'pk_dtype': cfg['pk_dtype'], 'maxIter': cfg['maxIter']})
+        pm.optimizeMatching()
+        for p, ((v, f), (v2, _)) in enumerate(prs):
+            ch, fr = NF.normal_heights(v, f, v2, f)
+            vbar = Kd.applyK(B.to_dev(v), pm.control_of(p).mean(0))
+            samples.append({"feat_normal": ch.cpu().numpy(),
+                            "vlocal": torch.einsum('nkj,nj->nk', fr, vbar).cpu().numpy()})
+        pm.pop.close()
+        if progress is not None:
+            progress(f"batched plain registrations: {len(samples)} / {len(seeds)}")
+    return samples
+
+
 def plain_registration(seed, cfg=DEFAULT):
     """Zero-momentum plain registration: every feature kind, the targets
     (abar, a0, v0), final data term, iterations, seconds."""

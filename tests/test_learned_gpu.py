@@ -97,12 +97,12 @@ def test_adam_step(net_mod):
 def test_trained_network_warm_start_gain(net_mod):
     """The committed network (ray-cast normal heights -> the plain optimiser's
     time-mean velocity in the local vertex frames, turned into a control by a
-    10-product conjugate-gradient solve; trained on 1,200 plain registrations)
+    10-product conjugate-gradient solve; trained on 20,000 plain registrations)
     on held-out pairs of the plain workload: every warm-started registration
     reaches the plain run's final varifold energy (1e-4 relative), and it
     takes at least 3x fewer BFGS iterations in total than the zero-momentum
-    plain runs.  Measured on 60 held-out pairs: 12.9 vs 49.9 iterations
-    (3.86x); on the six seeds below 13.5 vs 50."""
+    plain runs.  Measured on 1,000 held-out pairs: 12.04 vs 49.85 iterations
+    (4.14x); on the six seeds below 80 vs 299."""
     model, torch = net_mod
     import os
     from lddmm_amd.learned import warmstart as W
