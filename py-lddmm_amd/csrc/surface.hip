@@ -44,18 +44,14 @@ __global__ void k_face_features(const double* __restrict__ v, const int32_t* __r
 
 // Energy: E = fK * sum_i aw_i row_i, one block, fixed order (deterministic).
 template <typename T>
-__global__ void __launch_bounds__(1024) k_vf_energy(Partials<T> part, int nchunk, const FaceRec* f1, int n1,
-                                                    double fK, double* energy) {
-    __shared__ double red[1024];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n1; i += 1024) s += f1[i].aw * sum_range(part, 0, nchunk, 1, i, 0);
-    red[threadIdx.x] = s;
+__global__ void __launch_bounds__(256) k_vf_energy_rows(Partials<T> part, int nchunk, const FaceRec* f1, int n1,
+                                                        double fK, double* rows) {
+    __shared__ double red[kEpiLanes][kEpiRows];
+    const int fl = threadIdx.x % kEpiRows, lane = threadIdx.x / kEpiRows;
+    const int i = blockIdx.x * kEpiRows + fl;
+    red[lane][fl] = i < n1 ? sum_lane(part, 0, nchunk, 1, i, 0, lane) : 0.0;
     __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *energy = fK * red[0];
+    if (lane == 0 && i < n1) rows[i] = fK * f1[i].aw * (((red[0][fl] + red[1][fl]) + red[2][fl]) + red[3][fl]);
 }
 
 // Gradient epilogue per face: z1, dz1 and the energy row.
@@ -63,25 +59,40 @@ __global__ void __launch_bounds__(1024) k_vf_energy(Partials<T> part, int nchunk
 // = mean of 3 vertices).  Curves (base/curveDistances.py:280-336): zf = 2,
 // sd = sqrt(d), dzf = 1 (no 1/3 factor; the midpoint's 1/2 cancels the 2 of
 // the symmetric term).  z1 = fw zf (2 sd S + (A - B) n) is the weighted z1.
+// 64 faces per workgroup, 4 chunk lanes per face.
 template <typename T>
-__global__ void k_vf_grad_epi(Partials<T> part, int c1, int nchunk, const FaceRec* f1, int n1, double fK,
-                              double fD, double zf, double sd, double dzf, double* z1, double* dz1,
-                              double* erow) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n1) return;
-    double S[3], D[3];
-    for (int c = 0; c < 3; ++c) {
-        S[c] = sum_range(part, 0, nchunk, 8, i, c);
-        D[c] = sum_range(part, 0, nchunk, 8, i, 5 + c);
+__global__ void __launch_bounds__(256) k_vf_grad_epi(Partials<T> part, int c1, int nchunk, const FaceRec* f1, int n1,
+                                                     double fK, double fD, double zf, double sd, double dzf,
+                                                     double* z1, double* dz1, double* erow) {
+    constexpr int NV = 10;       // S[3], D[3], A1, A2, B1, B2
+    __shared__ double red[NV][kEpiLanes][kEpiRows];
+    const int fl = threadIdx.x % kEpiRows, lane = threadIdx.x / kEpiRows;
+    const int i = blockIdx.x * kEpiRows + fl;
+    double v[NV] = {};
+    if (i < n1) {
+        for (int c = 0; c < 3; ++c) {
+            v[c] = sum_lane(part, 0, nchunk, 8, i, c, lane);
+            v[3 + c] = sum_lane(part, 0, nchunk, 8, i, 5 + c, lane);
+        }
+        v[6] = sum_lane(part, 0, c1, 8, i, 3, lane);
+        v[7] = sum_lane(part, c1, nchunk, 8, i, 3, lane);
+        v[8] = sum_lane(part, 0, c1, 8, i, 4, lane);
+        v[9] = sum_lane(part, c1, nchunk, 8, i, 4, lane);
     }
-    double A1 = sum_range(part, 0, c1, 8, i, 3), A2 = sum_range(part, c1, nchunk, 8, i, 3);
-    double B1 = sum_range(part, 0, c1, 8, i, 4), B2 = sum_range(part, c1, nchunk, 8, i, 4);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) red[q][lane][fl] = v[q];
+    __syncthreads();
+    if (lane != 0 || i >= n1) return;
+    double t[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) t[q] = ((red[q][0][fl] + red[q][1][fl]) + red[q][2][fl]) + red[q][3][fl];
+    const double A1 = t[6], A2 = t[7], B1 = t[8], B2 = t[9];
     const FaceRec f = f1[i];
-    double AmB = (A1 + A2) - (B1 + B2);
-    double n[3] = {f.n0, f.n1, f.n2};
+    const double AmB = (A1 + A2) - (B1 + B2);
+    const double n[3] = {f.n0, f.n1, f.n2};
     for (int c = 0; c < 3; ++c) {
-        z1[3 * size_t(i) + c] = f.fw * fK * zf * (2.0 * sd * S[c] + AmB * n[c]);
-        dz1[3 * size_t(i) + c] = dzf * fD * f.aw * D[c];
+        z1[3 * size_t(i) + c] = f.fw * fK * zf * (2.0 * sd * t[c] + AmB * n[c]);
+        dz1[3 * size_t(i) + c] = dzf * fD * f.aw * t[3 + c];
     }
     // surface-2 sources carry weight -a_j here; the cross term of the energy is -2<S1,S2>
     erow[i] = f.aw * fK * ((A1 + B1) + 2.0 * (A2 + B2));
@@ -212,7 +223,7 @@ int varifold_core(const lddmm_kernel* kd, int n1, int n2, bool has_tgt, VfShape 
     } else {
         if (R1 == 2) LDDMM_TRY((plan_for<T, OpVarifoldE<MS>, 2>(n1, n1 + n2, &plan)));
         else LDDMM_TRY((plan_for<T, OpVarifoldE<MS>, 1>(n1, n1 + n2, &plan)));
-        need += partial_bytes<T>(plan, 1, n1);
+        need += partial_bytes<T>(plan, 1, n1) + size_t(n1) * sizeof(double) + 256;
     }
     void* ws;
     LDDMM_TRY(workspace(need, st, &ws));
@@ -231,7 +242,11 @@ int varifold_core(const lddmm_kernel* kd, int n1, int n2, bool has_tgt, VfShape 
         Partials<T> part = carve_partials<T>(base, cv, plan, 1, n1);
         if (R1 == 2) LDDMM_TRY((run_pair<T, OpVarifoldE<MS>, 2>(A, n1, n1 + n2, plan, part, st)));
         else LDDMM_TRY((run_pair<T, OpVarifoldE<MS>, 1>(A, n1, n1 + n2, plan, part, st)));
-        hipLaunchKernelGGL((k_vf_energy<T>), dim3(1), dim3(1024), 0, st, part, plan.nchunk, r1, n1, fK, energy);
+        double* rows = reinterpret_cast<double*>(base + cv.take(size_t(n1) * sizeof(double)));
+        hipLaunchKernelGGL((k_vf_energy_rows<T>), dim3(unsigned((n1 + kEpiRows - 1) / kEpiRows)), dim3(256), 0, st,
+                           part, plan.nchunk, r1, n1, fK, rows);
+        LDDMM_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(1024), 0, st, rows, n1, energy);
         LDDMM_CHECK_LAUNCH();
         return LDDMM_OK;
     }
@@ -245,7 +260,7 @@ int varifold_core(const lddmm_kernel* kd, int n1, int n2, bool has_tgt, VfShape 
     double* erow = reinterpret_cast<double*>(base + cv.take(size_t(n1) * sizeof(double)));
     if (R1 == 2) LDDMM_TRY((run_pair<T, OpVarifoldG<MS>, 2>(A, n1, n1 + n2, plan, part, st)));
     else LDDMM_TRY((run_pair<T, OpVarifoldG<MS>, 1>(A, n1, n1 + n2, plan, part, st)));
-    hipLaunchKernelGGL((k_vf_grad_epi<T>), dim3(unsigned((n1 + 255) / 256)), dim3(256), 0, st, part, c1,
+    hipLaunchKernelGGL((k_vf_grad_epi<T>), dim3(unsigned((n1 + kEpiRows - 1) / kEpiRows)), dim3(256), 0, st, part, c1,
                        plan.nchunk, r1, n1, fK, fD, shp.zf, shp.sd, shp.dzf, z1, dz1, erow);
     LDDMM_CHECK_LAUNCH();
     if (energy) {
