@@ -92,6 +92,17 @@ PairPlan plan_pair(int n, int m, int R, int num_cu, int occ) {
         double cost = double(rounds) * double(chunk + kTile / 2) + 0.25 * se;
         if (cost < best) { best = cost; p.nchunk = se; p.chunk = chunk; }
     }
+    // small problems: when whole-tile chunks leave workgroup slots empty, also
+    // try sub-tile chunks (the kernel's last partial tile carries padding)
+    if (long(p.bx) * p.nchunk < cap) {
+        for (int chunk : {kTile / 2, kTile / 4}) {
+            int se = (m + chunk - 1) / chunk;
+            long blocks = long(p.bx) * se;
+            long rounds = (blocks + cap - 1) / cap;
+            double cost = double(rounds) * double(chunk + kTile / 2) + 0.25 * se;
+            if (cost < best) { best = cost; p.nchunk = se; p.chunk = chunk; }
+        }
+    }
     return p;
 }
 

@@ -218,7 +218,7 @@ int launch_fill(double* p, size_t count, double v, hipStream_t st);
 struct PairPlan {
     int bx = 0;       // target blocks
     int nchunk = 0;   // source chunks (grid.y)
-    int chunk = 0;    // sources per chunk (multiple of kTile)
+    int chunk = 0;    // sources per chunk (a multiple of kTile, or kTile/2 or kTile/4 for small problems)
 };
 
 PairPlan plan_pair(int n, int m, int R, int num_cu, int occ_blocks);
