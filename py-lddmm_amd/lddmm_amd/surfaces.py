@@ -187,8 +187,13 @@ class Surface:
         a dense V x V table, base/surfaces.py:61-116)."""
         f = self.faces
         e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
-        es = np.sort(e, axis=1)
-        uniq, inv, cnt = np.unique(es, axis=0, return_inverse=True, return_counts=True)
+        es = np.sort(e, axis=1).astype(np.int64)
+        # one int64 key per edge, lo * nv + hi: its 1-D order is the lexicographic
+        # order of (lo, hi) rows, and a 1-D unique is far cheaper than unique(axis=0)
+        nv = max(int(self.vertices.shape[0]), int(es.max()) + 1 if es.size else 1)
+        keys, inv, cnt = np.unique(es[:, 0] * nv + es[:, 1], return_inverse=True, return_counts=True)
+        uniq = np.stack([keys // nv, keys % nv], 1).astype(f.dtype)
+        inv = inv.reshape(-1)
         self.edges = uniq
         nf = f.shape[0]
         self.faceEdges = inv.reshape(3, nf).T

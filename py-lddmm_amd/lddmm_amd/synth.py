@@ -14,6 +14,8 @@ workloads here need exact vertex counts, so they are built from closed forms:
   "width" is the Gaussian sigma of each bump and amp_sd its amplitude standard
   deviation.
 """
+import functools
+
 import numpy as np
 
 
@@ -30,6 +32,15 @@ def _orient_outward(v, f):
 
 
 def icosphere(subdiv, radius=1.0):
+    """Icosahedron subdivided `subdiv` times, projected to the sphere of the
+    given radius: V = 10 * 4^subdiv + 2 vertices, F = 20 * 4^subdiv outward faces.
+    The unit mesh of each subdivision level is built once and copied."""
+    v, f = _unit_icosphere(int(subdiv))
+    return v * radius, f.copy()
+
+
+@functools.lru_cache(maxsize=8)
+def _unit_icosphere(subdiv):
     t = (1.0 + 5 ** 0.5) / 2
     v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t),
          (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
@@ -54,8 +65,11 @@ def icosphere(subdiv, radius=1.0):
             ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
             nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
         faces = np.array(nf, dtype=np.int64)
-    V = np.array(verts) * radius
-    return V, _orient_outward(V, faces)
+    V = np.array(verts)
+    f = _orient_outward(V, faces)
+    V.flags.writeable = False
+    f.flags.writeable = False
+    return V, f
 
 
 def vertex_normals(v, f):
