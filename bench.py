@@ -328,6 +328,9 @@ def main():
     ap.add_argument("--probe", action="store_true",
                     help="only launch the dominant kernel (FP32 applyDiffKT plain, sigma 2.5) 3 times and a "
                          "256 MiB device copy 3 times, for rocprofv3 counter runs")
+    ap.add_argument("--registration", action="store_true",
+                    help="only run the plain registration (icosphere 4, float64 and float32) three times, "
+                         "for rocprofv3 kernel traces of the registration loop")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -346,6 +349,10 @@ def main():
         sys.exit(1)
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.registration:
+        for rep in range(3):
+            registration_cfg1(torch)
+        return
     N = M = args.n
     y_h = synth.uniform_cloud(N, seed=1)
     x_h = synth.uniform_cloud(M, seed=2)

@@ -399,6 +399,22 @@ int lddmm_lbfgs_pair_scalars(const double* s, const double* y, int64_t n, double
 int lddmm_lbfgs_two_loop(const double* g, int64_t n, int32_t m, const double* const* s,
                          const double* const* y, const double* const* rho, const double* c_newest,
                          double* q, void* stream);
+/* The same product in the compact form (Byrd-Nocedal-Schnabel), for m <= 64
+ * stored pairs, in four launches instead of 2m + 1 (base/bfgs.py:188-207
+ * computes it by the two-loop recursion; the results agree to rounding).
+ * lddmm_lbfgs_multidot: out[p * nt + t] = <probes[p], targets[t]> for
+ * np <= 2 probes and nt <= 128 targets of n doubles (host arrays of device
+ * pointers), reduced in a fixed order.  lddmm_lbfgs_compact: q = H g; s, y as
+ * in lddmm_lbfgs_two_loop; cols[j] (device, 2 * col_len[j] doubles) holds the
+ * dots of y_j with s_0..s_j then with y_0..y_j of the pairs stored up to j
+ * (older pairs that have since left the window may lead each half;
+ * col_len[j] >= j + 1); gdots = [<s_i, g>, <y_i, g>] (2m doubles); q may
+ * equal g. */
+int lddmm_lbfgs_multidot(const double* const* probes, int32_t np, const double* const* targets, int32_t nt,
+                         int64_t n, double* out, void* stream);
+int lddmm_lbfgs_compact(const double* g, int64_t n, int32_t m, const double* const* s, const double* const* y,
+                        const double* const* cols, const int32_t* col_len, const double* gdots, double* q,
+                        void* stream);
 
 /* ------------------------------------------------------------------------
  * Population registration: P template-target surface pairs of uneven sizes

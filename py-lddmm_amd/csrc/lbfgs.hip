@@ -25,9 +25,17 @@ static_assert(kLbGrid == kLbBlock, "a workgroup reduces one partial per lane");
 
 // Per-device state: two [kLbGrid] partial buffers (alternating launches),
 // [2*kLbMaxPairs + 8] scalars.
+constexpr int kLbcMaxPairs = 64;        // compact form: stored pairs per call
+constexpr int kLbcMaxProbes = 2;
+constexpr int kLbcMaxTargets = 2 * kLbcMaxPairs;
+constexpr int kLbcMaxSlices = 512;
+constexpr int kLbcGroup = 8;            // targets per multidot workgroup
+
 struct LbState {
     double* partials[2] = {nullptr, nullptr};
     double* scal = nullptr;
+    double* mdpart = nullptr;           // [kLbcMaxSlices][kLbcMaxProbes][kLbcMaxTargets]
+    double* coef = nullptr;             // [2 * kLbcMaxPairs + 1]
 };
 std::mutex g_lb_mutex;
 std::unordered_map<int, LbState> g_lb;
@@ -41,6 +49,9 @@ int lb_state(LbState* out) {
         LbState s;
         for (auto& p : s.partials) LDDMM_HIP(hipMalloc(reinterpret_cast<void**>(&p), kLbGrid * sizeof(double)));
         LDDMM_HIP(hipMalloc(reinterpret_cast<void**>(&s.scal), (2 * kLbMaxPairs + 8) * sizeof(double)));
+        LDDMM_HIP(hipMalloc(reinterpret_cast<void**>(&s.mdpart),
+                            size_t(kLbcMaxSlices) * kLbcMaxProbes * kLbcMaxTargets * sizeof(double)));
+        LDDMM_HIP(hipMalloc(reinterpret_cast<void**>(&s.coef), (2 * kLbcMaxPairs + 1) * sizeof(double)));
         it = g_lb.emplace(dev, s).first;
     }
     *out = it->second;
@@ -115,6 +126,145 @@ __global__ void __launch_bounds__(kLbBlock) k_lb_pair_scalars(const double* pys,
         *rho = 1.0 / ys;
         double v = ys / yy;
         *c = v < 1e-10 ? 1.0 : v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Compact form of the same inverse-Hessian product (Byrd, Nocedal and Schnabel,
+// "Representations of quasi-Newton matrices and their use in limited memory
+// methods", Math. Programming 63, 1994): with S = [s_0 .. s_{m-1}],
+// Y = [y_0 .. y_{m-1}] (oldest first), R = upper triangle of S^T Y (diagonal
+// included), D = diag(s_i . y_i) and H0 = gamma I,
+//   H g = gamma g + S w - gamma Y u,   u = R^{-1} S^T g,
+//   w = R^{-T} ((D + gamma Y^T Y) u - gamma Y^T g).
+// Equal to the two-loop recursion in exact arithmetic; it needs the m x m
+// Gram blocks (cached per stored pair by the caller), S^T g and Y^T g (one
+// batched dot launch) and one pass over the vectors.
+// ---------------------------------------------------------------------------
+struct LbcPtrs {
+    const double* v[kLbcMaxTargets];
+};
+
+// Pass 1 of the batched dots: workgroup (slice, group) adds, for up to two
+// probes and the group's kLbcGroup targets, the products over its slice of
+// the vectors; lanes are combined by a fixed tree.  part[(slice * np + p) * nt + t].
+__global__ void __launch_bounds__(kLbBlock) k_lbc_dots(LbcPtrs probes, int np, LbcPtrs targets, int nt, int64_t n,
+                                                       int64_t slice, double* __restrict__ part) {
+    __shared__ double red[kLbcMaxProbes * kLbcGroup][kLbBlock];
+    const int t0 = blockIdx.y * kLbcGroup;
+    const int64_t e0 = int64_t(blockIdx.x) * slice, e1 = e0 + slice < n ? e0 + slice : n;
+    double acc[kLbcMaxProbes][kLbcGroup];
+#pragma unroll
+    for (int p = 0; p < kLbcMaxProbes; ++p)
+#pragma unroll
+        for (int k = 0; k < kLbcGroup; ++k) acc[p][k] = 0.0;
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += kLbBlock) {
+        const double a0 = probes.v[0][e];
+        const double a1 = np > 1 ? probes.v[1][e] : 0.0;
+#pragma unroll
+        for (int k = 0; k < kLbcGroup; ++k) {
+            if (t0 + k < nt) {
+                const double b = targets.v[t0 + k][e];
+                acc[0][k] = fma(a0, b, acc[0][k]);
+                acc[1][k] = fma(a1, b, acc[1][k]);
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < kLbcMaxProbes; ++p)
+#pragma unroll
+        for (int k = 0; k < kLbcGroup; ++k) red[p * kLbcGroup + k][threadIdx.x] = acc[p][k];
+    __syncthreads();
+    for (int h = kLbBlock / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+#pragma unroll
+            for (int q = 0; q < kLbcMaxProbes * kLbcGroup; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x < kLbcMaxProbes * kLbcGroup) {
+        const int p = threadIdx.x / kLbcGroup, k = threadIdx.x % kLbcGroup;
+        if (p < np && t0 + k < nt) part[(size_t(blockIdx.x) * np + p) * nt + t0 + k] = red[threadIdx.x][0];
+    }
+}
+
+// Pass 2: out[p * nt + t] = sum over slices (one workgroup per dot, fixed tree).
+__global__ void __launch_bounds__(kLbBlock) k_lbc_dots_fold(const double* __restrict__ part, int nslice, int ndot,
+                                                            double* __restrict__ out) {
+    __shared__ double red[kLbBlock];
+    const int d = blockIdx.x;
+    double acc = 0.0;
+    for (int sl = threadIdx.x; sl < nslice; sl += kLbBlock) acc += part[size_t(sl) * ndot + d];
+    red[threadIdx.x] = acc;
+    lb_tree(red);
+    if (threadIdx.x == 0) out[d] = red[0];
+}
+
+// Column j of the cached Gram blocks: cols.v[j] holds 2 * len[j] dots of y_j
+// with s and y of the pairs up to j (window order when j was stored, j last).
+struct LbcCols {
+    const double* v[kLbcMaxPairs];
+    int len[kLbcMaxPairs];
+};
+
+// One wave: the m x m triangular solves; coef = [w (m), -gamma u (m), gamma].
+__global__ void __launch_bounds__(64) k_lbc_solve(int m, LbcCols cols, const double* __restrict__ gdots,
+                                                  double* __restrict__ coef) {
+    __shared__ double R[kLbcMaxPairs][kLbcMaxPairs + 1];
+    __shared__ double YY[kLbcMaxPairs][kLbcMaxPairs + 1];
+    __shared__ double u[kLbcMaxPairs], b[kLbcMaxPairs], w[kLbcMaxPairs];
+    const int t = threadIdx.x;
+    if (t < m) {
+        const int L = cols.len[t];
+        const double* c = cols.v[t];
+        for (int i = 0; i <= t; ++i) {
+            const int o = L - 1 - (t - i);
+            R[i][t] = c[o];
+            YY[i][t] = c[L + o];
+            YY[t][i] = c[L + o];
+        }
+        b[t] = gdots[t];                       // S^T g
+    }
+    __syncthreads();
+    double gamma = R[m - 1][m - 1] / YY[m - 1][m - 1];
+    if (gamma < 1e-10) gamma = 1.0;
+    // u = R^{-1} S^T g (back substitution, column by column)
+    for (int i = m - 1; i >= 0; --i) {
+        if (t == 0) u[i] = b[i] / R[i][i];
+        __syncthreads();
+        if (t < i) b[t] -= R[t][i] * u[i];
+        __syncthreads();
+    }
+    // b = (D + gamma Y^T Y) u - gamma Y^T g, then w = R^{-T} b (forward substitution)
+    if (t < m) {
+        double yu = 0.0;
+        for (int j = 0; j < m; ++j) yu = fma(YY[t][j], u[j], yu);
+        b[t] = R[t][t] * u[t] + gamma * yu - gamma * gdots[m + t];
+    }
+    __syncthreads();
+    for (int i = 0; i < m; ++i) {
+        if (t == 0) w[i] = b[i] / R[i][i];
+        __syncthreads();
+        if (t > i && t < m) b[t] -= R[i][t] * w[i];
+        __syncthreads();
+    }
+    if (t < m) {
+        coef[t] = w[t];
+        coef[m + t] = -gamma * u[t];
+    }
+    if (t == 0) coef[2 * m] = gamma;
+}
+
+// q = gamma g + sum_i coef[i] s_i + sum_i coef[m + i] y_i  (q may alias g).
+__global__ void __launch_bounds__(kLbBlock) k_lbc_combine(int64_t n, int m, const double* g, LbcPtrs sy,
+                                                          const double* __restrict__ coef, double* q) {
+    __shared__ double cf[kLbcMaxTargets + 1];
+    for (int k = threadIdx.x; k <= 2 * m; k += kLbBlock) cf[k] = coef[k];
+    __syncthreads();
+    const double gamma = cf[2 * m];
+    for (int64_t e = int64_t(blockIdx.x) * kLbBlock + threadIdx.x; e < n; e += int64_t(gridDim.x) * kLbBlock) {
+        double v = gamma * g[e];
+        for (int k = 0; k < 2 * m; ++k) v = fma(cf[k], sy.v[k][e], v);
+        q[e] = v;
     }
 }
 
@@ -206,5 +356,56 @@ extern "C" int lddmm_lbfgs_two_loop(const double* g, int64_t n, int32_t m, const
         last = S.partials[buf];
         buf ^= 1;
     }
+    return LDDMM_OK;
+}
+
+extern "C" int lddmm_lbfgs_multidot(const double* const* probes, int32_t np, const double* const* targets,
+                                    int32_t nt, int64_t n, double* out, void* stream) {
+    LDDMM_REQUIRE(n >= 1 && probes && targets && out, "lbfgs_multidot: bad arguments");
+    LDDMM_REQUIRE(np >= 1 && np <= kLbcMaxProbes, "lbfgs_multidot: np must be in [1, %d]", kLbcMaxProbes);
+    LDDMM_REQUIRE(nt >= 1 && nt <= kLbcMaxTargets, "lbfgs_multidot: nt must be in [1, %d]", kLbcMaxTargets);
+    LbcPtrs P{}, T{};
+    for (int p = 0; p < np; ++p) { LDDMM_REQUIRE(probes[p], "lbfgs_multidot: null probe"); P.v[p] = probes[p]; }
+    for (int t = 0; t < nt; ++t) { LDDMM_REQUIRE(targets[t], "lbfgs_multidot: null target"); T.v[t] = targets[t]; }
+    LbState S;
+    LDDMM_TRY(lb_state(&S));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // slices of whole workgroup passes; at most kLbcMaxSlices of them
+    int64_t slice = std::max<int64_t>(4 * kLbBlock, (n + kLbcMaxSlices - 1) / kLbcMaxSlices);
+    slice = (slice + kLbBlock - 1) / kLbBlock * kLbBlock;
+    const int nslice = int((n + slice - 1) / slice);
+    dim3 grid(unsigned(nslice), unsigned((nt + kLbcGroup - 1) / kLbcGroup));
+    hipLaunchKernelGGL(k_lbc_dots, grid, dim3(kLbBlock), 0, st, P, int(np), T, int(nt), n, slice, S.mdpart);
+    LDDMM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_lbc_dots_fold, dim3(unsigned(np * nt)), dim3(kLbBlock), 0, st, S.mdpart, nslice,
+                       int(np * nt), out);
+    LDDMM_CHECK_LAUNCH();
+    return LDDMM_OK;
+}
+
+extern "C" int lddmm_lbfgs_compact(const double* g, int64_t n, int32_t m, const double* const* s,
+                                   const double* const* y, const double* const* cols, const int32_t* col_len,
+                                   const double* gdots, double* q, void* stream) {
+    LDDMM_REQUIRE(n >= 1 && g && q && gdots, "lbfgs_compact: bad arguments");
+    LDDMM_REQUIRE(m >= 1 && m <= kLbcMaxPairs, "lbfgs_compact: m must be in [1, %d]", kLbcMaxPairs);
+    LDDMM_REQUIRE(s && y && cols && col_len, "lbfgs_compact: null history");
+    LbcPtrs V{};
+    LbcCols C{};
+    for (int k = 0; k < m; ++k) {
+        LDDMM_REQUIRE(s[k] && y[k] && cols[k], "lbfgs_compact: null history entry %d", k);
+        LDDMM_REQUIRE(col_len[k] >= k + 1, "lbfgs_compact: column %d holds %d pairs, needs %d", k, col_len[k], k + 1);
+        V.v[k] = s[k];
+        V.v[m + k] = y[k];
+        C.v[k] = cols[k];
+        C.len[k] = col_len[k];
+    }
+    LbState S;
+    LDDMM_TRY(lb_state(&S));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_lbc_solve, dim3(1), dim3(64), 0, st, int(m), C, gdots, S.coef);
+    LDDMM_CHECK_LAUNCH();
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n + kLbBlock - 1) / kLbBlock, 4096));
+    hipLaunchKernelGGL(k_lbc_combine, dim3(unsigned(blocks)), dim3(kLbBlock), 0, st, n, int(m), g, V, S.coef, q);
+    LDDMM_CHECK_LAUNCH();
     return LDDMM_OK;
 }

@@ -183,20 +183,46 @@ __device__ __forceinline__ double sum_chunks(const Partials<T>& p, int nchunk, i
     }
 }
 
+// Chunk sums spread over kEpiLanes lanes per row: lane l adds chunks
+// k0 + l, k0 + l + kEpiLanes, ...; epilogues combine the lanes in lane order.
+constexpr int kEpiLanes = 4;
+constexpr int kEpiRows = 64;     // rows per 256-lane epilogue workgroup
+
+template <typename T>
+__device__ __forceinline__ double sum_lane(const Partials<T>& p, int k0, int k1, int nch, int i, int c, int lane) {
+    if constexpr (kCompensated<T>) {
+        float h = 0.f, l = 0.f;
+        for (int k = k0 + lane; k < k1; k += kEpiLanes) {
+            size_t o = (size_t(k) * nch + c) * size_t(p.n) + i;
+            two_sum_acc(h, l, p.hi[o]);
+            l += p.lo[o];
+        }
+        return double(h) + double(l);
+    } else {
+        double s = 0.0;
+        for (int k = k0 + lane; k < k1; k += kEpiLanes) s += p.hi[(size_t(k) * nch + c) * size_t(p.n) + i];
+        return s;
+    }
+}
+
 struct ChanScale { double s[8]; };
 
 // out[o] = (base ? base[o] : (accumulate ? out[o] : 0)) + alpha * scale_c * v,
 // o = i*ldo + col0 + c, v = sum over chunks of channel c; partials hold `nch`
 // channels per chunk, of which the first `nout` are written.
-// One thread per (i, c): c = blockIdx.y.
+// 64 rows x 4 chunk lanes per workgroup; c = blockIdx.y.
 template <typename T>
-__global__ void k_reduce(Partials<T> part, int nchunk, int nch, int nout, int n, ChanScale sc,
-                         double* out, int ldo, int col0, int accumulate, const double* base,
-                         double alpha) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_reduce(Partials<T> part, int nchunk, int nch, int nout, int n, ChanScale sc,
+                                                double* out, int ldo, int col0, int accumulate, const double* base,
+                                                double alpha) {
+    __shared__ double red[kEpiLanes][kEpiRows];
+    const int fl = threadIdx.x % kEpiRows, lane = threadIdx.x / kEpiRows;
+    const int i = blockIdx.x * kEpiRows + fl;
     const int c = blockIdx.y;
-    if (i >= n || c >= nout) return;
-    double v = alpha * sum_chunks(part, nchunk, nch, i, c, nullptr) * sc.s[c];
+    red[lane][fl] = (i < n && c < nout) ? sum_lane(part, 0, nchunk, nch, i, c, lane) : 0.0;
+    __syncthreads();
+    if (lane != 0 || i >= n || c >= nout) return;
+    const double v = alpha * (((red[0][fl] + red[1][fl]) + red[2][fl]) + red[3][fl]) * sc.s[c];
     size_t o = size_t(i) * ldo + col0 + c;
     if (base) out[o] = base[o] + v;
     else if (accumulate) out[o] += v;
@@ -264,7 +290,7 @@ template <typename T>
 int run_reduce(const Partials<T>& part, int nchunk, int nch, int nout, int n, const ChanScale& sc,
                double* out, int ldo, int col0, int accumulate, const double* base, double alpha,
                hipStream_t st) {
-    int nb = (n + 255) / 256;
+    int nb = (n + kEpiRows - 1) / kEpiRows;
     hipLaunchKernelGGL((k_reduce<T>), dim3(nb, std::max(1, nout)), dim3(256), 0, st, part, nchunk, nch, nout, n,
                        sc, out,
                        ldo, col0, accumulate, base, alpha);

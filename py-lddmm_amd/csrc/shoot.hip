@@ -15,28 +15,37 @@ namespace lddmm {
 namespace {
 
 // x_next = x + dt v; dots[block] = scale * sum_i a_i . v_i   (v = summed chunk partials)
+// 64 rows x 4 chunk lanes per workgroup (lanes combined in lane order).
 template <typename T, int KW>
-__global__ void k_shoot_epi(Partials<T> part, int nchunk, int n, const double* __restrict__ x,
-                            const double* __restrict__ a, double dt, double* __restrict__ xn,
-                            double* __restrict__ dots, double dscale) {
-    __shared__ double red[256];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double d = 0.0;
-    if (i < n) {
-        for (int c = 0; c < KW; ++c) {
-            double v = sum_chunks(part, nchunk, KW, i, c, nullptr);
-            size_t o = size_t(i) * KW + c;
-            xn[o] = x[o] + dt * v;
-            d += a[o] * v;
-        }
-    }
-    red[threadIdx.x] = d;
+__global__ void __launch_bounds__(256) k_shoot_epi(Partials<T> part, int nchunk, int n, const double* __restrict__ x,
+                                                   const double* __restrict__ a, double dt, double* __restrict__ xn,
+                                                   double* __restrict__ dots, double dscale) {
+    __shared__ double red[KW][kEpiLanes][kEpiRows];
+    __shared__ double dred[kEpiRows];
+    const int fl = threadIdx.x % kEpiRows, lane = threadIdx.x / kEpiRows;
+    const int i = blockIdx.x * kEpiRows + fl;
+#pragma unroll
+    for (int c = 0; c < KW; ++c) red[c][lane][fl] = i < n ? sum_lane(part, 0, nchunk, KW, i, c, lane) : 0.0;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    if (lane == 0) {
+        double d = 0.0;
+        if (i < n) {
+#pragma unroll
+            for (int c = 0; c < KW; ++c) {
+                const double v = ((red[c][0][fl] + red[c][1][fl]) + red[c][2][fl]) + red[c][3][fl];
+                const size_t o = size_t(i) * KW + c;
+                xn[o] = x[o] + dt * v;
+                d += a[o] * v;
+            }
+        }
+        dred[fl] = d;
+    }
+    __syncthreads();
+    for (int s = kEpiRows / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) dred[threadIdx.x] += dred[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0 && dots) dots[blockIdx.x] = dscale * red[0];
+    if (threadIdx.x == 0 && dots) dots[blockIdx.x] = dscale * dred[0];
 }
 
 // out = alpha * a + beta * b (elementwise)
@@ -275,7 +284,7 @@ int shoot_forward_t(const lddmm_kernel* kd, const double* at, int T_, int n, dou
     using OPK = OpApplyK<KW, MS>;
     Job<T, OPK> jk;
     LDDMM_TRY(plan_job(jk, n, n, !MS));
-    const int nb = (n + 255) / 256;
+    const int nb = (n + kEpiRows - 1) / kEpiRows;              // blocks of k_shoot_epi
     const int nbv = int((int64_t(n) * KW + 255) / 256);       // blocks of k_shoot_epi_v
     const int ndot = ic ? nbv + 1 : nb + 1;                     // dot slots per step
     void* ws;

@@ -168,28 +168,6 @@ __device__ __forceinline__ double sum_range(const Partials<T>& p, int k0, int k1
     }
 }
 
-// Chunk sums spread over kEpiLanes lanes per row: lane l adds chunks
-// k0 + l, k0 + l + kEpiLanes, ...; epilogues combine the lanes in lane order.
-constexpr int kEpiLanes = 4;
-constexpr int kEpiRows = 64;     // rows per 256-lane epilogue workgroup
-
-template <typename T>
-__device__ __forceinline__ double sum_lane(const Partials<T>& p, int k0, int k1, int nch, int i, int c, int lane) {
-    if constexpr (kCompensated<T>) {
-        float h = 0.f, l = 0.f;
-        for (int k = k0 + lane; k < k1; k += kEpiLanes) {
-            size_t o = (size_t(k) * nch + c) * size_t(p.n) + i;
-            two_sum_acc(h, l, p.hi[o]);
-            l += p.lo[o];
-        }
-        return double(h) + double(l);
-    } else {
-        double s = 0.0;
-        for (int k = k0 + lane; k < k1; k += kEpiLanes) s += p.hi[(size_t(k) * nch + c) * size_t(p.n) + i];
-        return s;
-    }
-}
-
 // Face features of faces [0, nf) -> records / centres / weighted surfels / weights.
 __global__ void k_face_features(const double* __restrict__ v, const int32_t* __restrict__ f, int nf,
                                 const double* __restrict__ vw, FaceRec* __restrict__ rec,

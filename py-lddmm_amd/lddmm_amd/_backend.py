@@ -137,6 +137,8 @@ SIGNATURES = {
     "lddmm_normal_raycast": [_P, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P],
     "lddmm_lbfgs_pair_scalars": [_P, _P, _I64, _P, _P, _P],
     "lddmm_lbfgs_two_loop": [_P, _I64, _I32, _P, _P, _P, _P, _P, _P],
+    "lddmm_lbfgs_multidot": [_P, _I32, _P, _I32, _I64, _P, _P],
+    "lddmm_lbfgs_compact": [_P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P],
     "lddmm_population_create": [ctypes.POINTER(LddmmPopulationDesc), ctypes.POINTER(ctypes.c_void_p)],
     "lddmm_population_destroy": [_P],
     "lddmm_population_set_active": [_P, _P],

@@ -31,6 +31,10 @@ from .kernelFunctions import Kernel
 from .surfaces import Surface
 
 
+
+# stored pairs up to which the L-BFGS direction uses the compact form
+LBFGS_COMPACT_MAX = 64
+
 class Control(dict):
     def __init__(self):
         super().__init__()
@@ -385,33 +389,68 @@ class SurfaceMatching:
         return res
 
     def twoLoop(self, grd, storedGrad):
-        """L-BFGS two-loop recursion of bfgs.bfgs (base/bfgs.py:188-207) on the
-        device (lddmm_lbfgs_two_loop): the same operations in the same order,
-        with rho and alpha / beta kept in device memory, so no host
-        synchronisation per dot product.  storedGrad: [[s_k, y_k], ...] oldest
-        first (Control dicts); the pair scalars (rho, c) are computed once per
-        stored pair and kept with it."""
+        """L-BFGS direction of bfgs.bfgs (base/bfgs.py:188-207) on the device.
+        For up to LBFGS_COMPACT_MAX stored pairs it is computed in the compact
+        form (lddmm_lbfgs_multidot + lddmm_lbfgs_compact: the same product as
+        the two-loop recursion, equal to rounding, in four launches): each
+        stored pair keeps, from its first use, its flattened vectors and the
+        dots of its y with the s and y of the pairs up to it, so a call costs
+        one batched dot launch for the new pair's column and S^T g, Y^T g,
+        then the m x m solve and one pass over the vectors.  Longer memories
+        use the two-loop recursion itself (lddmm_lbfgs_two_loop: the reference's
+        operations in the reference's order, scalars in device memory).
+        storedGrad: [[s_k, y_k], ...] oldest first (Control dicts); the cache
+        is appended to each entry."""
         import ctypes
         keys = [k for k in ('at', 'ct') if grd.get(k) is not None]
         flat = (lambda c: c['at'].contiguous()) if keys == ['at'] else \
             (lambda c: torch.cat([c[k].reshape(-1) for k in keys]))
+        m = len(storedGrad)
+        g = flat(grd)
+        n = g.numel()
+        q = torch.empty_like(g)
+        PP = lambda ts: (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
         for entry in storedGrad:
             if len(entry) == 2:
-                sk, yk = flat(entry[0]), flat(entry[1])
-                sc = torch.empty(2, dtype=torch.float64, device=sk.device)
-                B.check(B.LIB.lddmm_lbfgs_pair_scalars(B.ptr(sk), B.ptr(yk), sk.numel(), B.ptr(sc[0:1]),
-                                                       B.ptr(sc[1:2]), B.stream_handle()), "lbfgs_pair_scalars")
-                entry.append((sk, yk, sc))
-        g = flat(grd)
-        q = torch.empty_like(g)
-        m = len(storedGrad)
-        P = ctypes.c_void_p * m
-        sp = P(*[e[2][0].data_ptr() for e in storedGrad])
-        yp = P(*[e[2][1].data_ptr() for e in storedGrad])
-        rp = P(*[e[2][2].data_ptr() for e in storedGrad])
-        cn = storedGrad[-1][2][2][1:2]
-        B.check(B.LIB.lddmm_lbfgs_two_loop(B.ptr(g), g.numel(), m, sp, yp, rp, B.ptr(cn), B.ptr(q),
-                                           B.stream_handle()), "lbfgs_two_loop")
+                entry.append({'s': flat(entry[0]), 'y': flat(entry[1])})
+        cache = [e[2] for e in storedGrad]
+        if m == 0:
+            q.copy_(g)
+        elif m <= LBFGS_COMPACT_MAX:
+            S = [c['s'] for c in cache]
+            Y = [c['y'] for c in cache]
+            new = [j for j, c in enumerate(cache) if 'col' not in c]
+            gd = torch.empty(2 * m, dtype=torch.float64, device=g.device)
+            if new == [m - 1]:
+                out = torch.empty((2, 2 * m), dtype=torch.float64, device=g.device)
+                B.check(B.LIB.lddmm_lbfgs_multidot(PP([Y[m - 1], g]), 2, PP(S + Y), 2 * m, n, B.ptr(out),
+                                                   B.stream_handle()), "lbfgs_multidot")
+                cache[m - 1]['col'] = out[0]
+                gd = out[1]
+            else:
+                for j in new:
+                    col = torch.empty(2 * (j + 1), dtype=torch.float64, device=g.device)
+                    B.check(B.LIB.lddmm_lbfgs_multidot(PP([Y[j]]), 1, PP(S[:j + 1] + Y[:j + 1]), 2 * (j + 1), n,
+                                                       B.ptr(col), B.stream_handle()), "lbfgs_multidot")
+                    cache[j]['col'] = col
+                B.check(B.LIB.lddmm_lbfgs_multidot(PP([g]), 1, PP(S + Y), 2 * m, n, B.ptr(gd), B.stream_handle()),
+                        "lbfgs_multidot")
+            cols = [c['col'] for c in cache]
+            clen = (ctypes.c_int32 * m)(*[c.numel() // 2 for c in cols])
+            B.check(B.LIB.lddmm_lbfgs_compact(B.ptr(g), n, m, PP(S), PP(Y), PP(cols), clen, B.ptr(gd), B.ptr(q),
+                                              B.stream_handle()), "lbfgs_compact")
+        else:
+            for c in cache:
+                if 'sc' not in c:
+                    sc = torch.empty(2, dtype=torch.float64, device=g.device)
+                    B.check(B.LIB.lddmm_lbfgs_pair_scalars(B.ptr(c['s']), B.ptr(c['y']), n, B.ptr(sc[0:1]),
+                                                           B.ptr(sc[1:2]), B.stream_handle()), "lbfgs_pair_scalars")
+                    c['sc'] = sc
+            rp = PP([c['sc'][0:1] for c in cache])
+            cn = cache[-1]['sc'][1:2]
+            B.check(B.LIB.lddmm_lbfgs_two_loop(B.ptr(g), n, m, PP([c['s'] for c in cache]),
+                                               PP([c['y'] for c in cache]), rp, B.ptr(cn), B.ptr(q),
+                                               B.stream_handle()), "lbfgs_two_loop")
         out = Control()
         if keys == ['at']:
             out['at'] = q
