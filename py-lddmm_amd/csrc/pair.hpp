@@ -190,17 +190,41 @@ constexpr int kEpiRows = 64;     // rows per 256-lane epilogue workgroup
 
 template <typename T>
 __device__ __forceinline__ double sum_lane(const Partials<T>& p, int k0, int k1, int nch, int i, int c, int lane) {
+    // this lane's chunks k0 + lane + kEpiLanes * q, q < cnt; loads issued kSumBatch at a time
+    const int cnt = k1 - k0 > lane ? (k1 - k0 - lane + kEpiLanes - 1) / kEpiLanes : 0;
+    const size_t stride = size_t(kEpiLanes) * size_t(nch) * size_t(p.n);
+    const size_t base = (size_t(k0 + lane) * nch + c) * size_t(p.n) + size_t(i);
     if constexpr (kCompensated<T>) {
         float h = 0.f, l = 0.f;
-        for (int k = k0 + lane; k < k1; k += kEpiLanes) {
-            size_t o = (size_t(k) * nch + c) * size_t(p.n) + i;
-            two_sum_acc(h, l, p.hi[o]);
-            l += p.lo[o];
+        for (int q0 = 0; q0 < cnt; q0 += kSumBatch) {
+            float vh[kSumBatch], vl[kSumBatch];
+#pragma unroll
+            for (int b = 0; b < kSumBatch; ++b) {
+                const int q = q0 + b;
+                vh[b] = q < cnt ? p.hi[base + size_t(q) * stride] : 0.f;
+                vl[b] = q < cnt ? p.lo[base + size_t(q) * stride] : 0.f;
+            }
+#pragma unroll
+            for (int b = 0; b < kSumBatch; ++b)
+                if (q0 + b < cnt) {
+                    two_sum_acc(h, l, vh[b]);
+                    l += vl[b];
+                }
         }
         return double(h) + double(l);
     } else {
         double s = 0.0;
-        for (int k = k0 + lane; k < k1; k += kEpiLanes) s += p.hi[(size_t(k) * nch + c) * size_t(p.n) + i];
+        for (int q0 = 0; q0 < cnt; q0 += kSumBatch) {
+            double v[kSumBatch];
+#pragma unroll
+            for (int b = 0; b < kSumBatch; ++b) {
+                const int q = q0 + b;
+                v[b] = q < cnt ? p.hi[base + size_t(q) * stride] : 0.0;
+            }
+#pragma unroll
+            for (int b = 0; b < kSumBatch; ++b)
+                if (q0 + b < cnt) s += v[b];
+        }
         return s;
     }
 }

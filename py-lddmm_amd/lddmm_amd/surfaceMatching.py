@@ -381,11 +381,27 @@ class SurfaceMatching:
         return grd
 
     def dotProduct_euclidean(self, g1, g2):
+        """Euclidean dot products <g1, g2[l]> (base/surfaceMatching.py dotProduct_euclidean),
+        one batched deterministic device reduction per control key
+        (lddmm_lbfgs_multidot)."""
+        import ctypes
         res = np.zeros(len(g2))
         for k in g1.keys():
-            if g1[k] is not None:
-                for ll, gr in enumerate(g2):
-                    res[ll] += float((g1[k] * gr[k]).sum())
+            if g1[k] is None:
+                continue
+            a = g1[k].contiguous()
+            bs = [gr[k].contiguous() for gr in g2]
+            if not bs:
+                continue
+            if a.numel() == 0 or len(bs) > 128:
+                for ll, b in enumerate(bs):
+                    res[ll] += float((a * b).sum())
+                continue
+            out = torch.empty(len(bs), dtype=torch.float64, device=a.device)
+            PP = lambda ts: (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+            B.check(B.LIB.lddmm_lbfgs_multidot(PP([a]), 1, PP(bs), len(bs), a.numel(), B.ptr(out),
+                                               B.stream_handle()), "lbfgs_multidot")
+            res += out.cpu().numpy()
         return res
 
     def twoLoop(self, grd, storedGrad):
