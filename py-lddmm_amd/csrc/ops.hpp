@@ -177,6 +177,60 @@ struct OpDiffKT {
 };
 
 // ---------------------------------------------------------------------------
+// One step of the first-order adjoint with the Euclidean gradient, fused:
+// channels 0..2 the lddmm covector sum of OpDiffKT<KW, true>
+// (sum_j g'(r2) (p_i.a_j + a_i.q_j) (y_i - x_j), q_j = p_j - 2 r a_j), channels
+// 3..3+KW the kernel sum of q: sum_j g(r2) q_j = -K(x) (2 r a - p), whose
+// negative is the Euclidean gradient direction K(x_t)(2 r a_t - p_{t+1})
+// (base/surfaceMatching.py:860-891, base/pointEvolution.py:167-236).  The pair's
+// distance and, for one Gaussian scale, its exponential serve both sums.
+// ---------------------------------------------------------------------------
+template <int KW, bool MS>
+struct OpAdjoint {
+    using Base = OpDiffKT<KW, true, MS>;
+    static constexpr int NCH = 3 + KW;
+    static constexpr int REC = Base::REC;
+    using Chunks = ContiguousChunks;
+    template <typename T>
+    using Args = typename Base::template Args<T>;
+    template <typename T>
+    using Tgt = typename Base::template Tgt<T>;
+    using Raw = typename Base::Raw;
+
+    template <typename T>
+    __device__ static void load_target(const Args<T>& A, int i, Tgt<T>& t) { Base::load_target(A, i, t); }
+    template <typename T>
+    __device__ static void load_raw(const Args<T>& A, int j, Raw& r) { Base::load_raw(A, j, r); }
+    template <typename T>
+    __device__ static void store_rec(const Args<T>& A, const Raw& r, T* rec) { Base::store_rec(A, r, rec); }
+    template <typename T>
+    __device__ static void pair(const Args<T>& A, const Tgt<T>& t, const T* rec, T* acc) {
+        Vec4<T> p = lds4(rec);
+        T dx = t.u0 - p.x, dy = t.u1 - p.y, dz = t.u2 - p.z;
+        T r2 = fmaT(dz, dz, fmaT(dy, dy, dx * dx));
+        T g = gD<T, MS>(A.sc, r2);
+        T K;
+        if constexpr (MS) K = gK<T, MS>(A.sc, r2);
+        else K = g;
+        Vec4<T> a = lds4(rec + 4);
+        Vec4<T> q = lds4(rec + 8);
+        T c = t.p[0] * a.x;
+        if constexpr (KW > 1) c = fmaT(t.p[1], a.y, c);
+        if constexpr (KW > 2) c = fmaT(t.p[2], a.z, c);
+        c = fmaT(t.a[0], q.x, c);
+        if constexpr (KW > 1) c = fmaT(t.a[1], q.y, c);
+        if constexpr (KW > 2) c = fmaT(t.a[2], q.z, c);
+        T w = g * c;
+        acc[0] = fmaT(w, dx, acc[0]);
+        acc[1] = fmaT(w, dy, acc[1]);
+        acc[2] = fmaT(w, dz, acc[2]);
+        acc[3] = fmaT(K, q.x, acc[3]);
+        if constexpr (KW > 1) acc[4] = fmaT(K, q.y, acc[4]);
+        if constexpr (KW > 2) acc[5] = fmaT(K, q.z, acc[5]);
+    }
+};
+
+// ---------------------------------------------------------------------------
 // applyktensor: out_i = sum_j g(r2)(ay_i ax_j + (by_i.bx_j)^2)
 //                                          (base/kernelFunctions_util.py:1475-1506)
 // ---------------------------------------------------------------------------

@@ -276,6 +276,30 @@ k_shoot_epi_v(int64_t count, const double* __restrict__ x, const double* __restr
     if (threadIdx.x == 0) dots[blockIdx.x] = dscale * red[0];
 }
 
+// Reduce of the fused adjoint step (OpAdjoint partials): channel c < KW gives
+// p_t = p_{t+1} + dt * fdiff * v_c, channel KW + c the gradient g = -dt * v.
+// 64 rows x 4 chunk lanes per workgroup; c = blockIdx.y in [0, 2 KW).
+template <typename T, int KW>
+__global__ void __launch_bounds__(256) k_reduce_adjoint(Partials<T> part, int nchunk, int n, double dt, double fdiff,
+                                                        const double* __restrict__ pn, double* __restrict__ pc,
+                                                        double* __restrict__ g) {
+    __shared__ double red[kEpiLanes][kEpiRows];
+    const int fl = threadIdx.x % kEpiRows, lane = threadIdx.x / kEpiRows;
+    const int i = blockIdx.x * kEpiRows + fl;
+    const int c = blockIdx.y;
+    const int ch = c < KW ? c : 3 + (c - KW);            // partial channel
+    red[lane][fl] = i < n ? sum_lane(part, 0, nchunk, 3 + KW, i, ch, lane) : 0.0;
+    __syncthreads();
+    if (lane != 0 || i >= n) return;
+    const double v = ((red[0][fl] + red[1][fl]) + red[2][fl]) + red[3][fl];
+    if (c < KW) {
+        const size_t o = size_t(i) * KW + c;
+        pc[o] = pn[o] + dt * v * fdiff;          // the order of reduce_job(..., fdiff, pc, 0, pn, dt)
+    } else {
+        g[size_t(i) * KW + (c - KW)] = -dt * v;
+    }
+}
+
 // ---------------------------------------------------------------------------
 template <typename T, bool MS, int KW>
 int shoot_forward_t(const lddmm_kernel* kd, const double* at, int T_, int n, double* xt, double r,
@@ -352,15 +376,22 @@ int shoot_adjoint_t(const lddmm_kernel* kd, const double* at, const double* xt, 
     const double fdiff = MS ? -1.0 : ss.diff;
     using OPD = OpDiffKT<KW, true, MS>;
     using OPK = OpApplyK<KW, MS>;
+    using OPA = OpAdjoint<KW, MS>;
     Job<T, OPD> jd;
     Job<T, OPK> jk;
+    Job<T, OPA> ja;
     LDDMM_TRY(plan_job(jd, n, n, !MS));
     LDDMM_TRY(plan_job(jk, n, n, !MS));
     const size_t stride = size_t(n) * KW;
     const bool aff = kd->affine != LDDMM_AFFINE_NONE;
+    // the fused step (one pair launch + one reduce) covers the Euclidean gradient
+    // without affine part or internal costs
+    const bool fused = euclidean && !aff && !ic;
+    if (fused) LDDMM_TRY(plan_job(ja, n, n, !MS));
     const AffineArgs AA = affine_args(kd);
     const size_t icb = ic ? (4 * stride + ic_scratch_doubles(ic->nf, true)) * sizeof(double) + 1280 : 0;
-    size_t need = 256 + std::max(partial_bytes<T>(jd.plan, 3, n), partial_bytes<T>(jk.plan, KW, n)) +
+    size_t need = 256 + std::max({partial_bytes<T>(jd.plan, 3, n), partial_bytes<T>(jk.plan, KW, n),
+                                  fused ? partial_bytes<T>(ja.plan, 3 + KW, n) : size_t(0)}) +
                   stride * sizeof(double) * (pxt ? 1 : 3) + (aff ? kAffBytes + 256 : 0) + icb + 1024;
     void* ws;
     LDDMM_TRY(workspace(need, st, &ws));
@@ -409,6 +440,17 @@ int shoot_adjoint_t(const lddmm_kernel* kd, const double* at, const double* xt, 
                               dlv, icws, st));
             LDDMM_TRY(axpby(stride, 1.0, pn, -1.0, lv, pe, st));
             pu = pe;
+        }
+        if (fused) {
+            Carve ca; ca.off = poff;
+            Partials<T> partA = carve_partials<T>(base, ca, ja.plan, 3 + KW, n);
+            ja.args = {x, x, pu, a, c0, KW, KW, 0, 2.0 * r, sc};
+            LDDMM_TRY(run_job(ja, n, n, partA, st));
+            hipLaunchKernelGGL((k_reduce_adjoint<T, KW>), dim3(unsigned((n + kEpiRows - 1) / kEpiRows), 2 * KW),
+                               dim3(256), 0, st, partA, ja.plan.nchunk, n, dt, fdiff, pn, pc,
+                               grad + size_t(t) * stride);
+            LDDMM_CHECK_LAUNCH();
+            continue;
         }
         Carve c2; c2.off = poff;
         Partials<T> partD = carve_partials<T>(base, c2, jd.plan, 3, n);
