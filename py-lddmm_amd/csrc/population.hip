@@ -92,6 +92,25 @@ __global__ void k_pop_reduce3(const T* __restrict__ phi, const T* __restrict__ p
     }
 }
 
+// Fused adjoint step (OpAdjoint partials, 6 channels), active rows:
+// pc = pn + dt * fdiff * sum_{0..2}, g = -dt * sum_{3..5}.
+template <typename T>
+__global__ void k_pop_reduce_adj(const T* __restrict__ phi, const T* __restrict__ plo,
+                                 const PopPair* __restrict__ pairs, const int32_t* __restrict__ rowpair,
+                                 const uint8_t* __restrict__ active, int nrows, double fdiff, double dt,
+                                 const double* __restrict__ pn, double* __restrict__ pc, double* __restrict__ g) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    const int p = rowpair[i];
+    if (!active[p]) return;
+    const PopPair pp = pairs[p];
+    for (int c = 0; c < 3; ++c) {
+        const size_t o = size_t(i) * 3 + c;
+        pc[o] = pn[o] + dt * fdiff * pop_sum(phi, plo, pp, 6, i, c, 0, pp.nchunk);
+        g[o] = -(dt * pop_sum(phi, plo, pp, 6, i, 3 + c, 0, pp.nchunk));
+    }
+}
+
 // x_next = x + dt v; rowacc_i += rdt * a_i.v_i  (v = summed partials), active rows
 template <typename T>
 __global__ void k_pop_shoot_epi(const T* __restrict__ phi, const T* __restrict__ plo,
@@ -307,7 +326,7 @@ __global__ void k_pop_vf_grad_epi(const T* __restrict__ phi, const T* __restrict
 namespace {
 
 enum PopFamily { kFamVert = 0, kFamFace = 1 };
-enum PopOp { kOpK = 0, kOpDiffKT = 1, kOpVfE = 2, kOpVfG = 3 };
+enum PopOp { kOpK = 0, kOpDiffKT = 1, kOpVfE = 2, kOpVfG = 3, kOpAdj = 4 };
 
 struct PopPlan {
     std::vector<PopItem> items;
@@ -870,7 +889,6 @@ int pop_adjoint_t(const lddmm_kernel* kd, lddmm_population* pop, const double* a
     ScaleSet<T> sc = make_scales<T>(kd, &ss);
     const double fdiff = MS ? -1.0 : ss.diff;
     using OPD = OpDiffKT<3, true, MS>;
-    using OPK = OpApplyK<3, MS>;
     const PopFixed fx = fixed_layout(pop);
     LDDMM_TRY(pop_scratch(pop, fx.total, st));
     const int64_t V = pop->V;
@@ -883,15 +901,15 @@ int pop_adjoint_t(const lddmm_kernel* kd, lddmm_population* pop, const double* a
         LDDMM_CHECK_LAUNCH();
         return LDDMM_OK;
     };
-    const bool tuned = pop->tune.count(tune_key<T>(kOpDiffKT, MS)) != 0 &&
-                       (!euclidean || pop->tune.count(tune_key<T>(kOpK, MS)) != 0);
-    PopLaunch<T> LD, LK;
+    // the Euclidean gradient takes the fused step: one pair launch computes the
+    // covector sum and K(x)(2 r a - p) (OpAdjoint), one reduce writes both
+    using OPA = OpAdjoint<3, MS>;
+    const bool tuned = euclidean ? pop->tune.count(tune_key<T>(kOpAdj, MS)) != 0
+                                 : pop->tune.count(tune_key<T>(kOpDiffKT, MS)) != 0;
+    PopLaunch<T> LD, LA;
     auto mkd = [&]() { return typename OPD::template Args<T>{xl, xl, px1, al, nullptr, 3, 3, 0, 2.0 * r, sc}; };
-    LDDMM_TRY((prepare_op<T, OPD>(pop, kOpDiffKT, kFamVert, MS, mkd, fx.total, prep, st, &LD)));
-    if (euclidean) {
-        auto mkk = [&]() { return typename OPK::template Args<T>{xl, xl, px1, nullptr, 3, 3, 0, sc}; };
-        LDDMM_TRY((prepare_op<T, OPK>(pop, kOpK, kFamVert, MS, mkk, fx.total, prep, st, &LK)));
-    }
+    if (euclidean) LDDMM_TRY((prepare_op<T, OPA>(pop, kOpAdj, kFamVert, MS, mkd, fx.total, prep, st, &LA)));
+    else LDDMM_TRY((prepare_op<T, OPD>(pop, kOpDiffKT, kFamVert, MS, mkd, fx.total, prep, st, &LD)));
     char* S = pop->scratch;
     const PopPartials<T> pp = pop_partials<T>(pop, fx.total);
     double* c0 = reinterpret_cast<double*>(S + fx.c0);
@@ -911,6 +929,14 @@ int pop_adjoint_t(const lddmm_kernel* kd, lddmm_population* pop, const double* a
             double* pn = pb[(t + 1) & 1];
             double* pc = pb[t & 1];
             typename OPD::template Args<T> args{x, x, pn, a, nullptr, 3, 3, 0, 2.0 * r, sc};
+            double* g = grad + size_t(t) * stride;
+            if (euclidean) {
+                LDDMM_TRY((launch_pop<T, OPA>(args, LA, c0, pp.phi, pp.plo, s)));
+                hipLaunchKernelGGL((k_pop_reduce_adj<T>), dim3(nblk(V)), dim3(256), 0, s, pp.phi, pp.plo,
+                                   LA.plan->d_pairs, pop->d_vpair, pop->d_active, int(V), fdiff, dt, pn, pc, g);
+                LDDMM_CHECK_LAUNCH();
+                continue;
+            }
             LDDMM_TRY((launch_pop<T, OPD>(args, LD, c0, pp.phi, pp.plo, s)));
             // dat = 2 r a_t - p_{t+1}  (before p_t overwrites the other buffer)
             hipLaunchKernelGGL(k_pop_axpby, dim3(nblk(V)), dim3(256), 0, s, int(V), pop->d_vpair, pop->d_active,
@@ -920,18 +946,9 @@ int pop_adjoint_t(const lddmm_kernel* kd, lddmm_population* pop, const double* a
             hipLaunchKernelGGL((k_pop_reduce3<T>), dim3(nblk(V)), dim3(256), 0, s, pp.phi, pp.plo,
                                LD.plan->d_pairs, pop->d_vpair, pop->d_active, int(V), 3, fdiff, dt, pn, pc);
             LDDMM_CHECK_LAUNCH();
-            double* g = grad + size_t(t) * stride;
-            if (euclidean) {
-                typename OPK::template Args<T> ak{x, x, dat, nullptr, 3, 3, 0, sc};
-                LDDMM_TRY((launch_pop<T, OPK>(ak, LK, c0, pp.phi, pp.plo, s)));
-                hipLaunchKernelGGL((k_pop_reduce3<T>), dim3(nblk(V)), dim3(256), 0, s, pp.phi, pp.plo,
-                                   LK.plan->d_pairs, pop->d_vpair, pop->d_active, int(V), 3, 1.0, dt, nullptr, g);
-                LDDMM_CHECK_LAUNCH();
-            } else {
-                hipLaunchKernelGGL(k_pop_axpby, dim3(nblk(V)), dim3(256), 0, s, int(V), pop->d_vpair, pop->d_active,
-                                   dt, dat, 0.0, nullptr, g);
-                LDDMM_CHECK_LAUNCH();
-            }
+            hipLaunchKernelGGL(k_pop_axpby, dim3(nblk(V)), dim3(256), 0, s, int(V), pop->d_vpair, pop->d_active, dt,
+                               dat, 0.0, nullptr, g);
+            LDDMM_CHECK_LAUNCH();
         }
         return LDDMM_OK;
     };
@@ -943,8 +960,7 @@ int pop_adjoint_t(const lddmm_kernel* kd, lddmm_population* pop, const double* a
     for (const void* p : {(const void*)at, (const void*)xt, (const void*)px1, (const void*)grad, (const void*)S,
                           (const void*)pp.plo})
         key_ptr(key, p);
-    key_launch(key, LD);
-    if (euclidean) key_launch(key, LK);
+    key_launch(key, euclidean ? LA : LD);
     return run_graphed(pop, key, pop->graphs_on && tuned, enqueue, st);
 }
 

@@ -139,7 +139,7 @@ class Population:
         buf = (ctypes.c_int32 * 64)()
         n = ctypes.c_int32()
         B.check(B.LIB.lddmm_population_tuning(self.h, buf, 16, ctypes.byref(n)), "population_tuning")
-        names = {0: "applyK", 1: "applyDiffKT", 2: "varifold_energy", 3: "varifold_gradient"}
+        names = {0: "applyK", 1: "applyDiffKT", 2: "varifold_energy", 3: "varifold_gradient", 4: "adjoint_step"}
         return [{"op": names.get(buf[4 * i], buf[4 * i]), "bytes": buf[4 * i + 1], "R": buf[4 * i + 2],
                  "items_per_workgroup": buf[4 * i + 3] / 10} for i in range(min(n.value, 16))]
 
