@@ -13,6 +13,7 @@ zero control is kept whenever it has the lower objective (the fallback),
 and BFGS then runs until the varifold data term matches the plain method's
 final value within 1e-4.
 """
+import functools
 import logging
 import time
 
@@ -32,9 +33,23 @@ DEFAULT = dict(subdiv=4, radius=10.0, nbumps=20, width=3.0, amp_sd=1.0, sigma_de
                timeStep=0.1, maxIter=50, pk_dtype='float64')
 
 
+@functools.lru_cache(maxsize=64)
+def _pair(seed, subdiv, radius, nbumps, width, amp_sd):
+    (v, f), (v2, _) = synth.plain_matching_pair(seed=seed, subdiv=subdiv, radius=radius, nbumps=nbumps, width=width,
+                                                amp_sd=amp_sd)
+    for arr in (v, f, v2):
+        arr.flags.writeable = False
+    return v, f, v2
+
+
+def pair_arrays(seed, cfg=DEFAULT):
+    """Template vertices, faces and target vertices of the seeded pair (cached, read-only)."""
+    return _pair(int(seed), cfg['subdiv'], float(cfg['radius']), cfg['nbumps'], float(cfg['width']),
+                 float(cfg['amp_sd']))
+
+
 def make_matching(seed, cfg=DEFAULT, at=None):
-    (v, f), (v2, _) = synth.plain_matching_pair(seed=seed, subdiv=cfg['subdiv'], radius=cfg['radius'],
-                                                nbumps=cfg['nbumps'], width=cfg['width'], amp_sd=cfg['amp_sd'])
+    v, f, v2 = pair_arrays(seed, cfg)
     opts = {'KparDiff': Kernel(name='gauss', sigma=cfg['sigma_def']),
             'KparDist': Kernel(name='gauss', sigma=cfg['sigma_var']),
             'errorType': 'varifold', 'timeStep': cfg['timeStep'], 'pk_dtype': cfg['pk_dtype'],

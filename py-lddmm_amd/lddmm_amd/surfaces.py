@@ -176,8 +176,12 @@ class Surface:
         self.computeCentersAreas()
 
     def surfVolume(self):
-        """sum_f det(v[f]) / 6  (base/surfaces.py:1645-1651)"""
-        return float(np.linalg.det(self.vertices[self.faces]).sum() / 6)
+        """sum_f det(v[f]) / 6  (base/surfaces.py:1645-1651), each determinant as
+        the triple product x1 . (x2 x x3)."""
+        x1 = self.vertices[self.faces[:, 0]]
+        x2 = self.vertices[self.faces[:, 1]]
+        x3 = self.vertices[self.faces[:, 2]]
+        return float((x1 * np.cross(x2, x3)).sum() / 6)
 
     def surfArea(self):
         return float(np.sqrt((self.surfel ** 2).sum(axis=1)).sum())
