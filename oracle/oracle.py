@@ -1109,3 +1109,37 @@ def secondOrderGradient(x0, a0, px1, pa1, sigma, timeStep, KP=-1, kernel=GAUSS):
     pxt, pat, xt, at = secondOrderCovector(x0, a0, px1, pa1, sigma, timeStep, KP, kernel=kernel)
     da0 = applyK(x0, x0, a0, sigma, KP, kernel=kernel) - pat[0]
     return da0, xt, at, pxt, pat
+
+
+def second_order_matching_objective(x0, w0, a0, target, sigma_diff, sigma_dist, timeStep, kind='point_measure',
+                                    sigma_error=1., kernel_diff=GAUSS):
+    """Second-order (geodesic) point-set matching objective at the initial
+    momentum a0: 1/2 a0.K(x0)a0 + (obj0 + dataTerm(x_T)) / sigma_err^2, with
+    x_T the endpoint of secondOrderEvolution
+    (base/secondOrderPointSetMatching.py:105-140 objectiveFunDef, no affine;
+    the data term and obj0 as in base/pointSetMatching.py).  kind
+    'point_measure' (w0 = template weights, target = (points, weights)) or
+    'L2' (target = points of the same shape; |x_T - y|^2 with obj0 = |y|^2)."""
+    a0 = _f64(a0)
+    xt, at = secondOrderEvolution(x0, a0, sigma_diff, timeStep, kernel=kernel_diff)
+    obj = 0.5 * (a0 * applyK(x0, x0, a0, sigma_diff, kernel=kernel_diff)).sum()
+    if kind == 'L2':
+        y = _f64(target)
+        dterm, obj0 = -2 * (xt[-1] * y).sum() + (xt[-1] ** 2).sum(), (y ** 2).sum()
+    else:
+        dterm, obj0 = _data_term(kind, (xt[-1], w0), target, sigma_dist)
+    return obj + (obj0 + dterm) / sigma_error ** 2, xt
+
+
+def second_order_matching_gradient(x0, w0, a0, target, sigma_diff, sigma_dist, timeStep, kind='point_measure',
+                                   sigma_error=1., kernel_diff=GAUSS):
+    """grad['a0'] = da0 of secondOrderGradient with px1 = -dataTerm'(x_T) /
+    sigma_err^2 and pa1 = 0 (base/secondOrderPointSetMatching.py:379-460)."""
+    a0 = _f64(a0)
+    xt, at = secondOrderEvolution(x0, a0, sigma_diff, timeStep, kernel=kernel_diff)
+    if kind == 'L2':
+        g = 2 * (xt[-1] - _f64(target))
+    else:
+        g = _data_grad(kind, (xt[-1], w0), target, sigma_dist)
+    px1 = -g / sigma_error ** 2
+    return secondOrderGradient(x0, a0, px1, np.zeros_like(a0), sigma_diff, timeStep, kernel=kernel_diff)[0]

@@ -418,8 +418,8 @@ class SurfaceMatching:
         storedGrad: [[s_k, y_k], ...] oldest first (Control dicts); the cache
         is appended to each entry."""
         import ctypes
-        keys = [k for k in ('at', 'ct') if grd.get(k) is not None]
-        flat = (lambda c: c['at'].contiguous()) if keys == ['at'] else \
+        keys = [k for k in grd.keys() if grd[k] is not None]       # 'at' (+ 'ct'), or 'a0'
+        flat = (lambda c: c[keys[0]].contiguous()) if len(keys) == 1 else \
             (lambda c: torch.cat([c[k].reshape(-1) for k in keys]))
         m = len(storedGrad)
         g = flat(grd)
@@ -468,8 +468,8 @@ class SurfaceMatching:
                                                PP([c['y'] for c in cache]), rp, B.ptr(cn), B.ptr(q),
                                                B.stream_handle()), "lbfgs_two_loop")
         out = Control()
-        if keys == ['at']:
-            out['at'] = q
+        if len(keys) == 1:
+            out[keys[0]] = q
         else:
             o = 0
             for k in keys:

@@ -433,6 +433,23 @@ def test_second_order_gradient_fd():
     np.testing.assert_allclose((da0 * d).sum(), _fd(J, a0, d), rtol=1e-6)
 
 
+@pytest.mark.parametrize("kind", ["point_measure", "L2"])
+def test_second_order_matching_gradient_fd(kind):
+    """Second-order point-set matching: da0 = FD of the objective in a0."""
+    x0 = rnd(30, 3) * 2
+    w0 = np.abs(rnd(30)) + 0.2
+    if kind == 'L2':
+        tgt = x0 + 0.3 * rnd(30, 3)
+    else:
+        tgt = (rnd(25, 3) * 2 + 0.3, np.abs(rnd(25)) + 0.2)
+    a0 = 0.1 * rnd(30, 3)
+    d = rnd(30, 3)
+    g = O.second_order_matching_gradient(x0, w0, a0, tgt, [1.5], [0.9], 0.2, kind=kind, sigma_error=0.7)
+    fd = _fd(lambda a_: O.second_order_matching_objective(x0, w0, a_, tgt, [1.5], [0.9], 0.2, kind=kind,
+                                                          sigma_error=0.7)[0], a0, d)
+    np.testing.assert_allclose((g * d).sum(), fd, rtol=1e-6)
+
+
 # ------------------------------------------------------ laplacian family
 def _matern(order, r):
     """Matern nu = order + 1/2 without its e^-r: order!/(2 order)! sum_i
