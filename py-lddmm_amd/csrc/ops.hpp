@@ -479,6 +479,196 @@ struct OpDDiffK {
 
 
 // ---------------------------------------------------------------------------
+// Geodesic (EPDiff) step, one Gaussian scale, fused: the sums of one forward
+// step (base/pointEvolution.py:320-379, :385-472) share the pair's distance and
+// exponential g = exp(-|x_i - x_j|^2 / 2s^2):
+//   channels 0..KW:     sum_j g a_j                          (K(x) a)
+//   channels KW..KW+3:  sum_j g (a_i.a_j) (x_i - x_j)        (DiffKT(x, a, a), plain)
+// ---------------------------------------------------------------------------
+template <int KW>
+struct OpEPDiffFwd {
+    static constexpr int NCH = KW + 3;
+    static constexpr int REC = 8;
+    using Chunks = ContiguousChunks;
+    template <typename T>
+    struct Args {
+        const double *x, *a, *c0;
+        ScaleSet<T> sc;
+    };
+    template <typename T>
+    struct Tgt { T u0, u1, u2, a[KW]; };
+    struct Raw { double x0, x1, x2, a[KW]; };
+
+    template <typename T>
+    __device__ static void load_target(const Args<T>& A, int i, Tgt<T>& t) {
+        t.u0 = coord<T>(A.x, i, KW, 0, A.c0, A.sc.cs);
+        t.u1 = coord<T>(A.x, i, KW, 1, A.c0, A.sc.cs);
+        t.u2 = coord<T>(A.x, i, KW, 2, A.c0, A.sc.cs);
+#pragma unroll
+        for (int c = 0; c < KW; ++c) t.a[c] = T(A.a[size_t(i) * KW + c]);
+    }
+    template <typename T>
+    __device__ static void load_raw(const Args<T>& A, int j, Raw& r) {
+        if (j < 0) {
+            r.x0 = A.c0[0]; r.x1 = A.c0[1]; r.x2 = A.c0[2];
+#pragma unroll
+            for (int c = 0; c < KW; ++c) r.a[c] = 0.0;
+            return;
+        }
+        const double* xp = A.x + size_t(j) * KW;
+        r.x0 = xp[0];
+        r.x1 = KW > 1 ? xp[1] : 0.0;
+        r.x2 = KW > 2 ? xp[2] : 0.0;
+#pragma unroll
+        for (int c = 0; c < KW; ++c) r.a[c] = A.a[size_t(j) * KW + c];
+    }
+    template <typename T>
+    __device__ static void store_rec(const Args<T>& A, const Raw& r, T* rec) {
+        rec[0] = T((r.x0 - A.c0[0]) * A.sc.cs);
+        rec[1] = T((r.x1 - A.c0[1]) * A.sc.cs);
+        rec[2] = T((r.x2 - A.c0[2]) * A.sc.cs);
+        rec[3] = T(0);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) rec[4 + c] = c < KW ? T(r.a[c < KW ? c : 0]) : T(0);
+    }
+    template <typename T>
+    __device__ static void pair(const Args<T>& A, const Tgt<T>& t, const T* rec, T* acc) {
+        Vec4<T> p = lds4(rec);
+        T dx = t.u0 - p.x, dy = t.u1 - p.y, dz = t.u2 - p.z;
+        T r2 = fmaT(dz, dz, fmaT(dy, dy, dx * dx));
+        T g = fast_exp2(-r2);
+        Vec4<T> a = lds4(rec + 4);
+        acc[0] = fmaT(g, a.x, acc[0]);
+        if constexpr (KW > 1) acc[1] = fmaT(g, a.y, acc[1]);
+        if constexpr (KW > 2) acc[2] = fmaT(g, a.z, acc[2]);
+        T c = t.a[0] * a.x;
+        if constexpr (KW > 1) c = fmaT(t.a[1], a.y, c);
+        if constexpr (KW > 2) c = fmaT(t.a[2], a.z, c);
+        T w = g * c;
+        acc[KW] = fmaT(w, dx, acc[KW]);
+        acc[KW + 1] = fmaT(w, dy, acc[KW + 1]);
+        acc[KW + 2] = fmaT(w, dz, acc[KW + 2]);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Geodesic adjoint step, one Gaussian scale, fused: the four sums of one
+// backward step of secondOrderCovector (base/pointEvolution.py:489-552)
+//   zpx = DiffKT_sym(x, px, a) - DDiffK11and12(x, a, a, pa)
+//   zpa = K(x) px - DiffK1and2(x, pa, a)
+// from one distance and exponential per pair; each sum's per-pair arithmetic
+// is that of OpDiffKT<KW, true>, OpDDiffK<KW, 11and12>, OpApplyK and
+// OpDiffK<KW, 1and2> (the directional term d.(pa_i - pa_j) is shared by the
+// last two).  Channels: [0, 3) DiffKT_sym, [3, 6) DDiffK, [6, 6+KW) K px,
+// [6+KW, 6+2KW) DiffK1and2.
+// ---------------------------------------------------------------------------
+template <int KW>
+struct OpEPDiffAdj {
+    static constexpr int NCH = 6 + 2 * KW;
+    static constexpr int REC = 16;
+    using Chunks = ContiguousChunks;
+    template <typename T>
+    struct Args {
+        const double *x, *a, *px, *pa, *c0;
+        T gamma;
+        ScaleSet<T> sc;
+    };
+    template <typename T>
+    struct Tgt { T u0, u1, u2, px[KW], a[KW], pa0, pa1, pa2; };
+    struct Raw { double x0, x1, x2, a[KW], px[KW], pa[KW]; };
+
+    template <typename T>
+    __device__ static void load_target(const Args<T>& A, int i, Tgt<T>& t) {
+        t.u0 = coord<T>(A.x, i, KW, 0, A.c0, A.sc.cs);
+        t.u1 = coord<T>(A.x, i, KW, 1, A.c0, A.sc.cs);
+        t.u2 = coord<T>(A.x, i, KW, 2, A.c0, A.sc.cs);
+#pragma unroll
+        for (int c = 0; c < KW; ++c) {
+            t.px[c] = T(A.px[size_t(i) * KW + c]);
+            t.a[c] = T(A.a[size_t(i) * KW + c]);
+        }
+        const double* pp = A.pa + size_t(i) * KW;
+        t.pa0 = T(pp[0]);
+        t.pa1 = KW > 1 ? T(pp[1]) : T(0);
+        t.pa2 = KW > 2 ? T(pp[2]) : T(0);
+    }
+    template <typename T>
+    __device__ static void load_raw(const Args<T>& A, int j, Raw& r) {
+        if (j < 0) {
+            r.x0 = A.c0[0]; r.x1 = A.c0[1]; r.x2 = A.c0[2];
+#pragma unroll
+            for (int c = 0; c < KW; ++c) r.a[c] = r.px[c] = r.pa[c] = 0.0;
+            return;
+        }
+        const double* xp = A.x + size_t(j) * KW;
+        r.x0 = xp[0];
+        r.x1 = KW > 1 ? xp[1] : 0.0;
+        r.x2 = KW > 2 ? xp[2] : 0.0;
+#pragma unroll
+        for (int c = 0; c < KW; ++c) {
+            r.a[c] = A.a[size_t(j) * KW + c];
+            r.px[c] = A.px[size_t(j) * KW + c];
+            r.pa[c] = A.pa[size_t(j) * KW + c];
+        }
+    }
+    template <typename T>
+    __device__ static void store_rec(const Args<T>& A, const Raw& r, T* rec) {
+        rec[0] = T((r.x0 - A.c0[0]) * A.sc.cs);
+        rec[1] = T((r.x1 - A.c0[1]) * A.sc.cs);
+        rec[2] = T((r.x2 - A.c0[2]) * A.sc.cs);
+        rec[3] = T(0);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int cc = c < KW ? c : 0;
+            rec[4 + c] = c < KW ? T(r.a[cc]) : T(0);
+            rec[8 + c] = c < KW ? T(r.px[cc]) : T(0);
+            rec[12 + c] = c < KW ? T(r.pa[cc]) : T(0);
+        }
+    }
+    template <typename T>
+    __device__ static void pair(const Args<T>& A, const Tgt<T>& t, const T* rec, T* acc) {
+        Vec4<T> p = lds4(rec);
+        T dx = t.u0 - p.x, dy = t.u1 - p.y, dz = t.u2 - p.z;
+        T r2 = fmaT(dz, dz, fmaT(dy, dy, dx * dx));
+        T g = fast_exp2(-r2);
+        Vec4<T> a = lds4(rec + 4);
+        Vec4<T> qx = lds4(rec + 8);          // px_j
+        Vec4<T> pl = lds4(rec + 12);         // pa_j
+        // DiffKT sym: c = px_i.a_j + a_i.px_j
+        T c = t.px[0] * a.x;
+        if constexpr (KW > 1) c = fmaT(t.px[1], a.y, c);
+        if constexpr (KW > 2) c = fmaT(t.px[2], a.z, c);
+        c = fmaT(t.a[0], qx.x, c);
+        if constexpr (KW > 1) c = fmaT(t.a[1], qx.y, c);
+        if constexpr (KW > 2) c = fmaT(t.a[2], qx.z, c);
+        T w = g * c;
+        acc[0] = fmaT(w, dx, acc[0]);
+        acc[1] = fmaT(w, dy, acc[1]);
+        acc[2] = fmaT(w, dz, acc[2]);
+        // DDiffK 11and12: K aa (gamma (d.q) d - q), q = pa_i - pa_j
+        T aa = t.a[0] * a.x;
+        if constexpr (KW > 1) aa = fmaT(t.a[1], a.y, aa);
+        if constexpr (KW > 2) aa = fmaT(t.a[2], a.z, aa);
+        const T q0 = t.pa0 - pl.x, q1 = t.pa1 - pl.y, q2 = t.pa2 - pl.z;
+        const T dq = fmaT(dz, q2, fmaT(dy, q1, dx * q0));
+        const T K = g * aa;
+        const T tq = A.gamma * dq;
+        acc[3] = fmaT(K, fmaT(tq, dx, -q0), acc[3]);
+        acc[4] = fmaT(K, fmaT(tq, dy, -q1), acc[4]);
+        acc[5] = fmaT(K, fmaT(tq, dz, -q2), acc[5]);
+        // K(x) px
+        acc[6] = fmaT(g, qx.x, acc[6]);
+        if constexpr (KW > 1) acc[7] = fmaT(g, qx.y, acc[7]);
+        if constexpr (KW > 2) acc[8] = fmaT(g, qx.z, acc[8]);
+        // DiffK1and2(x, pa, a): g (d.(pa_i - pa_j)) a_j
+        const T w4 = g * dq;
+        acc[6 + KW] = fmaT(w4, a.x, acc[6 + KW]);
+        if constexpr (KW > 1) acc[7 + KW] = fmaT(w4, a.y, acc[7 + KW]);
+        if constexpr (KW > 2) acc[8 + KW] = fmaT(w4, a.z, acc[8 + KW]);
+    }
+};
+
+// ---------------------------------------------------------------------------
 // Host helpers
 // ---------------------------------------------------------------------------
 template <class F>

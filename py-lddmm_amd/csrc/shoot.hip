@@ -300,6 +300,65 @@ __global__ void __launch_bounds__(256) k_reduce_adjoint(Partials<T> part, int nc
     }
 }
 
+// Fused geodesic forward step (OpEPDiffFwd partials): c < KW gives
+// x_{k+1} = x + dt K a, c >= KW gives a_{k+1} = a - dt fdiff DiffKT (the
+// operation order of two reduce_job calls).
+template <typename T, int KW>
+__global__ void __launch_bounds__(256) k_reduce_epdiff_fwd(Partials<T> part, int nchunk, int n, double dt,
+                                                           double fdiff, const double* __restrict__ x,
+                                                           const double* __restrict__ a, double* __restrict__ xn,
+                                                           double* __restrict__ an) {
+    __shared__ double red[kEpiLanes][kEpiRows];
+    const int fl = threadIdx.x % kEpiRows, lane = threadIdx.x / kEpiRows;
+    const int i = blockIdx.x * kEpiRows + fl;
+    const int c = blockIdx.y;
+    red[lane][fl] = i < n ? sum_lane(part, 0, nchunk, KW + 3, i, c, lane) : 0.0;
+    __syncthreads();
+    if (lane != 0 || i >= n) return;
+    const double v = ((red[0][fl] + red[1][fl]) + red[2][fl]) + red[3][fl];
+    if (c < KW) {
+        const size_t o = size_t(i) * KW + c;
+        xn[o] = x[o] + dt * v * 1.0;
+    } else {
+        const size_t o = size_t(i) * KW + (c - KW);
+        an[o] = a[o] + (-dt) * v * fdiff;
+    }
+}
+
+// Fused geodesic adjoint step (OpEPDiffAdj partials): c < KW gives
+// pxo = px + dt fdiff DiffKT_sym - dt fdd DDiffK, c >= KW gives
+// pao = pa + dt K px - dt fd1 DiffK1and2 (the operation order of the four
+// reduce_job calls of the unfused step).
+template <typename T, int KW>
+__global__ void __launch_bounds__(256) k_reduce_epdiff_adj(Partials<T> part, int nchunk, int n, double dt,
+                                                           double fdiff, double fdd, double fd1,
+                                                           const double* __restrict__ px,
+                                                           const double* __restrict__ pa, double* __restrict__ pxo,
+                                                           double* __restrict__ pao) {
+    constexpr int NCH = 6 + 2 * KW;
+    __shared__ double red[2][kEpiLanes][kEpiRows];
+    const int fl = threadIdx.x % kEpiRows, lane = threadIdx.x / kEpiRows;
+    const int i = blockIdx.x * kEpiRows + fl;
+    const int c = blockIdx.y;
+    const int cc = c < KW ? c : c - KW;
+    const int ch1 = c < KW ? c : 6 + cc;                 // DiffKT_sym / K px
+    const int ch2 = c < KW ? 3 + c : 6 + KW + cc;        // DDiffK / DiffK1and2
+    red[0][lane][fl] = i < n ? sum_lane(part, 0, nchunk, NCH, i, ch1, lane) : 0.0;
+    red[1][lane][fl] = i < n ? sum_lane(part, 0, nchunk, NCH, i, ch2, lane) : 0.0;
+    __syncthreads();
+    if (lane != 0 || i >= n) return;
+    const double v1 = ((red[0][0][fl] + red[0][1][fl]) + red[0][2][fl]) + red[0][3][fl];
+    const double v2 = ((red[1][0][fl] + red[1][1][fl]) + red[1][2][fl]) + red[1][3][fl];
+    const size_t o = size_t(i) * KW + cc;
+    if (c < KW) {
+        const double u = px[o] + dt * v1 * fdiff;
+        pxo[o] = u + (-dt) * v2 * fdd;
+    } else {
+        const double u = pa[o] + dt * v1 * 1.0;
+        pao[o] = u + (-dt) * v2 * fd1;
+    }
+}
+
 // ---------------------------------------------------------------------------
 template <typename T, bool MS, int KW>
 int shoot_forward_t(const lddmm_kernel* kd, const double* at, int T_, int n, double* xt, double r,
@@ -490,12 +549,16 @@ int epdiff_forward_t(const lddmm_kernel* kd, int T_, int n, double* xt, double* 
     const double fdiff = MS ? -1.0 : ss.diff;
     using OPK = OpApplyK<KW, MS>;
     using OPD = OpDiffKT<KW, false, MS>;
+    using OPF = OpEPDiffFwd<KW>;
     Job<T, OPK> jk;
     Job<T, OPD> jd;
+    Job<T, OPF> jf;
     LDDMM_TRY(plan_job(jk, n, n, !MS));
     LDDMM_TRY(plan_job(jd, n, n, !MS));
+    if (!MS) LDDMM_TRY(plan_job(jf, n, n, true));
     void* ws;
-    LDDMM_TRY(workspace(256 + std::max(partial_bytes<T>(jk.plan, KW, n), partial_bytes<T>(jd.plan, 3, n)) + 512,
+    LDDMM_TRY(workspace(256 + std::max({partial_bytes<T>(jk.plan, KW, n), partial_bytes<T>(jd.plan, 3, n),
+                                        MS ? size_t(0) : partial_bytes<T>(jf.plan, KW + 3, n)}) + 512,
                         st, &ws));
     char* base = static_cast<char*>(ws);
     Carve cv;
@@ -504,6 +567,22 @@ int epdiff_forward_t(const lddmm_kernel* kd, int T_, int n, double* xt, double* 
     const size_t stride = size_t(n) * KW;
     const double dt = 1.0 / T_;
     LDDMM_TRY(launch_center(xt, n, xt, 0, KW, c0, st));       // one centre (x_0) for all steps
+    if constexpr (!MS) {
+        // one Gaussian scale: both sums of a step in one pass (OpEPDiffFwd) + one reduce
+        for (int k = 0; k < T_; ++k) {
+            const double* x = xt + size_t(k) * stride;
+            const double* a = at + size_t(k) * stride;
+            Carve c1; c1.off = poff;
+            Partials<T> pf = carve_partials<T>(base, c1, jf.plan, KW + 3, n);
+            jf.args = {x, a, c0, sc};
+            LDDMM_TRY(run_job(jf, n, n, pf, st));
+            hipLaunchKernelGGL((k_reduce_epdiff_fwd<T, KW>), dim3(unsigned((n + kEpiRows - 1) / kEpiRows), 2 * KW),
+                               dim3(256), 0, st, pf, jf.plan.nchunk, n, dt, fdiff, x, a, xt + size_t(k + 1) * stride,
+                               at + size_t(k + 1) * stride);
+            LDDMM_CHECK_LAUNCH();
+        }
+        return LDDMM_OK;
+    }
     for (int k = 0; k < T_; ++k) {
         const double* x = xt + size_t(k) * stride;
         const double* a = at + size_t(k) * stride;
@@ -540,12 +619,16 @@ int epdiff_adjoint_t(const lddmm_kernel* kd, int T_, int n, const double* xt, co
     Job<T, OPDD> j2;
     Job<T, OPK> j3;
     Job<T, OPD12> j4;
+    using OPA = OpEPDiffAdj<KW>;
+    Job<T, OPA> ja;
     LDDMM_TRY(plan_job(j1, n, n, !MS));
     LDDMM_TRY(plan_job(j2, n, n, !MS));
     LDDMM_TRY(plan_job(j3, n, n, !MS));
     LDDMM_TRY(plan_job(j4, n, n, !MS));
+    if (!MS) LDDMM_TRY(plan_job(ja, n, n, true));
     size_t pb = std::max(std::max(partial_bytes<T>(j1.plan, 3, n), partial_bytes<T>(j2.plan, 3, n)),
                          std::max(partial_bytes<T>(j3.plan, KW, n), partial_bytes<T>(j4.plan, KW, n)));
+    if (!MS) pb = std::max(pb, partial_bytes<T>(ja.plan, 6 + 2 * KW, n));
     void* ws;
     LDDMM_TRY(workspace(256 + pb + 512, st, &ws));
     char* base = static_cast<char*>(ws);
@@ -568,6 +651,16 @@ int epdiff_adjoint_t(const lddmm_kernel* kd, int T_, int n, const double* xt, co
         double* pxo = pxt + size_t(t) * stride;
         double* pao = pat + size_t(t) * stride;
         Carve c; c.off = poff;
+        if constexpr (!MS) {
+            // one Gaussian scale: the four sums in one pass (OpEPDiffAdj) + one reduce
+            Partials<T> pa4 = carve_partials<T>(base, c, ja.plan, 6 + 2 * KW, n);
+            ja.args = {x, a, px, pa, c0, T(ss.gamma), sc};
+            LDDMM_TRY(run_job(ja, n, n, pa4, st));
+            hipLaunchKernelGGL((k_reduce_epdiff_adj<T, KW>), dim3(unsigned((n + kEpiRows - 1) / kEpiRows), 2 * KW),
+                               dim3(256), 0, st, pa4, ja.plan.nchunk, n, dt, fdiff, fdd, fd1, px, pa, pxo, pao);
+            LDDMM_CHECK_LAUNCH();
+            continue;
+        }
         Partials<T> part = carve_partials<T>(base, c, j1.plan, 3, n);
         // zpx = DiffKT_sym(x, px, a) - DDiffK11and12(x, a, a, pa)
         j1.args = {x, x, px, a, c0, KW, KW, 0, 0.0, sc};

@@ -147,6 +147,28 @@ def test_epdiff_forward_adjoint(mods, dtype, n, T, sig):
 
 
 @pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_epdiff_forward_adjoint_2d(mods, dtype):
+    """Planar points (dim 2) through the geodesic forward and adjoint."""
+    sd, pe, synth, Surface, Kernel = mods
+    rng = np.random.default_rng(21)
+    n, T, sig = 300, 6, [1.1]
+    x0 = rng.normal(size=(n, 2)) * 3
+    a0 = rng.normal(size=(n, 2)) * 0.3
+    px1, pa1 = rng.normal(size=(n, 2)), rng.normal(size=(n, 2))
+    K = Kernel(name='gauss', sigma=sig)
+    K.pk_dtype = dtype
+    st = pe.secondOrderEvolution(x0, a0, K, 1.0 / T)
+    xr, ar = O.secondOrderEvolution(x0, a0, sig, 1.0 / T)
+    assert rel(st['xt'], xr) < TOL[dtype]
+    assert rel(st['at'], ar) < TOL[dtype]
+    res = pe.secondOrderGradient(x0, a0, px1, pa1, K, 1.0 / T)
+    da0, _, _, pxr, par = O.secondOrderGradient(x0, a0, px1, pa1, sig, 1.0 / T)
+    assert rel(res['pxt'], pxr) < TOL[dtype] * 5
+    assert rel(res['pat'], par) < TOL[dtype] * 5
+    assert rel(res['da0'], da0) < TOL[dtype] * 5
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
 def test_geodesic_hamiltonian_conservation(mods, dtype):
     """H = 1/2 a.K(x)a along EPDiff, evaluated in float64 on the device: equals
     the oracle's H of the same trajectory, and its drift shrinks like dt."""
