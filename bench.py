@@ -142,6 +142,39 @@ def registration_cfg1(torch):
     return out
 
 
+def second_order_registration(torch):
+    """Geodesic point-set matching (SecondOrderPointSetMatching): the plain
+    workload's template and target vertices (2,562 points each) as point sets,
+    measure data term (sigma 1), Gaussian deformation kernel sigma 2.5, T = 10,
+    50 BFGS iterations on the initial momentum; wall seconds per registration
+    after one untimed run, float64 and float32."""
+    import logging
+    from lddmm_amd import synth
+    from lddmm_amd.pointSets import PointSet
+    from lddmm_amd.kernelFunctions import Kernel
+    from lddmm_amd.secondOrderPointSetMatching import SecondOrderPointSetMatching
+    logging.getLogger().setLevel(logging.WARNING)
+    (v, f), (v2, _) = synth.plain_matching_pair(seed=0)
+    out = {}
+    for pr in ("float64", "float32"):
+        def make():
+            opts = {'KparDiff': Kernel(name='gauss', sigma=2.5), 'KparDist': Kernel(name='gauss', sigma=1.0),
+                    'errorType': 'measure', 'timeStep': 0.1, 'pk_dtype': pr, 'maxIter': 50, 'mode': 'quiet'}
+            return SecondOrderPointSetMatching(Template=PointSet(v), Target=PointSet(v2), options=opts)
+        make().optimizeMatching()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m = make()
+        m.optimizeMatching()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[pr] = {"seconds": dt, "iterations": int(m.bfgs_iterations), "initial_objective": m.bfgs_history[0],
+                   "final_objective": m.bfgs_history[-1]}
+        log(f"second-order registration ({pr}): {dt:.3f} s, {m.bfgs_iterations} iterations, "
+            f"objective {m.bfgs_history[0]:.4f} -> {m.bfgs_history[-1]:.4f}")
+    return out
+
+
 def warmstart_cfg5(torch, pairs=8):
     """Learned warm start on held-out pairs of the plain workload (seeds
     900000..): the zero-momentum plain registration (float64, 50 BFGS
@@ -515,6 +548,7 @@ def main():
         extra["population_cfg4"] = population_cfg4(torch)
         extra["precision_cfg3"] = precision_cfg3(torch)
         extra["learned_warmstart_cfg5"] = warmstart_cfg5(torch)
+        extra["second_order_registration"] = second_order_registration(torch)
 
     if not args.no_cpu:
         log("cpu baseline ...")
