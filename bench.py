@@ -142,6 +142,57 @@ def registration_cfg1(torch):
     return out
 
 
+def population_registration(torch, npairs=256, iters=10, sample=16):
+    """Batched population registration (PopulationMatching, lockstep L-BFGS,
+    one persistent launch per kernel sum) of the population workload (256
+    pairs, 506-19,741 vertices, sigma_def 3, varifold sigma 1, T = 10, FP32),
+    `iters` BFGS iterations per pair, against the same registrations run one
+    pair at a time by SurfaceMatching on every (256 / sample)-th pair (pair
+    sizes are drawn independently, so the sample's time times 256 / sample
+    estimates the whole population's)."""
+    import logging
+    import numpy as np
+    from lddmm_amd import synth
+    from lddmm_amd.population import PopulationMatching
+    from lddmm_amd.surfaces import Surface
+    from lddmm_amd.kernelFunctions import Kernel
+    from lddmm_amd.surfaceMatching import SurfaceMatching
+    logging.getLogger().setLevel(logging.WARNING)
+    prs = synth.population_pairs(npairs, seed=4)
+
+    def opts():
+        return {'KparDiff': Kernel(name='gauss', sigma=3.0), 'KparDist': Kernel(name='gauss', sigma=1.0),
+                'timeStep': 0.1, 'pk_dtype': 'float32', 'maxIter': iters}
+    tm = [Surface((f, v)) for (v, f), _ in prs]
+    tg = [Surface((f2, v2)) for _, (v2, f2) in prs]
+    PopulationMatching(tm[:4], tg[:4], opts()).optimizeMatching()          # code objects, tuning
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pm = PopulationMatching(tm, tg, opts())
+    pm.optimizeMatching()
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    idx = list(range(0, npairs, npairs // sample))
+    o = dict(opts(), errorType='varifold', mode='quiet')
+    t_seq = 0.0
+    for i in idx:
+        (v, f), (v2, f2) = prs[i]
+        o['KparDiff'] = Kernel(name='gauss', sigma=3.0)
+        o['KparDist'] = Kernel(name='gauss', sigma=1.0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        SurfaceMatching(Template=Surface((f, v)), Target=Surface((f2, v2)), options=o).optimizeMatching()
+        torch.cuda.synchronize()
+        t_seq += time.perf_counter() - t1
+    t_seq_all = t_seq * npairs / len(idx)
+    out = {"pairs": npairs, "bfgs_iterations": iters, "batched_seconds": t_batch,
+           "batched_seconds_per_pair": t_batch / npairs, "sequential_sample_pairs": len(idx),
+           "sequential_seconds_estimated": t_seq_all, "batched_speedup": t_seq_all / t_batch}
+    log(f"population registration: {npairs} pairs x {iters} iterations batched {t_batch:.2f} s, "
+        f"one at a time ~{t_seq_all:.2f} s (sample of {len(idx)}), {t_seq_all / t_batch:.2f}x")
+    return out
+
+
 def second_order_registration(torch):
     """Geodesic point-set matching (SecondOrderPointSetMatching): the plain
     workload's template and target vertices (2,562 points each) as point sets,
@@ -549,6 +600,7 @@ def main():
         extra["precision_cfg3"] = precision_cfg3(torch)
         extra["learned_warmstart_cfg5"] = warmstart_cfg5(torch)
         extra["second_order_registration"] = second_order_registration(torch)
+        extra["population_registration"] = population_registration(torch)
 
     if not args.no_cpu:
         log("cpu baseline ...")

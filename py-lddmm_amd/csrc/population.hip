@@ -40,11 +40,14 @@ template <typename T, class OP, int R>
 __global__ void __launch_bounds__(kBlock)
 k_pair_pop(const typename OP::template Args<T> args, const PopItem* __restrict__ items,
            const int32_t* __restrict__ bstart, const PopPair* __restrict__ pairs, const double* __restrict__ c0s,
-           T* __restrict__ phi, T* __restrict__ plo) {
+           T* __restrict__ phi, T* __restrict__ plo, const uint8_t* __restrict__ active) {
     __shared__ __attribute__((aligned(16))) T lds[kTile * OP::REC];
     const int e = bstart[blockIdx.x + 1];
     for (int it = bstart[blockIdx.x]; it < e; ++it) {
         const PopItem w = items[it];
+        // items of pairs outside the active set are skipped (the item is the
+        // same for every lane of the workgroup, so the branch is uniform)
+        if (active && !active[w.pair]) continue;
         const PopPair pp = pairs[w.pair];
         typename OP::template Args<T> a = args;
         a.c0 = c0s + 4 * w.pair;
@@ -623,15 +626,17 @@ struct PopLaunch {
     int R = 1, G = 1;
 };
 
+// `active` (device, one flag per pair) restricts the work to the active pairs;
+// null runs every item (autotuning times the full population).
 template <typename T, class OP>
 int launch_pop(const typename OP::template Args<T>& args, const PopLaunch<T>& L, const double* c0s, T* phi, T* plo,
-               hipStream_t st) {
+               const uint8_t* active, hipStream_t st) {
     if (L.R == 2)
         hipLaunchKernelGGL((k_pair_pop<T, OP, 2>), dim3(unsigned(L.G)), dim3(kBlock), 0, st, args, L.plan->d_items,
-                           L.plan->d_bstart, L.plan->d_pairs, c0s, phi, plo);
+                           L.plan->d_bstart, L.plan->d_pairs, c0s, phi, plo, active);
     else
         hipLaunchKernelGGL((k_pair_pop<T, OP, 1>), dim3(unsigned(L.G)), dim3(kBlock), 0, st, args, L.plan->d_items,
-                           L.plan->d_bstart, L.plan->d_pairs, c0s, phi, plo);
+                           L.plan->d_bstart, L.plan->d_pairs, c0s, phi, plo, active);
     LDDMM_CHECK_LAUNCH();
     return LDDMM_OK;
 }
@@ -712,10 +717,10 @@ int prepare_op(lddmm_population* pop, int op, int fam, bool ms, MakeArgs&& make_
         int rc = LDDMM_OK;
         for (const Cand& c : cands) {
             // one untimed launch (instruction cache, tables), one timed
-            rc = launch_pop<T, OP>(args, c.L, c0s, pp.phi, pp.plo, st);
+            rc = launch_pop<T, OP>(args, c.L, c0s, pp.phi, pp.plo, nullptr, st);
             if (rc != LDDMM_OK) break;
             if (hipEventRecord(e0, st) != hipSuccess) { rc = LDDMM_EHIP; break; }
-            rc = launch_pop<T, OP>(args, c.L, c0s, pp.phi, pp.plo, st);
+            rc = launch_pop<T, OP>(args, c.L, c0s, pp.phi, pp.plo, nullptr, st);
             if (rc != LDDMM_OK) break;
             if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
                 rc = LDDMM_EHIP;
@@ -858,7 +863,7 @@ int pop_forward_t(const lddmm_kernel* kd, lddmm_population* pop, const double* a
             const double* x = xt + size_t(t) * stride;
             const double* a = at + size_t(t) * stride;
             typename OPK::template Args<T> args{x, x, a, nullptr, 3, 3, 0, sc};
-            LDDMM_TRY((launch_pop<T, OPK>(args, L, c0, pp.phi, pp.plo, s)));
+            LDDMM_TRY((launch_pop<T, OPK>(args, L, c0, pp.phi, pp.plo, pop->d_active, s)));
             hipLaunchKernelGGL((k_pop_shoot_epi<T>), dim3(nblk(V)), dim3(256), 0, s, pp.phi, pp.plo,
                                L.plan->d_pairs, pop->d_vpair, pop->d_active, int(V), x, a, dt,
                                xt + size_t(t + 1) * stride, rowacc, r * dt);
@@ -931,13 +936,13 @@ int pop_adjoint_t(const lddmm_kernel* kd, lddmm_population* pop, const double* a
             typename OPD::template Args<T> args{x, x, pn, a, nullptr, 3, 3, 0, 2.0 * r, sc};
             double* g = grad + size_t(t) * stride;
             if (euclidean) {
-                LDDMM_TRY((launch_pop<T, OPA>(args, LA, c0, pp.phi, pp.plo, s)));
+                LDDMM_TRY((launch_pop<T, OPA>(args, LA, c0, pp.phi, pp.plo, pop->d_active, s)));
                 hipLaunchKernelGGL((k_pop_reduce_adj<T>), dim3(nblk(V)), dim3(256), 0, s, pp.phi, pp.plo,
                                    LA.plan->d_pairs, pop->d_vpair, pop->d_active, int(V), fdiff, dt, pn, pc, g);
                 LDDMM_CHECK_LAUNCH();
                 continue;
             }
-            LDDMM_TRY((launch_pop<T, OPD>(args, LD, c0, pp.phi, pp.plo, s)));
+            LDDMM_TRY((launch_pop<T, OPD>(args, LD, c0, pp.phi, pp.plo, pop->d_active, s)));
             // dat = 2 r a_t - p_{t+1}  (before p_t overwrites the other buffer)
             hipLaunchKernelGGL(k_pop_axpby, dim3(nblk(V)), dim3(256), 0, s, int(V), pop->d_vpair, pop->d_active,
                                2.0 * r, a, -1.0, pn, dat);
@@ -1007,7 +1012,7 @@ int pop_varifold_t(const lddmm_kernel* kd, lddmm_population* pop, const double* 
         const PopPartials<T> pp = pop_partials<T>(pop, fx.total);
         double* erow = reinterpret_cast<double*>(pop->scratch + fx.erow);
         LDDMM_TRY(prep(st));
-        LDDMM_TRY((launch_pop<T, OPE>(mk(), L, c0p(), pp.phi, pp.plo, st)));
+        LDDMM_TRY((launch_pop<T, OPE>(mk(), L, c0p(), pp.phi, pp.plo, pop->d_active, st)));
         hipLaunchKernelGGL((k_pop_vf_erows<T>), dim3(nblk(F)), dim3(256), 0, st, pp.phi, pp.plo, L.plan->d_pairs,
                            pop->d_fpair, pop->d_active, int(F), r1p(), erow);
         LDDMM_CHECK_LAUNCH();
@@ -1032,7 +1037,7 @@ int pop_varifold_t(const lddmm_kernel* kd, lddmm_population* pop, const double* 
     double* dz1 = reinterpret_cast<double*>(pop->scratch + fx.dz1);
     double* erow = reinterpret_cast<double*>(pop->scratch + fx.erow);
     LDDMM_TRY(prep(st));
-    LDDMM_TRY((launch_pop<T, OPG>(mk(), L, c0p(), pp.phi, pp.plo, st)));
+    LDDMM_TRY((launch_pop<T, OPG>(mk(), L, c0p(), pp.phi, pp.plo, pop->d_active, st)));
     hipLaunchKernelGGL((k_pop_vf_grad_epi<T>), dim3(nblk(F)), dim3(256), 0, st, pp.phi, pp.plo, L.plan->d_pairs,
                        pop->d_fpair, pop->d_active, int(F), r1p(), fD, z1, dz1, erow);
     LDDMM_CHECK_LAUNCH();

@@ -19,8 +19,10 @@ requests are gathered into one batched evaluation per round.  The reference
 has no population driver; per pair, results equal SurfaceMatching's up to
 floating-point summation order.
 """
+import concurrent.futures as cf
 import ctypes
 import logging
+import os
 import time
 
 import numpy as np
@@ -218,12 +220,19 @@ class PopulationMatching:
         self.options = o
         self.fv0 = [Surface(surf=s) for s in templates]
         self.fv1 = [Surface(surf=s) for s in targets]
-        # orientation as SurfaceMatching.fix_orientation (base/surfaceMatching.py:284-318)
-        for a, b in zip(self.fv0, self.fv1):
+
+        # orientation as SurfaceMatching.fix_orientation (base/surfaceMatching.py:284-318);
+        # the pairs are independent and numpy's sorts release the GIL, so a
+        # thread pool checks them side by side
+        def orient(ab):
+            a, b = ab
             a.getEdges()
             b.getEdges()
             if a.bdry.max() == 0 and b.bdry.max() == 0 and a.surfVolume() * b.surfVolume() < 0:
                 b.flipFaces()
+        workers = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else 4))
+        with cf.ThreadPoolExecutor(workers) as ex:
+            list(ex.map(orient, zip(self.fv0, self.fv1)))
         self.pop = Population(self.fv0, self.fv1)
         self.pop.set_graphs(o['graphs'])
         self.P = self.pop.P

@@ -188,30 +188,35 @@ class Surface:
 
     def getEdges(self):
         """Edges, edge -> faces and boundary flags (sparse; the reference builds
-        a dense V x V table, base/surfaces.py:61-116)."""
+        a dense V x V table, base/surfaces.py:61-116).  One stable sort of an
+        int64 key per (face, side), lo * nv + hi, gives the edges in the
+        lexicographic order of (lo, hi), each edge's first and last face (in
+        face order) and its multiplicity."""
         f = self.faces
+        nf = f.shape[0]
         e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
         es = np.sort(e, axis=1).astype(np.int64)
-        # one int64 key per edge, lo * nv + hi: its 1-D order is the lexicographic
-        # order of (lo, hi) rows, and a 1-D unique is far cheaper than unique(axis=0)
         nv = max(int(self.vertices.shape[0]), int(es.max()) + 1 if es.size else 1)
-        keys, inv, cnt = np.unique(es[:, 0] * nv + es[:, 1], return_inverse=True, return_counts=True)
-        uniq = np.stack([keys // nv, keys % nv], 1).astype(f.dtype)
-        inv = inv.reshape(-1)
-        self.edges = uniq
-        nf = f.shape[0]
+        keys = es[:, 0] * nv + es[:, 1]
+        order = np.argsort(keys, kind='stable')
+        ks = keys[order]
+        first = np.ones(len(ks), dtype=bool)
+        first[1:] = ks[1:] != ks[:-1]
+        start = np.flatnonzero(first)
+        cnt = np.diff(np.append(start, len(ks)))
+        uk = ks[start]
+        self.edges = np.stack([uk // nv, uk % nv], 1).astype(f.dtype)
+        inv = np.empty(len(ks), dtype=np.int64)
+        inv[order] = np.cumsum(first) - 1
         self.faceEdges = inv.reshape(3, nf).T
-        self.edgeFaces = -np.ones((uniq.shape[0], 2), dtype=int)
         fid = np.tile(np.arange(nf), 3)
-        order = np.argsort(inv, kind='stable')
-        first = np.ones(len(order), dtype=bool)
-        first[1:] = inv[order][1:] != inv[order][:-1]
-        self.edgeFaces[inv[order][first], 0] = fid[order][first]
-        sec = ~first
-        self.edgeFaces[inv[order][sec], 1] = fid[order][sec]
+        self.edgeFaces = -np.ones((len(uk), 2), dtype=int)
+        self.edgeFaces[:, 0] = fid[order[start]]
+        two = cnt >= 2                  # second face: the last one in face order
+        self.edgeFaces[two, 1] = fid[order[start[two] + cnt[two] - 1]]
         self.bdry = (cnt == 1).astype(int)
         vb = np.zeros(self.vertices.shape[0], dtype=int)
-        vb[uniq[cnt == 1].ravel()] = 1
+        vb[self.edges[cnt == 1].ravel()] = 1
         self.vertices_bdry = vb
         return self.edges
 
