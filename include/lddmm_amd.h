@@ -377,7 +377,8 @@ int lddmm_nn_to_bf16(const float* w, uint16_t* wbf, int64_t count, void* stream)
 /* Warm-start input feature (no counterpart in the reference): for each ray
  * i (origin orig[i,:], direction dir[i,:], float64 [n,3]) the signed
  * parameter t[i] of the intersection orig + t dir with the triangle mesh
- * (tv [nv,3] float64, tf [nf,3] int32, every index in [0, nv)) that has the
+ * (tv [nv,3] float64, tf [nf,3] int32; a face with an index outside [0, nv)
+ * is treated as degenerate and never hit) that has the
  * smallest |t| over both directions, lowest face index on ties; hit[i] = 1,
  * or t[i] = 0 and hit[i] = 0 when the line meets no triangle.  With unit
  * template vertex normals as directions, t is the signed normal distance

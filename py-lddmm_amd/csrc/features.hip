@@ -22,7 +22,7 @@ constexpr double RC_BARY_TOL = 1e-9;
 
 __global__ void __launch_bounds__(RC_BLOCK) k_normal_raycast(const double* __restrict__ orig,
                                                              const double* __restrict__ dir, int64_t n,
-                                                             const double* __restrict__ tv,
+                                                             const double* __restrict__ tv, int64_t nv,
                                                              const int32_t* __restrict__ tf, int64_t nf,
                                                              double* __restrict__ t_out,
                                                              int32_t* __restrict__ hit) {
@@ -42,11 +42,16 @@ __global__ void __launch_bounds__(RC_BLOCK) k_normal_raycast(const double* __res
         if (threadIdx.x < cnt) {
             const int64_t f = f0 + threadIdx.x;
             const int32_t a = tf[3 * f], b = tf[3 * f + 1], c = tf[3 * f + 2];
-            const double ax = tv[3 * a], ay = tv[3 * a + 1], az = tv[3 * a + 2];
             double* r = tri[threadIdx.x];
-            r[0] = ax; r[1] = ay; r[2] = az;
-            r[3] = tv[3 * b] - ax; r[4] = tv[3 * b + 1] - ay; r[5] = tv[3 * b + 2] - az;
-            r[6] = tv[3 * c] - ax; r[7] = tv[3 * c + 1] - ay; r[8] = tv[3 * c + 2] - az;
+            if (a < 0 || b < 0 || c < 0 || a >= nv || b >= nv || c >= nv) {
+                // an index outside [0, nv): a degenerate triangle (det = 0, never hit)
+                for (int k = 0; k < 9; ++k) r[k] = 0.0;
+            } else {
+                const double ax = tv[3 * a], ay = tv[3 * a + 1], az = tv[3 * a + 2];
+                r[0] = ax; r[1] = ay; r[2] = az;
+                r[3] = tv[3 * b] - ax; r[4] = tv[3 * b + 1] - ay; r[5] = tv[3 * b + 2] - az;
+                r[6] = tv[3 * c] - ax; r[7] = tv[3 * c + 1] - ay; r[8] = tv[3 * c + 2] - az;
+            }
         }
         __syncthreads();
         if (!live) continue;
@@ -94,7 +99,7 @@ int lddmm_normal_raycast(const double* orig, const double* dir, int64_t n, const
     LDDMM_REQUIRE(nf == 0 || nv > 0, "normal_raycast: faces without vertices");
     if (n == 0) return LDDMM_OK;
     hipLaunchKernelGGL(k_normal_raycast, dim3(unsigned((n + RC_BLOCK - 1) / RC_BLOCK)), dim3(RC_BLOCK), 0,
-                       static_cast<hipStream_t>(stream), orig, dir, n, tv, tf, nf, t_out, hit);
+                       static_cast<hipStream_t>(stream), orig, dir, n, tv, nv, tf, nf, t_out, hit);
     LDDMM_CHECK_LAUNCH();
     return LDDMM_OK;
 }

@@ -66,3 +66,20 @@ def test_normal_heights_channels(F):
         G = np.exp(-D2 / (2 * s * s))
         np.testing.assert_allclose(ch[:, k + 1].cpu().numpy(), G @ h / G.sum(1), atol=1e-9)
     np.testing.assert_allclose(fr[:, 0].cpu().numpy(), n, atol=1e-15)
+
+
+def test_raycast_out_of_range_face_is_never_hit(F):
+    """Called through the C ABI directly (the Python wrapper rejects such faces):
+    a face with an index outside [0, nv) is degenerate, the others still hit."""
+    import torch
+    from lddmm_amd import _backend as B
+    tv = torch.tensor([[0.0, 0, 0], [1, 0, 0], [0, 1, 0]], dtype=torch.float64, device="cuda")
+    tf = torch.tensor([[0, 1, 5], [0, 1, 2], [-1, 1, 2]], dtype=torch.int32, device="cuda")
+    o = torch.tensor([[0.2, 0.2, 1.0], [3.0, 3.0, 1.0]], dtype=torch.float64, device="cuda")
+    d = torch.tensor([[0, 0, -1.0], [0, 0, -1.0]], dtype=torch.float64, device="cuda")
+    t = torch.empty(2, dtype=torch.float64, device="cuda")
+    hit = torch.empty(2, dtype=torch.int32, device="cuda")
+    B.check(B.LIB.lddmm_normal_raycast(B.ptr(o), B.ptr(d), 2, B.ptr(tv), 3, B.ptr(tf), 3, B.ptr(t), B.ptr(hit),
+                                       B.stream_handle()), "normal_raycast")
+    assert hit.cpu().tolist() == [1, 0]
+    assert abs(float(t[0]) - 1.0) < 1e-15 and float(t[1]) == 0.0
