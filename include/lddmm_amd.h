@@ -138,6 +138,13 @@ int lddmm_applyDiffKT(const lddmm_kernel* kd,
                       const double* p, const double* a, int32_t k,
                       int32_t mode, double regweight, double* out, void* stream);
 
+/* out[n] = sum_s s^(KP-1)/W sum_j -((y_i-x_j)/s) . a_j K_s(y_i,x_j): the divergence
+ * at y_i of the velocity field K(., x) a (a is [m,dim]).
+ * Replaces ku.applyDiv / applyDiv_numba (base/kernelFunctions_util.py:596-620),
+ * reached from Kernel.applyDivergence (base/kernelFunctions.py:638-663). */
+int lddmm_applyDiv(const lddmm_kernel* kd, const double* y, int64_t n, const double* x, int64_t m, int32_t dim,
+                   const double* a, double* out, void* stream);
+
 /* out[n] = sum_s w_s sum_j K_s(y_i,x_j) (ay_i ax_j + (betay_i . betax_j)^2)
  * Replaces ku.applyktensor (base/kernelFunctions_util.py:1424-1506),
  * reached from Kernel.applyKTensor (base/kernelFunctions.py:509-517). */

@@ -16,6 +16,10 @@ How each function is pinned (tests/test_oracle.py):
                   duplicated-sigma identity, adjoint symmetry
   applyDiffKT     central finite differences of <p, K(y,x) a> (plain),
                   of p^T K(x) a (sym) and p^T K a - r a^T K a (lddmm)
+  applyDiv        trace of the central-difference Jacobian (in y) of applyK;
+                  the affine part likewise from affineApplyK
+  jacobian        log det of the finite-difference Jacobian of the Euler flow
+                  map carried alongside, error shrinking like dt
   applyktensor    single pair closed form, adjoint symmetry, rigid invariance
   applykmat       reduces to applyK when beta[i,j,:] = a[j,:]
   applykdiffmat   reduces to applyDiffKT(plain) when beta[i,j] = p_i.a_j
@@ -55,7 +59,8 @@ import numpy as np
 
 __all__ = [
     "applyK", "applyDiffKT", "applyktensor", "applykmat", "applykdiffmat",
-    "applyDiffK", "applyDDiffK", "affineApplyK", "affineApplyDiffKT", "internal_cost", "internal_costs",
+    "applyDiffK", "applyDDiffK", "applyDiv", "affineApplyK", "affineApplyDiffKT", "affineApplyDiv",
+    "jacobian", "internal_cost", "internal_costs",
     "semiReducedEvolution", "matching_objective_unreduced", "matching_gradient_unreduced",
     "surface_geometry", "surface_volume",
     "varifoldNorm0", "varifoldNormDef", "varifoldNorm", "varifoldNormGradient",
@@ -173,6 +178,26 @@ def applyDiffKT(y, x, p, a, scale, regweight=1., sym=False, lddmm=False, KP=-1, 
             elif sym:
                 c = c + a[i0:i1] @ p.T
             res[i0:i1] += -(d * (K * c)[..., None]).sum(1) * sKP1[si]
+    return res / W
+
+
+def applyDiv(y, x, a, scale, KP=-1, kernel=GAUSS):
+    """out[i, 0] = sum_s s^(KP-1)/W sum_j -((y_i - x_j)/s).a_j K_s(y_i, x_j), the
+    divergence at y_i of the field K(., x) a.
+
+    Follows base/kernelFunctions_util.py:606-620 (applyDiv_numba) with
+    gauss_fun_diff / lap_fun_diff at :191-201.
+    """
+    y, x, a = _f64(y), _f64(x), _as2d(a)
+    s, sKP, W = _scales(scale, KP)
+    sKP1 = s ** (KP - 1)
+    res = np.zeros((y.shape[0], 1))
+    for si, sc in enumerate(s):
+        ys, xs = y / sc, x / sc
+        for i0, i1 in _row_blocks(y.shape[0], x.shape[0]):
+            d = ys[i0:i1, None, :] - xs[None, :, :]
+            K = _profile(d, kernel, 1)
+            res[i0:i1, 0] += -(K * (d * a[None, :, :]).sum(-1)).sum(1) * sKP1[si]
     return res / W
 
 
@@ -354,6 +379,20 @@ def affineApplyDiffKT(x, p, a, kaff, regweight=1., sym=False, lddmm=False):
         elif sym:
             z += w1 * (xx * ETp).sum() * ETa
     return z
+
+
+def affineApplyDiv(x, a, kaff):
+    """Affine part of Kernel.applyDivergence(x, a) (base/kernelFunctions.py:653-659),
+    the same at every target: w1 sum_j (x_j - c).a_j for 'affine'; the
+    Euclidean basis is skew (trace 0).  Returns a float."""
+    kind, w1, w2, c = kaff
+    x, a = _f64(x), _as2d(a)
+    if kind == 'affine':
+        return float(w1 * ((x - c) * a).sum())
+    z = 0.0
+    for E in _euclid_basis(x.shape[1]):
+        z += w1 * ((x - c) * (a @ E)).sum() * np.trace(E)
+    return float(z)
 
 
 # ---------------------------------------------------------------------------
@@ -731,6 +770,23 @@ def landmarkDirectEvolutionEuler(x0, at, sigma, KP=-1, kernel=GAUSS, kaff=None):
     for t in range(T):
         xt[t + 1] = xt[t] + dt * _kv(xt[t], at[t], sigma, KP, kernel, kaff)
     return xt
+
+
+def jacobian(xt, at, sigma, KP=-1, kernel=GAUSS, kaff=None):
+    """J_0 = 0, J_{t+1} = J_t + dt div v_t(x_t), v_t = K(., x_t) a_t, dt = 1/T with
+    T = len(at): the withJacobian output of base/pointEvolution.py:149-152
+    (direct evolution), :366-367 (landmarkEPDiff) and :459-463
+    (secondOrderEvolution)."""
+    xt, at = _f64(xt), _f64(at)
+    T = at.shape[0]
+    dt = 1.0 / T
+    Jt = np.zeros((T + 1, xt.shape[1], 1))
+    for t in range(T):
+        d = applyDiv(xt[t], xt[t], at[t], sigma, KP, kernel=kernel)
+        if kaff is not None:
+            d = d + affineApplyDiv(xt[t], at[t], kaff)
+        Jt[t + 1] = Jt[t] + dt * d
+    return Jt
 
 
 def landmarkHamiltonianCovector(x0, at, px1, sigma, regweight, KP=-1, kernel=GAUSS, kaff=None):

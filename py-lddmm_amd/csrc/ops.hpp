@@ -231,6 +231,68 @@ struct OpAdjoint {
 };
 
 // ---------------------------------------------------------------------------
+// Divergence of the velocity field K(., x) a at the targets:
+//   out_i = sum_j g'(r2) (y_i - x_j).a_j         (base/kernelFunctions_util.py:596-620)
+// the radial factor and its scaling are those of OpDiffKT (same derivative).
+// ---------------------------------------------------------------------------
+template <bool MS>
+struct OpDiv {
+    static constexpr int NCH = 1;
+    static constexpr int REC = 8;
+    using Chunks = ContiguousChunks;
+    template <typename T>
+    struct Args {
+        const double *y, *x, *a, *c0;
+        int dim;
+        ScaleSet<T> sc;
+    };
+    template <typename T>
+    struct Tgt { T u0, u1, u2; };
+    struct Raw { double x0, x1, x2, a0, a1, a2; };
+
+    template <typename T>
+    __device__ static void load_target(const Args<T>& A, int i, Tgt<T>& t) {
+        t.u0 = coord<T>(A.y, i, A.dim, 0, A.c0, A.sc.cs);
+        t.u1 = coord<T>(A.y, i, A.dim, 1, A.c0, A.sc.cs);
+        t.u2 = coord<T>(A.y, i, A.dim, 2, A.c0, A.sc.cs);
+    }
+    template <typename T>
+    __device__ static void load_raw(const Args<T>& A, int j, Raw& r) {
+        if (j < 0) {
+            r.x0 = A.c0[0]; r.x1 = A.c0[1]; r.x2 = A.c0[2];
+            r.a0 = r.a1 = r.a2 = 0.0;
+            return;
+        }
+        const double* xp = A.x + size_t(j) * A.dim;
+        const double* ap = A.a + size_t(j) * A.dim;
+        r.x0 = xp[0];
+        r.x1 = A.dim > 1 ? xp[1] : 0.0;
+        r.x2 = A.dim > 2 ? xp[2] : 0.0;
+        r.a0 = ap[0];
+        r.a1 = A.dim > 1 ? ap[1] : 0.0;
+        r.a2 = A.dim > 2 ? ap[2] : 0.0;
+    }
+    template <typename T>
+    __device__ static void store_rec(const Args<T>& A, const Raw& r, T* rec) {
+        rec[0] = T((r.x0 - A.c0[0]) * A.sc.cs);
+        rec[1] = T((r.x1 - A.c0[1]) * A.sc.cs);
+        rec[2] = T((r.x2 - A.c0[2]) * A.sc.cs);
+        rec[3] = T(0);
+        rec[4] = T(r.a0); rec[5] = T(r.a1); rec[6] = T(r.a2); rec[7] = T(0);
+    }
+    template <typename T>
+    __device__ static void pair(const Args<T>& A, const Tgt<T>& t, const T* rec, T* acc) {
+        Vec4<T> p = lds4(rec);
+        T dx = t.u0 - p.x, dy = t.u1 - p.y, dz = t.u2 - p.z;
+        T r2 = fmaT(dz, dz, fmaT(dy, dy, dx * dx));
+        T g = gD<T, MS>(A.sc, r2);
+        Vec4<T> a = lds4(rec + 4);
+        T s = fmaT(dz, a.z, fmaT(dy, a.y, dx * a.x));
+        acc[0] = fmaT(g, s, acc[0]);
+    }
+};
+
+// ---------------------------------------------------------------------------
 // applyktensor: out_i = sum_j g(r2)(ay_i ax_j + (by_i.bx_j)^2)
 //                                          (base/kernelFunctions_util.py:1475-1506)
 // ---------------------------------------------------------------------------

@@ -442,6 +442,40 @@ extern "C" int lddmm_applyDiffKT(const lddmm_kernel* kd, const double* y, int64_
 #undef LDDMM_DKT
 }
 
+extern "C" int lddmm_applyDiv(const lddmm_kernel* kd, const double* y, int64_t n, const double* x, int64_t m,
+                              int32_t dim, const double* a, double* out, void* stream) {
+    LDDMM_TRY(check_common(kd, n, m, dim));
+    if (n == 0) return LDDMM_OK;
+    LDDMM_REQUIRE(y && out, "null y/out");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (m == 0) return launch_fill(out, size_t(n), 0.0, st);
+    LDDMM_REQUIRE(x && a, "null x/a");
+    auto run = [&](auto Tz, auto MSc) -> int {
+        using T = decltype(Tz);
+        constexpr bool MS = decltype(MSc)::value;
+        SingleScaleConsts ss;
+        ScaleSet<T> sc = make_scales<T>(kd, &ss);
+        Job<T, OpDiv<MS>> j;
+        LDDMM_TRY(plan_job(j, int(n), int(m), !MS));
+        void* ws;
+        LDDMM_TRY(workspace(partial_bytes<T>(j.plan, 1, int(n)) + 512, st, &ws));
+        char* base = static_cast<char*>(ws);
+        Carve cv;
+        double* c0 = reinterpret_cast<double*>(base + cv.take(32));
+        LDDMM_TRY(launch_center(y, int(n), x, int(m), dim, c0, st));
+        Partials<T> part = carve_partials<T>(base, cv, j.plan, 1, int(n));
+        j.args = {y, x, a, c0, dim, sc};
+        LDDMM_TRY(run_job(j, int(n), int(m), part, st));
+        ChanScale cs{};
+        cs.s[0] = MS ? -1.0 : ss.diff;
+        return run_reduce(part, j.plan.nchunk, 1, 1, int(n), cs, out, 1, 0, 0, nullptr, 1.0, st);
+    };
+    const bool ms = general_path(kd);
+    if (kd->prec == LDDMM_PREC_F32C)
+        return ms ? run(float{}, std::true_type{}) : run(float{}, std::false_type{});
+    return ms ? run(double{}, std::true_type{}) : run(double{}, std::false_type{});
+}
+
 extern "C" int lddmm_applyktensor(const lddmm_kernel* kd, const double* y, int64_t n, const double* x,
                                   int64_t m, int32_t dim, const double* ay, const double* ax,
                                   const double* betay, const double* betax, int32_t kb, double* out,

@@ -100,6 +100,7 @@ _ICP = ctypes.POINTER(LddmmInternalCost)
 SIGNATURES = {
     "lddmm_applyK": [_KP, _P, _I64, _P, _I64, _I32, _P, _I32, _P, _P],
     "lddmm_applyDiffKT": [_KP, _P, _I64, _P, _I64, _I32, _P, _P, _I32, _I32, _D, _P, _P],
+    "lddmm_applyDiv": [_KP, _P, _I64, _P, _I64, _I32, _P, _P, _P],
     "lddmm_applyktensor": [_KP, _P, _I64, _P, _I64, _I32, _P, _P, _P, _P, _I32, _P, _P],
     "lddmm_applykmat": [_KP, _P, _I64, _P, _I64, _I32, _P, _I32, _P, _P],
     "lddmm_applykdiffmat": [_KP, _P, _I64, _P, _I64, _I32, _P, _P, _P],

@@ -230,6 +230,18 @@ class Kernel(KernelSpec):
                     zpx = zpx + self.w1 * (xx * ETp).sum() * ETa
         return zpx
 
+    # divergence of the field K(., x) a at y  (base/kernelFunctions.py:638-663)
+    def applyDivergence(self, x, a, firstVar=None):
+        y = x if firstVar is None else firstVar
+        zpx = ku.applyDiv(y, x, a, self.name, self.sigma, self.order, dtype=self.pk_dtype,
+                          KP=self.ms_exponent)
+        if self.affine == 'affine':
+            # div_y sum_j w1 ((y-c).(x_j-c)) a_j = w1 sum_j (x_j-c).a_j
+            xx = x - _like(self.center, x)
+            zpx = zpx + self.w1 * (xx * a).sum()
+        # 'euclidean': the basis matrices are skew (trace 0), the affine part adds nothing
+        return zpx
+
     def testDiffKT(self, y, x, p, a):
         """Finite-difference check of applyDiffKT (base/kernelFunctions.py:629-635)."""
         k0 = (p * self.applyK(x, a, firstVar=y)).sum()

@@ -2,7 +2,7 @@
 
 Drop-in for the dispatch functions of py-lddmm's base/kernelFunctions_util.py
 (applyK :443, applyDiffKT :765, applyktensor :1424, applykmat :1510,
-applykdiffmat :1631, applyDiffK :646, applyDDiffK :1084 and the
+applykdiffmat :1631, applyDiffK :646, applyDDiffK :1084, applyDiv :596 and the
 applykdiff1/2/1and2, applykdiff11/12/11and12 wrappers :948-955, :1249-1256).
 Every call runs on the GPU through the C ABI of liblddmm_amd.so.
 
@@ -77,6 +77,25 @@ def applyDiffKT(y, x, p, a, name, scale, order, regweight=1., sym=False, lddmm=F
     B.check(B.LIB.lddmm_applyDiffKT(kd, B.ptr(y_), n, B.ptr(x_), m, dim, B.ptr(p_), B.ptr(a_), k,
                                     mode, float(regweight), B.ptr(out), B.stream_handle()),
             "applyDiffKT")
+    return _out(out, tt)
+
+
+def applyDiv(y, x, a, name, scale, order, cpu=False, dtype='float64', KP=-1):
+    """out[i, 0] = sum_s s^(KP-1)/W sum_j -((y_i - x_j)/s).a_j K_s(y_i, x_j): the
+    divergence at y_i of the field K(., x) a; shape [N, 1]
+    (base/kernelFunctions_util.py:596-620)."""
+    tt = _is_torch(y, x, a)
+    y_, x_ = _pts(y), _pts(x)
+    n, m, dim = y_.shape[0], x_.shape[0], y_.shape[1]
+    if x_.shape[1] != dim:
+        raise ValueError("y and x must have the same dimension")
+    a_ = _w(a, m)
+    if a_.shape[1] != dim:
+        raise ValueError("a must have one column per coordinate")
+    out = torch.empty((n, 1), dtype=torch.float64, device=y_.device)
+    kd = B.kernel_for(name, scale, order, KP, dtype)
+    B.check(B.LIB.lddmm_applyDiv(kd, B.ptr(y_), n, B.ptr(x_), m, dim, B.ptr(a_), B.ptr(out),
+                                 B.stream_handle()), "applyDiv")
     return _out(out, tt)
 
 

@@ -8,9 +8,10 @@ The T-step loops run on the GPU (lddmm_shoot_* / lddmm_epdiff_*): the
 trajectory stays in HBM and only the requested outputs come back.  numpy
 inputs return numpy arrays; CUDA tensors return CUDA tensors.  A kernel's
 affine part (Kernel(affine='affine'|'euclidean')) is applied in first-order
-shooting and its adjoint.  Affine controls, Jacobians, normals, passenger
-point sets and splines are not built (NotImplementedError), as in the
-configurations this build targets.
+shooting and its adjoint.  The log-Jacobian along the trajectory
+(withJacobian) is tracked on the device by the kernel divergence.  Affine
+controls, normals, passenger point sets and splines are not built
+(NotImplementedError), as in the configurations this build targets.
 """
 import ctypes
 
@@ -93,17 +94,33 @@ def shoot_adjoint(xt, at, px1, KparDiff, regweight, euclidean=True, covector=Fal
     return g, pxt
 
 
+def jacobian(xt, at, KparDiff, T=None):
+    """Log-Jacobian along a trajectory: J_0 = 0, J_{t+1} = J_t + dt div v_t(x_t)
+    with v_t = K(., x_t) a_t and dt = 1/T (base/pointEvolution.py:149-152,
+    :366-367, :459-463).  xt [>=T, N, d], at [>=T, N, d] device tensors;
+    returns [T+1, N, 1] float64 on the device."""
+    T = at.shape[0] if T is None else T
+    n = xt.shape[1]
+    Jt = torch.zeros((T + 1, n, 1), dtype=torch.float64, device=xt.device)
+    dt = 1.0 / T
+    for t in range(T):
+        Jt[t + 1] = Jt[t] + dt * KparDiff.applyDivergence(xt[t], at[t])
+    return Jt
+
+
 def landmarkDirectEvolutionEuler(x0, at, KparDiff, affine=None, options=None):
     """x_{t+1} = x_t + (1/T) K(x_t) a_t   (base/pointEvolution.py:88-164)."""
     _no_affine(affine)
     opt = {'withJacobian': False, 'withNormals': None, 'withPointSet': None}
     if options:
         opt.update(options)
-    if opt['withJacobian'] or opt['withNormals'] is not None or opt['withPointSet'] is not None:
-        raise NotImplementedError("Jacobian / normals / point-set tracking are not built")
+    if opt['withNormals'] is not None or opt['withPointSet'] is not None:
+        raise NotImplementedError("normals / point-set tracking are not built")
     tt = _is_torch(x0, at)
-    xt, _ = shoot(x0, at, KparDiff)
-    return {'xt': _ret(xt, tt), 'Jt': None, 'yt': None, 'nt': None}
+    at_ = B.to_dev(at)
+    xt, _ = shoot(x0, at_, KparDiff)
+    Jt = _ret(jacobian(xt, at_, KparDiff), tt) if opt['withJacobian'] else None
+    return {'xt': _ret(xt, tt), 'Jt': Jt, 'yt': None, 'nt': None}
 
 
 def landmarkHamiltonianCovector(x0, at, px1, Kpardiff, regweight, affine=None, extraTerm=None):
@@ -152,12 +169,17 @@ def landmarkEPDiff(T, x0, a0, KparDiff, affine=None, withJacobian=False, withNor
     """Geodesic shooting of (x0, a0) over T steps (base/pointEvolution.py:320-379).
 
     Returns (xt [T+1], at [T]); at[T-1] is the last momentum used (the
-    reference does not update it past T-1)."""
+    reference does not update it past T-1).  With withJacobian, returns
+    [xt, at, Jt] with Jt [T+1, N, 1] (the reference's list output; its
+    update writes the [N, 1] divergence into an [N] slice, here it is
+    flattened as in secondOrderEvolution)."""
     _no_affine(affine)
-    if withJacobian or withNormals is not None or withPointSet is not None:
-        raise NotImplementedError("Jacobian / normals / point-set tracking are not built")
+    if withNormals is not None or withPointSet is not None:
+        raise NotImplementedError("normals / point-set tracking are not built")
     tt = _is_torch(x0, a0)
     xt, at = _epdiff(x0, a0, KparDiff, T)
+    if withJacobian:
+        return [_ret(xt, tt), _ret(at[:T], tt), _ret(jacobian(xt, at, KparDiff, T), tt)]
     return _ret(xt, tt), _ret(at[:T], tt)
 
 
@@ -188,12 +210,13 @@ def secondOrderEvolution(x0, a0, KparDiff, timeStep, withSpline=None, affine=Non
     if withSpline is not None:
         raise NotImplementedError("spline controls are not built")
     _no_affine(affine)
-    if options and (options.get('withJacobian') or options.get('withPointSet') is not None):
-        raise NotImplementedError("Jacobian / point-set tracking are not built")
+    if options and options.get('withPointSet') is not None:
+        raise NotImplementedError("point-set tracking is not built")
     T = int(np.round(1 / timeStep))
     tt = _is_torch(x0, a0)
     xt, at = _epdiff(x0, a0, KparDiff, T)
-    return {'xt': _ret(xt, tt), 'at': _ret(at, tt), 'Jt': None, 'yt': None}
+    Jt = _ret(jacobian(xt, at, KparDiff, T), tt) if options and options.get('withJacobian') else None
+    return {'xt': _ret(xt, tt), 'at': _ret(at, tt), 'Jt': Jt, 'yt': None}
 
 
 def secondOrderHamiltonian(x, a, px, pa, KparDiff, withSpline=None, affine=None):

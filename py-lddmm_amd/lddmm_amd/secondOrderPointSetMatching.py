@@ -70,14 +70,14 @@ class SecondOrderPointSetMatching(PointSetMatching):
     def objectiveFunDef(self, control, var=None, withTrajectory=True, withJacobian=False, display=False):
         """1/2 a0.K(x0)a0 (device float64 scalar) and the trajectory
         (base/secondOrderPointSetMatching.py:113-147)."""
-        if withJacobian:
-            raise NotImplementedError("Jacobian tracking is not built")
         if var is not None and 'Init' in var:
             raise NotImplementedError("a moving initial shape is not built")
         st = self.solveStateEquation(control=control)
+        if withJacobian:
+            st['Jt'] = evol.jacobian(st['xt'], st['at'], self.options['KparDiff'], self.Tsize)
         a0 = B.to_dev(control['a0'])
         obj = 0.5 * (a0 * self.options['KparDiff'].applyK(self.x0_dev, a0)).sum()
-        return (obj, st) if withTrajectory else obj
+        return (obj, st) if (withTrajectory or withJacobian) else obj
 
     def getGradient(self, coeff=1.0, update=None):
         """grad['a0'] = (K(x0) a0 - pa_0) / coeff with px1 = -dataTerm'(x_T),

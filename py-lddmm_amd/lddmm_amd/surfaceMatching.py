@@ -300,8 +300,8 @@ class SurfaceMatching:
     def objectiveFunDef(self, control, var=None, withTrajectory=True, withJacobian=False):
         """Deformation energy sum_t r dt a_t.K(x_t)a_t ds^2 and the trajectory
         (base/surfaceMatching.py:468-565); energy fused in the device shoot."""
-        if withJacobian:
-            raise NotImplementedError("Jacobian tracking is not built")
+        if withJacobian and self.unreduced:
+            raise NotImplementedError("Jacobian tracking of the semi-reduced flow is not built")
         if self.unreduced:
             xt, e = self._unreduced_forward(control, energy=True)
             st = State()
@@ -311,8 +311,10 @@ class SurfaceMatching:
                            regweight=self.options['regWeight'], energy=True, icost=self.icost)
         st = State()
         st['xt'] = xt
+        if withJacobian:
+            st['Jt'] = evol.jacobian(xt, B.to_dev(control['at']), self.options['KparDiff'])
         obj = e[0] * self.ds ** 2
-        return (obj, st) if withTrajectory else obj
+        return (obj, st) if (withTrajectory or withJacobian) else obj
 
     def objectiveFun(self):
         """base/surfaceMatching.py:568-594"""

@@ -727,3 +727,84 @@ def test_unreduced_matching_gradient_fd(spheres, variant):
     dc, da = rnd(*ct.shape), rnd(*at.shape)
     np.testing.assert_allclose((ga * da).sum(), _fd(lambda aa: obj(ct, aa), at, da), rtol=1e-6)
     np.testing.assert_allclose((gc * dc).sum(), _fd(lambda cc: obj(cc, at), ct, dc), rtol=1e-6)
+
+
+# ------------------------------------------------------------- divergence
+def _fd_div(f, y, eps=1e-6):
+    """Trace of the central-difference Jacobian of f: [N, d] -> [N, d]."""
+    n, dim = y.shape
+    div = np.zeros(n)
+    for k in range(dim):
+        e = np.zeros((n, dim))
+        e[:, k] = 1.0
+        div += ((f(y + eps * e) - f(y - eps * e)) / (2 * eps))[:, k]
+    return div
+
+
+@pytest.mark.parametrize("kern,sig,dim", [(O.GAUSS, [1.2], 3), (O.GAUSS, [0.9, 2.1], 3), (O.GAUSS, [1.0], 2),
+                                          (('laplacian', 1), [1.1], 3), (('laplacian', 3), [0.9, 2.0], 3),
+                                          (('laplacian', 4), [1.3], 2)])
+def test_applyDiv_is_trace_of_the_jacobian_of_applyK(kern, sig, dim):
+    rng = np.random.default_rng(len(sig) + 10 * dim)
+    y, x, a = rng.normal(size=(12, dim)), rng.normal(size=(17, dim)), rng.normal(size=(17, dim))
+    fd = _fd_div(lambda yy: O.applyK(yy, x, a, sig, kernel=kern), y)
+    got = O.applyDiv(y, x, a, sig, kernel=kern)
+    assert got.shape == (12, 1)
+    # the sum cancels (terms ~1, divergence ~1e-3): the difference quotient's
+    # rounding (~1e-9) sets the absolute floor; a wrong term or power is ~1e-2
+    np.testing.assert_allclose(got[:, 0], fd, rtol=1e-7, atol=1e-8)
+
+
+def test_applyDiv_laplacian_order0_vanishes():
+    """Order 0 has a zero first-derivative row in the reference's table
+    (base/kernelFunctions_util.py:27): the divergence sum is zero."""
+    x, a = rnd(9, 3), rnd(9, 3)
+    assert np.all(O.applyDiv(x, x, a, [1.0], kernel=('laplacian', 0)) == 0)
+
+
+@pytest.mark.parametrize("kaff", KAFF, ids=['affine', 'euclidean'])
+def test_affine_divergence_is_trace_of_the_jacobian(kaff):
+    rng = np.random.default_rng(77)
+    y, x, a = rng.normal(size=(11, 3)), rng.normal(size=(11, 3)), rng.normal(size=(11, 3))
+    fd = _fd_div(lambda yy: O.affineApplyK(yy, x, a, kaff), y)
+    got = O.affineApplyDiv(x, a, kaff)
+    np.testing.assert_allclose(np.full(11, got), fd, rtol=1e-7, atol=1e-9)
+    if kaff[0] == 'euclidean':          # rotations preserve volume
+        assert got == 0
+
+
+def _flow_logdet(x0, at, sig, kaff, h=1e-5):
+    """log det of the central-difference Jacobian of the Euler flow map at x0
+    (the perturbed copies ride along as passenger points)."""
+    xt = O.landmarkDirectEvolutionEuler(x0, at, sig, kaff=kaff)
+    T = at.shape[0]
+    n, dim = x0.shape
+    ys = np.concatenate([x0 + s * h * np.eye(dim)[k] for k in range(dim) for s in (1, -1)])
+    for t in range(T):
+        v = O.applyK(ys, xt[t], at[t], sig)
+        if kaff is not None:
+            v = v + O.affineApplyK(ys, xt[t], at[t], kaff)
+        ys = ys + v / T
+    ys = ys.reshape(dim, 2, n, dim)
+    D = np.stack([(ys[k, 0] - ys[k, 1]) / (2 * h) for k in range(dim)], axis=-1)   # [n, dim, dim]
+    return np.log(np.linalg.det(D))
+
+
+@pytest.mark.parametrize("kaff", [None, KAFF[0]], ids=['none', 'affine'])
+def test_jacobian_tracks_log_det_of_the_flow(kaff):
+    """J_T = sum_t dt div v_t(x_t) differs from log det D(phi_T) by
+    sum_t [log det(I + dt Dv_t) - dt tr Dv_t] = O(dt): halving dt halves it."""
+    rng = np.random.default_rng(78)
+    x0 = rng.normal(size=(7, 3))
+    # the affine field grows with x_t: a smaller momentum keeps the flow bounded
+    a = (0.4 if kaff is None else 0.06) * rng.normal(size=(7, 3))
+    errs = []
+    for T in (20, 40, 80):
+        at = np.repeat(a[None], T, axis=0)
+        xt = O.landmarkDirectEvolutionEuler(x0, at, [1.3], kaff=kaff)
+        J = O.jacobian(xt, at, [1.3], kaff=kaff)
+        assert J.shape == (T + 1, 7, 1) and np.all(J[0] == 0)
+        ld = _flow_logdet(x0, at, [1.3], kaff)
+        errs.append(np.abs(J[-1, :, 0] - ld).max())
+        assert errs[-1] < 0.05 * np.abs(ld).max()
+    assert 1.7 < errs[0] / errs[1] < 2.3 and 1.7 < errs[1] / errs[2] < 2.3
