@@ -18,6 +18,7 @@ the host once per evaluation.  `control_numpy()` / `fvDef` give host copies.
 Unsupported options raise NotImplementedError.
 """
 import logging
+import os
 from copy import deepcopy
 
 import numpy as np
@@ -27,6 +28,7 @@ from . import _backend as B
 from . import bfgs as bfgs_mod
 from . import pointEvolution as evol
 from . import surfaceDistances as sd
+from . import vtk_io
 from .kernelFunctions import Kernel
 from .surfaces import Surface
 
@@ -96,7 +98,7 @@ class SurfaceMatching:
             'scaleWeight': None, 'transWeight': 0.1, 'testGradient': False, 'subsampleTargetSize': -1,
             'affineOnly': False, 'passenger': None, 'Landmarks': None, 'saveFile': 'evolution',
             'saveRate': 10, 'mode': 'normal', 'verb': True, 'pplot': False, 'saveTrajectories': False,
-            'affine': None, 'outputDir': '.', 'symmetric': False, 'internalCost': None,
+            'affine': None, 'outputDir': '.', 'writeOutput': False, 'symmetric': False, 'internalCost': None,
             'internalWeight': 1.0, 'lineSearch': 'Weak_Wolfe', 'gradTol': -1., 'gradLB': 0.001,
             'reweightCells': False, 'unreducedResetRate': -1, 'fidelityWeight': 0.,
         }
@@ -583,9 +585,87 @@ class SurfaceMatching:
             self.resetPK('float64')
             self.pkBuffer = 0
 
+    def saveEvolution(self, fv0, state, passenger=None, fileName='evolution', velocity=None, orientation=None,
+                      with_area_displacement=False):
+        """One legacy VTK file per time step, `outputDir/fileName{t:03d}.vtk`
+        (base/surfaceMatching.py:1219-1290): the deformed template with point
+        scalars Jacobian = exp(J_t), Jacobian_T (vertex area / initial vertex
+        area), Jacobian_N (their ratio) when the state holds 'Jt', the
+        accumulated normal displacement sum_s dt v_s.n_s, and the velocity
+        vectors v_t = K(x_t) a_t.  Passengers are not built."""
+        if passenger is not None and passenger[0] is not None:
+            raise NotImplementedError("passenger surfaces / points are not built")
+        xt = state['xt']
+        xt = xt.cpu().numpy() if isinstance(xt, torch.Tensor) else np.asarray(xt)
+        Jt = state['Jt'] if 'Jt' in state else None
+        if isinstance(Jt, torch.Tensor):
+            Jt = Jt.cpu().numpy()
+        if velocity is None:
+            K, at = self.options['KparDiff'], B.to_dev(self.control['at'])
+            xd = B.to_dev(xt)
+            velocity = torch.stack([K.applyK(xd[t], at[t]) * self.ds for t in range(self.Tsize)]).cpu().numpy()
+        orientation = self.fv0ori if orientation is None else orientation
+        fn = [fileName + f'{kk:03d}' for kk in range(self.Tsize + 1)] if isinstance(fileName, str) else fileName
+        outdir = self.options['outputDir']
+        os.makedirs(outdir, exist_ok=True)
+        fvDef = Surface(surf=fv0)
+        AV0 = fvDef.computeVertexArea()
+        nvert = fv0.vertices.shape[0]
+        nu = orientation * fv0.computeVertexNormals()
+        v = velocity[0, :nvert, :]
+        displ = np.zeros(nvert)
+        area_displ = np.zeros((self.Tsize + 1, xt.shape[1]))
+        dt = 1.0 / self.Tsize
+        for kk in range(self.Tsize + 1):
+            fvDef.updateVertices(xt[kk, :nvert, :])
+            AV = fvDef.computeVertexArea()[0] / AV0[0]
+            vf = vtk_io.vtkFields()
+            if Jt is not None:
+                vf.scalars += ['Jacobian', np.exp(Jt[kk, :nvert, 0]), 'Jacobian_T', AV,
+                               'Jacobian_N', np.exp(Jt[kk, :nvert, 0]) / AV]
+            vf.scalars += ['displacement', displ]
+            if kk < self.Tsize:
+                nu = orientation * fvDef.computeVertexNormals()
+                v = velocity[kk, :nvert, :]
+                kkm = kk
+            else:
+                kkm = kk - 1
+            vf.vectors += ['velocity', velocity[kkm, :nvert]]
+            if with_area_displacement and kk > 0:
+                area_displ[kk, :] = area_displ[kk - 1, :] + dt * ((AV + 1) * (v * nu).sum(axis=1))[None, :]
+            fvDef.saveVTK2(os.path.join(outdir, fn[kk] + '.vtk'), vf)
+            displ = displ + dt * (v * nu).sum(axis=1)
+        return area_displ
+
+    def saveOutputs(self):
+        """The file output of base/surfaceMatching.py:1370-1432: re-shoot with the
+        Jacobian, optionally the point trajectories (saveTrajectories), then
+        the per-step surfaces (saveEvolution)."""
+        if not isinstance(getattr(self, 'fvInit', None), Surface):
+            raise NotImplementedError("file output (writeOutput) is built for surface matching only")
+        if self.unreduced:
+            xt, _ = self._unreduced_forward(self.control, energy=False)
+            st = State()
+            st['xt'] = xt
+        else:
+            _, st = self.objectiveFunDef(self.control, withTrajectory=True, withJacobian=True)
+        self.state['xt'] = st['xt']
+        if 'Jt' in st:
+            self.state['Jt'] = st['Jt']
+        outdir = self.options['outputDir']
+        os.makedirs(outdir, exist_ok=True)
+        if self.options['saveTrajectories']:
+            vtk_io.saveTrajectories(os.path.join(outdir, self.options['saveFile'] + 'curves.vtk'),
+                                    st['xt'].cpu().numpy())
+        self.saveEvolution(self.fvInit, st, fileName=self.options['saveFile'])
+
     def endOfIteration(self, forceSave=False):
-        """base/surfaceMatching.py:1348-1503 without file output."""
+        """base/surfaceMatching.py:1348-1503.  File output (saveOutputs) runs every
+        saveRate iterations and at the end when the option writeOutput is set;
+        by default nothing is written, so registration time has no file I/O."""
         self.iter += 1
+        if self.options['writeOutput'] and (forceSave or self.iter % self.saveRate == 0):
+            self.saveOutputs()
         if self.reset:
             obj0 = self.objectiveFun()
             self.obj = None

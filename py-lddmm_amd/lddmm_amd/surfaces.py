@@ -8,6 +8,7 @@ surfel, face_weights, component.  `device()` returns the C-ABI view of the
 surface (float64 vertices, int32 faces, vertex -> face adjacency) on the GPU,
 rebuilt only when the vertices or topology change.
 """
+import logging
 import os
 
 import numpy as np
@@ -168,6 +169,69 @@ class Surface:
         self.centers = (x1 + x2 + x3) / 3
         self.surfel = np.cross(x2 - x1, x3 - x1) / 2
         self._dev = None
+
+    def computeVertexArea(self, simpleMethod=False):
+        """Vertex areas AV [nv] and face areas AF [nf] (base/surfaces.py:364-440).
+
+        simpleMethod: each corner gets a third of its face.  Otherwise the mixed
+        Voronoi split of each face: when no angle is obtuse, corner i gets
+        (|e_ij|^2 cot k + |e_ik|^2 cot j) / 8, the part of the face nearer to it
+        than to the other corners; when the angle at corner i is obtuse, each
+        other corner j gets the right triangle that the perpendicular bisector
+        of edge ij cuts off the face, |e_ij|^2 / (8 cot j), and corner i keeps
+        the rest.  Either way the three parts add up to the face area."""
+        V, F = self.vertices, self.faces
+        x12 = V[F[:, 1]] - V[F[:, 0]]
+        x13 = V[F[:, 2]] - V[F[:, 0]]
+        x23 = V[F[:, 2]] - V[F[:, 1]]
+        AF = np.sqrt((np.cross(x12, x13) ** 2).sum(axis=1)) / 2
+        if simpleMethod:
+            u1 = u2 = u3 = AF / 3
+        else:
+            l12, l13, l23 = (x12 ** 2).sum(1), (x13 ** 2).sum(1), (x23 ** 2).sum(1)
+            n12, n13, n23 = np.sqrt(l12), np.sqrt(l13), np.sqrt(l23)
+            with np.errstate(divide='ignore', invalid='ignore'):
+                c1 = (x12 * x13).sum(1) / (n12 * n13)
+                c2 = -(x12 * x23).sum(1) / (n12 * n23)
+                c3 = (x13 * x23).sum(1) / (n13 * n23)
+                cot1, cot2, cot3 = (c / np.sqrt(1 - c ** 2) for c in (c1, c2, c3))
+                u1 = (l12 * cot3 + l13 * cot2) / 8
+                u2 = (l12 * cot3 + l23 * cot1) / 8
+                u3 = (l13 * cot2 + l23 * cot1) / 8
+                o1 = c1 < 0
+                o2 = ~o1 & (c2 < 0)
+                o3 = ~o1 & ~o2 & (c3 < 0)
+                u2 = np.where(o1, l12 / (8 * cot2), u2)
+                u3 = np.where(o1, l13 / (8 * cot3), u3)
+                u1 = np.where(o1, AF - u2 - u3, u1)
+                u1 = np.where(o2, l12 / (8 * cot1), u1)
+                u3 = np.where(o2, l23 / (8 * cot3), u3)
+                u2 = np.where(o2, AF - u1 - u3, u2)
+                u1 = np.where(o3, l13 / (8 * cot1), u1)
+                u2 = np.where(o3, l23 / (8 * cot2), u2)
+                u3 = np.where(o3, AF - u1 - u2, u3)
+        AV = np.zeros(V.shape[0])
+        np.add.at(AV, F[:, 0], u1)
+        np.add.at(AV, F[:, 1], u2)
+        np.add.at(AV, F[:, 2], u3)
+        lonely = np.flatnonzero(np.abs(AV) < 1e-10)
+        if lonely.size:
+            logging.info(f'Warning: vertices {lonely[:10].tolist()} have no face; use removeIsolated')
+        return AV, AF
+
+    def computeAreaWeightedVertexNormals(self):
+        """Sum of the incident faces' area vectors at each vertex (base/surfaces.py:456-465)."""
+        self.computeCentersAreas()
+        normals = np.zeros(self.vertices.shape)
+        for c in range(3):
+            np.add.at(normals, self.faces[:, c], self.surfel)
+        return normals
+
+    def computeVertexNormals(self):
+        """Unit vertex normals: the normalised sum of the incident faces' area
+        vectors (base/surfaces.py:442-454)."""
+        normals = self.computeAreaWeightedVertexNormals()
+        return normals / np.sqrt((normals ** 2).sum(axis=1))[:, None]
 
     def flipFaces(self):
         """base/surfaces.py:603-605"""

@@ -197,3 +197,50 @@ def test_matching_objective_with_jacobian(env):
     xr, ar = O.secondOrderEvolution(v, a0, [2.0], 0.1)
     r = rel(st['Jt'].cpu().numpy(), O.jacobian(xr, ar[:10], [2.0]))
     assert r < TOL['float64'], r
+
+
+def _vtk_field(path, kind, name, n):
+    lines = open(path).read().splitlines()
+    if kind == "SCALARS":
+        i = lines.index(f"SCALARS {name} float 1") + 2
+        return np.array([float(t) for t in lines[i:i + n]])
+    i = lines.index(f"VECTORS {name} float") + 1
+    return np.array([[float(u) for u in t.split()] for t in lines[i:i + n]])
+
+
+def test_registration_writes_the_evolution_with_its_jacobian(env, tmp_path):
+    """writeOutput: every saveRate iterations and at the end, one legacy VTK
+    file per time step (base/surfaceMatching.py:1219-1290, :1370-1432); the
+    last files hold exp(J_t), the vertex-area ratio and the velocity of the
+    final control, and the point trajectories (saveTrajectories)."""
+    from lddmm_amd import synth
+    from lddmm_amd.surfaces import Surface
+    from lddmm_amd.surfaceMatching import SurfaceMatching
+    Kernel = env["Kernel"]
+    v, f = synth.icosphere(2, 5.0)
+    sm = SurfaceMatching(Template=Surface((f, v)), Target=Surface((f, v * 1.15)),
+                         options={'KparDiff': Kernel('gauss', sigma=2.5), 'KparDist': Kernel('gauss', sigma=1.0),
+                                  'errorType': 'varifold', 'mode': 'quiet', 'timeStep': 0.2,
+                                  'pk_dtype': 'float64', 'maxIter': 4, 'saveRate': 2, 'writeOutput': True,
+                                  'outputDir': str(tmp_path / "out"), 'saveTrajectories': True})
+    sm.optimizeMatching()
+    names = sorted(p.name for p in (tmp_path / "out").iterdir())
+    assert names == ["evolution%03d.vtk" % k for k in range(6)] + ["evolutioncurves.vtk"]
+    at = sm.control['at'].cpu().numpy()
+    xr = O.landmarkDirectEvolutionEuler(v, at, [2.5])
+    Jr = O.jacobian(xr, at, [2.5])
+    nv = v.shape[0]
+    for k in (0, 3, 5):
+        path = str(tmp_path / "out" / ("evolution%03d.vtk" % k))
+        J = _vtk_field(path, "SCALARS", "Jacobian", nv)
+        assert rel(J, np.exp(Jr[k, :, 0])) < 1e-12
+        JT = _vtk_field(path, "SCALARS", "Jacobian_T", nv)
+        S = Surface((f, xr[k]))
+        assert rel(JT, S.computeVertexArea()[0] / Surface((f, v)).computeVertexArea()[0]) < 1e-10
+        assert rel(_vtk_field(path, "SCALARS", "Jacobian_N", nv), J / JT) < 1e-12
+        vel = _vtk_field(path, "VECTORS", "velocity", nv)
+        kk = min(k, 4)
+        assert rel(vel, O.applyK(xr[kk], xr[kk], at[kk], [2.5])) < 1e-11
+    assert np.all(_vtk_field(str(tmp_path / "out" / "evolution000.vtk"), "SCALARS", "displacement", nv) == 0)
+    # the template grows toward the larger target: the mean Jacobian exceeds 1
+    assert np.mean(_vtk_field(str(tmp_path / "out" / "evolution005.vtk"), "SCALARS", "Jacobian", nv)) > 1.0
