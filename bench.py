@@ -419,6 +419,9 @@ def main():
     ap.add_argument("--registration", action="store_true",
                     help="only run the plain registration (icosphere 4, float64 and float32) three times, "
                          "for rocprofv3 kernel traces of the registration loop")
+    ap.add_argument("--population", action="store_true",
+                    help="only run the batched population workload (256 pairs: forward, adjoint, varifold "
+                         "gradient, and the forward as 256 single-pair calls), for rocprofv3 kernel traces")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -440,6 +443,9 @@ def main():
     if args.registration:
         for rep in range(3):
             registration_cfg1(torch)
+        return
+    if args.population:
+        print(json.dumps(population_cfg4(torch)), flush=True)
         return
     N = M = args.n
     y_h = synth.uniform_cloud(N, seed=1)
