@@ -776,7 +776,8 @@ def jacobian(xt, at, sigma, KP=-1, kernel=GAUSS, kaff=None):
     """J_0 = 0, J_{t+1} = J_t + dt div v_t(x_t), v_t = K(., x_t) a_t, dt = 1/T with
     T = len(at): the withJacobian output of base/pointEvolution.py:149-152
     (direct evolution), :366-367 (landmarkEPDiff) and :459-463
-    (secondOrderEvolution)."""
+    (secondOrderEvolution); with xt = the control points c_t, the semi-reduced
+    flow's (base/pointEvolutionSemiReduced.py:78-81)."""
     xt, at = _f64(xt), _f64(at)
     T = at.shape[0]
     dt = 1.0 / T

@@ -302,13 +302,16 @@ class SurfaceMatching:
     def objectiveFunDef(self, control, var=None, withTrajectory=True, withJacobian=False):
         """Deformation energy sum_t r dt a_t.K(x_t)a_t ds^2 and the trajectory
         (base/surfaceMatching.py:468-565); energy fused in the device shoot."""
-        if withJacobian and self.unreduced:
-            raise NotImplementedError("Jacobian tracking of the semi-reduced flow is not built")
         if self.unreduced:
             xt, e = self._unreduced_forward(control, energy=True)
             st = State()
             st['xt'] = xt
-            return (e, st) if withTrajectory else e
+            if withJacobian:
+                # the reference takes the divergence at the control points c_t
+                # (base/pointEvolutionSemiReduced.py:78-81)
+                st['Jt'] = evol.jacobian(B.to_dev(control['ct']), B.to_dev(control['at']) * self.ds,
+                                         self.options['KparDiff'])
+            return (e, st) if (withTrajectory or withJacobian) else e
         xt, e = evol.shoot(self.x0_dev, control['at'], self.options['KparDiff'],
                            regweight=self.options['regWeight'], energy=True, icost=self.icost)
         st = State()
@@ -601,9 +604,16 @@ class SurfaceMatching:
         if isinstance(Jt, torch.Tensor):
             Jt = Jt.cpu().numpy()
         if velocity is None:
+            # v_t = K(x_t, x_t) a_t, or K(x_t, c_t) a_t with control points (the
+            # reference's self.v, base/surfaceMatching.py:527-545)
             K, at = self.options['KparDiff'], B.to_dev(self.control['at'])
             xd = B.to_dev(xt)
-            velocity = torch.stack([K.applyK(xd[t], at[t]) * self.ds for t in range(self.Tsize)]).cpu().numpy()
+            if self.unreduced:
+                ct = B.to_dev(self.control['ct'])
+                vel = [K.applyK(ct[t], at[t], firstVar=xd[t]) for t in range(self.Tsize)]
+            else:
+                vel = [K.applyK(xd[t], at[t]) for t in range(self.Tsize)]
+            velocity = (torch.stack(vel) * self.ds).cpu().numpy()
         orientation = self.fv0ori if orientation is None else orientation
         fn = [fileName + f'{kk:03d}' for kk in range(self.Tsize + 1)] if isinstance(fileName, str) else fileName
         outdir = self.options['outputDir']
@@ -643,12 +653,7 @@ class SurfaceMatching:
         the per-step surfaces (saveEvolution)."""
         if not isinstance(getattr(self, 'fvInit', None), Surface):
             raise NotImplementedError("file output (writeOutput) is built for surface matching only")
-        if self.unreduced:
-            xt, _ = self._unreduced_forward(self.control, energy=False)
-            st = State()
-            st['xt'] = xt
-        else:
-            _, st = self.objectiveFunDef(self.control, withTrajectory=True, withJacobian=True)
+        _, st = self.objectiveFunDef(self.control, withTrajectory=True, withJacobian=True)
         self.state['xt'] = st['xt']
         if 'Jt' in st:
             self.state['Jt'] = st['Jt']

@@ -62,13 +62,20 @@ def test_unreduced_objective_and_gradients(env, variant):
     gc, ga = O.matching_gradient_unreduced(v, f, ct, at, (v2, f), [2.5], [1.0], **kw)
     assert rel(g['at'].cpu(), ga) < 1e-10
     assert rel(g['ct'].cpu(), gc) < 1e-10
+    # Jacobian of the semi-reduced flow: the divergence at the control points
+    # (base/pointEvolutionSemiReduced.py:78-81)
+    _, st = sm.objectiveFunDef(sm.control, withJacobian=True)
+    Jr = O.jacobian(ct, at, [2.5], kernel=kw.get('kernel_diff', O.GAUSS), kaff=kw.get('kaff'))
+    assert st['Jt'].shape == (T + 1, N, 1)
+    assert rel(st['Jt'].cpu(), Jr) < 1e-10
 
 
-def test_unreduced_registration_decreases(env):
+def test_unreduced_registration_decreases(env, tmp_path):
     S, Kernel = env["Surface"], env["Kernel"]
     (v, f), (v2, _) = env["synth"].plain_matching_pair(seed=1, subdiv=3, radius=10.0, nbumps=20, width=3.0)
     opts = {'KparDiff': Kernel(name='gauss', sigma=2.5), 'KparDist': Kernel(name='gauss', sigma=1.0),
-            'errorType': 'varifold', 'pk_dtype': 'float64', 'maxIter': 8, 'mode': 'quiet', 'unreduced': True}
+            'errorType': 'varifold', 'pk_dtype': 'float64', 'maxIter': 8, 'mode': 'quiet', 'unreduced': True,
+            'writeOutput': True, 'saveRate': 4, 'outputDir': str(tmp_path)}
     sm = env["SurfaceMatching"](Template=S((f, v)), Target=S((f, v2)), options=opts)
     sm.optimizeMatching()
     h = sm.bfgs_history
@@ -76,3 +83,16 @@ def test_unreduced_registration_decreases(env):
     assert all(h[i + 1] <= h[i] for i in range(len(h) - 1))
     assert h[-1] < 0.8 * h[0]
     assert float((sm.control['ct'] - sm.x0_dev).abs().max()) > 0      # control points moved
+    # the evolution files of the final control: exp(J_t) from the divergence at
+    # c_t, velocity K(x_t, c_t) a_t
+    ct, at = sm.control['ct'].cpu().numpy(), sm.control['at'].cpu().numpy()
+    Jr = O.jacobian(ct, at, [2.5])
+    xr = O.semiReducedEvolution(v, ct, at, [2.5], 0.0, O.GAUSS, None)
+    nv = v.shape[0]
+    for k in (0, 7):
+        lines = open(tmp_path / ("evolution%03d.vtk" % k)).read().splitlines()
+        i = lines.index("SCALARS Jacobian float 1") + 2
+        assert rel([float(t) for t in lines[i:i + nv]], np.exp(Jr[k, :, 0])) < 1e-12
+        i = lines.index("VECTORS velocity float") + 1
+        vel = [[float(u) for u in t.split()] for t in lines[i:i + nv]]
+        assert rel(vel, O.applyK(xr[k], ct[k], at[k], [2.5])) < 1e-11
