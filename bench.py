@@ -426,7 +426,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the kernel-sum path shards by target rows with no "
-              "communication (see DESIGN.md); multi-GPU runs are not part of this build.",
+              "communication; multi-GPU runs are not built.",
               file=sys.stderr)
         sys.exit(2)
 

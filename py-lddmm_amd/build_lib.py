@@ -21,7 +21,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     "-mcode-object-version=5",
-    # packed FP32 VALU ops stay disabled (precision notes)
+    # packed FP32 VALU ops stay disabled: some of their forms have given
+    # wrong results on this hardware without any error
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",

@@ -11,7 +11,7 @@ affine part (Kernel(affine='affine'|'euclidean')) is applied in first-order
 shooting and its adjoint.  The log-Jacobian along the trajectory
 (withJacobian) is tracked on the device by the kernel divergence.  Affine
 controls, normals, passenger point sets and splines are not built
-(NotImplementedError), as in the configurations this build targets.
+(NotImplementedError); the registration drivers here do not use them.
 """
 import ctypes
 
