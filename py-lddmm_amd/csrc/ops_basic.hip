@@ -75,33 +75,35 @@ int launch_center(const double* y, int n, const double* x, int m, int dim, doubl
     return LDDMM_OK;
 }
 
-PairPlan plan_pair(int n, int m, int R, int num_cu, int occ) {
+PairPlan plan_pair(int n, int m, int R, int num_cu, int occ, int quantum) {
     PairPlan p;
     p.bx = (n + kBlock * R - 1) / (kBlock * R);
     if (m <= 0 || n <= 0) { p.nchunk = 0; p.chunk = kTile; return p; }
-    const long cap = std::max(1L, long(num_cu) * std::max(1, occ));
-    const int maxS = std::max(1, (m + kTile - 1) / kTile);
+    occ = std::max(1, occ);
+    num_cu = std::max(1, num_cu);
+    quantum = std::max(1, quantum);
+    // Time model (VALU-bound pair kernels): each CU runs its blocks `occ` at a
+    // time, so its time is its block count times one block's work (chunk plus a
+    // fixed cost of about half a tile for the target loads and partial
+    // stores); a round with fewer than min(4, occ) resident blocks per CU costs
+    // as much as one with that many (too few waves to hide latency).  The
+    // busiest CU sets the launch time, so blocks are spread evenly rather
+    // than filling num_cu * occ slots with some CUs one block ahead.
+    const long minres = std::min(4, occ);
+    const long min_chunk = std::max(quantum, 64);
+    const long smax = std::max(1L, (long(m) + min_chunk - 1) / min_chunk);
+    const long p0 = (long(p.bx) + num_cu - 1) / num_cu;
     double best = 1e300;
-    for (int S = 1; S <= maxS; ++S) {
-        int chunk = ((m + S - 1) / S + kTile - 1) / kTile * kTile;
-        int se = (m + chunk - 1) / chunk;
-        if (se != S) continue;
-        long blocks = long(p.bx) * se;
-        long rounds = (blocks + cap - 1) / cap;
-        // per-block fixed cost (target loads, partial stores) ~ half a tile
-        double cost = double(rounds) * double(chunk + kTile / 2) + 0.25 * se;
-        if (cost < best) { best = cost; p.nchunk = se; p.chunk = chunk; }
-    }
-    // small problems: when whole-tile chunks leave workgroup slots empty, also
-    // try sub-tile chunks (the kernel's last partial tile carries padding)
-    if (long(p.bx) * p.nchunk < cap) {
-        for (int chunk : {kTile / 2, kTile / 4}) {
-            int se = (m + chunk - 1) / chunk;
-            long blocks = long(p.bx) * se;
-            long rounds = (blocks + cap - 1) / cap;
-            double cost = double(rounds) * double(chunk + kTile / 2) + 0.25 * se;
-            if (cost < best) { best = cost; p.nchunk = se; p.chunk = chunk; }
-        }
+    // for each block count P on the busiest CU, the most chunks that keep it
+    for (long P = p0; P <= p0 + 3L * occ + 4; ++P) {
+        const long S = std::max(1L, std::min(smax, P * num_cu / p.bx));
+        const long chunk = ((m + S - 1) / S + quantum - 1) / quantum * quantum;
+        const long se = (m + chunk - 1) / chunk;
+        const long per_cu = (long(p.bx) * se + num_cu - 1) / num_cu;
+        const long full = per_cu / occ, rest = per_cu % occ;
+        const double t = double(full * occ + (rest ? std::max(rest, minres) : 0));
+        const double cost = t * double(chunk + kTile / 2) + 0.25 * double(se);
+        if (cost < best) { best = cost; p.nchunk = int(se); p.chunk = int(chunk); }
     }
     return p;
 }

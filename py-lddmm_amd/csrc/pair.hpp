@@ -262,16 +262,20 @@ int launch_center(const double* y, int n, const double* x, int m, int dim, doubl
 int launch_fill(double* p, size_t count, double v, hipStream_t st);
 
 // ---------------------------------------------------------------------------
-// Launch planning: pick the number of source chunks so that the grid fills
-// the GPU in whole rounds (blocks run in rounds of num_cu * occupancy).
+// Launch planning: pick the number of source chunks so that the busiest CU
+// carries as little work as possible (blocks spread evenly over the CUs,
+// enough of them resident to hide latency).
 // ---------------------------------------------------------------------------
 struct PairPlan {
     int bx = 0;       // target blocks
     int nchunk = 0;   // source chunks (grid.y)
-    int chunk = 0;    // sources per chunk (a multiple of kTile, or kTile/2 or kTile/4 for small problems)
+    int chunk = 0;    // sources per chunk: a multiple of the quantum, about 64 or more unless m is smaller
 };
 
-PairPlan plan_pair(int n, int m, int R, int num_cu, int occ_blocks);
+// quantum: chunk granularity in sources.  4 for the VALU kernels (a partial
+// tile runs only its used count, rounded up to 4); a multiple of 64 for
+// kernels that always process whole tiles.
+PairPlan plan_pair(int n, int m, int R, int num_cu, int occ_blocks, int quantum = 4);
 
 template <typename T, class OP, int R>
 int run_pair(const typename OP::template Args<T>& args, int n, int m, const PairPlan& plan,

@@ -122,7 +122,7 @@ int plan_mfma(int n, int m, PairPlan* plan) {
     LDDMM_TRY(device_info(&di));
     int occ = blocks_per_cu(reinterpret_cast<const void*>(&k_pair_mfma<OP, G>), kBlock, 0);
     // plan_pair's target block is kBlock * R lanes; here it is 64 * G targets
-    *plan = plan_pair(n, m, (64 * G) / kBlock > 0 ? (64 * G) / kBlock : 1, di.num_cu, occ);
+    *plan = plan_pair(n, m, (64 * G) / kBlock > 0 ? (64 * G) / kBlock : 1, di.num_cu, occ, 64);
     plan->bx = (n + 64 * G - 1) / (64 * G);
     return LDDMM_OK;
 }
