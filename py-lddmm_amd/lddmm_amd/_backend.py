@@ -184,13 +184,26 @@ def check(rc, what=""):
         raise LddmmError(f"lddmm_amd {what}: {kind}: {msg}")
 
 
+_gpu_checked = False
+
+
 def device():
-    if not torch.cuda.is_available():
-        raise LddmmError("lddmm_amd needs a HIP GPU (torch.cuda.is_available() is False)")
+    global _gpu_checked
+    if not _gpu_checked:
+        if not torch.cuda.is_available():
+            raise LddmmError("lddmm_amd needs a HIP GPU (torch.cuda.is_available() is False)")
+        _gpu_checked = True
     return torch.device("cuda", torch.cuda.current_device())
 
 
+# torch's current stream of a device as a raw hipStream_t, without building a
+# Stream object per call (the registration drivers make thousands of calls)
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def stream_handle():
+    if _raw_stream is not None:
+        return ctypes.c_void_p(_raw_stream(torch.cuda.current_device()))
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
@@ -198,12 +211,19 @@ def ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
 
 
+_NP_DTYPE = {torch.float64: np.float64, torch.float32: np.float32, torch.int32: np.int32, torch.int64: np.int64}
+
+
 def to_dev(a, dtype=torch.float64):
     """Contiguous device tensor of `dtype` (numpy arrays are copied host->device)."""
-    dev = device()
     if isinstance(a, torch.Tensor):
-        return a.to(device=dev, dtype=dtype).contiguous()
-    arr = np.ascontiguousarray(np.asarray(a), dtype=torch.empty((), dtype=dtype).numpy().dtype)
+        if (a.is_cuda and a.dtype == dtype and a.is_contiguous()
+                and a.device.index == torch.cuda.current_device()):
+            return a                     # what .to(...).contiguous() would return
+        return a.to(device=device(), dtype=dtype).contiguous()
+    dev = device()
+    npt = _NP_DTYPE.get(dtype) or torch.empty((), dtype=dtype).numpy().dtype
+    arr = np.ascontiguousarray(np.asarray(a), dtype=npt)
     return torch.from_numpy(arr).to(dev)
 
 

@@ -62,7 +62,7 @@ class KernelSpec:
             self.par = [sigma]
             self.kff = True
         elif name == 'laplacian':
-            # accepted for API compatibility; its sums raise NotImplementedError
+            # orders above 4 fall back to 3, as in the reference
             if self.order > 4:
                 self.order = 3
             self.par = [sigma, self.order]
@@ -259,7 +259,8 @@ class Kernel(KernelSpec):
         alpha * mean|p| < 1e-10, when the quadratic energy a.Ka - 2 xi.a stalls
         (relative change < 1e-20), or after iterMax products.  The products run
         on the device; the reference's per-iteration residual (an extra product
-        used only for logging) is not computed."""
+        used only for logging) is not computed.  The CG scalars stay on the
+        device; the two stop quantities come back in one transfer per product."""
         tt = isinstance(x, torch.Tensor) or isinstance(xi, torch.Tensor)
         from . import _backend as B
         x_ = B.to_dev(x)
@@ -271,21 +272,22 @@ class Kernel(KernelSpec):
         p = None
         mu = None
         for it in range(iterMax):
-            mu_new = float((r * r).sum())
+            mu_new = (r * r).sum()
             if it == 0:
-                if mu_new < 1e-10:
+                if float(mu_new) < 1e-10:
                     break
                 p = r.clone()
             else:
                 p = r + (mu_new / mu) * p
             mu = mu_new
             q = self.applyK(x_, p)
-            alpha = mu / float((p * q).sum())
+            alpha = mu / (p * q).sum()
             a = a + alpha * p
             r = r - alpha * q
             z = z + alpha * q
-            ener = float((z * a).sum())
-            if alpha * float(p.abs().mean()) < 1e-10:
+            chk = torch.stack([alpha * p.abs().mean(), (z * a).sum()]).cpu()
+            ener = float(chk[1])
+            if float(chk[0]) < 1e-10:
                 break
             if abs(ener - ener_old) < 1e-20 * abs(ener_old):
                 break

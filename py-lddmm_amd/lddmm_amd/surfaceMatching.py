@@ -326,8 +326,10 @@ class SurfaceMatching:
         if self.obj is None:
             self.obj0 = self.fun_obj0(self.fv1) / self.options['sigmaError'] ** 2
             obj, self.state = self.objectiveFunDef(self.control, withTrajectory=True)
-            total = obj + self.dataTerm_device(self.state['xt'][-1])
-            self.obj = self.obj0 + float(total.item())
+            xt = self.state['xt']
+            terms = self._host_terms(obj, self.dataTerm_device(xt[-1]))
+            self.obj = self.obj0 + float(terms[0] + terms[1])
+            self._data_cur = (xt, xt._version, self.obj0 + float(terms[1]))
         return self.obj
 
     def getVariable(self):
@@ -340,20 +342,30 @@ class SurfaceMatching:
             if dr[k] is not None:
                 controlTry[k] = self.control[k] - eps * dr[k]
         obj_, stateTry = self.objectiveFunDef(controlTry, withTrajectory=True)
-        objTry = self.obj0 + float((obj_ + self.dataTerm_device(stateTry['xt'][-1])).item())
+        xt = stateTry['xt']
+        terms = self._host_terms(obj_, self.dataTerm_device(xt[-1]))
+        objTry = self.obj0 + float(terms[0] + terms[1])
         if np.isnan(objTry):
             return 1e500
         if objRef is None or objTry < objRef:
             self.controlTry = controlTry
             self.stateTry = stateTry
             self.objTry = objTry
+            self._data_try = (xt, xt._version, self.obj0 + float(terms[1]))
         self._last_try = (dr, eps, controlTry, stateTry)
         return objTry
+
+    @staticmethod
+    def _host_terms(obj, data):
+        """(deformation term, data term) as one float64 host tensor: a single
+        device-to-host transfer for both."""
+        return torch.stack([obj.reshape(()), data.reshape(())]).cpu()
 
     def acceptVarTry(self):
         self.obj = self.objTry
         self.control = self.controlTry
         self.state = self.stateTry
+        self._data_cur = getattr(self, '_data_try', None)
 
     # ------------------------------------------------------------ gradient
     def getGradient(self, coeff=1.0, update=None):
@@ -739,5 +751,11 @@ class SurfaceMatching:
         return self.control['at'].cpu().numpy()
 
     def varifold_data(self):
-        """Current data term |def - tgt|^2 / sigmaError^2 (host float)."""
-        return self.obj0 + float(self.dataTerm_device(self.state['xt'][-1]).item())
+        """Current data term |def - tgt|^2 / sigmaError^2 (host float); the value
+        the last objective evaluation of the current trajectory computed is
+        reused while that trajectory is unchanged."""
+        xt = self.state['xt']
+        c = getattr(self, '_data_cur', None)
+        if c is not None and c[0] is xt and c[1] == xt._version:
+            return c[2]
+        return self.obj0 + float(self.dataTerm_device(xt[-1]).item())

@@ -225,3 +225,23 @@ def test_kernel_solve_matches_dense_solve(ku):
     assert rel(G @ a, xi) < 1e-6
     assert rel(a, expect) < 1e-4
     assert np.all(K.solve(x, np.zeros_like(xi), None) == 0)
+
+
+def test_stream_handle_is_torch_current_stream(ku):
+    """The C calls are issued on torch's current stream, also inside a side stream."""
+    import torch
+    from lddmm_amd import _backend as B
+    assert (B.stream_handle().value or 0) == torch.cuda.current_stream().cuda_stream
+    s = torch.cuda.Stream()
+    rng = np.random.default_rng(8)
+    y, a = rng.normal(size=(500, 3)), rng.normal(size=(500, 3))
+    yd, ad = torch.tensor(y, device="cuda"), torch.tensor(a, device="cuda")
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        assert (B.stream_handle().value or 0) == s.cuda_stream
+        out = ku.applyK(yd, yd, ad, 'gauss', [1.5], 10)
+    torch.cuda.current_stream().wait_stream(s)
+    assert rel(out.cpu().numpy(), O.applyK(y, y, a, [1.5])) < 1e-12
+    t = torch.tensor(y, device="cuda")
+    assert B.to_dev(t) is t                                   # already device float64 contiguous
+    assert B.to_dev(t.T).is_contiguous() and B.to_dev(t.float()).dtype == torch.float64
