@@ -389,7 +389,9 @@ int lddmm_nn_to_bf16(const float* w, uint16_t* wbf, int64_t count, void* stream)
  * smallest |t| over both directions, lowest face index on ties; hit[i] = 1,
  * or t[i] = 0 and hit[i] = 0 when the line meets no triangle.  With unit
  * template vertex normals as directions, t is the signed normal distance
- * from the template to the target surface. */
+ * from the template to the target surface.  Meshes of more than 256 faces
+ * are split into face chunks whose per-ray hits sit in the library's
+ * per-device workspace (calls sharing a GPU go on one stream). */
 int lddmm_normal_raycast(const double* orig, const double* dir, int64_t n, const double* tv, int64_t nv,
                          const int32_t* tf, int64_t nf, double* t_out, int32_t* hit, void* stream);
 
