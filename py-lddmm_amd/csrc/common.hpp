@@ -35,6 +35,11 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
 #define LDDMM_TRY(expr)                                                        \
     do { int rc_ = (expr); if (rc_ != LDDMM_OK) return rc_; } while (0)
 
+// Largest point count of a pair sum: the kernels index targets and sources in
+// int, and a target block's first index blockIdx.x * (256 lanes * 2 targets)
+// must stay below 2^31 for the last block too.
+constexpr int64_t kMaxPoints = (int64_t(1) << 31) - 4096;
+
 // ---------------------------------------------------------------------------
 // Device facts and workspace
 // ---------------------------------------------------------------------------

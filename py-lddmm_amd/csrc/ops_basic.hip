@@ -77,7 +77,9 @@ int launch_center(const double* y, int n, const double* x, int m, int dim, doubl
 
 PairPlan plan_pair(int n, int m, int R, int num_cu, int occ, int quantum) {
     PairPlan p;
-    p.bx = (n + kBlock * R - 1) / (kBlock * R);
+    // n <= kMaxPoints (checked by every entry point), so the block count and
+    // the kernels' first-target index blockIdx.x * kBlock * R stay in int
+    p.bx = int((int64_t(n) + kBlock * R - 1) / (kBlock * R));
     if (m <= 0 || n <= 0) { p.nchunk = 0; p.chunk = kTile; return p; }
     occ = std::max(1, occ);
     num_cu = std::max(1, num_cu);
@@ -116,7 +118,7 @@ namespace {
 int check_common(const lddmm_kernel* kd, int64_t n, int64_t m, int dim) {
     LDDMM_TRY(validate_kernel(kd));
     LDDMM_REQUIRE(n >= 0 && m >= 0, "negative sizes");
-    LDDMM_REQUIRE(n < (int64_t(1) << 31) && m < (int64_t(1) << 31), "sizes beyond int32");
+    LDDMM_REQUIRE(n <= kMaxPoints && m <= kMaxPoints, "sizes above %lld points", (long long)kMaxPoints);
     LDDMM_REQUIRE(dim >= 1 && dim <= 3, "dim %d outside [1, 3]", dim);
     return LDDMM_OK;
 }

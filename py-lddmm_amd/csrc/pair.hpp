@@ -298,9 +298,10 @@ int plan_for(int n, int m, PairPlan* plan) {
     return LDDMM_OK;
 }
 
+// Bytes carve_partials takes (each slice rounded up to Carve's 256 bytes).
 template <typename T>
 size_t partial_bytes(const PairPlan& p, int nch, int n) {
-    size_t e = size_t(p.nchunk) * nch * size_t(n) * sizeof(T);
+    size_t e = (size_t(p.nchunk) * nch * size_t(n) * sizeof(T) + 255) & ~size_t(255);
     return kCompensated<T> ? 2 * e : e;
 }
 

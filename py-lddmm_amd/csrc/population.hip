@@ -1103,7 +1103,7 @@ extern "C" int lddmm_population_create(const lddmm_population_desc* d, lddmm_pop
     }
     LDDMM_REQUIRE(d->voff[0] == 0 && d->foff[0] == 0 && d->tfoff[0] == 0, "population_create: offsets start at 0");
     const int64_t V = d->voff[P], F = d->foff[P], TF = d->tfoff[P];
-    LDDMM_REQUIRE(V < (int64_t(1) << 31) && F + TF < (int64_t(1) << 30), "population_create: sizes beyond int32");
+    LDDMM_REQUIRE(V <= kMaxPoints && F + TF < (int64_t(1) << 30), "population_create: sizes beyond int32");
     LDDMM_REQUIRE(F == 0 || d->faces, "population_create: null faces");
     LDDMM_REQUIRE(V == 0 || (d->adj_ptr && d->adj_idx), "population_create: adjacency required");
     LDDMM_REQUIRE(TF == 0 || (d->tvertices && d->tfaces), "population_create: null target arrays");

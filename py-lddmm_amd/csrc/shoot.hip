@@ -228,7 +228,7 @@ int axpby(size_t count, double alpha, const double* a, double beta, const double
 int check_shoot(const lddmm_kernel* kd, int T, int64_t n, int dim, unsigned flags) {
     LDDMM_TRY(validate_kernel(kd, flags));
     LDDMM_REQUIRE(T >= 1, "T must be >= 1 (got %d)", T);
-    LDDMM_REQUIRE(n >= 0 && n < (int64_t(1) << 31), "bad n");
+    LDDMM_REQUIRE(n >= 0 && n <= kMaxPoints, "bad n");
     LDDMM_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3 (got %d)", dim);
     return LDDMM_OK;
 }
