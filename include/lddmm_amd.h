@@ -127,6 +127,28 @@ int lddmm_applyK(const lddmm_kernel* kd,
                  const double* y, int64_t n, const double* x, int64_t m, int32_t dim,
                  const double* a, int32_t k, double* out, void* stream);
 
+/* lddmm_applyK without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and result as lddmm_applyK; the centre and the chunk partial sums
+ * live in one file-scope device array of LDDMM_NOALLOC_SCRATCH_BYTES per GPU
+ * (held until the process ends) instead of the grow-only workspace, and the
+ * call makes no hipMalloc / hipFree / allocating library call (host-side
+ * caches of device facts and launch shapes are filled on a device's first
+ * call).  Every launch writes all the scratch it reads, so nothing carries
+ * over between calls.  Sizes: n * min(k, 4) * 8 bytes of partials per source
+ * chunk must fit the array (n up to about 2.8M targets for k = 3); a plan
+ * with more chunks than fit is given fewer, larger chunks (the same sums in
+ * another summation order); larger n is rejected with LDDMM_EINVAL before
+ * any launch.  All calls on one GPU share its array: calls that overlap on
+ * one GPU (different streams or threads) race and must be serialised by the
+ * caller.  Always uses the VALU engine; not instrumented by
+ * lddmm_profile_enable. */
+#define LDDMM_NOALLOC_SCRATCH_BYTES (64u << 20)
+int lddmm_applyK_noalloc(const lddmm_kernel* kd,
+                         const double* y, int64_t n, const double* x, int64_t m, int32_t dim,
+                         const double* a, int32_t k, double* out, void* stream);
+
 /* out[n,dim] = sum_s s^(KP-1)/W sum_j -(y_i-x_j)/s K_s(y_i,x_j) c_ij with
  * c_ij = p_i.a_j (+ a_i.p_j for SYM, + a_i.p_j - 2 regweight a_i.a_j for LDDMM).
  * p is [n,k], a is [m,k]; SYM and LDDMM require n == m.

@@ -69,6 +69,14 @@ bool profiling();
 int profile_begin(hipStream_t st);
 int profile_end(hipStream_t st);
 
+// While an instance lives, profiling() is false on the calling thread (the
+// hooks create HIP events, which entry points that must not allocate skip).
+struct ProfileOff {
+    ProfileOff();
+    ~ProfileOff();
+    bool prev;
+};
+
 // Occupancy (blocks per CU) of a kernel, cached by function address.
 int blocks_per_cu(const void* func, int block, size_t dyn_lds);
 

@@ -755,10 +755,10 @@ struct Job {
 };
 
 template <typename T, class OP>
-int plan_job(Job<T, OP>& j, int n, int m, bool allow_r2) {
+int plan_job(Job<T, OP>& j, int n, int m, bool allow_r2, int max_nchunk = 1 << 30) {
     j.R = allow_r2 ? pick_R(n) : 1;
-    if (j.R == 2) return plan_for<T, OP, 2>(n, m, &j.plan);
-    return plan_for<T, OP, 1>(n, m, &j.plan);
+    if (j.R == 2) return plan_for<T, OP, 2>(n, m, &j.plan, max_nchunk);
+    return plan_for<T, OP, 1>(n, m, &j.plan, max_nchunk);
 }
 
 template <typename T, class OP>

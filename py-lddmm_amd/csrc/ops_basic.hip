@@ -4,6 +4,7 @@
 #include "ops_mfma.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 namespace lddmm {
@@ -75,7 +76,7 @@ int launch_center(const double* y, int n, const double* x, int m, int dim, doubl
     return LDDMM_OK;
 }
 
-PairPlan plan_pair(int n, int m, int R, int num_cu, int occ, int quantum) {
+PairPlan plan_pair(int n, int m, int R, int num_cu, int occ, int quantum, int max_nchunk) {
     PairPlan p;
     // n <= kMaxPoints (checked by every entry point), so the block count and
     // the kernels' first-target index blockIdx.x * kBlock * R stay in int
@@ -84,6 +85,7 @@ PairPlan plan_pair(int n, int m, int R, int num_cu, int occ, int quantum) {
     occ = std::max(1, occ);
     num_cu = std::max(1, num_cu);
     quantum = std::max(1, quantum);
+    max_nchunk = std::max(1, max_nchunk);
     // Time model (VALU-bound pair kernels): each CU runs its blocks `occ` at a
     // time, so its time is its block count times one block's work (chunk plus a
     // fixed cost of about half a tile for the target loads and partial
@@ -98,7 +100,7 @@ PairPlan plan_pair(int n, int m, int R, int num_cu, int occ, int quantum) {
     double best = 1e300;
     // for each block count P on the busiest CU, the most chunks that keep it
     for (long P = p0; P <= p0 + 3L * occ + 4; ++P) {
-        const long S = std::max(1L, std::min(smax, P * num_cu / p.bx));
+        const long S = std::max(1L, std::min(std::min(smax, long(max_nchunk)), P * num_cu / p.bx));
         const long chunk = ((m + S - 1) / S + quantum - 1) / quantum * quantum;
         const long se = (m + chunk - 1) / chunk;
         const long per_cu = (long(p.bx) * se + num_cu - 1) / num_cu;
@@ -345,6 +347,89 @@ int ddiffK_t(const lddmm_kernel* kd, const double* x, int n, int dim, const doub
 }
 
 // ---------------------------------------------------------------------------
+// applyK with no allocation: the centre and the chunk partials live in a
+// fixed device array, `scratch`, instead of the grow-only workspace.  Host
+// code never passes `scratch` itself to a kernel (in host code the name is a
+// stand-in, not a device address): each device's address of the array is
+// looked up once with hipGetSymbolAddress and kept in g_scratch_dev.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(LDDMM_NOALLOC_SCRATCH_BYTES);
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+char* g_scratch_dev[kMaxDevices] = {};
+
+// This device's address of `scratch` (looked up on the first call per device).
+int scratch_address(char** out) {
+    int dev = 0;
+    LDDMM_HIP(hipGetDevice(&dev));
+    LDDMM_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d the library tracks", dev, kMaxDevices);
+    std::lock_guard<std::mutex> g(g_scratch_mutex);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        LDDMM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        LDDMM_REQUIRE(p != nullptr, "no device address for the static scratch array");
+        g_scratch_dev[dev] = static_cast<char*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return LDDMM_OK;
+}
+
+// Same launches as applyK_t, with each weight group's partials carved from
+// `base` (cap bytes).  A group whose plan would not fit gets fewer, larger
+// source chunks (same sums, another summation order); one that does not fit
+// with a single chunk is rejected before anything is launched.
+template <typename T, bool MS>
+int applyK_fixed(const lddmm_kernel* kd, const double* y, int n, const double* x, int m, int dim,
+                 const double* a, int k, double* out, char* base, size_t cap, hipStream_t st) {
+    SingleScaleConsts ss;
+    ScaleSet<T> sc = make_scales<T>(kd, &ss);
+    Carve cv;
+    double* c0 = reinterpret_cast<double*>(base + cv.take(32));
+    const size_t off0 = cv.off;
+    const size_t room = cap - off0;
+    const int ng = (k + 3) / 4;
+    // every group is planned and checked before the first launch
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) LDDMM_TRY(launch_center(y, n, x, m, dim, c0, st));
+        for (int g = 0; g < ng; ++g) {
+            const int kw = std::min(4, k - 4 * g);
+            int rc = with_kw(kw, [&](auto KWc) {
+                constexpr int KW = decltype(KWc)::value;
+                Job<T, OpApplyK<KW, MS>> j;
+                LDDMM_TRY(plan_job(j, n, m, !MS));
+                if (partial_bytes<T>(j.plan, KW, n) > room) {
+                    PairPlan one = j.plan;
+                    one.nchunk = 1;
+                    const size_t per = partial_bytes<T>(one, KW, n);
+                    if (per > room) {
+                        set_error("applyK_noalloc: %lld targets x %d weight columns need %zu bytes of partials, "
+                                  "the static scratch holds %zu (n * min(k, 4) * 8 <= %zu)",
+                                  (long long)n, KW, per + off0, cap, room);
+                        return LDDMM_EINVAL;
+                    }
+                    LDDMM_TRY(plan_job(j, n, m, !MS, int(std::min<size_t>(room / per, 1 << 30))));
+                }
+                if (pass == 0) return LDDMM_OK;
+                j.args = {y, x, a, c0, dim, k, 4 * g, sc};
+                Carve c2;
+                c2.off = off0;
+                Partials<T> part = carve_partials<T>(base, c2, j.plan, KW, n);
+                LDDMM_TRY(run_job(j, n, m, part, st));
+                ChanScale cs{};
+                for (int c = 0; c < 8; ++c) cs.s[c] = 1.0;
+                return run_reduce(part, j.plan.nchunk, KW, KW, n, cs, out, k, 4 * g, 0, nullptr, 1.0, st);
+            });
+            if (rc) return rc;
+        }
+    }
+    return LDDMM_OK;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // Dense-weight sums (memory-bound on beta): one wave64 per target row, lanes
 // stride over sources, float64 arithmetic, wave reduction at the end.
 // ---------------------------------------------------------------------------
@@ -415,6 +500,27 @@ extern "C" int lddmm_applyK(const lddmm_kernel* kd, const double* y, int64_t n, 
                   : applyK_t<float, false>(kd, y, n, x, m, dim, a, k, out, st);
     return ms ? applyK_t<double, true>(kd, y, n, x, m, dim, a, k, out, st)
               : applyK_t<double, false>(kd, y, n, x, m, dim, a, k, out, st);
+}
+
+extern "C" int lddmm_applyK_noalloc(const lddmm_kernel* kd, const double* y, int64_t n, const double* x,
+                                    int64_t m, int32_t dim, const double* a, int32_t k, double* out,
+                                    void* stream) {
+    LDDMM_TRY(check_common(kd, n, m, dim));
+    LDDMM_REQUIRE(k >= 1, "k must be >= 1");
+    if (n == 0) return LDDMM_OK;
+    LDDMM_REQUIRE(y && out, "null y/out");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (m == 0) return launch_fill(out, size_t(n) * k, 0.0, st);
+    LDDMM_REQUIRE(x && a, "null x/a");
+    char* base = nullptr;
+    LDDMM_TRY(scratch_address(&base));
+    ProfileOff no_events;            // the timing hooks would create HIP events
+    const bool ms = general_path(kd);
+    if (kd->prec == LDDMM_PREC_F32C)
+        return ms ? applyK_fixed<float, true>(kd, y, n, x, m, dim, a, k, out, base, kNoAllocScratchBytes, st)
+                  : applyK_fixed<float, false>(kd, y, n, x, m, dim, a, k, out, base, kNoAllocScratchBytes, st);
+    return ms ? applyK_fixed<double, true>(kd, y, n, x, m, dim, a, k, out, base, kNoAllocScratchBytes, st)
+              : applyK_fixed<double, false>(kd, y, n, x, m, dim, a, k, out, base, kNoAllocScratchBytes, st);
 }
 
 extern "C" int lddmm_applyDiffKT(const lddmm_kernel* kd, const double* y, int64_t n, const double* x,

@@ -274,8 +274,10 @@ struct PairPlan {
 
 // quantum: chunk granularity in sources.  4 for the VALU kernels (a partial
 // tile runs only its used count, rounded up to 4); a multiple of 64 for
-// kernels that always process whole tiles.
-PairPlan plan_pair(int n, int m, int R, int num_cu, int occ_blocks, int quantum = 4);
+// kernels that always process whole tiles.  max_nchunk bounds the number of
+// source chunks (callers whose partial buffer has a fixed size).
+PairPlan plan_pair(int n, int m, int R, int num_cu, int occ_blocks, int quantum = 4,
+                   int max_nchunk = 1 << 30);
 
 template <typename T, class OP, int R>
 int run_pair(const typename OP::template Args<T>& args, int n, int m, const PairPlan& plan,
@@ -290,11 +292,11 @@ int run_pair(const typename OP::template Args<T>& args, int n, int m, const Pair
 }
 
 template <typename T, class OP, int R>
-int plan_for(int n, int m, PairPlan* plan) {
+int plan_for(int n, int m, PairPlan* plan, int max_nchunk = 1 << 30) {
     DeviceInfo di;
     LDDMM_TRY(device_info(&di));
     int occ = blocks_per_cu(reinterpret_cast<const void*>(&k_pair<T, OP, R>), kBlock, 0);
-    *plan = plan_pair(n, m, R, di.num_cu, occ);
+    *plan = plan_pair(n, m, R, di.num_cu, occ, 4, max_nchunk);
     return LDDMM_OK;
 }
 

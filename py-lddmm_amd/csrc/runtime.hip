@@ -127,7 +127,12 @@ hipEvent_t take_event() {
 }
 }  // namespace
 
-bool profiling() { return g_prof.on; }
+thread_local bool t_prof_off = false;
+
+bool profiling() { return g_prof.on && !t_prof_off; }
+
+ProfileOff::ProfileOff() : prev(t_prof_off) { t_prof_off = true; }
+ProfileOff::~ProfileOff() { t_prof_off = prev; }
 
 int profile_begin(hipStream_t st) {
     std::lock_guard<std::mutex> g(g_prof_mutex);

@@ -99,6 +99,7 @@ _ICP = ctypes.POINTER(LddmmInternalCost)
 # name -> argtypes (restype is int for every compute entry point)
 SIGNATURES = {
     "lddmm_applyK": [_KP, _P, _I64, _P, _I64, _I32, _P, _I32, _P, _P],
+    "lddmm_applyK_noalloc": [_KP, _P, _I64, _P, _I64, _I32, _P, _I32, _P, _P],
     "lddmm_applyDiffKT": [_KP, _P, _I64, _P, _I64, _I32, _P, _P, _I32, _I32, _D, _P, _P],
     "lddmm_applyDiv": [_KP, _P, _I64, _P, _I64, _I32, _P, _P, _P],
     "lddmm_applyktensor": [_KP, _P, _I64, _P, _I64, _I32, _P, _P, _P, _P, _I32, _P, _P],
