@@ -1,253 +1,298 @@
-"""L-BFGS driver (py-lddmm base/bfgs.py:55-312).
+"""Limited-memory BFGS driver for the matching objects.
 
-Same protocol as the reference: the optimisable object provides
-getVariable, objectiveFun, updateTry, acceptVarTry, getGradient and
-optionally startOfIteration / endOfIteration / endOfProcedure,
-dotProduct_euclidean, addProd / prod / copyDir, gradEps, gradCoeff, epsMax,
-burnIn, reset, twoLoop (the two-loop recursion computed by the object, e.g.
-with its scalars kept on the device).  Two-loop recursion with memory `memory`, t_init = 1/|q| on
-the first (and reset) iterations, weak-Wolfe search (c1 = 1e-4, c2 = 0.9,
-at most 10 gradients), stop on |obj - obj_old| < 1e-7 after burn-in or on
-|g| < gradEps.  Directions may hold numpy arrays or device tensors; dot
-products are reduced to Python floats.
+The protocol of the optimisable object is the reference's (described at
+base/bfgs.py:9-35): required getVariable, objectiveFun, updateTry,
+acceptVarTry, getGradient; optional startOptim, startOfIteration,
+endOfIteration, endOfProcedure, endOptim, stopCondition, dotProduct_euclidean
+(or dotProduct), addProd, prod, copyDir, randomDir and the attributes
+gradEps (or options['gradEps']), gradCoeff, epsMax, burnIn, reset.  Two
+hooks have no counterpart there: twoLoop(grad, pairs), which lets the object
+compute the quasi-Newton direction itself (e.g. with its scalars kept on the
+device), and stopNow(), an early stop checked after each counted iteration.
+
+The loop is organised as an explicit state (`_Run`) plus one decision
+function (`decide`) applied after every line search.  The decision rules are
+the reference's (base/bfgs.py:243-300):
+
+* a line search that fails right after a restart ends the run, not converged;
+* any other failed line search restarts: the memory is dropped, the objective
+  recomputed and the iteration repeated from the steepest-descent direction
+  with the first step 1/|q|;
+* |f - f_old| < 1e-7 ends the run as converged after the burn-in iterations
+  (or right after a restart) and restarts before them;
+* |g| < gradEps ends the run as converged;
+* otherwise the step is accepted and the iteration counted.
+
+The gradient threshold is fixed at iteration 0 and again at the end of the
+burn-in (base/bfgs.py:218-223); the first step of the weak-Wolfe search
+(c1 = 1e-4, c2 = 0.9, at most 10 gradients) is 1/|q| at iteration 0 and after
+a restart, 1 otherwise (:210-213); the quasi-Newton direction is the
+two-loop product with memory `memory` (:181-207).  Directions may hold numpy
+arrays or device tensors; dot products come back as Python floats.
 """
+import enum
 import logging
 import time
-from copy import deepcopy
 
 import numpy as np
 
 from .linesearch import line_search_weak_wolfe
 
-
-def _dot_default(x, y):
-    return [float((x * yy).sum()) for yy in y]
-
-
-def _addProd(x, y, a):
-    return x + a * y
+STALL_TOL = 1e-7          # |f - f_old| below which the run has stalled
+WOLFE_C1, WOLFE_C2 = 1e-4, 0.9
+WOLFE_MAX_GRADIENTS = 10
 
 
-def _prod(x, a):
-    return a * x
+class Outcome(enum.Enum):
+    ACCEPT = "accept"                  # step taken, iteration counted
+    RESTART = "restart"                # memory dropped, iteration repeated
+    STALLED = "small variation"        # converged: objective no longer moves
+    SMALL_GRADIENT = "small gradient"  # converged: |g| below the threshold
+    STUCK = "stuck"                    # no step even from steepest descent
 
 
-def _copyDir(x):
-    return x.clone() if hasattr(x, "clone") else np.copy(x)
+def decide(step_found, after_restart, it, burn_in, f, f_old, grad_norm, grad_eps, may_stop):
+    """End-of-iteration decision.  `may_stop` is the object's stopCondition;
+    it is consulted only for a stopping rule that holds, the stall rule first."""
+    if not step_found and after_restart:
+        return Outcome.STUCK
+    restart = not step_found
+    if f_old is not None and abs(f - f_old) < STALL_TOL and may_stop():
+        if it > burn_in or after_restart:
+            return Outcome.STALLED
+        restart = True
+    if grad_norm < grad_eps and may_stop():
+        return Outcome.SMALL_GRADIENT
+    return Outcome.RESTART if restart else Outcome.ACCEPT
+
+
+def gradient_threshold(current, grad_norm):
+    """The |g| stopping threshold: 1e-3 |g| (at least 1e-4) when none is
+    given, else the smaller of the given one and 1e-4 |g|."""
+    if current is None:
+        return max(0.001 * grad_norm, 0.0001)
+    return min(current, 0.0001 * grad_norm)
+
+
+class _VectorOps:
+    """Direction arithmetic, from the object's hooks or numpy defaults."""
+
+    def __init__(self, opt):
+        if hasattr(opt, 'dotProduct_euclidean'):
+            self._dots = opt.dotProduct_euclidean
+        elif hasattr(opt, 'dotProduct'):
+            self._dots = opt.dotProduct
+        else:
+            self._dots = lambda u, vs: [float((u * v).sum()) for v in vs]
+        self.axpy = getattr(opt, 'addProd', lambda u, v, c: u + c * v)        # u + c v
+        self.scale = getattr(opt, 'prod', lambda u, c: c * u)
+        self.copy = getattr(opt, 'copyDir', lambda u: u.clone() if hasattr(u, 'clone') else np.copy(u))
+
+    def dot(self, u, v):
+        return self._dots(u, [v])[0]
+
+
+class _Memory:
+    """The last `size` (s, y) pairs, oldest first, as [s, y] lists (twoLoop
+    hooks may append their own cached data to an entry)."""
+
+    def __init__(self, size):
+        self.size = size
+        self.pairs = []
+
+    def push(self, s, y):
+        self.pairs.append([s, y])
+        del self.pairs[:max(0, len(self.pairs) - self.size)]
+
+    def clear(self):
+        self.pairs = []
+
+    def __len__(self):
+        return len(self.pairs)
+
+
+def quasi_newton_direction(g, pairs, ops):
+    """H g for the inverse-Hessian estimate of the stored pairs (two-loop
+    product, Nocedal & Wright alg. 7.4), scaled by gamma = s.y / y.y of the
+    newest pair (1 when that ratio is below 1e-10)."""
+    q = ops.copy(g)
+    down = []
+    for s, y in reversed([p[:2] for p in pairs]):
+        rho = 1 / ops.dot(y, s)
+        a = rho * ops.dot(s, q)
+        q = ops.axpy(q, y, -a)
+        down.append((rho, a))
+    s, y = pairs[-1][:2]
+    gamma = ops.dot(s, y) / ops.dot(y, y)
+    q = ops.scale(q, 1 if gamma < 1e-10 else gamma)
+    for (s, y), (rho, a) in zip([p[:2] for p in pairs], reversed(down)):
+        b = rho * ops.dot(y, q)
+        q = ops.axpy(q, s, a - b)
+    return q
+
+
+def _finish(opt):
+    if hasattr(opt, 'endOfProcedure'):
+        opt.endOfProcedure()
+    elif hasattr(opt, 'endOfIteration'):
+        opt.endOfIteration()
+
+
+def _log_iteration(it, f, step, grad_norm, times=None):
+    text = f'iteration {it + 1:d}: obj = {f:.5f}, eps = ' + ('None' if step is None else f'{step:.5f}')
+    text += f', gradient: {grad_norm:.5f}'
+    if times is not None:
+        text += f', time = {times[0]:.04f}, {times[1]:.04f}'
+    logging.info(text)
+
+
+def _fd_gradient_check(opt, ops, grad, f, coeff):
+    """Central difference of the objective along a random direction next to
+    the gradient's prediction (the reference's TestGradient option)."""
+    d = opt.randomDir() if hasattr(opt, 'randomDir') else np.random.normal(size=np.shape(grad))
+    h = 1e-8
+    fp = opt.updateTry(d, h, f - 1e8)
+    fm = opt.updateTry(d, -h, f - 1e8)
+    logging.info('Test Gradient: %.4f %.4f' % ((fp - fm) / (2 * h), -ops.dot(grad, d) * coeff))
+
+
+class _Run:
+    """State of one optimisation."""
+
+    def __init__(self, opt, memory):
+        self.it = 0
+        self.f = None              # objective at the current variable
+        self.f_old = None          # objective before the last accepted step
+        self.step_taken = None     # s of the last accepted step (-t q)
+        self.grad_old = None
+        self.grad_next = None      # gradient the line search left at the accepted point
+        self.memory = _Memory(memory)
+        self.grad_eps = None
+        self.history = []
+        self.decisions = []        # (iteration, Outcome) after every line search
 
 
 def bfgs(opt, verb=True, maxIter=1000, TestGradient=False, epsInit=0.01, memory=25, Wolfe=True,
          lineSearch='Weak_Wolfe'):
-    for name in ('getVariable', 'objectiveFun', 'updateTry', 'acceptVarTry', 'getGradient'):
-        if not hasattr(opt, name):
-            logging.error('Error: required functions are not provided')
-            return
-    if lineSearch != "Weak_Wolfe":
+    """Minimise opt's objective; returns opt.getVariable().  epsInit and
+    Wolfe are accepted for the reference's signature: with the weak-Wolfe
+    search the first trial step is set as described in the module docstring."""
+    required = ('getVariable', 'objectiveFun', 'updateTry', 'acceptVarTry', 'getGradient')
+    if not all(hasattr(opt, name) for name in required):
+        logging.error('Error: required functions are not provided')
+        return None
+    if lineSearch != 'Weak_Wolfe':
         logging.warning(f'line search {lineSearch!r} is not built: using weak wolfe condition')
-    line_search = line_search_weak_wolfe
-
-    if hasattr(opt, 'dotProduct_euclidean'):
-        dotProduct = opt.dotProduct_euclidean
-    elif hasattr(opt, 'dotProduct'):
-        dotProduct = opt.dotProduct
-    else:
-        dotProduct = _dot_default
-    addProd = getattr(opt, 'addProd', _addProd)
-    prod = getattr(opt, 'prod', _prod)
-    copyDir = getattr(opt, 'copyDir', _copyDir)
-    stopCondition = getattr(opt, 'stopCondition', lambda: True)
-
+    ops = _VectorOps(opt)
+    may_stop = getattr(opt, 'stopCondition', lambda: True)
+    coeff = getattr(opt, 'gradCoeff', 1.0)
+    burn_in = getattr(opt, 'burnIn', 0)
     if hasattr(opt, 'startOptim'):
         opt.startOptim()
 
+    run = _Run(opt, memory)
     if hasattr(opt, 'gradEps'):
-        gradEps = opt.gradEps
-    elif hasattr(opt, 'options') and 'gradEps' in opt.options.keys():
-        gradEps = opt.options['gradEps']
-    else:
-        gradEps = None
-    gradCoeff = getattr(opt, 'gradCoeff', 1.0)
-    epsMax = getattr(opt, 'epsMax', 1.)
-    burnIn = getattr(opt, 'burnIn', 0)
-    eps = epsInit
+        run.grad_eps = opt.gradEps
+    elif hasattr(opt, 'options') and 'gradEps' in opt.options:
+        run.grad_eps = opt.options['gradEps']
     opt.converged = False
-
-    if hasattr(opt, 'reset') and opt.reset:
+    if getattr(opt, 'reset', False):
         opt.obj = None
-
-    obj = opt.objectiveFun()
+    run.f = opt.objectiveFun()
     opt.reset = False
-    logging.info('iteration 0: obj = {0: .5f}'.format(obj))
+    logging.info('iteration 0: obj = {0: .5f}'.format(run.f))
+    run.history.append(run.f)
+    clock = time.process_time()
 
-    storedGrad = []
-    it = 0
-    diffVar = None
-    grdOld = None
-    obj_old = None
-    gval = None
-    opt.reset = False
-    itt0 = time.process_time()
-    history = [obj]
-    while it < maxIter:
-        t0 = time.process_time()
+    while run.it < maxIter:
+        t_start = time.process_time()
         if hasattr(opt, 'startOfIteration'):
             opt.startOfIteration()
-        if opt.reset:
+        restarted = opt.reset
+        if restarted:
             opt.obj = None
-            obj = opt.objectiveFun()
+            run.f = opt.objectiveFun()
             if verb:
-                logging.info(f"recomputed objective {obj:.5f}")
-            obj_old = None
-            gval = None
+                logging.info(f"recomputed objective {run.f:.5f}")
+            run.f_old = None
+            run.grad_next = None
 
-        if gval is None:
-            grd = opt.getGradient(gradCoeff)
-        else:
-            grd = deepcopy(gval)
-
+        grad = opt.getGradient(coeff) if run.grad_next is None else ops.copy(run.grad_next)
         if TestGradient:
-            dirfoo = opt.randomDir()
-            epsfoo = 1e-8
-            objfoo1 = opt.updateTry(dirfoo, epsfoo, obj - 1e8)
-            objfoo2 = opt.updateTry(dirfoo, -epsfoo, obj - 1e8)
-            [grdfoo] = dotProduct(grd, [dirfoo])
-            logging.info('Test Gradient: %.4f %.4f' % ((objfoo1 - objfoo2) / (2 * epsfoo), -grdfoo * gradCoeff))
+            _fd_gradient_check(opt, ops, grad, run.f, coeff)
 
-        if (not opt.reset) and it > 0:
-            storedGrad.append([diffVar, addProd(grd, grdOld, -1)])
-            if len(storedGrad) > memory:
-                storedGrad.pop(0)
+        # direction: quasi-Newton from the stored pairs, steepest descent on
+        # the first iteration and after a restart
+        if restarted or run.it == 0:
+            run.memory.clear()
+            q = ops.copy(grad)
+        else:
+            run.memory.push(run.step_taken, ops.axpy(grad, run.grad_old, -1))
             if hasattr(opt, 'twoLoop'):
-                # same recursion with the scalars kept where the vectors live
-                q = opt.twoLoop(grd, storedGrad)
+                q = opt.twoLoop(grad, run.memory.pairs)
             else:
-                q = copyDir(grd)
-                rho = []
-                alpha = []
-                for m in reversed(storedGrad):
-                    rho.append(1 / dotProduct(m[1], [m[0]])[0])
-                    alpha.append(rho[-1] * dotProduct(m[0], [q])[0])
-                    q = addProd(q, m[1], -alpha[-1])
-                rho.reverse()
-                alpha.reverse()
-                m = storedGrad[-1]
-                c = dotProduct(m[0], [m[1]])[0] / dotProduct(m[1], [m[1]])[0]
-                if c < 1e-10:
-                    c = 1
-                q = prod(q, c)
-                for k, m in enumerate(storedGrad):
-                    beta = rho[k] * dotProduct(m[1], [q])[0]
-                    q = addProd(q, m[0], alpha[k] - beta)
-        else:
-            storedGrad = []
-            q = copyDir(grd)
+                q = quasi_newton_direction(grad, run.memory.pairs, ops)
+        grad_norm = np.sqrt(ops.dot(grad, grad))
+        q_norm = np.sqrt(np.maximum(1e-20, ops.dot(q, q)))
+        direction = ops.copy(q)
+        run.grad_old = ops.copy(grad)
+        if run.it == 0 or run.it == burn_in:
+            run.grad_eps = gradient_threshold(run.grad_eps, grad_norm)
+            logging.info(f'Gradient threshold: {run.grad_eps:.6f}')
 
-        grd2 = dotProduct(grd, [grd])[0]
-        grdTry = np.sqrt(np.maximum(1e-20, dotProduct(q, [q])[0]))
-        dir0 = copyDir(q)
-        if it == 0 or opt.reset:
-            t_init = 1. / grdTry
-        else:
-            t_init = 1.
-
-        grdOld = copyDir(grd)
-
-        if it == 0 or it == burnIn:
-            if gradEps is None:
-                gradEps = max(0.001 * np.sqrt(grd2), 0.0001)
-            else:
-                gradEps = min(gradEps, 0.0001 * np.sqrt(grd2))
-            logging.info(f'Gradient threshold: {gradEps:.6f}')
-
-        if it < burnIn:
-            Wolfe = False
-            eps = epsInit
-        else:
-            if Wolfe:
-                eps = 1.
-            else:
-                epsBig = epsMax / grdTry
-                if eps > epsBig:
-                    eps = epsBig
-
-        eps, fc, gc, phi_star, old_fval, gval = line_search(opt, dir0, gfk=grd, old_fval=obj,
-                                                            old_old_fval=obj_old, c1=1e-4, c2=0.9,
-                                                            amax=None, t_init=t_init, maxiter=10)
-        if eps is not None:
-            diffVar = prod(dir0, -eps)
-            obj_old = obj
+        step, _, _, f_new, _, run.grad_next = line_search_weak_wolfe(
+            opt, direction, gfk=grad, old_fval=run.f, old_old_fval=run.f_old, c1=WOLFE_C1, c2=WOLFE_C2,
+            amax=None, t_init=(1. / q_norm) if (run.it == 0 or restarted) else 1., maxiter=WOLFE_MAX_GRADIENTS)
+        if step is not None:
+            run.step_taken = ops.scale(direction, -step)
+            run.f_old = run.f
             opt.acceptVarTry()
-            obj = phi_star
-            newreset = False
+            run.f = f_new
         else:
             logging.info('Wolfe search unsuccessful')
-            if opt.reset:
-                logging.info('Cannot go any further')
-                opt.converged = False
-                if hasattr(opt, 'endOfProcedure'):
-                    opt.endOfProcedure()
-                elif hasattr(opt, 'endOfIteration'):
-                    opt.endOfIteration()
-                break
-            newreset = True
-            gval = None
+            run.grad_next = None
 
-        if (obj_old is not None and np.fabs(obj - obj_old) < 1e-7) and stopCondition():
-            logging.info(f'iteration {it + 1:d}: obj = {obj:.5f}, eps = {eps:.5f}, gradient: {np.sqrt(grd2):.5f}')
-            if it > burnIn or opt.reset:
-                logging.info('Stopping Gradient Descent: small variation')
-                opt.converged = True
-                history.append(obj)
-                if hasattr(opt, 'endOfProcedure'):
-                    opt.endOfProcedure()
-                elif hasattr(opt, 'endOfIteration'):
-                    opt.endOfIteration()
-                break
-            else:
-                newreset = True
-
-        tt0 = time.process_time()
-        t1 = tt0 - t0
-        tt1 = tt0 - itt0
-        itt0 = tt0
-        if verb | (it == maxIter):
-            if eps is None:
-                logging.info(f'iteration {it + 1:d}: obj = {obj:.5f}, eps = None, gradient: {np.sqrt(grd2):.5f}, '
-                             f'time = {t1:.04f}, {tt1:.04f}')
-            else:
-                logging.info(f'iteration {it + 1:d}: obj = {obj:.5f}, eps = {eps:.5f}, gradient: '
-                             f'{np.sqrt(grd2):.5f}, time = {t1:.04f}, {tt1:.04f}')
-
-        if np.sqrt(grd2) < gradEps and stopCondition():
+        outcome = decide(step is not None, restarted, run.it, burn_in, run.f, run.f_old, grad_norm,
+                         run.grad_eps, may_stop)
+        run.decisions.append((run.it, outcome))
+        if outcome is Outcome.STUCK:
+            logging.info('Cannot go any further')
+            opt.converged = False
+            _finish(opt)
+            break
+        if outcome is Outcome.STALLED:
+            _log_iteration(run.it, run.f, step, grad_norm)
+            logging.info('Stopping Gradient Descent: small variation')
+            opt.converged = True
+            run.history.append(run.f)
+            _finish(opt)
+            break
+        now = time.process_time()
+        if verb:
+            _log_iteration(run.it, run.f, step, grad_norm, (now - t_start, now - clock))
+        clock = now
+        if outcome is Outcome.SMALL_GRADIENT:
             logging.info('Stopping Gradient Descent: small gradient')
             opt.converged = True
-            history.append(obj)
-            if hasattr(opt, 'endOfProcedure'):
-                opt.endOfProcedure()
-            elif hasattr(opt, 'endOfIteration'):
-                opt.endOfIteration()
+            run.history.append(run.f)
+            _finish(opt)
             break
-        if eps is not None:
-            eps = np.minimum(100 * eps, epsMax)
-        else:
-            eps = epsMax
 
-        opt.reset = newreset
+        opt.reset = outcome is Outcome.RESTART
         if hasattr(opt, 'endOfIteration'):
             opt.endOfIteration()
         if not opt.reset:
-            it += 1
-            history.append(obj)
-        # optional early stop requested by the optimisable object (no
-        # counterpart in the reference loop; absent -> identical behaviour)
+            run.it += 1
+            run.history.append(run.f)
         if hasattr(opt, 'stopNow') and opt.stopNow():
             opt.converged = True
             break
 
-    if it == maxIter and hasattr(opt, 'endOfProcedure'):
+    if run.it == maxIter and hasattr(opt, 'endOfProcedure'):
         opt.endOfProcedure()
     if hasattr(opt, 'endOptim'):
         opt.endOptim()
-    opt.bfgs_iterations = it
-    opt.bfgs_history = history
+    opt.bfgs_iterations = run.it
+    opt.bfgs_history = run.history
+    opt.bfgs_decisions = run.decisions
     return opt.getVariable()

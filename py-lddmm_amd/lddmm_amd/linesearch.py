@@ -1,87 +1,100 @@
-"""Weak-Wolfe line search (py-lddmm base/linesearch.py:66-163).
+"""Weak-Wolfe line search along a descent direction.
 
-The optimisable object provides updateTry(dir, t) (objective at control -
-t*dir, stored as the trial variable) and getGradient(coeff, update=[dir, t])
-(gradient at the trial control); directions are "descent directions to be
-subtracted", so phi'(t) = -<grad(t), dir>.  Steps, tests and the quadratic
-interpolation follow the reference line by line.
+The step rules are the reference's (base/linesearch.py:66-163), which the
+iterates of the matching drivers depend on: a trial step t is accepted when
+it gives sufficient decrease (Armijo, c1) and a slope no steeper than c2
+times the initial one; below the sufficient-decrease bound it becomes the
+lower end of a bracket, above it the upper end.  With no upper end yet the
+step doubles; with one, the next trial is the bracket midpoint after a
+sufficient-decrease trial, else the minimiser of the quadratic through
+f(0), f'(0), f(t), clipped to [lo + (hi - lo)/10, (lo + hi)/2].  After
+`maxiter` slope evaluations it falls back to halving until sufficient
+decrease; 100 trials without success return no step.
+
+The optimisable object provides updateTry(dir, t) (objective at the control
+minus t*dir) and getGradient(coeff, update=[dir, t]) (gradient there);
+directions are subtracted, so the slope along the step is -<grad, dir>.
+Decrease tests use gradCoeff-scaled slopes, as the reference does.
 """
 import logging
+
+MAX_TRIALS = 100
+
+
+class _Probe:
+    """Objective and slope of t -> f(x - t d), counting evaluations."""
+
+    def __init__(self, opt, direction, euclidean):
+        self.opt = opt
+        self.direction = direction
+        self.coeff = getattr(opt, "gradCoeff", 1)
+        self._dots = opt.dotProduct_euclidean if euclidean else opt.dotProduct
+        self.n_values = 0
+        self.n_slopes = 0
+        self.last_grad = None
+
+    def slope_of(self, grad):
+        return -self._dots(grad, [self.direction])[0]
+
+    def value(self, t):
+        self.n_values += 1
+        return self.opt.updateTry(self.direction, t)
+
+    def slope(self, t):
+        self.n_slopes += 1
+        self.last_grad = self.opt.getGradient(self.coeff, update=[self.direction, t])
+        return self.slope_of(self.last_grad)
+
+
+def _next_trial(t, f_t, f0, slope0, lo, hi, decreased):
+    if hi < 1e-16:                      # no upper end yet
+        return 2 * t
+    if decreased:
+        return 0.5 * hi + 0.5 * lo
+    t_quad = -(t ** 2 * slope0) / (2 * (f_t - f0 - t * slope0))
+    return max(0.9 * lo + 0.1 * hi, min(0.5 * hi + 0.5 * lo, t_quad))
 
 
 def line_search_weak_wolfe(opt, pk, gfk=None, old_fval=None, old_old_fval=None, c1=0.001, c2=0.9,
                            amax=None, maxiter=100, euclidean=True, t_init=1.):
-    fc = [0]
-    gc = [0]
-    gval = [None]
-
-    def dot(g, d):
-        if euclidean:
-            return opt.dotProduct_euclidean(g, [d])[0]
-        return opt.dotProduct(g, [d])[0]
-
-    def phi(alpha):
-        fc[0] += 1
-        return opt.updateTry(pk, alpha)
-
-    def derphi(alpha):
-        gc[0] += 1
-        gval[0] = opt.getGradient(opt.gradCoeff, update=[pk, alpha])
-        return -dot(gval[0], pk)
-
+    """Returns (t or None, objective evaluations, slope evaluations, f(t),
+    f(0), gradient at t or None) for the direction pk (amax and old_old_fval
+    are part of the reference's signature and unused)."""
+    probe = _Probe(opt, pk, euclidean)
     if gfk is None:
-        gfk = opt.getGradient(opt.gradCoeff)
-    derphi0 = -dot(gfk, pk)
+        gfk = opt.getGradient(probe.coeff)
+    slope0 = probe.slope_of(gfk)
+    f0 = probe.value(0) if old_fval is None else old_fval
+    armijo = lambda t, f_t: f_t < f0 + c1 * t * slope0 * probe.coeff
 
-    it = 0
-    itGrad = 0
-    amax = 0
-    amin = 0
-    if old_fval is None:
-        old_fval = phi(0)
-
+    lo = hi = 0
+    trials = 0
     t = t_init
-    gradCoeff = getattr(opt, "gradCoeff", 1)
-    maxiter2 = 100
-    fval = phi(t)
-    df = None
-    while it < maxiter2 and itGrad < maxiter:
-        armijo = False
-        if fval < old_fval + c1 * t * derphi0 * gradCoeff:
-            df = derphi(t)
-            armijo = True
-            itGrad += 1
-            if df > c2 * derphi0 * gradCoeff:
-                break
-            else:
-                amin = t
+    f_t = probe.value(t)
+    while trials < MAX_TRIALS and probe.n_slopes < maxiter:
+        decreased = armijo(t, f_t)
+        if decreased:
+            if probe.slope(t) > c2 * slope0 * probe.coeff:
+                break                   # both conditions hold
+            lo = t
         else:
-            amax = t
-        if amax < 1e-16:
-            t *= 2
-        else:
-            if armijo:
-                t = 0.5 * amax + 0.5 * amin
-            else:
-                tt = -(t ** 2 * derphi0) / (2 * (fval - old_fval - t * derphi0))
-                t = max(0.9 * amin + 0.1 * amax, min(0.5 * amax + 0.5 * amin, tt))
-        fval = phi(t)
-        it += 1
+            hi = t
+        t = _next_trial(t, f_t, f0, slope0, lo, hi, decreased)
+        f_t = probe.value(t)
+        trials += 1
 
-    if itGrad == maxiter:
+    if probe.n_slopes == maxiter:
         logging.warning('Weak Wolfe condition: maximum gradient computations attained: switching to backtracking')
-        gval[0] = None
-        while it < maxiter2:
-            if fval < old_fval + c1 * t * derphi0 * gradCoeff:
-                df = derphi(t)
+        probe.last_grad = None
+        while trials < MAX_TRIALS:
+            if armijo(t, f_t):
+                probe.slope(t)
                 break
-            else:
-                t /= 2
-            fval = phi(t)
-            it += 1
+            t /= 2
+            f_t = probe.value(t)
+            trials += 1
 
-    if it == maxiter2:
-        logging.warning(f'Weak Wolfe condition: maximum number of iterations attained {fval:.5f}--{old_fval:.5f}')
+    if trials == MAX_TRIALS:
+        logging.warning(f'Weak Wolfe condition: maximum number of iterations attained {f_t:.5f}--{f0:.5f}')
         t = None
-
-    return t, fc[0], gc[0], fval, old_fval, gval[0]
+    return t, probe.n_values, probe.n_slopes, f_t, f0, probe.last_grad
