@@ -156,27 +156,35 @@ def applyK(y, x, a, scale, KP=-1, kernel=GAUSS):
     return res / W
 
 
-def applyDiffKT(y, x, p, a, scale, regweight=1., sym=False, lddmm=False, KP=-1, kernel=GAUSS):
+def applyDiffKT(y, x, p, a, scale, regweight=1., sym=False, lddmm=False, KP=-1, kernel=GAUSS, rows=None):
     """out[i] = sum_s s^(KP-1)/W sum_j -(y_i - x_j)/s * K_s(y_i, x_j) * c_ij.
 
     c_ij = p_i.a_j (plain), + a_i.p_j (sym), + a_i.p_j - 2 r a_i.a_j (lddmm).
     Follows base/kernelFunctions_util.py:773-793 (applyDiffKT_numba) with
     gauss_fun_diffT at :232-236.  sym / lddmm need y and x of the same size.
+    rows (index array) returns only those target rows (y_i, p_i and, for
+    sym / lddmm, a_i taken at the rows; every source j kept): out[rows].
     """
     y, x, p, a = _f64(y), _f64(x), _as2d(p), _as2d(a)
+    if rows is None:
+        yt, pt, at = y, p, a
+    else:
+        rows = np.asarray(rows)
+        yt, pt = y[rows], p[rows]
+        at = a[rows] if (sym or lddmm) else None
     s, sKP, W = _scales(scale, KP)
     sKP1 = s ** (KP - 1)
-    res = np.zeros(y.shape)
+    res = np.zeros(yt.shape)
     for si, sc in enumerate(s):
-        ys, xs = y / sc, x / sc
-        for i0, i1 in _row_blocks(y.shape[0], x.shape[0]):
+        ys, xs = yt / sc, x / sc
+        for i0, i1 in _row_blocks(yt.shape[0], x.shape[0]):
             d = ys[i0:i1, None, :] - xs[None, :, :]
             K = _profile(d, kernel, 1)
-            c = p[i0:i1] @ a.T
+            c = pt[i0:i1] @ a.T
             if lddmm:
-                c = c + a[i0:i1] @ p.T - 2 * regweight * (a[i0:i1] @ a.T)
+                c = c + at[i0:i1] @ p.T - 2 * regweight * (at[i0:i1] @ a.T)
             elif sym:
-                c = c + a[i0:i1] @ p.T
+                c = c + at[i0:i1] @ p.T
             res[i0:i1] += -(d * (K * c)[..., None]).sum(1) * sKP1[si]
     return res / W
 
@@ -472,11 +480,13 @@ def _scatter_faces(nv, faces, per_face_vertex):
     return px
 
 
-def varifoldNormGradient(fvDef, fv1, sigma, KP=-1):
+def varifoldNormGradient(fvDef, fv1, sigma, KP=-1, vertex_rows=None):
     """Gradient of |fvDef - fv1|^2 (varifold, f0(u) = 1 + u^2) wrt fvDef vertices.
 
     Follows base/surfaceDistances.py:403-468 (varifoldNormGradient_numpy),
-    with the dense beta/u temporaries evaluated per row block.
+    with the dense beta/u temporaries evaluated per row block.  vertex_rows
+    (index array) returns only those vertices' rows, computing the face sums
+    of their incident faces only.
     """
     xDef, f, c1, S1, fw1 = _surf(fvDef)
     _, _, c2, S2, fw2 = _surf(fv1)
@@ -484,26 +494,34 @@ def varifoldNormGradient(fvDef, fv1, sigma, KP=-1):
     _, n2, a2w = _varifold_features(S2, fw2)
     s, sKP, W = _scales(sigma, KP)
     sKP1 = s ** (KP - 1)
-    F = c1.shape[0]
+    if vertex_rows is None:
+        sel = np.arange(c1.shape[0])
+    else:
+        vertex_rows = np.asarray(vertex_rows)
+        sel = np.nonzero(np.isin(f, vertex_rows).any(axis=1))[0]
+    F = sel.size
     z1 = np.zeros((F, 3))
     dz1 = np.zeros((F, 3))
+    ct, nt, at = c1[sel], n1[sel], a1w[sel]
     for (cx, nx, ax, sign) in ((c1, n1, a1w, 1.), (c2, n2, a2w, -1.)):
         for i0, i1 in _row_blocks(F, cx.shape[0]):
-            cc = n1[i0:i1] @ nx.T                                 # cr1cr_
-            beta = a1w[i0:i1, None] * ax[None, :] * (1 + cc ** 2)
-            u = (2 * cc[..., None] * (nx[None, :, :] - cc[..., None] * n1[i0:i1, None, :])
-                 + (1 + cc ** 2)[..., None] * n1[i0:i1, None, :]) * ax[None, :, None]
+            cc = nt[i0:i1] @ nx.T                                 # cr1cr_
+            beta = at[i0:i1, None] * ax[None, :] * (1 + cc ** 2)
+            u = (2 * cc[..., None] * (nx[None, :, :] - cc[..., None] * nt[i0:i1, None, :])
+                 + (1 + cc ** 2)[..., None] * nt[i0:i1, None, :]) * ax[None, :, None]
             for si, sc in enumerate(s):
-                d = c1[i0:i1, None, :] / sc - cx[None, :, :] / sc
+                d = ct[i0:i1, None, :] / sc - cx[None, :, :] / sc
                 K = np.exp(-(d ** 2).sum(-1) / 2)
                 z1[i0:i1] += sign * (K[..., None] * u).sum(1) * sKP[si] / W
                 dz1[i0:i1] -= sign * (d * (K * beta)[..., None]).sum(1) * sKP1[si] / W
-    z1 *= fw1[:, None]
+    z1 *= fw1[sel, None]
     dz1 *= 2. / 3.
-    x1, x2, x3 = xDef[f[:, 0]], xDef[f[:, 1]], xDef[f[:, 2]]
-    return _scatter_faces(xDef.shape[0], f, (dz1 - np.cross(x3 - x2, z1),
-                                            dz1 - np.cross(x1 - x3, z1),
-                                            dz1 - np.cross(x2 - x1, z1)))
+    fs = f[sel]
+    x1, x2, x3 = xDef[fs[:, 0]], xDef[fs[:, 1]], xDef[fs[:, 2]]
+    px = _scatter_faces(xDef.shape[0], fs, (dz1 - np.cross(x3 - x2, z1),
+                                           dz1 - np.cross(x1 - x3, z1),
+                                           dz1 - np.cross(x2 - x1, z1)))
+    return px if vertex_rows is None else px[vertex_rows]
 
 
 def currentNorm0(fv1, sigma, KP=-1):

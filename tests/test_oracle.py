@@ -281,6 +281,23 @@ def test_varifold_gradient_fd(spheres, sig):
     np.testing.assert_allclose((g * dv).sum(), fd, rtol=1e-6)
 
 
+def test_varifold_gradient_vertex_rows_equal_full(spheres):
+    A, Bs = spheres
+    rows = np.array([0, 5, 17, 40, 41, A[0].shape[0] - 1])
+    full = O.varifoldNormGradient(A, Bs, [0.7])
+    np.testing.assert_array_equal(O.varifoldNormGradient(A, Bs, [0.7], vertex_rows=rows), full[rows])
+
+
+@pytest.mark.parametrize("mode", ["plain", "sym", "lddmm"])
+def test_diffKT_target_rows_equal_full(mode):
+    x, p, a = rnd(40, 3), rnd(40, 3), rnd(40, 3)
+    kw = dict(sym=mode == 'sym', lddmm=mode == 'lddmm', regweight=0.7)
+    rows = np.array([3, 0, 39, 11])
+    full = O.applyDiffKT(x, x, p, a, [1.1, 2.0], **kw)
+    np.testing.assert_allclose(O.applyDiffKT(x, x, p, a, [1.1, 2.0], rows=rows, **kw), full[rows],
+                               rtol=1e-14, atol=1e-15)
+
+
 def test_current_orientation_and_gradient(spheres):
     A, Bs = spheres
     v, f = A
