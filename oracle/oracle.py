@@ -480,13 +480,24 @@ def _scatter_faces(nv, faces, per_face_vertex):
     return px
 
 
-def varifoldNormGradient(fvDef, fv1, sigma, KP=-1, vertex_rows=None):
+def _vertex_sums(nv, faces, per_face):
+    """sum over the faces of each vertex of a per-face scalar."""
+    out = np.zeros(nv)
+    for j in range(faces.shape[1]):
+        np.add.at(out, faces[:, j], per_face)
+    return out
+
+
+def varifoldNormGradient(fvDef, fv1, sigma, KP=-1, vertex_rows=None, with_weights=False):
     """Gradient of |fvDef - fv1|^2 (varifold, f0(u) = 1 + u^2) wrt fvDef vertices.
 
     Follows base/surfaceDistances.py:403-468 (varifoldNormGradient_numpy),
     with the dense beta/u temporaries evaluated per row block.  vertex_rows
     (index array) returns only those vertices' rows, computing the face sums
-    of their incident faces only.
+    of their incident faces only.  with_weights also returns the weight part
+    of base/surfaceDistances.py:455-467: per vertex the sum over its faces of
+    (2/3) fw_f a_f (sum_j K f0 a1w_j - sum_j K f0 a2w_j) (the reference keeps
+    the face weight fw_f in this factor).
     """
     xDef, f, c1, S1, fw1 = _surf(fvDef)
     _, _, c2, S2, fw2 = _surf(fv1)
@@ -521,7 +532,16 @@ def varifoldNormGradient(fvDef, fv1, sigma, KP=-1, vertex_rows=None):
     px = _scatter_faces(xDef.shape[0], fs, (dz1 - np.cross(x3 - x2, z1),
                                            dz1 - np.cross(x1 - x3, z1),
                                            dz1 - np.cross(x2 - x1, z1)))
-    return px if vertex_rows is None else px[vertex_rows]
+    if vertex_rows is not None:
+        px = px[vertex_rows]
+    if not with_weights:
+        return px
+    a1 = np.sqrt((S1 ** 2).sum(axis=1) + 1e-10)
+    ones = np.ones(c1.shape[0])
+    s11 = applyktensor(c1, c1, ones, a1w, n1, n1 * np.sqrt(a1w)[:, None], sigma, KP)
+    s12 = applyktensor(c1, c2, ones, a2w, n1, n2 * np.sqrt(a2w)[:, None], sigma, KP)
+    pxw = _vertex_sums(xDef.shape[0], f, (2 / 3) * fw1 * a1 * (s11 - s12))
+    return px, (pxw if vertex_rows is None else pxw[vertex_rows])
 
 
 def currentNorm0(fv1, sigma, KP=-1):
@@ -548,8 +568,10 @@ def currentNorm(fvDef, fv1, sigma, KP=-1):
     return currentNormDef(fvDef, fv1, sigma, KP) + currentNorm0(fv1, sigma, KP)
 
 
-def currentNormGradient(fvDef, fv1, sigma, KP=-1):
-    """base/surfaceDistances.py:87-139 (currentNormGradient)."""
+def currentNormGradient(fvDef, fv1, sigma, KP=-1, with_weights=False):
+    """base/surfaceDistances.py:87-139 (currentNormGradient); with_weights also
+    returns the vertex-weight gradient (:130-137): per vertex the sum over its
+    faces of (2/3) S_f . z1_f, S the unweighted surfel."""
     xDef, f, c1, S1, w1 = _surf(fvDef)
     _, _, c2, S2, w2 = _surf(fv1)
     wcr1 = w1[:, None] * S1
@@ -559,22 +581,29 @@ def currentNormGradient(fvDef, fv1, sigma, KP=-1):
     dz1 = (2 / 3) * (applyDiffKT(c1, c1, wcr1, wcr1, sigma, KP=KP) -
                      applyDiffKT(c1, c2, wcr1, wcr2, sigma, KP=KP))
     x1, x2, x3 = xDef[f[:, 0]], xDef[f[:, 1]], xDef[f[:, 2]]
-    return _scatter_faces(xDef.shape[0], f, (dz1 - np.cross(x3 - x2, wz1),
-                                            dz1 - np.cross(x1 - x3, wz1),
-                                            dz1 - np.cross(x2 - x1, wz1)))
+    px = _scatter_faces(xDef.shape[0], f, (dz1 - np.cross(x3 - x2, wz1),
+                                          dz1 - np.cross(x1 - x3, wz1),
+                                          dz1 - np.cross(x2 - x1, wz1)))
+    if not with_weights:
+        return px
+    return px, _vertex_sums(xDef.shape[0], f, (2 / 3) * (S1 * z1).sum(axis=1))
 
 
-def currentMagnitudeGradient(fvDef, sigma, KP=-1):
-    """base/surfaceDistances.py:28-63 (currentMagnitudeGradient)."""
+def currentMagnitudeGradient(fvDef, sigma, KP=-1, with_weights=False):
+    """base/surfaceDistances.py:28-63 (currentMagnitudeGradient); with_weights
+    as currentNormGradient (:61-67)."""
     xDef, f, c1, S1, w1 = _surf(fvDef)
     wcr1 = w1[:, None] * S1
     z1 = applyK(c1, c1, wcr1, sigma, KP)
     wz1 = w1[:, None] * z1
     dz1 = (2 / 3) * applyDiffKT(c1, c1, wcr1, wcr1, sigma, KP=KP)
     x1, x2, x3 = xDef[f[:, 0]], xDef[f[:, 1]], xDef[f[:, 2]]
-    return _scatter_faces(xDef.shape[0], f, (dz1 - np.cross(x3 - x2, wz1),
-                                            dz1 - np.cross(x1 - x3, wz1),
-                                            dz1 - np.cross(x2 - x1, wz1)))
+    px = _scatter_faces(xDef.shape[0], f, (dz1 - np.cross(x3 - x2, wz1),
+                                          dz1 - np.cross(x1 - x3, wz1),
+                                          dz1 - np.cross(x2 - x1, wz1)))
+    if not with_weights:
+        return px
+    return px, _vertex_sums(xDef.shape[0], f, (2 / 3) * (S1 * z1).sum(axis=1))
 
 
 # Measures ------------------------------------------------------------------
@@ -600,8 +629,15 @@ def measureNormDef(fvDef, fv1, sigma, KP=-1):
     return (cr1 * applyK(c1, c1, cr1, sigma, KP)).sum() - 2 * (cr1 * applyK(c1, c2, cr2, sigma, KP)).sum()
 
 
-def measureNormGradient(fvDef, fv1, sigma, KP=-1):
-    """base/surfaceDistances.py:231-278 (with_weights=False)."""
+def measureNorm(fvDef, fv1, sigma, KP=-1):
+    """base/surfaceDistances.py:226-227."""
+    return measureNormDef(fvDef, fv1, sigma, KP) + measureNorm0(fv1, sigma, KP)
+
+
+def measureNormGradient(fvDef, fv1, sigma, KP=-1, with_weights=False):
+    """base/surfaceDistances.py:231-278; with_weights also returns the
+    vertex-weight gradient (:269-276): per vertex the sum over its faces of
+    (2/3) a_f z1_f."""
     xDef, f, c1, S1, w1 = _surf(fvDef)
     _, _, c2, S2, w2 = _surf(fv1)
     a1, cr1, a1w = _measure_features(S1, w1)
@@ -611,9 +647,12 @@ def measureNormGradient(fvDef, fv1, sigma, KP=-1):
     dz1 = (2 / 3) * (applyDiffKT(c1, c1, a1w[:, None], a1w[:, None], sigma, KP=KP) -
                      applyDiffKT(c1, c2, a1w[:, None], a2w[:, None], sigma, KP=KP))
     x1, x2, x3 = xDef[f[:, 0]], xDef[f[:, 1]], xDef[f[:, 2]]
-    return _scatter_faces(xDef.shape[0], f, (dz1 - np.cross(x3 - x2, wz1),
-                                            dz1 - np.cross(x1 - x3, wz1),
-                                            dz1 - np.cross(x2 - x1, wz1)))
+    px = _scatter_faces(xDef.shape[0], f, (dz1 - np.cross(x3 - x2, wz1),
+                                          dz1 - np.cross(x1 - x3, wz1),
+                                          dz1 - np.cross(x2 - x1, wz1)))
+    if not with_weights:
+        return px
+    return px, _vertex_sums(xDef.shape[0], f, (2 / 3) * a1 * z1[:, 0])
 
 
 def measureNormPS0(points, weights, sigma, KP=-1):

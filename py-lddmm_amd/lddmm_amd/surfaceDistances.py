@@ -84,12 +84,23 @@ def varifoldNorm(fvDef, fv1, KparDist, fun=None):
 
 
 def varifoldNormGradient(fvDef, fv1, KparDist, with_weights=False, fun=None, cpu=False, dtype='float64'):
-    """Gradient of |fvDef - fv1|^2 wrt fvDef's vertices (base/surfaceDistances.py:394-468)."""
+    """Gradient of |fvDef - fv1|^2 wrt fvDef's vertices (base/surfaceDistances.py:394-468).
+    with_weights also returns the vertex-weight part (:455-467): per vertex
+    the sum over its faces of (2/3) fw_f a_f sum_j K f0 (a1w_j - a2w_j terms),
+    i.e. (2/3) times the face's row of |S1|^2 minus its row of <S1, S2>
+    (applyktensor rows; the reference keeps fw_f in this factor)."""
     _check_fun(fun)
-    if with_weights:
-        raise NotImplementedError("varifold gradient wrt vertex weights is not built")
     _, g = varifold_device(_kd(KparDist), fvDef, fv1, energy=False, grad=True)
-    return g.cpu().numpy()
+    if not with_weights:
+        return g.cpu().numpy()
+    c1, n1, a1, w1, dk = _measure_data(fvDef)
+    c2, n2, a2, w2, _ = _measure_data(fv1)
+    a1w, a2w = a1 * w1, a2 * w2
+    b1, b2 = n1 * torch.sqrt(a1w)[:, None], n2 * torch.sqrt(a2w)[:, None]
+    kw = _ap(KparDist)
+    r11 = ku.applyktensor(c1, c1, a1w, a1w, b1, b1, **kw)
+    r12 = ku.applyktensor(c1, c2, a1w, a2w, b1, b2, **kw)
+    return g.cpu().numpy(), _vertex_sums(dk, (2 / 3) * (r11 - r12)).cpu().numpy()
 
 
 # ------------------------------------------------------------------ currents
@@ -112,6 +123,14 @@ def _gather(dk, dz1, z1):
     B.check(B.LIB.lddmm_surface_face_gather(ctypes.byref(d), B.ptr(dz1_), B.ptr(z1_), B.ptr(px),
                                             B.stream_handle()), "surface_face_gather")
     return px
+
+
+def _vertex_sums(dk, per_face):
+    """Per vertex, the sum over its faces of a per-face scalar (the CSR face
+    gather with a zero cross-product term: deterministic order)."""
+    dz1 = torch.zeros((per_face.shape[0], 3), dtype=torch.float64, device=per_face.device)
+    dz1[:, 0] = per_face
+    return _gather(dk, dz1, torch.zeros_like(dz1))[:, 0].contiguous()
 
 
 def _ap(KparDist):
@@ -164,9 +183,10 @@ def measureNorm(fvDef, fv1, KparDist):
     return measureNormDef(fvDef, fv1, KparDist) + measureNorm0(fv1, KparDist)
 
 
-def measureNormGradient_device(fvDef, fv1, KparDist, vertices=None):
+def measureNormGradient_device(fvDef, fv1, KparDist, vertices=None, with_weights=False):
     """z1 = K(c1,c1) a1w - K(c1,c2) a2w, w z1 n gathered with the cross-product
-    chain rule, dz1 = (2/3)(DiffKT terms) (base/surfaceDistances.py:231-271)."""
+    chain rule, dz1 = (2/3)(DiffKT terms) (base/surfaceDistances.py:231-271);
+    with_weights also the vertex sums of (2/3) a z1 (:269-276)."""
     c1, n1, a1, w1, dk = _measure_data(fvDef, vertices)
     c2, _, a2, w2, _ = _measure_data(fv1)
     a1w, a2w = (a1 * w1)[:, None], (a2 * w2)[:, None]
@@ -174,13 +194,17 @@ def measureNormGradient_device(fvDef, fv1, KparDist, vertices=None):
     z1 = ku.applyK(c1, c1, a1w, **kw) - ku.applyK(c1, c2, a2w, **kw)
     wz1 = w1[:, None] * z1 * n1
     dz1 = (2 / 3) * (ku.applyDiffKT(c1, c1, a1w, a1w, **kw) - ku.applyDiffKT(c1, c2, a1w, a2w, **kw))
-    return _gather(dk, dz1, wz1)
+    px = _gather(dk, dz1, wz1)
+    if not with_weights:
+        return px
+    return px, _vertex_sums(dk, (2 / 3) * a1 * z1[:, 0])
 
 
 def measureNormGradient(fvDef, fv1, KparDist, with_weights=False):
     """base/surfaceDistances.py:231-278"""
     if with_weights:
-        raise NotImplementedError("measure gradient wrt vertex weights is not built")
+        px, pxw = measureNormGradient_device(fvDef, fv1, KparDist, with_weights=True)
+        return px.cpu().numpy(), pxw.cpu().numpy()
     return measureNormGradient_device(fvDef, fv1, KparDist).cpu().numpy()
 
 
@@ -250,33 +274,50 @@ def currentNorm(fvDef, fv1, KparDist, weight=1.):
     return currentNormDef(fvDef, fv1, KparDist) + currentNorm0(fv1, KparDist)
 
 
-def currentNormGradient_device(fvDef, fv1, KparDist, vertices=None):
+def _surfel(dk):
+    """Unweighted surfels [F,3] of a device surface view."""
+    d, keep = dk
+    S = torch.empty((d.nf, 3), dtype=torch.float64, device=B.device())
+    vw = d.vweights
+    d.vweights = None
+    B.check(B.LIB.lddmm_surface_features(ctypes.byref(d), None, B.ptr(S), None, B.stream_handle()),
+            "surface_features")
+    d.vweights = vw
+    return S
+
+
+def currentNormGradient_device(fvDef, fv1, KparDist, vertices=None, with_weights=False):
+    """with_weights also returns the vertex sums of (2/3) S.z1, S the unweighted
+    surfel (base/surfaceDistances.py:130-137)."""
     c1, wcr1, w1, dk = _face_data(fvDef, vertices)
-    c2, wcr2, _, _ = _face_data(fv1)
     kw = _ap(KparDist)
-    z1 = ku.applyK(c1, c1, wcr1, **kw) - ku.applyK(c1, c2, wcr2, **kw)
-    wz1 = w1[:, None] * z1
-    dz1 = (2 / 3) * (ku.applyDiffKT(c1, c1, wcr1, wcr1, **kw) - ku.applyDiffKT(c1, c2, wcr1, wcr2, **kw))
-    return _gather(dk, dz1, wz1)
+    z1 = ku.applyK(c1, c1, wcr1, **kw)
+    if fv1 is not None:
+        c2, wcr2, _, _ = _face_data(fv1)
+        z1 = z1 - ku.applyK(c1, c2, wcr2, **kw)
+        dz1 = (2 / 3) * (ku.applyDiffKT(c1, c1, wcr1, wcr1, **kw) - ku.applyDiffKT(c1, c2, wcr1, wcr2, **kw))
+    else:
+        dz1 = (2 / 3) * ku.applyDiffKT(c1, c1, wcr1, wcr1, **kw)
+    px = _gather(dk, dz1, w1[:, None] * z1)
+    if not with_weights:
+        return px
+    return px, _vertex_sums(dk, (2 / 3) * (_surfel(dk) * z1).sum(1))
 
 
 def currentNormGradient(fvDef, fv1, KparDist, weight=1., with_weights=False):
     """base/surfaceDistances.py:87-139"""
     if with_weights:
-        raise NotImplementedError("current gradient wrt vertex weights is not built")
+        px, pxw = currentNormGradient_device(fvDef, fv1, KparDist, with_weights=True)
+        return px.cpu().numpy(), pxw.cpu().numpy()
     return currentNormGradient_device(fvDef, fv1, KparDist).cpu().numpy()
 
 
 def currentMagnitudeGradient(fvDef, KparDist, with_weights=False):
-    """base/surfaceDistances.py:28-63"""
+    """base/surfaceDistances.py:28-63 (the current gradient with no target)."""
     if with_weights:
-        raise NotImplementedError("current gradient wrt vertex weights is not built")
-    c1, wcr1, w1, dk = _face_data(fvDef)
-    kw = _ap(KparDist)
-    z1 = ku.applyK(c1, c1, wcr1, **kw)
-    wz1 = w1[:, None] * z1
-    dz1 = (2 / 3) * ku.applyDiffKT(c1, c1, wcr1, wcr1, **kw)
-    return _gather(dk, dz1, wz1).cpu().numpy()
+        px, pxw = currentNormGradient_device(fvDef, None, KparDist, with_weights=True)
+        return px.cpu().numpy(), pxw.cpu().numpy()
+    return currentNormGradient_device(fvDef, None, KparDist).cpu().numpy()
 
 
 # --------------------------------------------------------------------------

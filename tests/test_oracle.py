@@ -298,6 +298,47 @@ def test_diffKT_target_rows_equal_full(mode):
                                rtol=1e-14, atol=1e-15)
 
 
+def _weighted(spheres, rng_seed=5):
+    (v, f), (v2, f2) = spheres
+    r = np.random.default_rng(rng_seed)
+    return v, f, r.uniform(0.5, 1.5, v.shape[0]), v2, f2, r.uniform(0.5, 1.5, v2.shape[0])
+
+
+@pytest.mark.parametrize("kind", ["current", "measure"])
+def test_vertex_weight_gradients_fd(spheres, kind):
+    """current / measure: the weight part is the derivative of the distance
+    in the template's vertex weights (base/surfaceDistances.py:130-137, :269-276)."""
+    v, f, w, v2, f2, w2 = _weighted(spheres)
+    norm = {'current': O.currentNorm, 'measure': O.measureNorm}[kind]
+    grad = {'current': O.currentNormGradient, 'measure': O.measureNormGradient}[kind]
+    px, pxw = grad((v, f, w), (v2, f2, w2), [0.9], with_weights=True)
+    np.testing.assert_array_equal(px, grad((v, f, w), (v2, f2, w2), [0.9]))
+    dw = rnd(v.shape[0])
+    fd = _fd(lambda ww: norm((v, f, ww), (v2, f2, w2), [0.9]), w, dw, eps=1e-6)
+    np.testing.assert_allclose((pxw * dw).sum(), fd, rtol=1e-6)
+    pxm, pxwm = O.currentMagnitudeGradient((v, f, w), [0.9], with_weights=True)
+    fdm = _fd(lambda ww: O.currentNorm0((v, f, ww), [0.9]), w, dw, eps=1e-6)
+    np.testing.assert_allclose((pxwm * dw).sum(), fdm, rtol=1e-6)
+
+
+@pytest.mark.parametrize("c", [1.0, 1.7])
+def test_varifold_vertex_weight_gradient(spheres, c):
+    """The varifold weight part keeps the face weight fw_f in its per-face
+    factor (base/surfaceDistances.py:455-462): with uniform template weights c
+    it is c times the derivative of the distance in the vertex weights."""
+    (v, f), (v2, f2) = spheres
+    w = np.full(v.shape[0], c)
+    w2 = np.random.default_rng(8).uniform(0.5, 1.5, v2.shape[0])
+    px, pxw = O.varifoldNormGradient((v, f, w), (v2, f2, w2), [0.8], with_weights=True)
+    np.testing.assert_array_equal(px, O.varifoldNormGradient((v, f, w), (v2, f2, w2), [0.8]))
+    dw = rnd(v.shape[0])
+    fd = _fd(lambda ww: O.varifoldNorm((v, f, ww), (v2, f2, w2), [0.8]), w, dw, eps=1e-6)
+    np.testing.assert_allclose((pxw * dw).sum(), c * fd, rtol=1e-6)
+    rows = np.array([1, 7, 30])
+    pr, pwr = O.varifoldNormGradient((v, f, w), (v2, f2, w2), [0.8], vertex_rows=rows, with_weights=True)
+    np.testing.assert_array_equal(pwr, pxw[rows])
+
+
 def test_current_orientation_and_gradient(spheres):
     A, Bs = spheres
     v, f = A
@@ -569,7 +610,7 @@ def test_laplacian_dense_helpers_and_tensor(order):
     y, x, a, p = rnd(12, 3), rnd(17, 3), rnd(17, 3), rnd(12, 3)
     beta = np.broadcast_to(a[None], (12, 17, 3))
     np.testing.assert_allclose(O.applykmat(y, x, beta, [1.3, 2.0], kernel=kern),
-                               O.applyK(y, x, a, [1.3, 2.0], kernel=kern), rtol=1e-13)
+                               O.applyK(y, x, a, [1.3, 2.0], kernel=kern), rtol=1e-13, atol=1e-14)
     np.testing.assert_allclose(O.applykdiffmat(y, x, p @ a.T, [1.3], kernel=kern),
                                O.applyDiffKT(y, x, p, a, [1.3], kernel=kern), rtol=1e-12, atol=1e-300)
     y1, x1 = np.zeros((1, 3)), np.array([[0.3, 0.2, -0.1]])
