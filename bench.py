@@ -414,8 +414,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the varifold and registration measurements reported under `extra`")
     ap.add_argument("--probe", action="store_true",
-                    help="only launch the dominant kernel (FP32 applyDiffKT plain, sigma 2.5) 3 times and a "
-                         "256 MiB device copy 3 times, for rocprofv3 counter runs")
+                    help="only launch the pair kernels (applyK and plain applyDiffKT, sigma 2.5, FP32 then "
+                         "FP64) 3 times each and a 256 MiB device copy 3 times, for rocprofv3 counter runs")
     ap.add_argument("--registration", action="store_true",
                     help="only run the plain registration (icosphere 4, float64 and float32) three times, "
                          "for rocprofv3 kernel traces of the registration loop")
@@ -474,14 +474,18 @@ def main():
             applyDiffKT(s, pr, outD)
 
     if args.probe:
-        for _ in range(3):
-            applyDiffKT(SIGMAS[0], "float32", outD)
+        for pr in ("float32", "float64"):
+            for _ in range(3):
+                applyK(SIGMAS[0], pr, outK)
+            for _ in range(3):
+                applyDiffKT(SIGMAS[0], pr, outD)
         src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)     # 256 MiB
         dst = torch.empty_like(src)
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
-        log("probe done: 3 applyDiffKT launches (N=M=%d), 3 copies of 256 MiB" % N)
+        log("probe done: 3 launches each of applyK and applyDiffKT in FP32 then FP64 (N=M=%d), "
+            "3 copies of 256 MiB" % N)
         return
 
     log(f"warmup {args.warmup} steps, N=M={N}")
@@ -588,7 +592,7 @@ def main():
             "n": N, "m": M, "sigmas": list(SIGMAS), "pairs_per_step": pairs_per_step,
         },
         "roofline": {
-            "bound": "mfma",
+            "bound": "valu",
             "engine": "FP32 VALU + v_exp_f32 (compute-bound; packed FP32 disabled)",
             "kernel": "k_pair<float, OpDiffKT<3,false,false>, 2> (applyDiffKT plain)",
             "achieved": achievedD, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",

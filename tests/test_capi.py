@@ -232,3 +232,20 @@ def test_internal_cost_descriptor_checks(lib):
                                         None) == B.EINVAL            # adjacency required
     with pytest.raises(NotImplementedError):
         B.internal_cost_desc([('bending', 1.0)])
+
+
+def test_exp2_polynomial_matches_its_generator():
+    """The float64 2^f coefficients in common.hpp are the ones exp2_coeffs.py
+    fits, and their Horner evaluation stays within 5e-16 relative on [-1/2, 1/2]."""
+    import importlib.util
+    here = os.path.join(ROOT, "py-lddmm_amd", "csrc")
+    spec = importlib.util.spec_from_file_location("exp2_coeffs", os.path.join(here, "exp2_coeffs.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    cs, _ = gen.coefficients()
+    src = open(os.path.join(here, "common.hpp")).read()
+    block = src[src.index("kExp2Poly[11] = {"):]
+    block = block[block.index("{") + 1:block.index("}")]
+    in_header = [float(t) for t in block.replace("\n", " ").split(",") if t.strip()]
+    assert in_header == cs
+    assert gen.worst_relative_error(cs, points=2001) < 5e-16

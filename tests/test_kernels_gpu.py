@@ -245,3 +245,25 @@ def test_stream_handle_is_torch_current_stream(ku):
     t = torch.tensor(y, device="cuda")
     assert B.to_dev(t) is t                                   # already device float64 contiguous
     assert B.to_dev(t.T).is_contiguous() and B.to_dev(t.float()).dtype == torch.float64
+
+
+@pytest.mark.parametrize("sigma", [1.0, 2.5])
+def test_fp64_gaussian_factor_over_its_range(ku, sigma):
+    """One source and targets at distances whose Gaussian goes from 1 down to
+    the subnormal range (exponent e = r^2 / 2 sigma^2 up to 750): each output
+    is a single factor exp(-e), checked element by element against numpy.
+    The float64 path rounds the centred, scaled coordinates, which perturbs
+    e by a few ulp of e itself, hence the e-proportional tolerance."""
+    rng = np.random.default_rng(17)
+    e = np.concatenate([[0.0, 1e-12, 0.5, 1.0], rng.uniform(0, 750, 4000)])
+    r = np.sqrt(2 * e) * sigma
+    dirs = rng.normal(size=(e.size, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    x = np.array([[3.0, -2.0, 1.0]])
+    y = x + r[:, None] * dirs
+    got = ku.applyK(y, x, np.ones((1, 1)), 'gauss', [sigma], 10, dtype='float64')[:, 0]
+    ee = ((y - x) ** 2).sum(1) / (2 * sigma ** 2)
+    ref = np.exp(-ee)
+    tol = (2e-15 + 4e-15 * ee) * ref + 1e-322
+    assert np.all(np.abs(got - ref) <= tol), np.max(np.abs(got - ref) / np.maximum(ref, 1e-300))
+    assert got[0] == 1.0

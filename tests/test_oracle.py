@@ -866,3 +866,23 @@ def test_jacobian_tracks_log_det_of_the_flow(kaff):
         errs.append(np.abs(J[-1, :, 0] - ld).max())
         assert errs[-1] < 0.05 * np.abs(ld).max()
     assert 1.7 < errs[0] / errs[1] < 2.3 and 1.7 < errs[1] / errs[2] < 2.3
+
+
+# ------------------------------------------- C restatement (CPU baseline)
+@pytest.mark.parametrize("sig", [[1.3], [0.8, 2.5]])
+def test_cpu_reference_matches_numpy_oracle(sig):
+    """oracle/cpu_ref.c (the timed CPU baseline) against the numpy oracle:
+    N != M, one and two scales, the three applyDiffKT modes, k = 3 and 1."""
+    from oracle import cpu
+    r = np.random.default_rng(12)
+    y, x = r.normal(size=(37, 3)) * 2, r.normal(size=(53, 3)) * 2
+    a, p = r.normal(size=(53, 3)), r.normal(size=(37, 3))
+    np.testing.assert_allclose(cpu.applyK(y, x, a, sig), O.applyK(y, x, a, sig), rtol=1e-13, atol=1e-14)
+    np.testing.assert_allclose(cpu.applyK(y, x, a[:, :1], sig), O.applyK(y, x, a[:, :1], sig),
+                               rtol=1e-13, atol=1e-14)
+    np.testing.assert_allclose(cpu.applyDiffKT(y, x, p, a, sig), O.applyDiffKT(y, x, p, a, sig),
+                               rtol=1e-13, atol=1e-14)
+    xs, ps, as_ = r.normal(size=(41, 3)), r.normal(size=(41, 3)), r.normal(size=(41, 3))
+    for kw in ({'sym': True}, {'lddmm': True, 'regweight': 0.6}):
+        np.testing.assert_allclose(cpu.applyDiffKT(xs, xs, ps, as_, sig, **kw),
+                                   O.applyDiffKT(xs, xs, ps, as_, sig, **kw), rtol=1e-13, atol=1e-14)
