@@ -1232,8 +1232,12 @@ def second_order_matching_objective(x0, w0, a0, target, sigma_diff, sigma_dist, 
     x_T the endpoint of secondOrderEvolution
     (base/secondOrderPointSetMatching.py:105-140 objectiveFunDef, no affine;
     the data term and obj0 as in base/pointSetMatching.py).  kind
-    'point_measure' (w0 = template weights, target = (points, weights)) or
-    'L2' (target = points of the same shape; |x_T - y|^2 with obj0 = |y|^2)."""
+    'point_measure' (w0 = template weights, target = (points, weights)),
+    'L2' (target = points of the same shape; |x_T - y|^2 with obj0 = |y|^2),
+    or a surface data term 'varifold' / 'current' / 'measure' (w0 = template
+    faces, target = (vertices, faces)), the objective of
+    base/secondOrderSurfaceMatching.py (SurfaceMatching's data term over the
+    second-order control)."""
     a0 = _f64(a0)
     xt, at = secondOrderEvolution(x0, a0, sigma_diff, timeStep, kernel=kernel_diff)
     obj = 0.5 * (a0 * applyK(x0, x0, a0, sigma_diff, kernel=kernel_diff)).sum()

@@ -8,7 +8,9 @@ algorithm 'bfgs', affine None (no affine controls), a single target.
 
 The geodesic forward and its adjoint run on the device (lddmm_epdiff_forward
 / lddmm_epdiff_adjoint, T steps each in one call); the control, its BFGS
-memory and the trajectory stay on the device as in SurfaceMatching.
+memory and the trajectory stay on the device as in SurfaceMatching.  The
+geodesic control itself is `GeodesicControl`, shared with
+SecondOrderSurfaceMatching (the same control over the surface data terms).
 """
 import torch
 
@@ -18,7 +20,15 @@ from .pointSetMatching import PointSetMatching
 from .surfaceMatching import Control, State
 
 
-class SecondOrderPointSetMatching(PointSetMatching):
+class GeodesicControl:
+    """The second-order control of base/secondOrderPointSetMatching.py placed
+    in front of a first-order driver (PointSetMatching or SurfaceMatching)
+    in the method order: initial momentum a0 instead of the time-dependent
+    momenta, geodesic trajectory, objective 1/2 a0.K(x0)a0 + data term,
+    gradient K(x0)a0 - pa_0.  The driver supplies the data term
+    (dataTerm_device / endPointGradient_device), the BFGS hooks and the
+    device L-BFGS direction."""
+    unreduced_supported = False
 
     def getDefaultOptions(self):
         """base/secondOrderPointSetMatching.py:53-57."""
@@ -117,3 +127,27 @@ class SecondOrderPointSetMatching(PointSetMatching):
 
     def control_numpy(self):
         return self.control['a0'].cpu().numpy()
+
+    def saveOutputs(self):
+        """Per-step files of the geodesic (base/secondOrderSurfaceMatching.py:
+        819-868): the deformed template with the log-Jacobian fields and the
+        velocities v_t = K(x_t) a_t of the geodesic's momenta."""
+        if not hasattr(self, 'saveEvolution') or not hasattr(self.fvInit, 'faces'):
+            raise NotImplementedError("file output (writeOutput) is built for surface matching only")
+        _, st = self.objectiveFunDef(self.control, withTrajectory=True, withJacobian=True)
+        K = self.options['KparDiff']
+        vel = torch.stack([K.applyK(st['xt'][t], st['at'][t]) for t in range(self.Tsize)])
+        self.saveEvolution(self.fvInit, st, fileName=self.options['saveFile'], velocity=vel.cpu().numpy())
+
+    def endOfIteration(self, forceSave=False):
+        """base/secondOrderPointSetMatching.py:538-560: count the iteration and
+        write the per-step files every saveRate iterations (with the option
+        writeOutput); the precision switched to float64 after a restart is
+        kept (no return to pk_dtype, unlike the first-order drivers)."""
+        self.iter += 1
+        if self.options['writeOutput'] and (forceSave or self.iter % self.saveRate == 0):
+            self.saveOutputs()
+
+
+class SecondOrderPointSetMatching(GeodesicControl, PointSetMatching):
+    pass

@@ -508,6 +508,21 @@ def test_second_order_matching_gradient_fd(kind):
     np.testing.assert_allclose((g * d).sum(), fd, rtol=1e-6)
 
 
+@pytest.mark.parametrize("kind", ["varifold", "current", "measure"])
+def test_second_order_surface_matching_gradient_fd(spheres, kind):
+    """Second-order surface matching (base/secondOrderSurfaceMatching.py with the
+    surface data terms): da0 = FD of the objective in a0 (the template's
+    faces take the place of the point weights)."""
+    (v, f), (v2, f2) = spheres
+    a0 = 0.05 * rnd(*v.shape)
+    d = rnd(*v.shape)
+    kw = dict(kind=kind, sigma_error=0.7)
+    g = O.second_order_matching_gradient(v, f, a0, (v2, f2), [1.5], [0.8], 0.2, **kw)
+    fd = _fd(lambda a_: O.second_order_matching_objective(v, f, a_, (v2, f2), [1.5], [0.8], 0.2, **kw)[0],
+             a0, d)
+    np.testing.assert_allclose((g * d).sum(), fd, rtol=1e-6)
+
+
 # ------------------------------------------------------ laplacian family
 def _matern(order, r):
     """Matern nu = order + 1/2 without its e^-r: order!/(2 order)! sum_i
