@@ -7,6 +7,7 @@
 //   EPDiff  : base/pointEvolution.py:385-472 (forward), :489-618 (adjoint)
 #include "ops.hpp"
 #include "icost.hpp"
+#include "step.hpp"
 
 #include <algorithm>
 
@@ -367,11 +368,14 @@ int shoot_forward_t(const lddmm_kernel* kd, const double* at, int T_, int n, dou
     using OPK = OpApplyK<KW, MS>;
     Job<T, OPK> jk;
     LDDMM_TRY(plan_job(jk, n, n, !MS));
-    const int nb = (n + kEpiRows - 1) / kEpiRows;              // blocks of k_shoot_epi
+    const bool aff = kd->affine != LDDMM_AFFINE_NONE;
+    // small sets: one launch per step (pair sums and epilogue in one workgroup)
+    const bool small = !ic && !aff && int64_t(n) <= kStepMaxN;
+    const int nb = small ? (n + kStepTgt - 1) / kStepTgt      // blocks of k_step_shoot
+                         : (n + kEpiRows - 1) / kEpiRows;     // blocks of k_shoot_epi
     const int nbv = int((int64_t(n) * KW + 255) / 256);       // blocks of k_shoot_epi_v
     const int ndot = ic ? nbv + 1 : nb + 1;                     // dot slots per step
     void* ws;
-    const bool aff = kd->affine != LDDMM_AFFINE_NONE;
     const AffineArgs AA = affine_args(kd);
     const size_t icb = ic ? (size_t(n) * KW + ic_scratch_doubles(ic->nf, false)) * sizeof(double) + 512 : 0;
     LDDMM_TRY(workspace(256 + partial_bytes<T>(jk.plan, KW, n) + size_t(T_) * ndot * sizeof(double) +
@@ -395,6 +399,12 @@ int shoot_forward_t(const lddmm_kernel* kd, const double* at, int T_, int n, dou
         const double* x = xt + t * stride;
         const double* a = at + t * stride;
         jk.args = {x, x, a, c0, KW, KW, 0, sc};
+        if (small) {
+            hipLaunchKernelGGL((k_step_shoot<T, KW, MS>), dim3(unsigned(nb)), dim3(kStepThreads), 0, st, jk.args, n,
+                               dt, x, a, xt + (t + 1) * stride, dots + size_t(t) * nb, r * dt);
+            LDDMM_CHECK_LAUNCH();
+            continue;
+        }
         LDDMM_TRY(run_job(jk, n, n, part, st));
         if (!ic) {
             hipLaunchKernelGGL((k_shoot_epi<T, KW>), dim3(unsigned(nb)), dim3(256), 0, st, part, jk.plan.nchunk, n,
@@ -501,9 +511,17 @@ int shoot_adjoint_t(const lddmm_kernel* kd, const double* at, const double* xt, 
             pu = pe;
         }
         if (fused) {
+            ja.args = {x, x, pu, a, c0, KW, KW, 0, 2.0 * r, sc};
+            if (int64_t(n) <= kStepMaxN) {
+                // small sets: one launch per step
+                hipLaunchKernelGGL((k_step_adjoint<T, KW, MS>), dim3(unsigned((n + kStepTgt - 1) / kStepTgt)),
+                                   dim3(kStepThreads), 0, st, ja.args, n, dt, fdiff, pn, pc,
+                                   grad + size_t(t) * stride);
+                LDDMM_CHECK_LAUNCH();
+                continue;
+            }
             Carve ca; ca.off = poff;
             Partials<T> partA = carve_partials<T>(base, ca, ja.plan, 3 + KW, n);
-            ja.args = {x, x, pu, a, c0, KW, KW, 0, 2.0 * r, sc};
             LDDMM_TRY(run_job(ja, n, n, partA, st));
             hipLaunchKernelGGL((k_reduce_adjoint<T, KW>), dim3(unsigned((n + kEpiRows - 1) / kEpiRows), 2 * KW),
                                dim3(256), 0, st, partA, ja.plan.nchunk, n, dt, fdiff, pn, pc,
