@@ -67,6 +67,11 @@ def main(argv=None):
                     help="evaluate this saved network (no data generation or training)")
     ap.add_argument("--feature-only-pairs", type=int, default=None,
                     help="held-out pairs that also run the feature-only baseline (default: all)")
+    ap.add_argument("--refined", action="store_true",
+                    help="train on refined-velocity residuals (target 'vresid', no registrations run)")
+    ap.add_argument("--refine-pairs", type=int, default=0,
+                    help="held-out pairs that also run the re-ray-cast velocity without a network "
+                         "(one and REFINE_STEPS refinements)")
     ap.add_argument("--batched", type=int, default=0,
                     help="generate the training registrations this many pairs at a time with the lockstep "
                          "population registration (features 'normal', target 'vlocal' only)")
@@ -88,9 +93,15 @@ def main(argv=None):
         from .model import MomentumNet
         a.train = 0
         a.variants = ""
-    if a.batched and a.train:
+    if a.refined:
+        if ":vresid" not in a.variants:
+            a.variants = "normal:vresid"
+        if a.train:
+            samples = W.generate_refined([10000 + k for k in range(a.train)], cfg, progress=log)
+        a.train = 0
+    elif a.batched and a.train:
         samples = W.generate_batched([10000 + k for k in range(a.train)], cfg, batch=a.batched, progress=log)
-    for k in range(a.train if not a.batched else 0):
+    for k in range(a.train if not (a.batched or a.refined) else 0):
         seed = 10000 + k
         sm, feats, sec = plain_with_trace(seed, cfg)
         smp = dict(feats)
@@ -143,19 +154,29 @@ def main(argv=None):
                         f"E_warm_{target}": r["data"], f"E0_warm_{target}": r["data0"],
                         f"obj_ratio_{target}": r["objective"] / obj_plain,
                         f"reached_{target}": r["reached"], f"used_warm_{target}": r["used_warm"]})
-            tg = W.targets_of(sm, sm.control['at'])[W.TARGET_KEY[net.target]]
-            fx = W.features(W.make_matching(seed, cfg), net.features)
+            if net.target == 'vresid':
+                smr = W.make_matching(seed, cfg)
+                ch, fr = W.ray_cast_heights(smr)
+                res = W.refined_velocity(smr, heights=(ch, fr)) - ch[:, :1] * fr[:, 0]
+                tg = torch.einsum('nkj,nj->nk', fr, res).cpu().numpy()
+                fx = W.height_inputs(smr, net.features, (ch, fr))
+            else:
+                tg = W.targets_of(sm, sm.control['at'])[W.TARGET_KEY[net.target]]
+                fx = W.features(W.make_matching(seed, cfg), net.features)
             pred = net.predict(fx, [fx.shape[0]]).double().cpu().numpy()
             row[f"pred_rel_err_{target}"] = float(np.linalg.norm(pred - tg) / np.linalg.norm(tg))
         if "normal" in a.variants and (a.feature_only_pairs is None or k < a.feature_only_pairs):
             # the same pipeline without the network: the ray-cast normal height
             # itself as the velocity, solved with the same truncated CG
-            sm0 = W.make_matching(seed, cfg)
-            ch, fr = W.NF.normal_heights(sm0.fv0.vertices, sm0.fv0.faces, sm0.fv1.vertices, sm0.fv1.faces)
-            a_f = sm0.options['KparDiff'].solve(sm0.x0_dev, ch[:, :1] * fr[:, 0], iterMax=W.SOLVE_ITERS)
-            r = W.register(seed, at=a_f[None].expand(sm0.Tsize, -1, -1).contiguous(), target_data=E, cfg=cfg)
+            r = W.register(seed, start=lambda s_: W.feature_only_control(s_)[0], target_data=E, cfg=cfg)
             row.update({"it_feature_only": r["iterations"], "obj_ratio_feature_only": r["objective"] / obj_plain,
                         "sec_feature_only": r["seconds"]})
+        if k < a.refine_pairs:
+            # the re-ray-cast velocity itself (no network), one and REFINE_STEPS refinements
+            for steps in sorted({1, W.REFINE_STEPS}):
+                r = W.register(seed, start=lambda s_: W.time_constant(s_, W.solve_control(s_, W.refined_velocity(
+                    s_, steps))), target_data=E, cfg=cfg)
+                row[f"it_refined{steps}"] = r["iterations"]
         if k < a.oracle_check:
             # upper bounds: the plain run's own time-mean control (perfect predictor),
             # and its velocity K abar turned back into abar by truncated CG, exact
@@ -207,6 +228,12 @@ def main(argv=None):
                                        "iteration_ratio": float(itpf.mean() / max(itf.mean(), 1e-9)),
                                        "mean_objective_ratio": float(np.mean([r["obj_ratio_feature_only"]
                                                                               for r in fo]))}
+    rf = [r for r in rows if "it_refined1" in r]
+    if rf:
+        rep["test"]["refined_no_network"] = {"pairs": len(rf)}
+        for key in rf[0]:
+            if key.startswith("it_refined"):
+                rep["test"]["refined_no_network"]["mean_" + key[3:]] = float(np.mean([r[key] for r in rf]))
     ora = [r for r in rows if "it_warm_true_abar" in r]
     if ora:
         rep["test"]["abar_error_tolerance"] = {k: float(np.mean([r[k] for r in ora])) for k in ora[0]

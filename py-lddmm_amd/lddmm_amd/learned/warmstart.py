@@ -2,16 +2,27 @@
 
 Pairs are the plain surface-matching workload (icosphere subdivision 4,
 R = 10, seeded Gaussian bumps) with one seed per pair; no vertex
-correspondence between template and target is assumed.  The default model
-(features 'normal', target 'vlocal') reads per vertex the ray-cast normal
-height of the target over the template and its smoothed versions, and
-predicts the velocity K(x0) abar of the plain optimiser's time-mean control
-abar in the vertex's local frame; the warm control is the time-constant
-abar recovered from that velocity by a truncated conjugate-gradient solve
-(Kernel.solve, 10 products; the truncation keeps the control smooth).  The
-zero control is kept whenever it has the lower objective (the fallback),
-and BFGS then runs until the varifold data term matches the plain method's
-final value within 1e-4.
+correspondence between template and target is assumed.
+
+Feature-only control (no network): the ray-cast normal height h of the
+target over each template vertex, taken as the velocity h n and turned into
+a time-constant control by a conjugate-gradient solve on K(x0) truncated at
+SOLVE_ITERS products (Kernel.solve; the truncation keeps the control smooth).
+
+Default model (features 'normal', target 'vresid'): the network reads per
+vertex h and its smoothed versions and predicts, in the vertex's local frame,
+the residual that turns h n into the refined velocity: the fixed point of
+re-ray-casting from the surface that the current velocity's control reaches
+(refined_velocity).  The warm control is then solved from h n plus the
+prediction the same way, so a zero prediction reproduces the feature-only
+control.  The zero control is kept whenever it has the lower objective (the
+fallback), and BFGS then runs until the varifold data term matches the plain
+method's final value within 1e-4.
+
+Older targets ('vlocal': the plain optimiser's time-mean velocity in the
+local frame, and the world-frame targets) are kept for the experiment
+driver; they learn the plain optimiser's balanced solution, which through
+the truncated solve is a worse starting point than h n itself.
 """
 import functools
 import logging
@@ -57,10 +68,12 @@ def make_matching(seed, cfg=DEFAULT, at=None):
     return SurfaceMatching(Template=Surface((f, v)), Target=Surface((f, v2)), options=opts)
 
 
-FEATURES = ('force', 'meanshift', 'kforce', 'normal')
-TARGETS = ('abar', 'vbar', 'vlocal', 'momentum', 'velocity')
-TARGET_KEY = {"abar": "abar", "vbar": "vbar", "vlocal": "vlocal", "momentum": "a0", "velocity": "v0"}
-SOLVE_ITERS = 10      # conjugate-gradient products that turn a predicted velocity into abar
+FEATURES = ('force', 'meanshift', 'kforce', 'normal', 'normal_lin')
+TARGETS = ('vresid', 'abar', 'vbar', 'vlocal', 'momentum', 'velocity')
+TARGET_KEY = {"vresid": "vresid", "abar": "abar", "vbar": "vbar", "vlocal": "vlocal", "momentum": "a0",
+              "velocity": "v0"}
+SOLVE_ITERS = 10      # conjugate-gradient products that turn a velocity into a time-constant control
+REFINE_STEPS = 2      # re-ray-casts behind the 'vresid' training target
 MEANSHIFT_SIGMA = 1.5
 
 
@@ -71,10 +84,12 @@ def features(sm, kind='force'):
                 template vertex minus the vertex (width MEANSHIFT_SIGMA);
     kforce    - the force smoothed by the deformation kernel, K(x0) force;
     normal    - ray-cast normal height of the target over the template and
-                its normalised Gaussian averages (learned.features)."""
-    if kind == 'normal':
-        ch, _ = NF.normal_heights(sm.fv0.vertices, sm.fv0.faces, sm.fv1.vertices, sm.fv1.faces)
-        return ch
+                its normalised Gaussian averages (learned.features);
+    normal_lin - the height h, the part of it that the feature-only
+                control's initial velocity misses, h - n.K(x0) a_f
+                (linear_miss), and h averaged at width 2.5."""
+    if kind in ('normal', 'normal_lin'):
+        return height_inputs(sm, kind)
     if kind == 'force':
         return -sm.endPointGradient_device(sm.x0_dev)
     if kind == 'kforce':
@@ -91,6 +106,85 @@ def features(sm, kind='force'):
 def frame(sm):
     """Local frames [N, 3, 3] (rows: normal, two tangents) of the template vertices (device)."""
     return B.to_dev(NF.local_frame(synth.vertex_normals(sm.fv0.vertices, sm.fv0.faces)))
+
+
+def time_constant(sm, a):
+    """Control [T, N, 3] that repeats the momentum a at every time step."""
+    return a[None].expand(sm.Tsize, -1, -1).contiguous()
+
+
+def solve_control(sm, v, iters=SOLVE_ITERS):
+    """Momentum a with K(x0) a ~ v: conjugate gradients truncated at `iters` products."""
+    return sm.options['KparDiff'].solve(sm.x0_dev, v, iterMax=iters)
+
+
+def ray_cast_heights(sm):
+    """Normal-height channels [N, 3] and local frames [N, 3, 3] of the pair (device)."""
+    return NF.normal_heights(sm.fv0.vertices, sm.fv0.faces, sm.fv1.vertices, sm.fv1.faces)
+
+
+def linear_miss(sm, ch, fr, a=None):
+    """h - n.K(x0) a_f [N, 1]: the normal height that the initial velocity of
+    the feature-only control a_f (solved from h n) does not reproduce."""
+    n = fr[:, 0]
+    if a is None:
+        a = solve_control(sm, ch[:, :1] * n)
+    return ch[:, :1] - (sm.options['KparDiff'].applyK(sm.x0_dev, a) * n).sum(1, keepdim=True)
+
+
+def height_inputs(sm, kind='normal', heights=None):
+    """Network inputs [N, 3] built on the ray-cast heights (features 'normal'
+    and 'normal_lin'); `heights` = the pair's (channels, frames) when known."""
+    ch, fr = ray_cast_heights(sm) if heights is None else heights
+    if kind == 'normal':
+        return ch
+    if kind == 'normal_lin':
+        return torch.cat([ch[:, :1], linear_miss(sm, ch, fr), ch[:, 1:2]], 1)
+    raise ValueError(f"feature kind {kind!r} is not built on the ray-cast heights")
+
+
+def feature_only_control(sm):
+    """The warm control without a network: the velocity h n of the ray-cast
+    normal heights, solved for a time-constant control.  Returns (at, a)."""
+    ch, fr = ray_cast_heights(sm)
+    a = solve_control(sm, ch[:, :1] * fr[:, 0])
+    return time_constant(sm, a), a
+
+
+def refined_velocity(sm, steps=REFINE_STEPS, heights=None):
+    """Velocity refined by re-ray-casting.  Start from v = h n; each step
+    shoots the time-constant control solved from v, ray-casts the target along
+    the unit vertex normals n_T of the surface it reaches (a missed ray has
+    height 0) and adds that height: v <- v + h_T n_T.  `heights` = the pair's
+    (channels, frames) when already computed.  Returns v [N, 3] (device)."""
+    ch, fr = ray_cast_heights(sm) if heights is None else heights
+    K = sm.options['KparDiff']
+    faces = sm.fv0.faces
+    v = ch[:, :1] * fr[:, 0]
+    for _ in range(steps):
+        xt, _ = evol.shoot(sm.x0_dev, time_constant(sm, solve_control(sm, v)), K)
+        xT = xt[-1].cpu().numpy()
+        nT = synth.vertex_normals(xT, faces)
+        hT, _ = NF.normal_raycast(xT, nT, sm.fv1.vertices, sm.fv1.faces)
+        v = v + hT[:, None] * B.to_dev(nT)
+    return v
+
+
+def generate_refined(seeds, cfg=DEFAULT, steps=REFINE_STEPS, progress=None):
+    """Training samples {feat_normal, feat_normal_lin, vresid} of the pairs `seeds`: the
+    residual refined_velocity - h n in the local frames.  No registration is
+    run: each pair costs `steps` shoots and ray casts."""
+    samples = []
+    for k, s in enumerate(seeds):
+        sm = make_matching(s, cfg)
+        ch, fr = ray_cast_heights(sm)
+        r = refined_velocity(sm, steps, heights=(ch, fr)) - ch[:, :1] * fr[:, 0]
+        samples.append({"feat_normal": ch.cpu().numpy(),
+                        "feat_normal_lin": height_inputs(sm, 'normal_lin', (ch, fr)).cpu().numpy(),
+                        "vresid": torch.einsum('nkj,nj->nk', fr, r).cpu().numpy()})
+        if progress is not None and (k + 1) % 1000 == 0:
+            progress(f"refined targets: {k + 1} / {len(seeds)}")
+    return samples
 
 
 def velocity(sm, a0):
@@ -170,7 +264,7 @@ def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50, target='a
     feat_scale = float(np.sqrt((F ** 2).mean())) or 1.0
     out_scale = float(np.sqrt((A ** 2).mean())) or 1.0
     net = MomentumNet(seed=seed, feat_scale=feat_scale, out_scale=out_scale, target=target, features=features,
-                      solve_iters=SOLVE_ITERS if target == 'vlocal' else 0)
+                      solve_iters=SOLVE_ITERS if target in ('vlocal', 'vresid') else 0)
     dev = net.dev
     Xs = [torch.tensor(s[fkey] / feat_scale, dtype=torch.float32, device=dev) for s in samples]
     Ys = [torch.tensor(s[key] / out_scale, dtype=torch.float32, device=dev) for s in samples]
@@ -221,7 +315,10 @@ def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50, target='a
 def warm_control(sm, net, cg_iters=None):
     """Initial control [T, N, 3] from the network's prediction.
 
-    target 'vlocal': the prediction is K(x0) abar in each vertex's local
+    target 'vresid': the prediction is added, in each vertex's local frame, to
+    the ray-cast velocity h n, and the sum is solved for a time-constant
+    control with the network's solve_iters products (SOLVE_ITERS), as the
+    feature-only control is; 'vlocal': the prediction is K(x0) abar in each vertex's local
     frame (normal, tangents), turned into abar by a conjugate-gradient solve
     truncated at the network's solve_iters products (SOLVE_ITERS) and used as a
     time-constant control; 'abar': the prediction is the time-mean of the plain optimiser's
@@ -230,12 +327,18 @@ def warm_control(sm, net, cg_iters=None):
     truncated at cg_iters products) and used as a time-constant control;
     'momentum': a0, shot with EPDiff; 'velocity': K(x0) a0, solved for a0 the
     same way and shot with EPDiff."""
-    feats = features(sm, getattr(net, 'features', 'force'))
-    pred = net.predict(feats, [feats.shape[0]])
-    T = sm.Tsize
     target = getattr(net, 'target', 'momentum')
     if cg_iters is None:
         cg_iters = getattr(net, 'solve_iters', None) or 20
+    if target == 'vresid':
+        ch, fr = ray_cast_heights(sm)
+        X = height_inputs(sm, getattr(net, 'features', 'normal'), (ch, fr))
+        pred = net.predict(X, [X.shape[0]])
+        a = solve_control(sm, ch[:, :1] * fr[:, 0] + torch.einsum('nkj,nk->nj', fr, pred), cg_iters)
+        return time_constant(sm, a), a
+    feats = features(sm, getattr(net, 'features', 'force'))
+    pred = net.predict(feats, [feats.shape[0]])
+    T = sm.Tsize
     if target == 'vlocal':
         v = torch.einsum('nkj,nk->nj', frame(sm), pred)
         a = sm.options['KparDiff'].solve(sm.x0_dev, v, iterMax=cg_iters)
@@ -253,9 +356,12 @@ def warm_control(sm, net, cg_iters=None):
     return at, a0
 
 
-def register(seed, net=None, target_data=None, cfg=DEFAULT, max_iter=None, a0=None, cg_iters=None, at=None):
-    """Registration (plain, warm-started by `net`, or warm-started from a given
-    initial momentum `a0`) until the data term reaches target_data (1e-4 rel).
+def register(seed, net=None, target_data=None, cfg=DEFAULT, max_iter=None, a0=None, cg_iters=None, at=None,
+             start=None):
+    """Registration (plain, warm-started by `net`, from a given control `at`,
+    from a given initial momentum `a0`, or from the control `start(sm)`
+    returns for the new matching) until the data term reaches target_data
+    (1e-4 rel).  The timed region covers set-up, the warm control and BFGS.
 
     The warm control is kept only when its objective is below the zero
     control's (zero-momentum fallback).  `data0` is the data term at the
@@ -267,8 +373,10 @@ def register(seed, net=None, target_data=None, cfg=DEFAULT, max_iter=None, a0=No
     t0 = time.perf_counter()
     sm = make_matching(seed, c)
     used_warm = False
-    if net is not None or a0 is not None or at is not None:
-        if at is not None:
+    if net is not None or a0 is not None or at is not None or start is not None:
+        if start is not None:
+            at = start(sm)
+        elif at is not None:
             at = B.to_dev(at)
         elif a0 is None:
             at, _ = warm_control(sm, net, cg_iters=cg_iters)

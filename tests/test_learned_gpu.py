@@ -95,31 +95,38 @@ def test_adam_step(net_mod):
 
 
 def test_trained_network_warm_start_gain(net_mod):
-    """The committed network (ray-cast normal heights -> the plain optimiser's
-    time-mean velocity in the local vertex frames, turned into a control by a
-    10-product conjugate-gradient solve; trained on 20,000 plain registrations)
-    on held-out pairs of the plain workload: every warm-started registration
-    reaches the plain run's final varifold energy (1e-4 relative), and it
-    takes at least 3x fewer BFGS iterations in total than the zero-momentum
-    plain runs.  Measured on 1,000 held-out pairs: 12.04 vs 49.85 iterations
-    (4.14x); on the six seeds below 80 vs 299."""
+    """The committed network on held-out pairs of the plain workload.  It reads
+    the ray-cast normal height h, the part of h n that the feature-only
+    control's initial velocity misses and h averaged at width 2.5, and
+    predicts the residual of the re-ray-cast (refined) velocity over h n in
+    the local vertex frames; trained on 20,000 pairs.  Each warm-started
+    registration reaches the plain run's final varifold energy (1e-4
+    relative) from the warm control.  In total over six seeds it takes no more
+    BFGS iterations than the feature-only control (h n solved the same way,
+    no network) and at least 3x fewer than the zero-momentum plain runs.
+    Measured on 60 held-out pairs: 4.63 (network) vs 6.05 (feature-only) vs
+    49.85 (plain) iterations; on the six seeds below 28 vs 34 vs 299."""
     model, torch = net_mod
     import os
     from lddmm_amd.learned import warmstart as W
     from lddmm_amd.learned.experiment import plain_with_trace, iterations_to
     path = os.path.join(os.path.dirname(W.__file__), "momentum_net.npz")
     net = model.MomentumNet.load(path)
-    assert net.target == 'vlocal' and net.features == 'normal' and net.solve_iters == W.SOLVE_ITERS
-    its_p, its_w = [], []
+    assert net.target == 'vresid' and net.features == 'normal_lin' and net.solve_iters == W.SOLVE_ITERS
+    its_p, its_w, its_f = [], [], []
     for seed in range(900000, 900006):
         sm, _, _ = plain_with_trace(seed, W.DEFAULT)
         E = sm.varifold_data()
         it_plain = iterations_to(sm.data_trace, E)
         r = W.register(seed, net=net, target_data=E)
-        print(f"[warmstart] seed {seed}: plain {it_plain} it, warm {r['iterations']} it", flush=True)
-        assert r["reached"]
+        rf = W.register(seed, start=lambda sm: W.feature_only_control(sm)[0], target_data=E)
+        print(f"[warmstart] seed {seed}: plain {it_plain} it, network {r['iterations']} it, "
+              f"feature-only {rf['iterations']} it", flush=True)
+        assert r["reached"] and rf["reached"]
         assert r["used_warm"]
         assert r["iterations"] < it_plain
         its_p.append(it_plain)
         its_w.append(r["iterations"])
+        its_f.append(rf["iterations"])
+    assert sum(its_w) <= sum(its_f)
     assert sum(its_p) >= 3 * sum(its_w)
