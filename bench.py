@@ -30,6 +30,7 @@ SIGMAS = (2.5, 5.0, 10.0)
 # algorithmic work per pair (FMA = 2 FLOP) and the MI355X published FP32 peak
 FLOP_PER_PAIR = {"applyK": 15, "applyDiffKT": 22}
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
 PEAK_EXP_PER_S = 256 * 4 * 16 / 2 * 2.4e9   # v_exp_f32 issues at half the FMA rate (profiles/gfx950_rates.md)
 
 
@@ -594,6 +595,11 @@ def main():
         B.profile_enable(False)
         extra["fp64_pair_evals_per_s"] = 2 * N * M / t64
         extra["fp64_pair_kernel_ms"] = [round(v, 3) for v in ms64]
+        if len(ms64) == 2:
+            tf64 = {op: N * M * FLOP_PER_PAIR[op] / (ms * 1e-3) / 1e12
+                    for op, ms in zip(("applyK", "applyDiffKT"), ms64)}
+            extra["fp64_tflops"] = {op: round(v, 2) for op, v in tf64.items()}
+            extra["fp64_frac_of_peak"] = {op: round(v / PEAK_FP64_TFLOPS, 3) for op, v in tf64.items()}
         extra["fp32_rel_err_vs_fp64"] = errs
         log(f"fp64: {2 * N * M / t64:.4g} pair-evals/s; max fp32 rel err {max(errs.values()):.2e}")
 
