@@ -138,7 +138,7 @@ def registration_cfg1(torch):
         dt = time.perf_counter() - t0
         out[pr] = {"seconds": dt, "iterations": int(sm.bfgs_iterations), "final_objective": sm.obj,
                    "final_data_term": sm.varifold_data(), "initial_objective": sm.bfgs_history[0]}
-        log(f"registration cfg1 ({pr}): {dt:.2f} s, {sm.bfgs_iterations} iterations, "
+        log(f"registration cfg1 ({pr}): {dt * 1e3:.1f} ms, {sm.bfgs_iterations} iterations, "
             f"objective {sm.bfgs_history[0]:.4f} -> {sm.obj:.4f}")
     return out
 

@@ -78,8 +78,10 @@ class _VectorOps:
     """Direction arithmetic, from the object's hooks or numpy defaults."""
 
     def __init__(self, opt):
+        self._table = None
         if hasattr(opt, 'dotProduct_euclidean'):
             self._dots = opt.dotProduct_euclidean
+            self._table = getattr(opt, 'dotTable', None)
         elif hasattr(opt, 'dotProduct'):
             self._dots = opt.dotProduct
         else:
@@ -90,6 +92,30 @@ class _VectorOps:
 
     def dot(self, u, v):
         return self._dots(u, [v])[0]
+
+    def table(self, probes, targets):
+        """A function returning table[i][j] = dot(probes[i], targets[j]).
+        With the object's dotTable the products are launched now and read
+        back, in one transfer, at the first call (the same values as its dot
+        products)."""
+        if self._table is not None:
+            return _Deferred(self._table(probes, targets, later=True))
+        tab = [[self.dot(p, t) for t in targets] for p in probes]
+        return lambda: tab
+
+
+class _Deferred:
+    """A value read back at the first call and kept."""
+
+    def __init__(self, read):
+        self._read = read
+        self._value = None
+
+    def __call__(self):
+        if self._read is not None:
+            self._value = self._read()
+            self._read = None
+        return self._value
 
 
 class _Memory:
@@ -232,17 +258,21 @@ def bfgs(opt, verb=True, maxIter=1000, TestGradient=False, epsInit=0.01, memory=
                 q = opt.twoLoop(grad, run.memory.pairs)
             else:
                 q = quasi_newton_direction(grad, run.memory.pairs, ops)
-        grad_norm = np.sqrt(ops.dot(grad, grad))
-        q_norm = np.sqrt(np.maximum(1e-20, ops.dot(q, q)))
+        # |g|^2, |q|^2 and the slope <g, q> along the new direction, launched
+        # together; read back when first needed (on most iterations after the
+        # line search's first trial, whose own read-back waits for them)
+        dots = ops.table([grad, q], [grad, q])
         direction = ops.copy(q)
         run.grad_old = ops.copy(grad)
         if run.it == 0 or run.it == burn_in:
-            run.grad_eps = gradient_threshold(run.grad_eps, grad_norm)
+            run.grad_eps = gradient_threshold(run.grad_eps, np.sqrt(dots()[0][0]))
             logging.info(f'Gradient threshold: {run.grad_eps:.6f}')
+        t_init = (1. / np.sqrt(np.maximum(1e-20, dots()[1][1]))) if (run.it == 0 or restarted) else 1.
 
         step, _, _, f_new, _, run.grad_next = line_search_weak_wolfe(
             opt, direction, gfk=grad, old_fval=run.f, old_old_fval=run.f_old, c1=WOLFE_C1, c2=WOLFE_C2,
-            amax=None, t_init=(1. / q_norm) if (run.it == 0 or restarted) else 1., maxiter=WOLFE_MAX_GRADIENTS)
+            amax=None, t_init=t_init, maxiter=WOLFE_MAX_GRADIENTS, slope0=lambda: -dots()[0][1])
+        grad_norm = np.sqrt(dots()[0][0])
         if step is not None:
             run.step_taken = ops.scale(direction, -step)
             run.f_old = run.f

@@ -22,13 +22,20 @@ MAX_TRIALS = 100
 
 
 class _Probe:
-    """Objective and slope of t -> f(x - t d), counting evaluations."""
+    """Objective and slope of t -> f(x - t d), counting evaluations.
+
+    When the object provides updateTryWithSlope, a trial's gradient and slope
+    are computed together with its objective (one read-back instead of two);
+    the slope is then counted, and its gradient kept, only if the search asks
+    for it, so the counts and every decision are those of separate calls."""
 
     def __init__(self, opt, direction, euclidean):
         self.opt = opt
         self.direction = direction
         self.coeff = getattr(opt, "gradCoeff", 1)
         self._dots = opt.dotProduct_euclidean if euclidean else opt.dotProduct
+        self._with_slope = getattr(opt, "updateTryWithSlope", None) if euclidean else None
+        self._pending = None       # (t, slope, gradient) of the last trial
         self.n_values = 0
         self.n_slopes = 0
         self.last_grad = None
@@ -38,10 +45,19 @@ class _Probe:
 
     def value(self, t):
         self.n_values += 1
+        self._pending = None
+        if self._with_slope is not None and t != 0:
+            f, slope, grad = self._with_slope(self.direction, t, self.coeff)
+            self._pending = (t, slope, grad)
+            return f
         return self.opt.updateTry(self.direction, t)
 
     def slope(self, t):
         self.n_slopes += 1
+        if self._pending is not None and self._pending[0] == t:
+            _, slope, self.last_grad = self._pending
+            self._pending = None
+            return slope
         self.last_grad = self.opt.getGradient(self.coeff, update=[self.direction, t])
         return self.slope_of(self.last_grad)
 
@@ -56,21 +72,27 @@ def _next_trial(t, f_t, f0, slope0, lo, hi, decreased):
 
 
 def line_search_weak_wolfe(opt, pk, gfk=None, old_fval=None, old_old_fval=None, c1=0.001, c2=0.9,
-                           amax=None, maxiter=100, euclidean=True, t_init=1.):
+                           amax=None, maxiter=100, euclidean=True, t_init=1., slope0=None):
     """Returns (t or None, objective evaluations, slope evaluations, f(t),
     f(0), gradient at t or None) for the direction pk (amax and old_old_fval
-    are part of the reference's signature and unused)."""
+    are part of the reference's signature and unused).  slope0, when given,
+    is the slope at t = 0, -<gfk, pk>, computed by the caller, or a function
+    returning it; the function is called after the first trial's objective,
+    where the slope is first needed."""
     probe = _Probe(opt, pk, euclidean)
-    if gfk is None:
-        gfk = opt.getGradient(probe.coeff)
-    slope0 = probe.slope_of(gfk)
+    if slope0 is None:
+        if gfk is None:
+            gfk = opt.getGradient(probe.coeff)
+        slope0 = probe.slope_of(gfk)
     f0 = probe.value(0) if old_fval is None else old_fval
-    armijo = lambda t, f_t: f_t < f0 + c1 * t * slope0 * probe.coeff
 
     lo = hi = 0
     trials = 0
     t = t_init
     f_t = probe.value(t)
+    if callable(slope0):
+        slope0 = slope0()
+    armijo = lambda t, f_t: f_t < f0 + c1 * t * slope0 * probe.coeff
     while trials < MAX_TRIALS and probe.n_slopes < maxiter:
         decreased = armijo(t, f_t)
         if decreased:

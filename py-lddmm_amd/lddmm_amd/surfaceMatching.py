@@ -279,7 +279,8 @@ class SurfaceMatching:
             kw = sd._ap(K)
             from . import kernelFunctions_util as ku
             e = ((cr1 * ku.applyK(c1, c1, cr1, **kw)).sum() - 2 * (cr1 * ku.applyK(c1, c2, cr2, **kw)).sum())
-        return e / self.options['sigmaError'] ** 2
+        s2 = self.options['sigmaError'] ** 2
+        return e if s2 == 1 else e / s2
 
     def dataTerm(self, _fvDef, var=None):
         return float(self.dataTerm_device(B.to_dev(_fvDef.vertices)).item())
@@ -292,7 +293,8 @@ class SurfaceMatching:
             g = sd.measureNormGradient_device(self.fv0, self.fv1, K, vertices=xT)
         else:
             g = sd.currentNormGradient_device(self.fv0, self.fv1, K, vertices=xT)
-        return g / self.options['sigmaError'] ** 2
+        s2 = self.options['sigmaError'] ** 2
+        return g if s2 == 1 else g / s2
 
     def endPointGradient(self, endPoint=None):
         xT = self.state['xt'][-1] if endPoint is None else B.to_dev(endPoint.vertices)
@@ -335,15 +337,17 @@ class SurfaceMatching:
     def getVariable(self):
         return self.control
 
-    def updateTry(self, dr, eps, objRef=None):
-        """Objective at control - eps * dr (base/surfaceMatching.py:638-675)."""
+    def _trial(self, dr, eps):
+        """Control - eps * dr, its trajectory and its (deformation, data) terms
+        as device scalars; nothing is read back."""
         controlTry = Control()
         for k in dr.keys():
             if dr[k] is not None:
                 controlTry[k] = self.control[k] - eps * dr[k]
         obj_, stateTry = self.objectiveFunDef(controlTry, withTrajectory=True)
-        xt = stateTry['xt']
-        terms = self._host_terms(obj_, self.dataTerm_device(xt[-1]))
+        return controlTry, stateTry, obj_, self.dataTerm_device(stateTry['xt'][-1])
+
+    def _record_try(self, dr, eps, controlTry, stateTry, terms, objRef):
         objTry = self.obj0 + float(terms[0] + terms[1])
         if np.isnan(objTry):
             return 1e500
@@ -351,9 +355,32 @@ class SurfaceMatching:
             self.controlTry = controlTry
             self.stateTry = stateTry
             self.objTry = objTry
+            xt = stateTry['xt']
             self._data_try = (xt, xt._version, self.obj0 + float(terms[1]))
         self._last_try = (dr, eps, controlTry, stateTry)
         return objTry
+
+    def updateTry(self, dr, eps, objRef=None):
+        """Objective at control - eps * dr (base/surfaceMatching.py:638-675)."""
+        controlTry, stateTry, obj_, data = self._trial(dr, eps)
+        return self._record_try(dr, eps, controlTry, stateTry, self._host_terms(obj_, data), objRef)
+
+    def updateTryWithSlope(self, dr, eps, coeff=1.0):
+        """updateTry(dr, eps) with the gradient at the trial control and the
+        slope along dr, in one read-back.  The gradient's kernels are queued
+        behind the trial's before anything comes back to the host, and the
+        objective terms and the dot product return in one transfer.  Returns
+        (objective, slope, gradient) with the values of updateTry(dr, eps),
+        -dotProduct_euclidean(g, [dr])[0] and g = getGradient(coeff,
+        update=[dr, eps]).  The line search uses it to evaluate the slope of
+        a trial without a second round trip."""
+        controlTry, stateTry, obj_, data = self._trial(dr, eps)
+        self._last_try = (dr, eps, controlTry, stateTry)
+        grad = self.getGradient(coeff, update=[dr, eps])
+        dot = self._dot_table_device([grad], [dr])
+        host = torch.cat([obj_.reshape(1), data.reshape(1), dot.reshape(1)]).cpu()
+        objTry = self._record_try(dr, eps, controlTry, stateTry, host[:2], None)
+        return objTry, -float(host[2]), grad
 
     @staticmethod
     def _host_terms(obj, data):
@@ -396,7 +423,7 @@ class SurfaceMatching:
         g, _ = evol.shoot_adjoint(xt, control['at'], px1, self.options['KparDiff'], self.options['regWeight'],
                                   euclidean=self.euclideanGradient, icost=self.icost)
         grd = Control()
-        grd['at'] = g / coeff
+        grd['at'] = g if coeff == 1 else g / coeff
         return grd
 
     def dotProduct_euclidean(self, g1, g2):
@@ -422,6 +449,39 @@ class SurfaceMatching:
                                                B.stream_handle()), "lbfgs_multidot")
             res += out.cpu().numpy()
         return res
+
+    def _dot_table_device(self, probes, targets):
+        """[len(probes), len(targets)] device table of the Euclidean dot
+        products <p, t>, summed over the control keys; each entry is the same
+        deterministic reduction that dotProduct_euclidean reads back
+        (lddmm_lbfgs_multidot computes every entry independently of the
+        others).  At most 2 probes."""
+        import ctypes
+        PP = lambda ts: (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+        tab = None
+        for k in probes[0].keys():
+            if probes[0][k] is None:
+                continue
+            ps = [p[k].contiguous() for p in probes]
+            ts = [t[k].contiguous() for t in targets]
+            if ps[0].numel() == 0:
+                continue
+            out = torch.empty((len(ps), len(ts)), dtype=torch.float64, device=ps[0].device)
+            B.check(B.LIB.lddmm_lbfgs_multidot(PP(ps), len(ps), PP(ts), len(ts), ps[0].numel(), B.ptr(out),
+                                               B.stream_handle()), "lbfgs_multidot")
+            tab = out if tab is None else tab + out
+        if tab is None:
+            tab = torch.zeros((len(probes), len(targets)), dtype=torch.float64, device=B.device())
+        return tab
+
+    def dotTable(self, probes, targets, later=False):
+        """Table [len(probes), len(targets)] of dotProduct_euclidean values
+        (numpy), read back in one transfer; with later=True the products are
+        launched now and a function is returned that reads the table."""
+        tab = self._dot_table_device(probes, targets)
+        if later:
+            return lambda: tab.cpu().numpy()
+        return tab.cpu().numpy()
 
     def twoLoop(self, grd, storedGrad):
         """L-BFGS direction of bfgs.bfgs (base/bfgs.py:188-207) on the device.

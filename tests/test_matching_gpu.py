@@ -64,3 +64,30 @@ def test_short_registration_decreases(env):
     # the final objective is the one of the final control
     sm.obj = None
     assert abs(sm.objectiveFun() - h[-1]) <= 1e-9 * abs(h[-1])
+
+
+@pytest.mark.parametrize("pk_dtype", ["float64", "float32"])
+def test_fused_trial_slope_keeps_the_iterates(env, pk_dtype, monkeypatch):
+    """The line search's fused trial (objective, gradient and slope in one
+    read-back: updateTryWithSlope) and BFGS's batched dot table (dotTable)
+    change the number of host synchronisations only: the same registration
+    without them takes bit-identical iterates (objective history, line-search
+    decisions and final control)."""
+    synth, SurfaceMatching, Surface, Kernel, torch = env
+    (v, f), (v2, _) = synth.plain_matching_pair(seed=3, subdiv=3, radius=10.0, nbumps=20, width=3.0)
+
+    def run():
+        opts = {'KparDiff': Kernel(name='gauss', sigma=2.5), 'KparDist': Kernel(name='gauss', sigma=1.0),
+                'errorType': 'varifold', 'pk_dtype': pk_dtype, 'maxIter': 25, 'mode': 'quiet'}
+        sm = SurfaceMatching(Template=Surface((f, v)), Target=Surface((f, v2)), options=opts)
+        sm.optimizeMatching()
+        return sm.bfgs_history, sm.bfgs_decisions, sm.control['at'].cpu().numpy()
+
+    h1, d1, a1 = run()
+    monkeypatch.setattr(SurfaceMatching, "updateTryWithSlope", None)
+    monkeypatch.setattr(SurfaceMatching, "dotTable", None)
+    h2, d2, a2 = run()
+    assert len(h1) >= 10
+    assert h1 == h2
+    assert d1 == d2
+    np.testing.assert_array_equal(a1, a2)
