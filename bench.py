@@ -109,6 +109,27 @@ def varifold_microbench(B, torch, synth, np):
         res[pr] = {"seconds": dt, "face_pairs_per_s": pairs / dt, "varifold_norm": vals[pr]}
         log(f"varifold 100k ({pr}): {dt * 1e3:.1f} ms, {pairs / dt:.3g} face-pairs/s, norm {vals[pr]:.6g}")
     res["fp32_rel_err_vs_fp64"] = abs(vals["float32"] - vals["float64"]) / abs(vals["float64"])
+    # A/B: the float32 energy with the cross terms on the matrix cores
+    kd = B.kernel_desc([1.0], -1, "float32")
+    res["mfma_ab"] = {}
+    for engine in ("mfma_cos", "mfma_cos_r2"):
+        B.set_varifold_engine(engine)
+        try:
+            sd.varifold_device(kd, S1, S2)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            eD, _ = sd.varifold_device(kd, S1, S2)
+            e0, _ = sd.varifold_device(kd, S2)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+        finally:
+            B.set_varifold_engine("valu")
+        val = float(eD.item()) + float(e0.item())
+        res["mfma_ab"][engine] = {"seconds": dt, "face_pairs_per_s": pairs / dt,
+                                  "speedup_vs_valu": res["float32"]["seconds"] / dt,
+                                  "rel_err_vs_fp64": abs(val - vals["float64"]) / abs(vals["float64"])}
+        log(f"varifold 100k ({engine}): {dt * 1e3:.1f} ms, {pairs / dt:.3g} face-pairs/s, "
+            f"rel err vs fp64 {res['mfma_ab'][engine]['rel_err_vs_fp64']:.2e}")
     return res
 
 

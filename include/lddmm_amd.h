@@ -119,6 +119,18 @@ int lddmm_profile_read(double* ms_each, int64_t cap, int64_t* launches);
 #define LDDMM_ENGINE_MFMA 1
 int lddmm_set_pair_engine(int32_t engine);
 
+/* Engine of the float32 single-scale Gaussian varifold energy sums (no
+ * counterpart in the reference): LDDMM_VARIFOLD_VALU (default) evaluates each
+ * face pair on the vector ALUs; LDDMM_VARIFOLD_MFMA_COS takes the 16x16 blocks
+ * of normal cosines n_i.n_j from v_mfma_f32_16x16x4f32; LDDMM_VARIFOLD_MFMA_COS_R2
+ * also the squared centre distances (relative to an origin per block of 16
+ * target faces).  The gradient and the float64 path stay on the VALU.
+ * Process-wide setting. */
+#define LDDMM_VARIFOLD_VALU 0
+#define LDDMM_VARIFOLD_MFMA_COS 1
+#define LDDMM_VARIFOLD_MFMA_COS_R2 2
+int lddmm_set_varifold_engine(int32_t engine);
+
 /* out[n,k] = sum_s w_s sum_j K_s(y_i, x_j) a[j,:]
  * Replaces ku.applyK / applyK_numba / applyK_pykeops
  * (base/kernelFunctions_util.py:443-466, :499-535), reached from

@@ -2,8 +2,11 @@
 // varifold energy and its fused gradient, and the deterministic face ->
 // vertex gather (py-lddmm base/surfaceDistances.py:286-468, :87-139).
 #include "varifold.hpp"
+#include "varifold_mfma.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <type_traits>
 #include <cmath>
 
 namespace lddmm {
@@ -139,6 +142,8 @@ __global__ void k_face_gather(const double* __restrict__ x, const int32_t* __res
 
 namespace {
 
+std::atomic<int> g_vf_engine{LDDMM_VARIFOLD_VALU};
+
 int check_surface(const lddmm_surface* s, bool need_adj, const char* what) {
     LDDMM_REQUIRE(s != nullptr, "%s: null surface", what);
     LDDMM_REQUIRE(s->nv >= 0 && s->nf >= 0, "%s: negative sizes", what);
@@ -206,6 +211,10 @@ int varifold_core(const lddmm_kernel* kd, int n1, int n2, bool has_tgt, VfShape 
     const double fK = 1.0;
     const double fD = MS ? -1.0 : ss.diff;
     const int R1 = n1 >= 8192 && !MS ? 2 : 1;
+    // matrix-core energy engine: float32, single-scale Gaussian only
+    int vf_engine = LDDMM_VARIFOLD_VALU;
+    if constexpr (std::is_same<T, float>::value && !MS)
+        if (kd->kind == LDDMM_KERNEL_GAUSS) vf_engine = g_vf_engine.load();
 
     PairPlan plan;
     size_t need = 256 + size_t(n1 + n2) * sizeof(FaceRec) + 256;
@@ -221,8 +230,14 @@ int varifold_core(const lddmm_kernel* kd, int n1, int n2, bool has_tgt, VfShape 
         plan.nchunk = c1 + c2;
         need += partial_bytes<T>(plan, 8, n1) + size_t(n1) * (7 * sizeof(double)) + 1024;
     } else {
-        if (R1 == 2) LDDMM_TRY((plan_for<T, OpVarifoldE<MS>, 2>(n1, n1 + n2, &plan)));
-        else LDDMM_TRY((plan_for<T, OpVarifoldE<MS>, 1>(n1, n1 + n2, &plan)));
+        if (vf_engine) {
+            if (vf_engine == LDDMM_VARIFOLD_MFMA_COS_R2) LDDMM_TRY((plan_vf_energy_mfma<2, true>(n1, n1 + n2, &plan)));
+            else LDDMM_TRY((plan_vf_energy_mfma<2, false>(n1, n1 + n2, &plan)));
+        } else if (R1 == 2) {
+            LDDMM_TRY((plan_for<T, OpVarifoldE<MS>, 2>(n1, n1 + n2, &plan)));
+        } else {
+            LDDMM_TRY((plan_for<T, OpVarifoldE<MS>, 1>(n1, n1 + n2, &plan)));
+        }
         need += partial_bytes<T>(plan, 1, n1) + size_t(n1) * sizeof(double) + 256;
     }
     void* ws;
@@ -240,8 +255,16 @@ int varifold_core(const lddmm_kernel* kd, int n1, int n2, bool has_tgt, VfShape 
         typename OpVarifoldE<MS>::template Args<T> A;
         A.f1 = r1; A.f2 = r2; A.n1 = n1; A.n2 = n2; A.w2 = -2.0; A.c1 = 0; A.c0 = c0; A.sd = shp.sd; A.sc = sc;
         Partials<T> part = carve_partials<T>(base, cv, plan, 1, n1);
-        if (R1 == 2) LDDMM_TRY((run_pair<T, OpVarifoldE<MS>, 2>(A, n1, n1 + n2, plan, part, st)));
-        else LDDMM_TRY((run_pair<T, OpVarifoldE<MS>, 1>(A, n1, n1 + n2, plan, part, st)));
+        if constexpr (std::is_same<T, float>::value && !MS) {
+            if (vf_engine == LDDMM_VARIFOLD_MFMA_COS_R2)
+                LDDMM_TRY((run_vf_energy_mfma<2, true>(A, n1, n1 + n2, plan, part, st)));
+            else if (vf_engine == LDDMM_VARIFOLD_MFMA_COS)
+                LDDMM_TRY((run_vf_energy_mfma<2, false>(A, n1, n1 + n2, plan, part, st)));
+        }
+        if (!vf_engine) {
+            if (R1 == 2) LDDMM_TRY((run_pair<T, OpVarifoldE<MS>, 2>(A, n1, n1 + n2, plan, part, st)));
+            else LDDMM_TRY((run_pair<T, OpVarifoldE<MS>, 1>(A, n1, n1 + n2, plan, part, st)));
+        }
         double* rows = reinterpret_cast<double*>(base + cv.take(size_t(n1) * sizeof(double)));
         hipLaunchKernelGGL((k_vf_energy_rows<T>), dim3(unsigned((n1 + kEpiRows - 1) / kEpiRows)), dim3(256), 0, st,
                            part, plan.nchunk, r1, n1, fK, rows);
@@ -391,6 +414,15 @@ int curve_varifold_t(const lddmm_kernel* kd, const lddmm_curve* c1, const lddmm_
 }  // namespace lddmm
 
 using namespace lddmm;
+
+extern "C" int lddmm_set_varifold_engine(int32_t engine) {
+    if (engine != LDDMM_VARIFOLD_VALU && engine != LDDMM_VARIFOLD_MFMA_COS && engine != LDDMM_VARIFOLD_MFMA_COS_R2) {
+        lddmm::set_error("unknown varifold engine %d", engine);
+        return LDDMM_EINVAL;
+    }
+    lddmm::g_vf_engine.store(engine);
+    return LDDMM_OK;
+}
 
 extern "C" int lddmm_surface_features(const lddmm_surface* s, double* centers, double* wsurfel,
                                       double* face_weights, void* stream) {

@@ -127,6 +127,7 @@ SIGNATURES = {
     "lddmm_profile_read": [_P, _I64, _P],
     "lddmm_device_count": [],
     "lddmm_set_pair_engine": [_I32],
+    "lddmm_set_varifold_engine": [_I32],
     "lddmm_nn_linear": [_P, _I64, _I32, _P, _I32, _P, _P, _P, _I32, _I32, _P, _P],
     "lddmm_nn_relu_bwd": [_P, _P, _P, _I64, _P],
     "lddmm_nn_wgrad": [_P, _P, _I64, _I32, _I32, _P, _P],
@@ -235,6 +236,18 @@ def set_pair_engine(engine):
     """Accumulation engine of the float32 applyK / plain applyDiffKT sums."""
     code = {'valu': ENGINE_VALU, 'mfma': ENGINE_MFMA}.get(engine, engine)
     check(LIB.lddmm_set_pair_engine(int(code)), "set_pair_engine")
+
+
+VARIFOLD_VALU, VARIFOLD_MFMA_COS, VARIFOLD_MFMA_COS_R2 = 0, 1, 2
+
+
+def set_varifold_engine(engine):
+    """Engine of the float32 single-scale varifold energy: 'valu' (default),
+    'mfma_cos' (normal cosines on the matrix cores) or 'mfma_cos_r2' (cosines
+    and squared distances)."""
+    code = {'valu': VARIFOLD_VALU, 'mfma_cos': VARIFOLD_MFMA_COS,
+            'mfma_cos_r2': VARIFOLD_MFMA_COS_R2}.get(engine, engine)
+    check(LIB.lddmm_set_varifold_engine(int(code)), "set_varifold_engine")
 
 
 def profile_enable(on=True):
