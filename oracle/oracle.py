@@ -389,6 +389,34 @@ def affineApplyDiffKT(x, p, a, kaff, regweight=1., sym=False, lddmm=False):
     return z
 
 
+def affineApplyDDiffK11and12(x, a, p, kaff):
+    """Affine part of Kernel.applyDDiffK11and12(x, n, a, p) (base/kernelFunctions.py:665-674;
+    n does not enter it)."""
+    kind, w1, w2, c = kaff
+    a, p = _as2d(a), _as2d(p)
+    if kind == 'affine':
+        return w1 * (a @ (a.T @ p))
+    z = np.zeros(a.shape)
+    for E in _euclid_basis(a.shape[1]):
+        ETa = a @ E
+        z += w1 * (p * ETa).sum() * ETa
+    return z
+
+
+def affineApplyDiffK1and2(x, a1, a2, kaff):
+    """Affine part of Kernel.applyDiffK1and2(x, a1, a2) (base/kernelFunctions.py:549-561)."""
+    kind, w1, w2, c = kaff
+    x, a1, a2 = _f64(x), _as2d(a1), _as2d(a2)
+    xx = x - c
+    if kind == 'affine':
+        return w1 * (a1 @ (xx.T @ a2) + (xx @ (a1.T @ a2)))
+    z = np.zeros(x.shape)
+    for E in _euclid_basis(x.shape[1]):
+        Exx, Ea1, ETa2 = xx @ E.T, a1 @ E.T, a2 @ E
+        z += w1 * ((xx * ETa2).sum() * Ea1 + (a1 * ETa2).sum() * Exx)
+    return z
+
+
 def affineApplyDiv(x, a, kaff):
     """Affine part of Kernel.applyDivergence(x, a) (base/kernelFunctions.py:653-659),
     the same at every target: w1 sum_j (x_j - c).a_j for 'affine'; the
@@ -1175,8 +1203,10 @@ def landmarkEPDiff(T, x0, a0, sigma, KP=-1, kernel=GAUSS):
     return xt, at
 
 
-def secondOrderEvolution(x0, a0, sigma, timeStep, KP=-1, kernel=GAUSS):
-    """base/pointEvolution.py:385-472 (no spline, no affine, no point set)."""
+def secondOrderEvolution(x0, a0, sigma, timeStep, KP=-1, kernel=GAUSS, kaff=None):
+    """base/pointEvolution.py:385-472 (no spline, no affine control, no point
+    set); kaff = the kernel's affine part, which Kernel.applyK and
+    Kernel.applyDiffKT add."""
     x0, a0 = _f64(x0), _f64(a0)
     T = int(np.round(1 / timeStep))
     xt = np.zeros((T + 1,) + x0.shape)
@@ -1186,6 +1216,9 @@ def secondOrderEvolution(x0, a0, sigma, timeStep, KP=-1, kernel=GAUSS):
         x, a = xt[k], at[k]
         zx = applyK(x, x, a, sigma, KP, kernel=kernel)
         za = -applyDiffKT(x, x, a, a, sigma, KP=KP, kernel=kernel)
+        if kaff is not None:
+            zx = zx + affineApplyK(x, x, a, kaff)
+            za = za - affineApplyDiffKT(x, a, a, kaff)
         xt[k + 1] = x + timeStep * zx
         at[k + 1] = a + timeStep * za
     return xt, at
@@ -1198,10 +1231,11 @@ def secondOrderHamiltonian(x, a, px, pa, sigma, KP=-1, kernel=GAUSS):
     return Ht
 
 
-def secondOrderCovector(x0, a0, px1, pa1, sigma, timeStep, KP=-1, kernel=GAUSS):
-    """base/pointEvolution.py:489-552 (single target, no affine)."""
+def secondOrderCovector(x0, a0, px1, pa1, sigma, timeStep, KP=-1, kernel=GAUSS, kaff=None):
+    """base/pointEvolution.py:489-552 (single target, no affine control; kaff
+    = the kernel's affine part, added by the Kernel methods the loop calls)."""
     T = int(np.round(1 / timeStep))
-    xt, at = secondOrderEvolution(x0, a0, sigma, timeStep, KP, kernel=kernel)
+    xt, at = secondOrderEvolution(x0, a0, sigma, timeStep, KP, kernel=kernel, kaff=kaff)
     pxt = np.zeros(xt.shape)
     pat = np.zeros(xt.shape)
     pxt[T], pat[T] = px1, pa1
@@ -1213,20 +1247,27 @@ def secondOrderCovector(x0, a0, px1, pa1, sigma, timeStep, KP=-1, kernel=GAUSS):
         # Kernel.applyDiffK1and2 -> applykdiff1and2, which forces KP = -1
         zpa = (applyK(x, x, px, sigma, KP, kernel=kernel)
                - applyDiffK(x, pa, a, sigma, KP=-1, option='1and2', kernel=kernel))
+        if kaff is not None:
+            zpx = zpx + affineApplyDiffKT(x, px, a, kaff, sym=True) - affineApplyDDiffK11and12(x, a, pa, kaff)
+            zpa = zpa + affineApplyK(x, x, px, kaff) - affineApplyDiffK1and2(x, pa, a, kaff)
         pxt[T - t - 1] = px + timeStep * zpx
         pat[T - t - 1] = pa + timeStep * zpa
     return pxt, pat, xt, at
 
 
-def secondOrderGradient(x0, a0, px1, pa1, sigma, timeStep, KP=-1, kernel=GAUSS):
-    """da0 = K(x0) a0 - pa_0; base/pointEvolution.py:555-618 (no affine)."""
-    pxt, pat, xt, at = secondOrderCovector(x0, a0, px1, pa1, sigma, timeStep, KP, kernel=kernel)
-    da0 = applyK(x0, x0, a0, sigma, KP, kernel=kernel) - pat[0]
+def secondOrderGradient(x0, a0, px1, pa1, sigma, timeStep, KP=-1, kernel=GAUSS, kaff=None):
+    """da0 = K(x0) a0 - pa_0; base/pointEvolution.py:555-618 (no affine
+    control; K with the kernel's affine part kaff)."""
+    pxt, pat, xt, at = secondOrderCovector(x0, a0, px1, pa1, sigma, timeStep, KP, kernel=kernel, kaff=kaff)
+    Ka = applyK(x0, x0, a0, sigma, KP, kernel=kernel)
+    if kaff is not None:
+        Ka = Ka + affineApplyK(x0, x0, a0, kaff)
+    da0 = Ka - pat[0]
     return da0, xt, at, pxt, pat
 
 
 def second_order_matching_objective(x0, w0, a0, target, sigma_diff, sigma_dist, timeStep, kind='point_measure',
-                                    sigma_error=1., kernel_diff=GAUSS):
+                                    sigma_error=1., kernel_diff=GAUSS, kaff=None):
     """Second-order (geodesic) point-set matching objective at the initial
     momentum a0: 1/2 a0.K(x0)a0 + (obj0 + dataTerm(x_T)) / sigma_err^2, with
     x_T the endpoint of secondOrderEvolution
@@ -1239,8 +1280,11 @@ def second_order_matching_objective(x0, w0, a0, target, sigma_diff, sigma_dist, 
     base/secondOrderSurfaceMatching.py (SurfaceMatching's data term over the
     second-order control)."""
     a0 = _f64(a0)
-    xt, at = secondOrderEvolution(x0, a0, sigma_diff, timeStep, kernel=kernel_diff)
-    obj = 0.5 * (a0 * applyK(x0, x0, a0, sigma_diff, kernel=kernel_diff)).sum()
+    xt, at = secondOrderEvolution(x0, a0, sigma_diff, timeStep, kernel=kernel_diff, kaff=kaff)
+    Ka = applyK(x0, x0, a0, sigma_diff, kernel=kernel_diff)
+    if kaff is not None:
+        Ka = Ka + affineApplyK(x0, x0, a0, kaff)
+    obj = 0.5 * (a0 * Ka).sum()
     if kind == 'L2':
         y = _f64(target)
         dterm, obj0 = -2 * (xt[-1] * y).sum() + (xt[-1] ** 2).sum(), (y ** 2).sum()
@@ -1250,14 +1294,15 @@ def second_order_matching_objective(x0, w0, a0, target, sigma_diff, sigma_dist, 
 
 
 def second_order_matching_gradient(x0, w0, a0, target, sigma_diff, sigma_dist, timeStep, kind='point_measure',
-                                   sigma_error=1., kernel_diff=GAUSS):
+                                   sigma_error=1., kernel_diff=GAUSS, kaff=None):
     """grad['a0'] = da0 of secondOrderGradient with px1 = -dataTerm'(x_T) /
     sigma_err^2 and pa1 = 0 (base/secondOrderPointSetMatching.py:379-460)."""
     a0 = _f64(a0)
-    xt, at = secondOrderEvolution(x0, a0, sigma_diff, timeStep, kernel=kernel_diff)
+    xt, at = secondOrderEvolution(x0, a0, sigma_diff, timeStep, kernel=kernel_diff, kaff=kaff)
     if kind == 'L2':
         g = 2 * (xt[-1] - _f64(target))
     else:
         g = _data_grad(kind, (xt[-1], w0), target, sigma_dist)
     px1 = -g / sigma_error ** 2
-    return secondOrderGradient(x0, a0, px1, np.zeros_like(a0), sigma_diff, timeStep, kernel=kernel_diff)[0]
+    return secondOrderGradient(x0, a0, px1, np.zeros_like(a0), sigma_diff, timeStep, kernel=kernel_diff,
+                               kaff=kaff)[0]

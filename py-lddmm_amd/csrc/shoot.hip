@@ -217,6 +217,139 @@ int aff_apply(const AffineArgs& A, int n, int dim, int mode, double r, const dou
     return LDDMM_OK;
 }
 
+// Affine part of a geodesic (EPDiff) step: Kernel.applyK, applyDiffKT,
+// applyDDiffK11and12 and applyDiffK1and2 each add theirs
+// (base/kernelFunctions.py:476-505, :549-561, :601-625, :665-674).  With
+// M(u) = sum_j xx_j u_j^T, S(u) = sum_j u_j, N = sum_j a_j pa_j^T and f as
+// above (a row vector r times a matrix B: (r B)[q] = sum_p r[p] B[p][q]):
+//   forward:  x_{k+1} += dt (xx_i w1 f(M(a)) + w2 S(a))
+//             a_{k+1} -= dt w1 a_i f(M(a))^T
+//   adjoint:  px_k += dt w1 (px_i f(M(a))^T + a_i f(M(px))^T - a_i f(N))
+//             pa_k += dt (w1 xx_i f(M(px)) + w2 S(px) - w1 (pa_i f(M(a)) + xx_i f(N)^T))
+// ('euclidean': each of the reference's sums <E, X> E over the orthonormal
+// antisymmetric basis is the projection f(X)).  Moments: a fixed grid for a
+// given n, per-lane sums, in-wave shuffles, waves and then workgroups added
+// in order (deterministic).
+constexpr int kEpdAffVals = 33;   // M(a) 9, S(a) 3, M(px) 9, S(px) 3, N 9
+
+__global__ void __launch_bounds__(256)
+k_epd_aff_partial(int n, int dim, const double* __restrict__ x, AffineArgs A, const double* __restrict__ a,
+                  const double* __restrict__ px, const double* __restrict__ pa, double* __restrict__ part) {
+    __shared__ double red[4][kEpdAffVals];
+    double acc[kEpdAffVals];
+#pragma unroll
+    for (int v = 0; v < kEpdAffVals; ++v) acc[v] = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        double xx[3], aa[3], pp[3], qq[3];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const bool in = p < dim;
+            const size_t o = size_t(i) * dim + p;
+            xx[p] = in ? x[o] - A.c[p] : 0.0;
+            aa[p] = in ? a[o] : 0.0;
+            pp[p] = in && px ? px[o] : 0.0;
+            qq[p] = in && pa ? pa[o] : 0.0;
+        }
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                acc[p * 3 + q] = fma(xx[p], aa[q], acc[p * 3 + q]);
+                acc[12 + p * 3 + q] = fma(xx[p], pp[q], acc[12 + p * 3 + q]);
+                acc[24 + p * 3 + q] = fma(aa[p], qq[q], acc[24 + p * 3 + q]);
+            }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            acc[9 + q] += aa[q];
+            acc[21 + q] += pp[q];
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int v = 0; v < kEpdAffVals; ++v) {
+        double s = acc[v];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) red[wave][v] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < kEpdAffVals) {
+        const int v = threadIdx.x;
+        part[size_t(blockIdx.x) * kEpdAffVals + v] = ((red[0][v] + red[1][v]) + red[2][v]) + red[3][v];
+    }
+}
+
+__global__ void __launch_bounds__(64)
+k_epd_aff_final(const double* __restrict__ part, int nblocks, double* __restrict__ mom) {
+    const int v = threadIdx.x;
+    if (v >= kEpdAffVals) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += part[size_t(b) * kEpdAffVals + v];
+    mom[v] = s;
+}
+
+// mode 0: forward step (x, a -> out1 = x_{k+1}, out2 = a_{k+1}); mode 1:
+// adjoint step (x, a, px = px_{k+1}, pa = pa_{k+1} -> out1 = px_k, out2 = pa_k).
+__global__ void __launch_bounds__(256)
+k_epd_aff_apply(int n, int dim, int mode, AffineArgs A, double dt, const double* __restrict__ mom,
+                const double* __restrict__ x, const double* __restrict__ a, const double* __restrict__ px,
+                const double* __restrict__ pa, double* __restrict__ out1, double* __restrict__ out2) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double Fa[9], Fp[9], Fn[9], xx[3] = {0, 0, 0}, aa[3] = {0, 0, 0}, pp[3] = {0, 0, 0}, qq[3] = {0, 0, 0};
+    aff_f(mom, A.kind, false, Fa);
+    for (int p = 0; p < dim; ++p) {
+        const size_t o = size_t(i) * dim + p;
+        xx[p] = x[o] - A.c[p];
+        aa[p] = a[o];
+        if (mode == 1) {
+            pp[p] = px[o];
+            qq[p] = pa[o];
+        }
+    }
+    if (mode == 0) {
+        for (int q = 0; q < dim; ++q) {
+            double zx = A.w2 * mom[9 + q], za = 0.0;
+            for (int p = 0; p < 3; ++p) {
+                zx = fma(A.w1 * xx[p], Fa[p * 3 + q], zx);
+                za = fma(A.w1 * aa[p], Fa[q * 3 + p], za);
+            }
+            out1[size_t(i) * dim + q] += dt * zx;
+            out2[size_t(i) * dim + q] += -dt * za;
+        }
+        return;
+    }
+    aff_f(mom + 12, A.kind, false, Fp);
+    aff_f(mom + 24, A.kind, false, Fn);
+    for (int q = 0; q < dim; ++q) {
+        double zpx = 0.0, zpa = A.w2 * mom[21 + q];
+        for (int p = 0; p < 3; ++p) {
+            zpx = fma(A.w1, pp[p] * Fa[q * 3 + p] + aa[p] * Fp[q * 3 + p] - aa[p] * Fn[p * 3 + q], zpx);
+            zpa = fma(A.w1, xx[p] * Fp[p * 3 + q] - qq[p] * Fa[p * 3 + q] - xx[p] * Fn[q * 3 + p], zpa);
+        }
+        out1[size_t(i) * dim + q] += dt * zpx;
+        out2[size_t(i) * dim + q] += dt * zpa;
+    }
+}
+
+// Workspace of the geodesic affine part: [kAffGrid * kEpdAffVals partials][kEpdAffVals moments].
+constexpr size_t kEpdAffBytes = size_t(kAffGrid + 1) * kEpdAffVals * sizeof(double);
+
+int epd_aff_step(const AffineArgs& A, int n, int dim, int mode, double dt, const double* x, const double* a,
+                 const double* px, const double* pa, double* out1, double* out2, double* ws, hipStream_t st) {
+    const int grid = std::max(1, std::min(kAffGrid, (n + 255) / 256));
+    double* mom = ws + size_t(kAffGrid) * kEpdAffVals;
+    hipLaunchKernelGGL(k_epd_aff_partial, dim3(unsigned(grid)), dim3(256), 0, st, n, dim, x, A, a,
+                       mode == 1 ? px : nullptr, mode == 1 ? pa : nullptr, ws);
+    LDDMM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_epd_aff_final, dim3(1), dim3(64), 0, st, ws, grid, mom);
+    LDDMM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_epd_aff_apply, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, n, dim, mode, A, dt, mom,
+                       x, a, px, pa, out1, out2);
+    LDDMM_CHECK_LAUNCH();
+    return LDDMM_OK;
+}
+
 int axpby(size_t count, double alpha, const double* a, double beta, const double* b, double* out,
           hipStream_t st) {
     if (count == 0) return LDDMM_OK;
@@ -574,46 +707,48 @@ int epdiff_forward_t(const lddmm_kernel* kd, int T_, int n, double* xt, double* 
     LDDMM_TRY(plan_job(jk, n, n, !MS));
     LDDMM_TRY(plan_job(jd, n, n, !MS));
     if (!MS) LDDMM_TRY(plan_job(jf, n, n, true));
+    const bool aff = kd->affine != LDDMM_AFFINE_NONE;
+    const AffineArgs AA = affine_args(kd);
     void* ws;
-    LDDMM_TRY(workspace(256 + std::max({partial_bytes<T>(jk.plan, KW, n), partial_bytes<T>(jd.plan, 3, n),
-                                        MS ? size_t(0) : partial_bytes<T>(jf.plan, KW + 3, n)}) + 512,
+    LDDMM_TRY(workspace(256 + (aff ? kEpdAffBytes + 256 : 0) +
+                            std::max({partial_bytes<T>(jk.plan, KW, n), partial_bytes<T>(jd.plan, 3, n),
+                                      MS ? size_t(0) : partial_bytes<T>(jf.plan, KW + 3, n)}) + 512,
                         st, &ws));
     char* base = static_cast<char*>(ws);
     Carve cv;
     double* c0 = reinterpret_cast<double*>(base + cv.take(32));
+    double* aws = aff ? reinterpret_cast<double*>(base + cv.take(kEpdAffBytes)) : nullptr;
     size_t poff = cv.off;
     const size_t stride = size_t(n) * KW;
     const double dt = 1.0 / T_;
     LDDMM_TRY(launch_center(xt, n, xt, 0, KW, c0, st));       // one centre (x_0) for all steps
-    if constexpr (!MS) {
-        // one Gaussian scale: both sums of a step in one pass (OpEPDiffFwd) + one reduce
-        for (int k = 0; k < T_; ++k) {
-            const double* x = xt + size_t(k) * stride;
-            const double* a = at + size_t(k) * stride;
+    for (int k = 0; k < T_; ++k) {
+        const double* x = xt + size_t(k) * stride;
+        const double* a = at + size_t(k) * stride;
+        double* xn = xt + size_t(k + 1) * stride;
+        double* an = at + size_t(k + 1) * stride;
+        if constexpr (!MS) {
+            // one Gaussian scale: both sums of a step in one pass (OpEPDiffFwd) + one reduce
             Carve c1; c1.off = poff;
             Partials<T> pf = carve_partials<T>(base, c1, jf.plan, KW + 3, n);
             jf.args = {x, a, c0, sc};
             LDDMM_TRY(run_job(jf, n, n, pf, st));
             hipLaunchKernelGGL((k_reduce_epdiff_fwd<T, KW>), dim3(unsigned((n + kEpiRows - 1) / kEpiRows), 2 * KW),
-                               dim3(256), 0, st, pf, jf.plan.nchunk, n, dt, fdiff, x, a, xt + size_t(k + 1) * stride,
-                               at + size_t(k + 1) * stride);
+                               dim3(256), 0, st, pf, jf.plan.nchunk, n, dt, fdiff, x, a, xn, an);
             LDDMM_CHECK_LAUNCH();
+        } else {
+            Carve c1; c1.off = poff;
+            Partials<T> pk = carve_partials<T>(base, c1, jk.plan, KW, n);
+            jk.args = {x, x, a, c0, KW, KW, 0, sc};
+            LDDMM_TRY(run_job(jk, n, n, pk, st));
+            LDDMM_TRY(reduce_job(jk, pk, n, KW, 1.0, xn, 0, x, dt, st));
+            Carve c2; c2.off = poff;
+            Partials<T> pd = carve_partials<T>(base, c2, jd.plan, 3, n);
+            jd.args = {x, x, a, a, c0, KW, KW, 0, 0.0, sc};
+            LDDMM_TRY(run_job(jd, n, n, pd, st));
+            LDDMM_TRY(reduce_job(jd, pd, n, KW, fdiff, an, 0, a, -dt, st));
         }
-        return LDDMM_OK;
-    }
-    for (int k = 0; k < T_; ++k) {
-        const double* x = xt + size_t(k) * stride;
-        const double* a = at + size_t(k) * stride;
-        Carve c1; c1.off = poff;
-        Partials<T> pk = carve_partials<T>(base, c1, jk.plan, KW, n);
-        jk.args = {x, x, a, c0, KW, KW, 0, sc};
-        LDDMM_TRY(run_job(jk, n, n, pk, st));
-        LDDMM_TRY(reduce_job(jk, pk, n, KW, 1.0, xt + size_t(k + 1) * stride, 0, x, dt, st));
-        Carve c2; c2.off = poff;
-        Partials<T> pd = carve_partials<T>(base, c2, jd.plan, 3, n);
-        jd.args = {x, x, a, a, c0, KW, KW, 0, 0.0, sc};
-        LDDMM_TRY(run_job(jd, n, n, pd, st));
-        LDDMM_TRY(reduce_job(jd, pd, n, KW, fdiff, at + size_t(k + 1) * stride, 0, a, -dt, st));
+        if (aff) LDDMM_TRY(epd_aff_step(AA, n, KW, 0, dt, x, a, nullptr, nullptr, xn, an, aws, st));
     }
     return LDDMM_OK;
 }
@@ -647,11 +782,14 @@ int epdiff_adjoint_t(const lddmm_kernel* kd, int T_, int n, const double* xt, co
     size_t pb = std::max(std::max(partial_bytes<T>(j1.plan, 3, n), partial_bytes<T>(j2.plan, 3, n)),
                          std::max(partial_bytes<T>(j3.plan, KW, n), partial_bytes<T>(j4.plan, KW, n)));
     if (!MS) pb = std::max(pb, partial_bytes<T>(ja.plan, 6 + 2 * KW, n));
+    const bool aff = kd->affine != LDDMM_AFFINE_NONE;
+    const AffineArgs AA = affine_args(kd);
     void* ws;
-    LDDMM_TRY(workspace(256 + pb + 512, st, &ws));
+    LDDMM_TRY(workspace(256 + (aff ? kEpdAffBytes + 256 : 0) + pb + 512, st, &ws));
     char* base = static_cast<char*>(ws);
     Carve cv;
     double* c0 = reinterpret_cast<double*>(base + cv.take(32));
+    double* aws = aff ? reinterpret_cast<double*>(base + cv.take(kEpdAffBytes)) : nullptr;
     size_t poff = cv.off;
     const size_t stride = size_t(n) * KW;
     const double dt = 1.0 / T_;
@@ -677,6 +815,7 @@ int epdiff_adjoint_t(const lddmm_kernel* kd, int T_, int n, const double* xt, co
             hipLaunchKernelGGL((k_reduce_epdiff_adj<T, KW>), dim3(unsigned((n + kEpiRows - 1) / kEpiRows), 2 * KW),
                                dim3(256), 0, st, pa4, ja.plan.nchunk, n, dt, fdiff, fdd, fd1, px, pa, pxo, pao);
             LDDMM_CHECK_LAUNCH();
+            if (aff) LDDMM_TRY(epd_aff_step(AA, n, KW, 1, dt, x, a, px, pa, pxo, pao, aws, st));
             continue;
         }
         Partials<T> part = carve_partials<T>(base, c, j1.plan, 3, n);
@@ -700,6 +839,7 @@ int epdiff_adjoint_t(const lddmm_kernel* kd, int T_, int n, const double* xt, co
         j4.args = {x, pa, a, c0, KW, KW, 0, sc1};
         LDDMM_TRY(run_job(j4, n, n, part, st));
         LDDMM_TRY(reduce_job(j4, part, n, KW, fd1, pao, 1, nullptr, -dt, st));
+        if (aff) LDDMM_TRY(epd_aff_step(AA, n, KW, 1, dt, x, a, px, pa, pxo, pao, aws, st));
     }
     // da0 = K(x0) a0 - pa_0
     LDDMM_TRY(axpby(stride, -1.0, pat, 0.0, nullptr, da0, st));
@@ -708,6 +848,11 @@ int epdiff_adjoint_t(const lddmm_kernel* kd, int T_, int n, const double* xt, co
     j3.args = {xt, xt, at, c0, KW, KW, 0, sc};
     LDDMM_TRY(run_job(j3, n, n, part, st));
     LDDMM_TRY(reduce_job(j3, part, n, KW, 1.0, da0, 1, nullptr, 1.0, st));
+    if (aff) {
+        // the affine part of K(x0) a0
+        LDDMM_TRY(aff_moments(AA, n, KW, xt, at, nullptr, aws, nullptr, 0.0, st));
+        LDDMM_TRY(aff_apply(AA, n, KW, 0, 0.0, aws, xt, nullptr, nullptr, 1.0, da0, st));
+    }
     return LDDMM_OK;
 }
 
@@ -793,7 +938,7 @@ extern "C" int lddmm_shoot_adjoint_ic(const lddmm_kernel* kd, const lddmm_surfac
 
 extern "C" int lddmm_epdiff_forward(const lddmm_kernel* kd, int32_t T, int64_t n, int32_t dim, double* xt,
                                     double* at, void* stream) {
-    LDDMM_TRY(check_shoot(kd, T, n, dim, 0u));
+    LDDMM_TRY(check_shoot(kd, T, n, dim, kValAffine));
     if (n == 0) return LDDMM_OK;
     LDDMM_REQUIRE(xt && at, "null pointer");
     hipStream_t st = static_cast<hipStream_t>(stream);
@@ -803,7 +948,7 @@ extern "C" int lddmm_epdiff_forward(const lddmm_kernel* kd, int32_t T, int64_t n
 extern "C" int lddmm_epdiff_adjoint(const lddmm_kernel* kd, int32_t T, int64_t n, int32_t dim, const double* xt,
                                     const double* at, const double* px1, const double* pa1, double* pxt,
                                     double* pat, double* da0, void* stream) {
-    LDDMM_TRY(check_shoot(kd, T, n, dim, 0u));
+    LDDMM_TRY(check_shoot(kd, T, n, dim, kValAffine));
     if (n == 0) return LDDMM_OK;
     LDDMM_REQUIRE(xt && at && px1 && pa1 && pxt && pat && da0, "null pointer");
     hipStream_t st = static_cast<hipStream_t>(stream);

@@ -8,7 +8,7 @@ The T-step loops run on the GPU (lddmm_shoot_* / lddmm_epdiff_*): the
 trajectory stays in HBM and only the requested outputs come back.  numpy
 inputs return numpy arrays; CUDA tensors return CUDA tensors.  A kernel's
 affine part (Kernel(affine='affine'|'euclidean')) is applied in first-order
-shooting and its adjoint.  The log-Jacobian along the trajectory
+and geodesic (EPDiff) shooting and their adjoints.  The log-Jacobian along the trajectory
 (withJacobian) is tracked on the device by the kernel divergence.  Affine
 controls, normals, passenger point sets and splines are not built
 (NotImplementedError); the registration drivers here do not use them.
@@ -24,7 +24,7 @@ from . import kernelFunctions_util as ku
 
 def _kd(K):
     """Kernel descriptor, with the kernel's affine part (applied by the
-    first-order shooting loops; the EPDiff entry points reject it)."""
+    first-order and geodesic shooting loops)."""
     return B.kernel_of(K, with_affine=True)
 
 
@@ -227,8 +227,12 @@ def secondOrderHamiltonian(x, a, px, pa, KparDiff, withSpline=None, affine=None)
     kw = dict(name=KparDiff.name, scale=KparDiff.sigma, order=KparDiff.order, dtype=KparDiff.pk_dtype,
               KP=KparDiff.ms_exponent)
     x_, a_ = B.to_dev(x), B.to_dev(a)
-    Ht = (B.to_dev(px) * ku.applyK(x_, x_, a_, **kw)).sum()
-    Ht = Ht - (B.to_dev(pa) * ku.applyDiffKT(x_, x_, a_, a_, **kw)).sum()
+    if getattr(KparDiff, 'affine', 'none') in ('affine', 'euclidean'):
+        # Kernel.applyK / applyDiffKT add the kernel's affine part
+        Ka, Da = KparDiff.applyK(x_, a_), KparDiff.applyDiffKT(x_, a_, a_)
+    else:
+        Ka, Da = ku.applyK(x_, x_, a_, **kw), ku.applyDiffKT(x_, x_, a_, a_, **kw)
+    Ht = (B.to_dev(px) * Ka).sum() - (B.to_dev(pa) * Da).sum()
     return float(Ht.item())
 
 

@@ -21,7 +21,8 @@ energy and gradient (lddmm_surface_varifold or the current / measure kernel
 sums), the L-BFGS direction (compact form).  As in the reference, internal
 costs are not applied to second-order methods (a given internalCost is
 dropped with a warning, base/secondOrderSurfaceMatching.py:44-47), and
-affine controls are not built.
+affine controls are not built; a kernel with an affine part (affineKernel)
+shoots with that part, as the reference's Kernel methods add it.
 """
 import logging
 
@@ -30,11 +31,6 @@ from .surfaceMatching import SurfaceMatching
 
 
 class SecondOrderSurfaceMatching(GeodesicControl, SurfaceMatching):
-
-    def setInitialOptions(self, options):
-        super().setInitialOptions(options)
-        if self.options['affineKernel']:
-            raise NotImplementedError("affine kernel terms are not built in geodesic (EPDiff) shooting")
 
     def set_internal_cost(self):
         """base/secondOrderSurfaceMatching.py:44-47."""

@@ -187,8 +187,8 @@ def test_population_schedule_covers_every_tile_chunk_within_bounds(lib):
 
 
 def test_affine_descriptor_checks(lib):
-    """The affine part is accepted by the first-order shooting entry points
-    only; kinds and weights are checked before any device call."""
+    """The affine part is accepted by the first-order and geodesic shooting
+    entry points only; kinds and weights are checked before any device call."""
     B = lib
     null = ctypes.c_void_p(0)
     kd = B.kernel_desc([1.0])
@@ -196,7 +196,10 @@ def test_affine_descriptor_checks(lib):
     kd.w1, kd.w2 = 0.5, 0.1
     assert B.LIB.lddmm_applyK(kd, null, 1, null, 1, 3, null, 3, null, null) == B.EUNSUPPORTED
     assert b"affine" in B.LIB.lddmm_last_error()
-    assert B.LIB.lddmm_epdiff_forward(kd, 3, 4, 3, null, null, null) == B.EUNSUPPORTED
+    assert B.LIB.lddmm_epdiff_forward(kd, 3, 0, 3, null, null, null) == B.OK                 # n == 0
+    assert B.LIB.lddmm_epdiff_forward(kd, 3, 4, 3, null, null, null) == B.EINVAL             # null arrays
+    assert B.LIB.lddmm_epdiff_adjoint(kd, 3, 4, 3, null, null, null, null, null, null, null, null) == B.EINVAL
+    assert B.LIB.lddmm_applyDiffKT(kd, null, 1, null, 1, 3, null, null, 3, 0, 1.0, null, null) == B.EUNSUPPORTED
     assert B.LIB.lddmm_shoot_forward(kd, null, 3, 0, 3, null, 1.0, None, null) == B.OK      # n == 0
     kd.affine = 5
     assert B.LIB.lddmm_shoot_forward(kd, null, 3, 4, 3, null, 1.0, None, null) == B.EINVAL

@@ -491,6 +491,47 @@ def test_second_order_gradient_fd():
     np.testing.assert_allclose((da0 * d).sum(), _fd(J, a0, d), rtol=1e-6)
 
 
+@pytest.mark.parametrize("aff", ["affine", "euclidean"])
+@pytest.mark.parametrize("dim", [3, 2])
+def test_second_order_gradient_affine_kernel_fd(aff, dim):
+    """The geodesic adjoint with the kernel's affine part (Kernel.applyK,
+    applyDiffKT, applyDDiffK11and12 and applyDiffK1and2 each add theirs):
+    J(a0) = 1/2 a0.K a0 + |x_T - tgt|^2 + w.a_T, so px1 = -2 (x_T - tgt) and
+    pa1 = -w (every adjoint term is exercised); da0 = FD of J."""
+    x0, a0 = rnd(10, dim), rnd(10, dim, scale=0.3)
+    tgt, w = rnd(10, dim), rnd(10, dim)
+    kaff = (aff, 0.7, 0.4, np.full(dim, 0.2))
+    T = 5
+
+    def J(a):
+        xt, at = O.secondOrderEvolution(x0, a, [1.2], 1 / T, kaff=kaff)
+        Ka = O.applyK(x0, x0, a, [1.2]) + O.affineApplyK(x0, x0, a, kaff)
+        return 0.5 * (a * Ka).sum() + ((xt[-1] - tgt) ** 2).sum() + (w * at[-1]).sum()
+    xt, at = O.secondOrderEvolution(x0, a0, [1.2], 1 / T, kaff=kaff)
+    da0 = O.secondOrderGradient(x0, a0, -2 * (xt[-1] - tgt), -w, [1.2], 1 / T, kaff=kaff)[0]
+    d = rnd(10, dim)
+    np.testing.assert_allclose((da0 * d).sum(), _fd(J, a0, d), rtol=1e-6)
+    # the affine part moves the trajectory
+    xt0, _ = O.secondOrderEvolution(x0, a0, [1.2], 1 / T)
+    assert np.abs(xt[-1] - xt0[-1]).max() > 1e-3
+
+
+@pytest.mark.parametrize("aff", ["affine", "euclidean"])
+def test_second_order_matching_affine_kernel_gradient_fd(aff):
+    """Second-order point-set matching with the kernel's affine part
+    (affineKernel): da0 = FD of the objective in a0."""
+    x0 = rnd(20, 3) * 2
+    w0 = np.abs(rnd(20)) + 0.2
+    tgt = (rnd(18, 3) * 2 + 0.3, np.abs(rnd(18)) + 0.2)
+    a0 = 0.1 * rnd(20, 3)
+    d = rnd(20, 3)
+    kaff = (aff, 0.5, 0.3, x0.mean(0))
+    kw = dict(kind='point_measure', sigma_error=0.7, kaff=kaff)
+    g = O.second_order_matching_gradient(x0, w0, a0, tgt, [1.5], [0.9], 0.2, **kw)
+    fd = _fd(lambda a_: O.second_order_matching_objective(x0, w0, a_, tgt, [1.5], [0.9], 0.2, **kw)[0], a0, d)
+    np.testing.assert_allclose((g * d).sum(), fd, rtol=1e-6)
+
+
 @pytest.mark.parametrize("kind", ["point_measure", "L2"])
 def test_second_order_matching_gradient_fd(kind):
     """Second-order point-set matching: da0 = FD of the objective in a0."""
