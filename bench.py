@@ -187,13 +187,18 @@ def population_registration(torch, npairs=256, iters=10, sample=16):
                 'timeStep': 0.1, 'pk_dtype': 'float32', 'maxIter': iters}
     tm = [Surface((f, v)) for (v, f), _ in prs]
     tg = [Surface((f2, v2)) for _, (v2, f2) in prs]
-    PopulationMatching(tm[:4], tg[:4], opts()).optimizeMatching()          # code objects, tuning
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    pm = PopulationMatching(tm, tg, opts())
-    pm.optimizeMatching()
-    torch.cuda.synchronize()
-    t_batch = time.perf_counter() - t0
+    # first run: the launch-shape autotuner times its candidates on each op's
+    # first call; the second run of the same population shape reuses them
+    t_runs = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pm = PopulationMatching(tm, tg, opts())
+        pm.optimizeMatching()
+        torch.cuda.synchronize()
+        t_runs.append(time.perf_counter() - t0)
+        pm.pop.close()
+    t_batch = t_runs[1]
     idx = list(range(0, npairs, npairs // sample))
     o = dict(opts(), errorType='varifold', mode='quiet')
     t_seq = 0.0
@@ -208,10 +213,12 @@ def population_registration(torch, npairs=256, iters=10, sample=16):
         t_seq += time.perf_counter() - t1
     t_seq_all = t_seq * npairs / len(idx)
     out = {"pairs": npairs, "bfgs_iterations": iters, "batched_seconds": t_batch,
+           "batched_seconds_first_run_with_tuning": t_runs[0],
            "batched_seconds_per_pair": t_batch / npairs, "sequential_sample_pairs": len(idx),
            "sequential_seconds_estimated": t_seq_all, "batched_speedup": t_seq_all / t_batch}
-    log(f"population registration: {npairs} pairs x {iters} iterations batched {t_batch:.2f} s, "
-        f"one at a time ~{t_seq_all:.2f} s (sample of {len(idx)}), {t_seq_all / t_batch:.2f}x")
+    log(f"population registration: {npairs} pairs x {iters} iterations batched {t_batch:.2f} s "
+        f"(first run, tuning included, {t_runs[0]:.2f} s), one at a time ~{t_seq_all:.2f} s "
+        f"(sample of {len(idx)}), {t_seq_all / t_batch:.2f}x")
     return out
 
 

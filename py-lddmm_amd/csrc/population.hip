@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <queue>
 #include <vector>
 
@@ -181,21 +182,44 @@ __global__ void __launch_bounds__(256) k_pop_segsum(const double* __restrict__ r
     if (threadIdx.x == 0) out[p] = scale * s;
 }
 
-// out[p] = sum_t sum_{i in pair p} a[t,i,:].b[t,i,:]  (one block per pair)
-__global__ void __launch_bounds__(256) k_pop_segdot(const double* __restrict__ a, const double* __restrict__ b,
-                                                    int T, int64_t V, const int64_t* __restrict__ off,
-                                                    const uint8_t* __restrict__ active, double* __restrict__ out) {
+// out[p] = sum_t sum_{i in pair p} a[t,i,:].b[t,i,:] in two stages: workgroup
+// (p, c) sums chunk c (kSegChunk consecutive elements of the pair's
+// t-major element list) into part[p * nc + c], then one thread per pair adds
+// its chunks in order (deterministic; every pair's chunks run in parallel).
+constexpr int64_t kSegChunk = 16384;
+
+__global__ void __launch_bounds__(256) k_pop_segdot_chunks(const double* __restrict__ a, const double* __restrict__ b,
+                                                           int T, int64_t V, const int64_t* __restrict__ off,
+                                                           const uint8_t* __restrict__ active, int nc,
+                                                           double* __restrict__ part) {
     __shared__ double red[256];
     const int p = blockIdx.x;
+    const int c = blockIdx.y;
     if (!active[p]) return;
-    const int64_t e0 = off[p] * 3, e1 = off[p + 1] * 3;
+    const int64_t e0 = off[p] * 3, len = off[p + 1] * 3 - e0;
+    const int64_t k0 = int64_t(c) * kSegChunk, k1 = min(k0 + kSegChunk, len * T);
+    if (k0 >= k1) return;                                  // beyond this pair's elements
     double s = 0.0;
-    for (int t = 0; t < T; ++t) {
-        const size_t ts = size_t(t) * size_t(V) * 3;
-        for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) s += a[ts + e] * b[ts + e];
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += 256) {
+        const int64_t t = k / len;
+        const size_t o = size_t(t) * size_t(V) * 3 + size_t(e0 + (k - t * len));
+        s = fma(a[o], b[o], s);
     }
     s = block_sum256(s, red);
-    if (threadIdx.x == 0) out[p] = s;
+    if (threadIdx.x == 0) part[size_t(p) * nc + c] = s;
+}
+
+__global__ void __launch_bounds__(256) k_pop_segdot_fold(const double* __restrict__ part, int P, int T,
+                                                         const int64_t* __restrict__ off,
+                                                         const uint8_t* __restrict__ active, int nc,
+                                                         double* __restrict__ out) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= P || !active[p]) return;
+    const int64_t n = (off[p + 1] - off[p]) * 3 * T;
+    const int64_t used = (n + kSegChunk - 1) / kSegChunk;
+    double s = 0.0;
+    for (int64_t c = 0; c < used; ++c) s += part[size_t(p) * nc + c];
+    out[p] = s;
 }
 
 // y[t,i,:] += alpha[pair(i)] x[t,i,:]
@@ -379,6 +403,8 @@ struct lddmm_population {
     size_t scratch_bytes = 0;
     bool graphs_on = true;
     hipStream_t cap_stream = nullptr;
+    double* d_segpart = nullptr;          // segment dot chunk sums (grow-only)
+    size_t segpart_cap = 0;
     std::map<std::vector<uint64_t>, lddmm::GraphRec> graphs;
 };
 
@@ -653,6 +679,31 @@ std::vector<int64_t> tune_key(int op, bool ms) {
     return {op, int64_t(sizeof(T)), ms ? 1 : 0};
 }
 
+// Tuning choices shared by the populations of one process: a population with
+// the same device and segment tables (FNV-1a hash of the vertex, face and
+// target-face offsets) reuses the choice instead of timing the candidates
+// again.  The choice only picks a launch shape; every shape gives the same
+// sums' values up to the chunking of the float64 row sums.
+std::mutex g_tune_mu;
+std::map<std::vector<int64_t>, TuneChoice> g_tune_shared;
+
+std::vector<int64_t> shared_tune_key(const lddmm_population* pop, const std::vector<int64_t>& key) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](int64_t v) {
+        for (int b = 0; b < 8; ++b) {
+            h ^= (uint64_t(v) >> (8 * b)) & 0xffu;
+            h *= 1099511628211ull;
+        }
+    };
+    for (const auto* tab : {&pop->voff, &pop->foff, &pop->tfoff}) {
+        mix(int64_t(tab->size()));
+        for (int64_t v : *tab) mix(v);
+    }
+    std::vector<int64_t> k = key;
+    k.insert(k.end(), {int64_t(pop->device), int64_t(pop->P), pop->V, pop->F, pop->TF, int64_t(h)});
+    return k;
+}
+
 // Partial buffers sit after the fixed region: phi at its start, plo at the
 // middle of what remains, so both stay put while the plans change (a graph
 // captured for one active set replays for any other that fits).
@@ -690,6 +741,15 @@ int prepare_op(lddmm_population* pop, int op, int fam, bool ms, MakeArgs&& make_
                hipStream_t st, PopLaunch<T>* out) {
     auto key = tune_key<T>(op, ms);
     auto it = pop->tune.find(key);
+    const std::vector<int64_t> skey = shared_tune_key(pop, key);
+    if (it == pop->tune.end()) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto g = g_tune_shared.find(skey);
+        if (g != g_tune_shared.end()) {
+            pop->tune[key] = g->second;
+            it = pop->tune.find(key);
+        }
+    }
     if (it == pop->tune.end()) {
         struct Cand { PopLaunch<T> L; int f10; };
         std::vector<Cand> cands;
@@ -735,6 +795,8 @@ int prepare_op(lddmm_population* pop, int op, int fam, bool ms, MakeArgs&& make_
         if (rc != LDDMM_OK) return rc;
         pop->tune[key] = bc;
         it = pop->tune.find(key);
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_shared[skey] = bc;
     }
     PopLaunch<T> L;
     L.R = it->second.R;
@@ -1071,7 +1133,8 @@ void free_pop(lddmm_population* pop) {
         if (kv.second.d_pairs) (void)hipFree(kv.second.d_pairs);
     }
     for (void* p : {(void*)pop->d_voff, (void*)pop->d_foff, (void*)pop->d_tfoff, (void*)pop->d_vpair,
-                    (void*)pop->d_fpair, (void*)pop->d_active, (void*)pop->d_trec, (void*)pop->scratch})
+                    (void*)pop->d_fpair, (void*)pop->d_active, (void*)pop->d_trec, (void*)pop->scratch,
+                    (void*)pop->d_segpart})
         if (p) (void)hipFree(p);
     if (pop->cap_stream) (void)hipStreamDestroy(pop->cap_stream);
     delete pop;
@@ -1239,8 +1302,23 @@ extern "C" int lddmm_population_seg_dot(lddmm_population* pop, const double* a, 
     LDDMM_REQUIRE(T >= 1 && out && (pop->V == 0 || (a && b)), "population_seg_dot: bad arguments");
     hipStream_t st = static_cast<hipStream_t>(stream);
     LDDMM_TRY(upload_active(pop, st));
-    hipLaunchKernelGGL(k_pop_segdot, dim3(unsigned(pop->P)), dim3(256), 0, st, a, b, int(T), pop->V, pop->d_voff,
-                       pop->d_active, out);
+    int64_t maxn = 0;
+    for (int q = 0; q < pop->P; ++q) maxn = std::max(maxn, pop->voff[size_t(q) + 1] - pop->voff[size_t(q)]);
+    const int nc = int(std::max<int64_t>(1, (maxn * 3 * T + kSegChunk - 1) / kSegChunk));
+    const size_t need = size_t(pop->P) * size_t(nc);
+    if (need > pop->segpart_cap) {
+        LDDMM_TRY(pop_sync(st));
+        if (pop->d_segpart) LDDMM_HIP(hipFree(pop->d_segpart));
+        pop->d_segpart = nullptr;
+        pop->segpart_cap = 0;
+        LDDMM_TRY(dev_alloc(&pop->d_segpart, need));
+        pop->segpart_cap = need;
+    }
+    hipLaunchKernelGGL(k_pop_segdot_chunks, dim3(unsigned(pop->P), unsigned(nc)), dim3(256), 0, st, a, b, int(T),
+                       pop->V, pop->d_voff, pop->d_active, nc, pop->d_segpart);
+    LDDMM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_pop_segdot_fold, dim3(unsigned((pop->P + 255) / 256)), dim3(256), 0, st, pop->d_segpart,
+                       pop->P, int(T), pop->d_voff, pop->d_active, nc, out);
     LDDMM_CHECK_LAUNCH();
     return LDDMM_OK;
 }
