@@ -6,8 +6,8 @@ errorType 'varifold', 'current' or 'measure', algorithm 'bfgs' (weak-Wolfe line 
 affine None (or affineKernel=True with affine 'euclidean' / 'affine', which
 become terms of the deformation kernel), internal costs 'h1' / 'elastic' /
 'displacement' (device kernels inside the shooting loops), unreduced
-(control points 'ct' beside 'at', semi-reduced flow; no control-point
-resets), not symmetric.
+(control points 'ct' beside 'at', semi-reduced flow, control points reset
+onto the trajectory every unreducedResetRate iterations), not symmetric.
 
 The control `at` [T, N, 3], the trajectories and the BFGS memory live on
 the GPU (float64 tensors); every objective is one device forward shoot with
@@ -82,6 +82,9 @@ class SurfaceMatching:
         # base/surfaceMatching.py:93: the control-point penalty scales with 1000 / N
         self.options['unreducedWeight'] *= 1000.0 / self.fv0.vertices.shape[0]
         self.fidelityWeight = self.options['fidelityWeight']
+        self.unreducedResetRate = self.options['unreducedResetRate']
+        self.obj_unreduced_save = np.inf
+        self.ct_resets = []
         self.initialize_variables()
         self.gradCoeff = 1.
         self.ds = 1.
@@ -115,8 +118,6 @@ class SurfaceMatching:
             raise NotImplementedError("affine registration is not built in lddmm_amd")
         if o['unreduced'] and not self.unreduced_supported:
             raise NotImplementedError(f"unreduced=True is not built for {type(self).__name__}")
-        if o['unreduced'] and o['unreducedResetRate'] > 0:
-            raise NotImplementedError("unreducedResetRate > 0 (control-point resets) is not built")
         for key in ('symmetric', 'pplot'):
             if o[key]:
                 raise NotImplementedError(f"option {key}=True is not built in lddmm_amd")
@@ -752,12 +753,46 @@ class SurfaceMatching:
             obj2 = self.objectiveFun()
             self.resetPK()
             logging.info(f"recomputing Objective {obj0:.5f} {obj:.5f} {obj2:.5f}")
+        self._reset_control_points()
         if self.pkBuffer > 50:
             if self.options['KparDiff'].pk_dtype != self.Kdiff_dtype:
                 logging.info("return to original pk_dtype")
             self.resetPK()
         else:
             self.pkBuffer += 1
+
+    def _reset_control_points(self):
+        """Unreduced form: every unreducedResetRate iterations, when the
+        objective is the lowest seen at a reset check, move the control points
+        onto the current trajectory (c_t <- x_t, t < T) if some control point
+        lies further than 5% of the largest template displacement from its
+        trajectory point; the optimiser then restarts from steepest descent
+        (base/surfaceMatching.py:1462-1489, with its step rho = 1).  Returns
+        True when the control points moved."""
+        rate = self.unreducedResetRate
+        if not (self.unreduced and rate > 0 and self.iter % rate == 0 and self.obj is not None
+                and self.obj < self.obj_unreduced_save):
+            return False
+        self.obj_unreduced_save = self.obj
+        xt, ct = self.state['xt'], self.control['ct']
+        # one transfer for both distances
+        d = torch.stack([((xt[0] - xt[-1]) ** 2).sum(dim=-1).max(),
+                         ((ct - xt[:-1]) ** 2).sum(dim=-1).max()]).sqrt().cpu()
+        dist0, dist = float(d[0]), float(d[1])
+        if not dist > 0.05 * dist0:
+            return False
+        logging.info(f'Resetting trajectories: max distance = {dist:.4f} trajectory distance = {dist0:.4f} '
+                     f'rho = {1.:0.4f}')
+        control = Control()
+        control.update(self.control)
+        control['ct'] = xt[:-1].clone()
+        self.control = control
+        self.controlTry = Control()
+        self.controlTry.update(control)
+        self.controlTry['ct'] = control['ct'].clone()
+        self.ct_resets.append(self.iter)
+        self.reset = True
+        return True
 
     def endOfProcedure(self):
         self.endOfIteration(forceSave=True)
