@@ -161,6 +161,25 @@ int lddmm_applyK_noalloc(const lddmm_kernel* kd,
                          const double* y, int64_t n, const double* x, int64_t m, int32_t dim,
                          const double* a, int32_t k, double* out, void* stream);
 
+/* a[n,k] with K(x, x) a = b by conjugate gradients, all on the device.
+ * Replaces Kernel.solve (base/kernelFunctions.py:676-680), which runs linearcg
+ * (base/conjugateGradient.py:351-408) with Kernel.applyK as the operator.
+ * Zero start; a = 0 when |b|^2 < 1e-10; stops when alpha mean|p| < 1e-10, when
+ * the energy a.Ka - 2 b.a changes by less than 1e-20 of its previous value, or
+ * after iter_max products.  sequence LDDMM_CG_REFERENCE reproduces linearcg as
+ * it executes: its first direction is the residual array itself, which the
+ * in-place residual update also moves, so the second direction is (1 + beta)
+ * times the residual; LDDMM_CG_TEXTBOOK keeps them apart (textbook conjugate
+ * gradients).  work: caller-provided device buffer of 4 n k + 8 doubles
+ * (residual, direction, product, K a - 2 b, CG scalars).  Stream-ordered; the
+ * host reads the stop flag every 16 products to skip the rest after a stop.
+ * Affine kernel parts are not applied (affine must be LDDMM_AFFINE_NONE). */
+#define LDDMM_CG_REFERENCE 0
+#define LDDMM_CG_TEXTBOOK  1
+int lddmm_kernel_solve(const lddmm_kernel* kd, const double* x, int64_t n, int32_t dim,
+                       const double* b, int32_t k, int32_t iter_max, int32_t sequence, double* a,
+                       double* work, void* stream);
+
 /* out[n,dim] = sum_s s^(KP-1)/W sum_j -(y_i-x_j)/s K_s(y_i,x_j) c_ij with
  * c_ij = p_i.a_j (+ a_i.p_j for SYM, + a_i.p_j - 2 regweight a_i.a_j for LDDMM).
  * p is [n,k], a is [m,k]; SYM and LDDMM require n == m.

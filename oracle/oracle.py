@@ -25,6 +25,11 @@ How each function is pinned (tests/test_oracle.py):
   applykdiffmat   reduces to applyDiffKT(plain) when beta[i,j] = p_i.a_j
   applyDiffK      finite differences of applyK along the point set
   applyDDiffK     finite differences of applyDiffKT(plain) along the point set
+  linearcg        'textbook': the k-th iterate minimises the quadratic energy
+                  over the Krylov space of b (dense projection), and the
+                  converged solve equals a dense LU solve; 'reference': the
+                  reference loop's own statements run with numpy's in-place
+                  updates on the aliased direction/residual arrays
   surface / curve varifold and current norms: ||S - S|| = 0, rigid-motion
                   invariance, face-flip invariance (varifold), single pair
                   closed forms, finite-difference gradients
@@ -72,7 +77,7 @@ __all__ = [
     "landmarkDirectEvolutionEuler", "landmarkHamiltonianCovector",
     "landmarkHamiltonianGradient", "matching_objective", "matching_gradient",
     "landmarkEPDiff", "secondOrderEvolution", "secondOrderHamiltonian",
-    "secondOrderCovector", "secondOrderGradient",
+    "secondOrderCovector", "secondOrderGradient", "linearcg", "kernel_solve",
 ]
 
 _BLOCK_PAIRS = 1 << 22   # pairs held in memory per block (~100 MB of f64 temporaries)
@@ -832,6 +837,63 @@ def curveCurrentNormGradient(fvDef, fv1, sigma, KP=-1):
     wz1 = w1[:, None] * z1
     dz1 = applyDiffKT(c1, c1, wcr1, wcr1, sigma, KP=KP) - applyDiffKT(c1, c2, wcr1, wcr2, sigma, KP=KP)
     return _scatter_faces(xDef.shape[0], f, (dz1 - wz1, dz1 + wz1))
+
+
+# ---------------------------------------------------------------------------
+# Kernel.solve: conjugate gradients on K(x, x)
+# ---------------------------------------------------------------------------
+
+def linearcg(op, b, iter_max=100, sequence='reference'):
+    """Solve op(a) = b from a = 0 (base/conjugateGradient.py:351-408, no x0 /
+    param).  The whole array is one vector (dot products over all entries).
+    Stops: |b|^2 < 1e-10 -> zero; alpha * mean|p| < 1e-10; energy a.op(a) - 2 b.a
+    changing by less than 1e-20 of its previous value; iter_max products.
+
+    sequence 'reference' is linearcg as numpy executes it: at the first
+    iteration `p = r` binds the direction to the residual array, and the
+    in-place `r -= alpha * q` moves both, so that iteration's stopping test
+    reads the updated residual and the second direction is r1 + beta r1.
+    'textbook' keeps the direction and the residual apart.  Returns
+    (a, products)."""
+    b = _f64(b)
+    a = np.zeros(b.shape)
+    z = -2.0 * b                       # op(a) - 2 b at a = 0
+    r = b.copy()
+    ener_old = 0.0
+    mu = 0.0
+    p = None
+    products = 0
+    for it in range(iter_max):
+        mu_new = float((r * r).sum())
+        if it == 0:
+            if mu_new < 1e-10:
+                break
+            p = r.copy()
+        else:
+            p = r + (mu_new / mu) * p
+        mu = mu_new
+        q = op(p)
+        products += 1
+        alpha = mu / float((p * q).sum())
+        a = a + alpha * p
+        r = r - alpha * q
+        z = z + alpha * q
+        if it == 0 and sequence == 'reference':
+            p = r.copy()               # the direction is the residual array
+        ener = float((z * a).sum())
+        if alpha * float(np.abs(p).mean()) < 1e-10:
+            break
+        if abs(ener - ener_old) < 1e-20 * abs(ener_old):
+            break
+        ener_old = ener
+    return a, products
+
+
+def kernel_solve(x, b, scale, iter_max=100, sequence='reference', KP=-1, kernel=GAUSS):
+    """Kernel.solve(x, b) (base/kernelFunctions.py:676-680): linearcg with the
+    operator a -> K(x, x) a.  Returns (a, products)."""
+    x = _f64(x)
+    return linearcg(lambda v: applyK(x, x, v, scale, KP, kernel=kernel), b, iter_max, sequence)
 
 
 # ---------------------------------------------------------------------------

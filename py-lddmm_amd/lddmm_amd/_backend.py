@@ -22,6 +22,7 @@ OK, EINVAL, EUNSUPPORTED, EHIP = 0, -1, -2, -3
 
 AFFINE_NONE, AFFINE_FULL, AFFINE_EUCLIDEAN = 0, 1, 2
 DIFFKT_PLAIN, DIFFKT_SYM, DIFFKT_LDDMM = 0, 1, 2
+CG_REFERENCE, CG_TEXTBOOK = 0, 1
 DIFFK_1, DIFFK_2, DIFFK_1AND2 = 0, 1, 2
 DDIFFK_11, DDIFFK_12, DDIFFK_11AND12 = 0, 1, 2
 
@@ -101,6 +102,7 @@ SIGNATURES = {
     "lddmm_applyK": [_KP, _P, _I64, _P, _I64, _I32, _P, _I32, _P, _P],
     "lddmm_applyK_noalloc": [_KP, _P, _I64, _P, _I64, _I32, _P, _I32, _P, _P],
     "lddmm_applyDiffKT": [_KP, _P, _I64, _P, _I64, _I32, _P, _P, _I32, _I32, _D, _P, _P],
+    "lddmm_kernel_solve": [_KP, _P, _I64, _I32, _P, _I32, _I32, _I32, _P, _P, _P],
     "lddmm_applyDiv": [_KP, _P, _I64, _P, _I64, _I32, _P, _P, _P],
     "lddmm_applyktensor": [_KP, _P, _I64, _P, _I64, _I32, _P, _P, _P, _P, _I32, _P, _P],
     "lddmm_applykmat": [_KP, _P, _I64, _P, _I64, _I32, _P, _I32, _P, _P],
@@ -307,6 +309,23 @@ def kernel_of(K, prec=None, with_affine=False):
     for i in range(3):
         kd.center[i] = c[i]
     return kd
+
+
+def kernel_solve(K, x, b, iter_max, sequence='reference'):
+    """a with K(x, x) a = b by conjugate gradients on the device
+    (lddmm_kernel_solve).  x [n, d] and b [n] or [n, k] float64 device
+    tensors; K a Kernel without affine part.  Returns a with b's shape."""
+    seq = {'reference': CG_REFERENCE, 'textbook': CG_TEXTBOOK}[sequence]
+    x = to_dev(x)
+    b2 = to_dev(b).reshape(b.shape[0], -1).contiguous()
+    n, k = b2.shape
+    if x.ndim != 2 or x.shape[0] != n:
+        raise ValueError("kernel_solve: x must be [n, d] with n rows of b")
+    a = torch.empty_like(b2)
+    work = torch.empty(4 * n * k + 8, dtype=torch.float64, device=b2.device)
+    check(LIB.lddmm_kernel_solve(kernel_of(K), ptr(x), n, x.shape[1], ptr(b2), k, int(iter_max), seq, ptr(a),
+                                 ptr(work), stream_handle()), "kernel_solve")
+    return a.reshape(b.shape)
 
 
 def precision_code(prec):

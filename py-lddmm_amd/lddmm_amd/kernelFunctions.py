@@ -252,19 +252,37 @@ class Kernel(KernelSpec):
         logging.info('test diffKT: {0:.5f}, {1:.5f}'.format((k1 - k0) / eps, (grd * dy).sum()))
         return (k1 - k0) / eps, (grd * dy).sum()
 
-    def solve(self, x, xi, b=None, iterMax=100):
+    def solve(self, x, xi, b=None, iterMax=100, sequence='reference'):
         """Solve K(x, x) a = xi by conjugate gradients (base/kernelFunctions.py:676-680;
-        `b` is unused there too).  Follows linearcg (base/conjugateGradient.py:351-408):
+        `b` is unused there too), following linearcg (base/conjugateGradient.py:351-408):
         zero start; return at once when |xi|^2 < 1e-10; stop when
         alpha * mean|p| < 1e-10, when the quadratic energy a.Ka - 2 xi.a stalls
-        (relative change < 1e-20), or after iterMax products.  The products run
-        on the device; the reference's per-iteration residual (an extra product
-        used only for logging) is not computed.  The CG scalars stay on the
-        device; the two stop quantities come back in one transfer per product."""
+        (relative change < 1e-20), or after iterMax products.
+
+        sequence 'reference' (default) gives linearcg's iterates as they run:
+        its first search direction is the residual array itself, which the
+        in-place residual update also moves, so the second direction is
+        (1 + beta) times the residual; 'textbook' keeps the two apart
+        (conjugate gradients proper; this build's warm start uses it).  The
+        reference's per-iteration residual (an extra product used only for
+        logging) is not computed.  Without affine part the whole solve runs on
+        the device (lddmm_kernel_solve: CG scalars and stop flag in device
+        memory, no host round trip per product); with one, the products are
+        Kernel.applyK calls and the scalars come back once per product."""
+        if sequence not in ('reference', 'textbook'):
+            raise ValueError(f"unknown CG sequence {sequence!r}")
         tt = isinstance(x, torch.Tensor) or isinstance(xi, torch.Tensor)
         from . import _backend as B
         x_ = B.to_dev(x)
         rhs = B.to_dev(xi)
+        if self.affine == 'none':
+            a = B.kernel_solve(self, x_, rhs, iterMax, sequence)
+        else:
+            a = self._solve_host_loop(x_, rhs, iterMax, sequence)
+        return a if tt else a.cpu().numpy()
+
+    def _solve_host_loop(self, x_, rhs, iterMax, sequence):
+        """linearcg with Kernel.applyK products (affine part included)."""
         a = torch.zeros_like(rhs)
         z = -2 * rhs                      # K a - 2 xi, tracked with the iterate
         r = rhs.clone()
@@ -285,6 +303,8 @@ class Kernel(KernelSpec):
             a = a + alpha * p
             r = r - alpha * q
             z = z + alpha * q
+            if it == 0 and sequence == 'reference':
+                p = r.clone()             # the reference's direction is its residual array
             chk = torch.stack([alpha * p.abs().mean(), (z * a).sum()]).cpu()
             ener = float(chk[1])
             if float(chk[0]) < 1e-10:
@@ -292,7 +312,7 @@ class Kernel(KernelSpec):
             if abs(ener - ener_old) < 1e-20 * abs(ener_old):
                 break
             ener_old = ener
-        return a if tt else a.cpu().numpy()
+        return a
 
     # (base/kernelFunctions.py:665-674)
     def applyDDiffK11and12(self, x, n, a, p):

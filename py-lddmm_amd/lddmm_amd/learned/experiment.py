@@ -186,7 +186,7 @@ def main(argv=None):
             r = W.register(seed, at=abar[None].expand(T_, -1, -1).contiguous(), target_data=E, cfg=cfg)
             row.update({"it_warm_true_abar": r["iterations"], "E0_true_abar": r["data0"]})
             # perfect prediction through the default pipeline: K abar -> truncated CG
-            a_v = sm.options['KparDiff'].solve(sm.x0_dev, W.velocity(sm, abar), iterMax=W.SOLVE_ITERS)
+            a_v = sm.options['KparDiff'].solve(sm.x0_dev, W.velocity(sm, abar), iterMax=W.SOLVE_ITERS, sequence='textbook')
             r = W.register(seed, at=a_v[None].expand(T_, -1, -1).contiguous(), target_data=E, cfg=cfg)
             row["it_true_vbar_solve"] = r["iterations"]
             # tolerance of the time-mean control to a smooth error: abar + e with

@@ -115,7 +115,7 @@ def time_constant(sm, a):
 
 def solve_control(sm, v, iters=SOLVE_ITERS):
     """Momentum a with K(x0) a ~ v: conjugate gradients truncated at `iters` products."""
-    return sm.options['KparDiff'].solve(sm.x0_dev, v, iterMax=iters)
+    return sm.options['KparDiff'].solve(sm.x0_dev, v, iterMax=iters, sequence='textbook')
 
 
 def ray_cast_heights(sm):
@@ -341,15 +341,15 @@ def warm_control(sm, net, cg_iters=None):
     T = sm.Tsize
     if target == 'vlocal':
         v = torch.einsum('nkj,nk->nj', frame(sm), pred)
-        a = sm.options['KparDiff'].solve(sm.x0_dev, v, iterMax=cg_iters)
+        a = sm.options['KparDiff'].solve(sm.x0_dev, v, iterMax=cg_iters, sequence='textbook')
         return a[None].expand(T, -1, -1).contiguous(), a
     if target == 'abar':
         return pred[None].expand(T, -1, -1).contiguous(), pred
     if target == 'vbar':
-        a = sm.options['KparDiff'].solve(sm.x0_dev, pred, iterMax=cg_iters)
+        a = sm.options['KparDiff'].solve(sm.x0_dev, pred, iterMax=cg_iters, sequence='textbook')
         return a[None].expand(T, -1, -1).contiguous(), a
     if target == 'velocity':
-        a0 = sm.options['KparDiff'].solve(sm.x0_dev, pred, iterMax=cg_iters)
+        a0 = sm.options['KparDiff'].solve(sm.x0_dev, pred, iterMax=cg_iters, sequence='textbook')
     else:
         a0 = pred
     _, at = evol.landmarkEPDiff(T, sm.x0_dev, a0, sm.options['KparDiff'])

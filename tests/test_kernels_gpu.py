@@ -210,14 +210,15 @@ def test_mfma_engine_parity(ku, n, m, dim, k, sig):
 
 
 def test_kernel_solve_matches_dense_solve(ku):
-    """Kernel.solve (conjugate gradients on device products) against a dense
-    LU solve of the Gram matrix; K(x, solve(xi)) must reproduce xi."""
+    """Kernel.solve with the textbook sequence (conjugate gradients on the
+    device) against a dense LU solve of the Gram matrix; K(x, solve(xi)) must
+    reproduce xi."""
     from lddmm_amd.kernelFunctions import Kernel
     rng = np.random.default_rng(21)
     x = rng.uniform(0, 12, size=(250, 3))
     xi = rng.normal(size=(250, 3))
     K = Kernel(name='gauss', sigma=1.0)
-    a = K.solve(x, xi, None, iterMax=400)
+    a = K.solve(x, xi, None, iterMax=400, sequence='textbook')
     d2 = ((x[:, None, :] - x[None, :, :]) ** 2).sum(-1)
     G = np.exp(-d2 / 2.0)
     expect = np.linalg.solve(G, xi)

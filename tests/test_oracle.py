@@ -942,3 +942,97 @@ def test_cpu_reference_matches_numpy_oracle(sig):
     for kw in ({'sym': True}, {'lddmm': True, 'regweight': 0.6}):
         np.testing.assert_allclose(cpu.applyDiffKT(xs, xs, ps, as_, sig, **kw),
                                    O.applyDiffKT(xs, xs, ps, as_, sig, **kw), rtol=1e-13, atol=1e-14)
+
+
+# ------------------------------------------------- Kernel.solve (linearcg)
+def _gram(x, s):
+    d2 = ((x[:, None, :] - x[None, :, :]) ** 2).sum(-1)
+    return np.exp(-d2 / (2 * s * s))
+
+
+def _solve_problem(n=40, seed=5):
+    r = np.random.default_rng(seed)
+    x = r.uniform(0, 6, size=(n, 3))
+    return x, r.normal(size=(n, 3)), _gram(x, 1.0)
+
+
+def test_linearcg_textbook_iterates_minimise_over_krylov_space():
+    """CG's k-th iterate minimises a.Ga/2 - b.a over span{b, Gb, .., G^(k-1) b}
+    (whole [n, 3] arrays as one vector); computed here by a dense projection."""
+    x, b, G = _solve_problem()
+    V = [b]
+    for _ in range(5):
+        V.append(G @ V[-1])
+    for k in range(1, 6):
+        B = np.stack([v.ravel() for v in V[:k]], 1)
+        GB = np.stack([(G @ v).ravel() for v in V[:k]], 1)
+        c = np.linalg.solve(B.T @ GB, B.T @ b.ravel())
+        expect = (B @ c).reshape(b.shape)
+        got, products = O.kernel_solve(x, b, [1.0], iter_max=k, sequence='textbook')
+        assert products == k
+        np.testing.assert_allclose(got, expect, rtol=1e-8, atol=1e-10 * np.abs(expect).max())
+
+
+def test_linearcg_converged_solve_is_dense_solve():
+    x, b, G = _solve_problem(n=30, seed=6)
+    expect = np.linalg.solve(G, b)
+    for seq in ('textbook', 'reference'):
+        got, _ = O.kernel_solve(x, b, [1.0], iter_max=3000, sequence=seq)
+        assert np.linalg.norm(G @ got - b) / np.linalg.norm(b) < 1e-7, seq
+        assert np.linalg.norm(got - expect) / np.linalg.norm(expect) < 1e-4, seq
+
+
+def _aliased_cg(op, b, iters):
+    """The reference loop's statements with numpy's own semantics: the first
+    direction is bound to the residual array and both are updated in place."""
+    sol = np.zeros(b.shape)
+    zz = op(sol) - 2 * b
+    res = -zz - b
+    e_prev = (zz * sol).sum()
+    m = 0.
+    d = None
+    for k in range(iters):
+        if k == 0:
+            m = (res * res).sum()
+            if m < 1e-10:
+                return sol
+            d = res
+        else:
+            m_prev = m
+            m = (res * res).sum()
+            d = res + (m / m_prev) * d
+        qq = op(d)
+        step = m / (d * qq).sum()
+        sol += step * d
+        res -= step * qq
+        zz += step * qq
+        e = (zz * sol).sum()
+        if step * np.mean(np.fabs(d)) < 1e-10:
+            return sol
+        if np.fabs(e - e_prev) < 1e-20 * np.fabs(e_prev):
+            return sol
+        e_prev = e
+    return sol
+
+
+@pytest.mark.parametrize("iters", [1, 2, 3, 10, 60])
+def test_linearcg_reference_sequence_follows_the_aliased_loop(iters):
+    x, b, G = _solve_problem(n=50, seed=7)
+    got, _ = O.kernel_solve(x, b, [1.0], iter_max=iters, sequence='reference')
+    expect = _aliased_cg(lambda v: G @ v, b, iters)
+    np.testing.assert_allclose(got, expect, rtol=1e-9, atol=1e-12 * np.abs(expect).max())
+    tb, _ = O.kernel_solve(x, b, [1.0], iter_max=iters, sequence='textbook')
+    if iters == 1:
+        np.testing.assert_allclose(got, tb, rtol=1e-13)        # first step: steepest descent in both
+    else:
+        assert np.linalg.norm(got - tb) > 1e-3 * np.linalg.norm(tb)
+
+
+def test_linearcg_zero_rhs_and_stop_counts():
+    x, b, G = _solve_problem(n=20, seed=8)
+    for seq in ('textbook', 'reference'):
+        a, products = O.kernel_solve(x, 1e-7 * b, [1.0], iter_max=10, sequence=seq)
+        assert products == 0 and np.all(a == 0)
+    # the energy test stops a converged textbook solve well before the cap
+    _, products = O.kernel_solve(x, b, [1.0], iter_max=5000, sequence='textbook')
+    assert products < 5000
