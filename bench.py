@@ -261,11 +261,14 @@ def warmstart_cfg5(torch, pairs=8):
     iterations) sets the target data term E*; each warm-started registration
     runs until its varifold data term is within 1e-4 of E*.  Side by side on
     the same pairs:
-      network      - the committed network (ray-cast heights -> residual of
-                     the refined velocity over h n -> truncated CG control);
+      network      - the committed pipeline: the network (ray-cast heights ->
+                     residual of the refined velocity over h n), then one
+                     re-ray-cast from the surface that velocity's control
+                     reaches, then the truncated CG control;
+      network_only - the network's velocity solved directly (no re-ray-cast);
       feature_only - h n solved the same way, no network;
-      refined1     - h n refined by one re-ray-cast from the surface its
-                     control reaches (one extra shoot and ray cast), no network.
+      refined1 / refined2 - h n refined by one / two re-ray-casts (one extra
+                     shoot and ray cast each), no network.
     Wall seconds cover matching set-up, features, network, solve and BFGS
     (the synthetic pair is generated before the timed region)."""
     import logging
@@ -279,14 +282,19 @@ def warmstart_cfg5(torch, pairs=8):
     def feature_only(sm):
         return W.feature_only_control(sm)[0]
 
-    def refined1(sm):
-        return W.time_constant(sm, W.solve_control(sm, W.refined_velocity(sm, 1)))
+    def refined(steps):
+        return lambda sm: W.time_constant(sm, W.solve_control(sm, W.refined_velocity(sm, steps)))
 
-    starts = {"network": None, "feature_only": feature_only, "refined1": refined1}
+    def network_only(sm):
+        return W.warm_control(sm, net, refine=0)[0]
+
+    starts = {"network": None, "network_only": network_only, "feature_only": feature_only,
+              "refined1": refined(1), "refined2": refined(2)}
     # untimed warm-up of every start on a pair outside the measured seeds (first kernel loads)
     W.register(899999, net=net, max_iter=1)
-    for make in (feature_only, refined1):
-        W.register(899999, start=make, max_iter=1)
+    for name, make in starts.items():
+        if make is not None:
+            W.register(899999, start=make, max_iter=1)
     rows = []
     for k in range(pairs):
         seed = 900000 + k

@@ -75,6 +75,9 @@ def main(argv=None):
     ap.add_argument("--batched", type=int, default=0,
                     help="generate the training registrations this many pairs at a time with the lockstep "
                          "population registration (features 'normal', target 'vlocal' only)")
+    ap.add_argument("--net-refine", type=int, default=None,
+                    help="re-ray-casts after the network's prediction (target 'vresid'; default "
+                         "NET_REFINE_STEPS)")
     ap.add_argument("--cg-iters", type=int, default=None,
                     help="CG products of the velocity -> control solve (default: the network's own)")
     ap.add_argument("--oracle-check", type=int, default=5,
@@ -149,7 +152,7 @@ def main(argv=None):
         if k < 3:
             row["plain_trace"] = list(sm.data_trace)
         for target, net in nets.items():
-            r = W.register(seed, net=net, target_data=E, cfg=cfg, cg_iters=a.cg_iters)
+            r = W.register(seed, net=net, target_data=E, cfg=cfg, cg_iters=a.cg_iters, refine=a.net_refine)
             row.update({f"it_warm_{target}": r["iterations"], f"sec_warm_{target}": r["seconds"],
                         f"E_warm_{target}": r["data"], f"E0_warm_{target}": r["data0"],
                         f"obj_ratio_{target}": r["objective"] / obj_plain,

@@ -74,6 +74,7 @@ TARGET_KEY = {"vresid": "vresid", "abar": "abar", "vbar": "vbar", "vlocal": "vlo
               "velocity": "v0"}
 SOLVE_ITERS = 10      # conjugate-gradient products that turn a velocity into a time-constant control
 REFINE_STEPS = 2      # re-ray-casts behind the 'vresid' training target
+NET_REFINE_STEPS = 1  # re-ray-casts after the network's prediction in the warm start
 MEANSHIFT_SIGMA = 1.5
 
 
@@ -152,15 +153,20 @@ def feature_only_control(sm):
 
 
 def refined_velocity(sm, steps=REFINE_STEPS, heights=None):
-    """Velocity refined by re-ray-casting.  Start from v = h n; each step
-    shoots the time-constant control solved from v, ray-casts the target along
-    the unit vertex normals n_T of the surface it reaches (a missed ray has
-    height 0) and adds that height: v <- v + h_T n_T.  `heights` = the pair's
-    (channels, frames) when already computed.  Returns v [N, 3] (device)."""
+    """Velocity refined by re-ray-casting, started from v = h n (see
+    refine_velocity).  `heights` = the pair's (channels, frames) when already
+    computed.  Returns v [N, 3] (device)."""
     ch, fr = ray_cast_heights(sm) if heights is None else heights
+    return refine_velocity(sm, ch[:, :1] * fr[:, 0], steps)
+
+
+def refine_velocity(sm, v, steps):
+    """`steps` re-ray-casts of the velocity v [N, 3]: each shoots the
+    time-constant control solved from v, ray-casts the target along the unit
+    vertex normals n_T of the surface it reaches (a missed ray has height 0)
+    and adds that height: v <- v + h_T n_T."""
     K = sm.options['KparDiff']
     faces = sm.fv0.faces
-    v = ch[:, :1] * fr[:, 0]
     for _ in range(steps):
         xt, _ = evol.shoot(sm.x0_dev, time_constant(sm, solve_control(sm, v)), K)
         xT = xt[-1].cpu().numpy()
@@ -312,13 +318,14 @@ def train(samples, epochs=300, batch=8, lr=2e-3, seed=0, log_every=50, target='a
     return net, losses
 
 
-def warm_control(sm, net, cg_iters=None):
+def warm_control(sm, net, cg_iters=None, refine=None):
     """Initial control [T, N, 3] from the network's prediction.
 
     target 'vresid': the prediction is added, in each vertex's local frame, to
-    the ray-cast velocity h n, and the sum is solved for a time-constant
-    control with the network's solve_iters products (SOLVE_ITERS), as the
-    feature-only control is; 'vlocal': the prediction is K(x0) abar in each vertex's local
+    the ray-cast velocity h n; the sum is re-ray-cast `refine` times
+    (refine_velocity, default NET_REFINE_STEPS) and solved for a
+    time-constant control with the network's solve_iters products
+    (SOLVE_ITERS), as the feature-only control is; 'vlocal': the prediction is K(x0) abar in each vertex's local
     frame (normal, tangents), turned into abar by a conjugate-gradient solve
     truncated at the network's solve_iters products (SOLVE_ITERS) and used as a
     time-constant control; 'abar': the prediction is the time-mean of the plain optimiser's
@@ -334,7 +341,9 @@ def warm_control(sm, net, cg_iters=None):
         ch, fr = ray_cast_heights(sm)
         X = height_inputs(sm, getattr(net, 'features', 'normal'), (ch, fr))
         pred = net.predict(X, [X.shape[0]])
-        a = solve_control(sm, ch[:, :1] * fr[:, 0] + torch.einsum('nkj,nk->nj', fr, pred), cg_iters)
+        v = ch[:, :1] * fr[:, 0] + torch.einsum('nkj,nk->nj', fr, pred)
+        v = refine_velocity(sm, v, NET_REFINE_STEPS if refine is None else refine)
+        a = solve_control(sm, v, cg_iters)
         return time_constant(sm, a), a
     feats = features(sm, getattr(net, 'features', 'force'))
     pred = net.predict(feats, [feats.shape[0]])
@@ -357,7 +366,7 @@ def warm_control(sm, net, cg_iters=None):
 
 
 def register(seed, net=None, target_data=None, cfg=DEFAULT, max_iter=None, a0=None, cg_iters=None, at=None,
-             start=None):
+             start=None, refine=None):
     """Registration (plain, warm-started by `net`, from a given control `at`,
     from a given initial momentum `a0`, or from the control `start(sm)`
     returns for the new matching) until the data term reaches target_data
@@ -379,7 +388,7 @@ def register(seed, net=None, target_data=None, cfg=DEFAULT, max_iter=None, a0=No
         elif at is not None:
             at = B.to_dev(at)
         elif a0 is None:
-            at, _ = warm_control(sm, net, cg_iters=cg_iters)
+            at, _ = warm_control(sm, net, cg_iters=cg_iters, refine=refine)
         else:
             _, at = evol.landmarkEPDiff(sm.Tsize, sm.x0_dev, B.to_dev(a0), sm.options['KparDiff'])
         obj_zero = sm.objectiveFun()
