@@ -379,7 +379,13 @@ class SurfaceMatching:
         self._last_try = (dr, eps, controlTry, stateTry)
         grad = self.getGradient(coeff, update=[dr, eps])
         dot = self._dot_table_device([grad], [dr])
-        host = torch.cat([obj_.reshape(1), data.reshape(1), dot.reshape(1)]).cpu()
+        parts = [obj_.reshape(1), data.reshape(1), dot.reshape(1)]
+        pending = getattr(self, '_pending_table', None)
+        if pending is not None and pending[1] is None:
+            parts.append(pending[0].reshape(-1))        # BFGS's dot table, launched before the trial
+        host = torch.cat(parts).cpu()
+        if len(parts) == 4:
+            pending[1] = host[3:].numpy().reshape(tuple(pending[0].shape))
         objTry = self._record_try(dr, eps, controlTry, stateTry, host[:2], None)
         return objTry, -float(host[2]), grad
 
@@ -478,10 +484,15 @@ class SurfaceMatching:
     def dotTable(self, probes, targets, later=False):
         """Table [len(probes), len(targets)] of dotProduct_euclidean values
         (numpy), read back in one transfer; with later=True the products are
-        launched now and a function is returned that reads the table."""
+        launched now and a function is returned that reads the table.  The
+        next trial of updateTryWithSlope brings the pending table back with
+        its own values (one transfer for both); the function then returns
+        that copy."""
         tab = self._dot_table_device(probes, targets)
         if later:
-            return lambda: tab.cpu().numpy()
+            pending = [tab, None]
+            self._pending_table = pending
+            return lambda: pending[1] if pending[1] is not None else tab.cpu().numpy()
         return tab.cpu().numpy()
 
     def twoLoop(self, grd, storedGrad):
