@@ -504,7 +504,13 @@ int shoot_forward_t(const lddmm_kernel* kd, const double* at, int T_, int n, dou
     const bool aff = kd->affine != LDDMM_AFFINE_NONE;
     // small sets: one launch per step (pair sums and epilogue in one workgroup)
     const bool small = !ic && !aff && int64_t(n) <= kStepMaxN;
-    const int nb = small ? (n + kStepTgt - 1) / kStepTgt      // blocks of k_step_shoot
+    int tpw = kStepTgtMax;                                      // targets per k_step_shoot workgroup
+    if (small) {
+        DeviceInfo di;
+        LDDMM_TRY(device_info(&di));
+        tpw = step_targets(n, di.num_cu);
+    }
+    const int nb = small ? (n + tpw - 1) / tpw                  // blocks of k_step_shoot
                          : (n + kEpiRows - 1) / kEpiRows;     // blocks of k_shoot_epi
     const int nbv = int((int64_t(n) * KW + 255) / 256);       // blocks of k_shoot_epi_v
     const int ndot = ic ? nbv + 1 : nb + 1;                     // dot slots per step
@@ -534,7 +540,7 @@ int shoot_forward_t(const lddmm_kernel* kd, const double* at, int T_, int n, dou
         jk.args = {x, x, a, c0, KW, KW, 0, sc};
         if (small) {
             hipLaunchKernelGGL((k_step_shoot<T, KW, MS>), dim3(unsigned(nb)), dim3(kStepThreads), 0, st, jk.args, n,
-                               dt, x, a, xt + (t + 1) * stride, dots + size_t(t) * nb, r * dt);
+                               tpw, dt, x, a, xt + (t + 1) * stride, dots + size_t(t) * nb, r * dt);
             LDDMM_CHECK_LAUNCH();
             continue;
         }
@@ -647,8 +653,11 @@ int shoot_adjoint_t(const lddmm_kernel* kd, const double* at, const double* xt, 
             ja.args = {x, x, pu, a, c0, KW, KW, 0, 2.0 * r, sc};
             if (int64_t(n) <= kStepMaxN) {
                 // small sets: one launch per step
-                hipLaunchKernelGGL((k_step_adjoint<T, KW, MS>), dim3(unsigned((n + kStepTgt - 1) / kStepTgt)),
-                                   dim3(kStepThreads), 0, st, ja.args, n, dt, fdiff, pn, pc,
+                DeviceInfo di;
+                LDDMM_TRY(device_info(&di));
+                const int tpw = step_targets(n, di.num_cu);
+                hipLaunchKernelGGL((k_step_adjoint<T, KW, MS>), dim3(unsigned((n + tpw - 1) / tpw)),
+                                   dim3(kStepThreads), 0, st, ja.args, n, tpw, dt, fdiff, pn, pc,
                                    grad + size_t(t) * stride);
                 LDDMM_CHECK_LAUNCH();
                 continue;

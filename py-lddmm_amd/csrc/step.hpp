@@ -1,48 +1,68 @@
 // One launch per shooting step for small point sets (the registration sizes):
-// each workgroup owns kStepTgt targets and sums over ALL sources, so the step's
+// each workgroup owns `tpw` targets and sums over ALL sources, so the step's
 // epilogue (x_{t+1} = x_t + dt v and the energy dot, or p_t and the gradient)
 // runs in the same kernel and no chunk partials go through HBM.
 //
-// Workgroup: kStepThreads lanes (16 wave64).  Lane tid takes target
-// tid % kStepTgt and source slice tid / kStepTgt (kStepSlices slices): in each
-// LDS tile of kStepTile source records it applies the op's pair function to
-// records slice, slice + kStepSlices, ...  A wave holds 4 slices of the 16
-// targets, so each LDS read has 4 distinct addresses, each broadcast to 16
-// lanes.  Tile sums are folded into a (hi, lo) two-sum accumulator in float32
-// (compensated form) or added in float64; the slices of a target are then
-// combined in float64, first across the wave's 4 slices by lane shuffles,
-// then across the 16 waves in wave order through LDS (a fixed order: the
-// result does not depend on scheduling).
+// tpw = ceil(n / CUs) (at most kStepTgtMax): the workgroups cover the CUs
+// once, each CU summing about n / CUs targets (2,562 points on 256 CUs: 11
+// targets, 233 workgroups; a power-of-two count would leave CUs idle or give
+// some CUs two workgroups).  Workgroup: kStepThreads lanes (16 wave64).  Lane
+// tid takes target tid % tpw and source slice tid / tpw (nsl =
+// kStepThreads / tpw slices; lanes past tpw * nsl only stage sources): in
+// each LDS tile of kStepTile source records it applies the op's pair
+// function to records slice, slice + nsl, ...  Tile sums are folded into a
+// (hi, lo) two-sum accumulator in float32 (compensated form) or added in
+// float64; the slices of a target are then combined in float64 through LDS
+// (the tile buffer, reused): kStepGroups lanes per target each add every
+// kStepGroups-th slice in slice order, then one lane per target adds the group
+// sums in group order (a fixed order: the result does not depend on
+// scheduling).
 //
 // For larger sets the chunked kernels (pair.hpp) are faster: each LDS record
-// (one wave-uniform read) feeds two targets per lane there, against one pair
-// per four-address read here.  Measured on MI355X (forward and adjoint steps,
-// float32 and float64): equal or faster up to 4,000 points, slower by 6,000
-// (1.5x at 10,000); kStepMaxN is where the shooting loops switch.
+// (one wave-uniform read) feeds two targets per lane there.  Measured on
+// MI355X (forward and adjoint steps, float32 and float64): equal or faster up
+// to 4,000 points, slower by 6,000 (1.5x at 10,000); kStepMaxN is where the
+// shooting loops switch.
 #pragma once
 
 #include "ops.hpp"
 
 namespace lddmm {
 
-constexpr int kStepTgt = 16;                         // targets per workgroup
+constexpr int kStepTgtMax = 16;                      // largest targets per workgroup
 constexpr int kStepThreads = 1024;                   // lanes per workgroup
-constexpr int kStepSlices = kStepThreads / kStepTgt; // source slices (64)
 constexpr int kStepTile = 512;                       // source records per LDS tile
-constexpr int kStepWaves = kStepThreads / 64;
+constexpr int kStepGroups = 32;                      // partial sums per target in the final combine
 constexpr int64_t kStepMaxN = 4096;                  // largest n served by the one-launch steps
 
-// Per-target sums over all m sources: on return, lanes tid < kStepTgt hold the
-// NCH channel sums of target blockIdx.x * kStepTgt + tid in v[] (float64).
+// Targets per workgroup for n points on num_cu CUs.
+inline int step_targets(int n, int num_cu) {
+    int t = num_cu > 0 ? (n + num_cu - 1) / num_cu : kStepTgtMax;
+    return t < 1 ? 1 : (t > kStepTgtMax ? kStepTgtMax : t);
+}
+
 template <typename T, class OP>
-__device__ __forceinline__ void step_sums(const typename OP::template Args<T>& args, int n, int m, T* lds,
-                                          double (*red)[OP::NCH][kStepTgt], double* v) {
+constexpr size_t step_smem_bytes() {
+    constexpr size_t tile = size_t(kStepTile) * OP::REC * sizeof(T);
+    constexpr size_t red = size_t(kStepThreads) * OP::NCH * sizeof(double);
+    return tile > red ? tile : red;
+}
+
+// Per-target sums over all m sources: on return, lanes tid < tpw hold the NCH
+// channel sums of target blockIdx.x * tpw + tid in v[] (float64).  smem holds
+// step_smem_bytes<T, OP>() bytes, 16-byte aligned.
+template <typename T, class OP>
+__device__ __forceinline__ void step_sums(const typename OP::template Args<T>& args, int n, int m, int tpw,
+                                          char* smem, double* v) {
     constexpr int NCH = OP::NCH;
     constexpr int REC = OP::REC;
+    T* lds = reinterpret_cast<T*>(smem);
     const int tid = threadIdx.x;
-    const int tg = tid % kStepTgt;
-    const int slice = tid / kStepTgt;
-    const int i = blockIdx.x * kStepTgt + tg;
+    const int nsl = kStepThreads / tpw;
+    const int tg = tid % tpw;
+    const int slice = tid / tpw;
+    const bool active = slice < nsl;
+    const int i = blockIdx.x * tpw + tg;
 
     typename OP::template Tgt<T> tgt;
     OP::load_target(args, i < n ? i : n - 1, tgt);
@@ -61,14 +81,14 @@ __device__ __forceinline__ void step_sums(const typename OP::template Args<T>& a
             const int jn = jt + kStepTile + tid;         // prefetch the next tile
             OP::load_raw(args, jn < m ? jn : -1, raw);
         }
-        // records past m are zero-weight padding; every slice runs the same count
         const int cnt = m - jt < kStepTile ? m - jt : kStepTile;
-        const int tn = (cnt + kStepSlices - 1) / kStepSlices * kStepSlices;
         T acc[NCH];
 #pragma unroll
         for (int c = 0; c < NCH; ++c) acc[c] = T(0);
+        if (active) {
 #pragma unroll 2
-        for (int r = slice; r < tn; r += kStepSlices) OP::pair(args, tgt, &lds[r * REC], acc);
+            for (int r = slice; r < cnt; r += nsl) OP::pair(args, tgt, &lds[r * REC], acc);
+        }
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             if constexpr (kCompensated<T>) two_sum_acc(hi[c], lo[c], acc[c]);
@@ -76,31 +96,49 @@ __device__ __forceinline__ void step_sums(const typename OP::template Args<T>& a
         }
     }
 
-    // slices of one target: across the wave (lanes tg, tg + 16, tg + 32, tg + 48) ...
-    const int wave = tid / 64;
+    // combine the slices of each target: red[(slice * NCH + c) * tpw + tg]
+    double* red = reinterpret_cast<double*>(smem);
+    __syncthreads();                                     // last tile consumed
+    if (active) {
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-        double s = kCompensated<T> ? double(hi[c]) + double(lo[c]) : double(hi[c]);
-        s += __shfl_xor(s, 16, 64);
-        s += __shfl_xor(s, 32, 64);
-        if ((tid & 63) < kStepTgt) red[wave][c][tg] = s;
+        for (int c = 0; c < NCH; ++c)
+            red[(slice * NCH + c) * tpw + tg] = kCompensated<T> ? double(hi[c]) + double(lo[c]) : double(hi[c]);
     }
     __syncthreads();
-    // ... then across the waves, in wave order
-    if (tid < kStepTgt) {
+    // group sums: lane (g, tg) for g < kStepGroups adds slices g, g + kStepGroups, ...
+    double gs[NCH];
+    const int gtg = tid % tpw;
+    const int g = tid / tpw;
+    const bool gl = g < kStepGroups;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) gs[c] = 0.0;
+    if (gl) {
+        for (int sl = g; sl < nsl; sl += kStepGroups) {
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) gs[c] += red[(sl * NCH + c) * tpw + gtg];
+        }
+    }
+    __syncthreads();                                     // every slice sum read
+    if (gl) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) red[(g * NCH + c) * tpw + gtg] = gs[c];
+    }
+    __syncthreads();
+    if (tid < tpw) {
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
             double s = 0.0;
-            for (int w = 0; w < kStepWaves; ++w) s += red[w][c][tid];
+            for (int q = 0; q < kStepGroups; ++q) s += red[(q * NCH + c) * tpw + tid];
             v[c] = s;
         }
     }
 }
 
-// Sum over the first kStepTgt lanes (one wave), lane 0 holds the result.
+// Sum of d over the first 64 lanes (one wave; lanes past the workgroup's
+// targets hold 0), lane 0 holds the result.
 __device__ __forceinline__ double step_lane_sum(double d) {
 #pragma unroll
-    for (int o = kStepTgt / 2; o > 0; o >>= 1) d += __shfl_xor(d, o, 64);
+    for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o, 64);
     return d;
 }
 
@@ -108,19 +146,18 @@ __device__ __forceinline__ double step_lane_sum(double d) {
 // over the workgroup's targets (the energy terms, summed later in a fixed order).
 template <typename T, int KW, bool MS>
 __global__ void __launch_bounds__(kStepThreads)
-k_step_shoot(const typename OpApplyK<KW, MS>::template Args<T> args, int n, double dt,
+k_step_shoot(const typename OpApplyK<KW, MS>::template Args<T> args, int n, int tpw, double dt,
              const double* __restrict__ x, const double* __restrict__ a, double* __restrict__ xn,
              double* __restrict__ dots, double dscale) {
     using OP = OpApplyK<KW, MS>;
-    __shared__ __attribute__((aligned(16))) T lds[kStepTile * OP::REC];
-    __shared__ double red[kStepWaves][OP::NCH][kStepTgt];
+    __shared__ __attribute__((aligned(16))) char smem[step_smem_bytes<T, OP>()];
     double v[OP::NCH];
-    step_sums<T, OP>(args, n, n, lds, red, v);
+    step_sums<T, OP>(args, n, n, tpw, smem, v);
     const int tid = threadIdx.x;
     if (tid < 64) {
-        const int i = blockIdx.x * kStepTgt + tid;
+        const int i = blockIdx.x * tpw + tid;
         double d = 0.0;
-        if (tid < kStepTgt && i < n) {
+        if (tid < tpw && i < n) {
 #pragma unroll
             for (int c = 0; c < KW; ++c) {
                 const size_t o = size_t(i) * KW + c;
@@ -137,16 +174,15 @@ k_step_shoot(const typename OpApplyK<KW, MS>::template Args<T> args, int n, doub
 // p_t = p_{t+1} + dt fdiff (lddmm covector sum), grad_t = -dt (kernel sum of q).
 template <typename T, int KW, bool MS>
 __global__ void __launch_bounds__(kStepThreads)
-k_step_adjoint(const typename OpAdjoint<KW, MS>::template Args<T> args, int n, double dt, double fdiff,
-               const double* __restrict__ pn, double* __restrict__ pc, double* __restrict__ g) {
+k_step_adjoint(const typename OpAdjoint<KW, MS>::template Args<T> args, int n, int tpw, double dt,
+               double fdiff, const double* __restrict__ pn, double* __restrict__ pc, double* __restrict__ g) {
     using OP = OpAdjoint<KW, MS>;
-    __shared__ __attribute__((aligned(16))) T lds[kStepTile * OP::REC];
-    __shared__ double red[kStepWaves][OP::NCH][kStepTgt];
+    __shared__ __attribute__((aligned(16))) char smem[step_smem_bytes<T, OP>()];
     double v[OP::NCH];
-    step_sums<T, OP>(args, n, n, lds, red, v);
+    step_sums<T, OP>(args, n, n, tpw, smem, v);
     const int tid = threadIdx.x;
-    const int i = blockIdx.x * kStepTgt + tid;
-    if (tid < kStepTgt && i < n) {
+    const int i = blockIdx.x * tpw + tid;
+    if (tid < tpw && i < n) {
 #pragma unroll
         for (int c = 0; c < KW; ++c) {
             const size_t o = size_t(i) * KW + c;
