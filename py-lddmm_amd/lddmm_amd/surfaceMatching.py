@@ -747,6 +747,53 @@ class SurfaceMatching:
             vtk_io.saveTrajectories(os.path.join(outdir, self.options['saveFile'] + 'curves.vtk'),
                                     st['xt'].cpu().numpy())
         self.saveEvolution(self.fvInit, st, fileName=self.options['saveFile'])
+        self.saveHdf5(os.path.join(outdir, 'output.h5'))
+
+    def saveHdf5(self, fileName):
+        """The result file of base/surfaceMatching.py:1506-1555: the same groups
+        and datasets under the same paths ('LDDMM Results/parameters/Time
+        steps', ..., 'LDDMM Results/descriptors/Displacement'), written as a
+        numpy .npz archive keyed by those paths ('.npz' is appended to a name
+        without it); h5py, which the reference writes with, is not part of
+        this build's environment.  The descriptors come from the trajectory of
+        the current control: 'Jacobian' is J_T (the log-Jacobian carried by the
+        flow), 'Surface Jacobian' the vertex areas at x_T over those at x_0,
+        'Displacement' x_T - x_0.  Returns the path written."""
+        if not isinstance(self.fv1, Surface):
+            raise NotImplementedError("saveHdf5 is built for surface targets")
+        Kd, Kv = self.options['KparDiff'], self.options['KparDist']
+        _, st = self.objectiveFunDef(self.control, withTrajectory=True, withJacobian=True)
+        xt = st['xt'].cpu().numpy()
+        Jt = st['Jt'].cpu().numpy()
+        fvDef = Surface(surf=self.fvInit)
+        fvDef.updateVertices(xt[-1])
+        AV = fvDef.computeVertexArea()[0] / self.fv0.computeVertexArea()[0]
+        at = self.control['at']
+        at = at.cpu().numpy() if isinstance(at, torch.Tensor) else np.asarray(at)
+        r = 'LDDMM Results/'
+        data = {
+            r + 'parameters/Time steps': np.array(self.Tsize),
+            r + 'parameters/Deformation Kernel type': np.array(Kd.name),
+            r + 'parameters/Deformation Kernel width': np.asarray(Kd.sigma),
+            r + 'parameters/Deformation Kernel order': np.array(Kd.order),
+            r + 'parameters/Spatial Varifold Kernel type': np.array(Kv.name),
+            r + 'parameters/Spatial Varifold width': np.asarray(Kv.sigma),
+            r + 'parameters/Spatial Varifold order': np.array(Kv.order),
+            r + 'template/vertices': np.asarray(self.fv0.vertices),
+            r + 'template/faces': np.asarray(self.fv0.faces),
+            r + 'target/vertices': np.asarray(self.fv1.vertices),
+            r + 'target/faces': np.asarray(self.fv1.faces),
+            r + 'deformedTemplate/vertices': xt[-1],
+            r + 'variables/alpha': at,
+            r + 'variables/affine': np.array('None'),
+            r + 'descriptors/Jacobian': Jt[-1],
+            r + 'descriptors/Surface Jacobian': AV,
+            r + 'descriptors/Displacement': xt[-1] - xt[0],
+        }
+        path = fileName if fileName.endswith('.npz') else fileName + '.npz'
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        np.savez(path, **data)
+        return path
 
     def endOfIteration(self, forceSave=False):
         """base/surfaceMatching.py:1348-1503.  File output (saveOutputs) runs every

@@ -212,7 +212,8 @@ def test_registration_writes_the_evolution_with_its_jacobian(env, tmp_path):
     """writeOutput: every saveRate iterations and at the end, one legacy VTK
     file per time step (base/surfaceMatching.py:1219-1290, :1370-1432); the
     last files hold exp(J_t), the vertex-area ratio and the velocity of the
-    final control, and the point trajectories (saveTrajectories)."""
+    final control, and the point trajectories (saveTrajectories); the result
+    file holds the datasets of the reference's saveHdf5 under its paths."""
     from lddmm_amd import synth
     from lddmm_amd.surfaces import Surface
     from lddmm_amd.surfaceMatching import SurfaceMatching
@@ -225,7 +226,7 @@ def test_registration_writes_the_evolution_with_its_jacobian(env, tmp_path):
                                   'outputDir': str(tmp_path / "out"), 'saveTrajectories': True})
     sm.optimizeMatching()
     names = sorted(p.name for p in (tmp_path / "out").iterdir())
-    assert names == ["evolution%03d.vtk" % k for k in range(6)] + ["evolutioncurves.vtk"]
+    assert names == ["evolution%03d.vtk" % k for k in range(6)] + ["evolutioncurves.vtk", "output.h5.npz"]
     at = sm.control['at'].cpu().numpy()
     xr = O.landmarkDirectEvolutionEuler(v, at, [2.5])
     Jr = O.jacobian(xr, at, [2.5])
@@ -244,3 +245,20 @@ def test_registration_writes_the_evolution_with_its_jacobian(env, tmp_path):
     assert np.all(_vtk_field(str(tmp_path / "out" / "evolution000.vtk"), "SCALARS", "displacement", nv) == 0)
     # the template grows toward the larger target: the mean Jacobian exceeds 1
     assert np.mean(_vtk_field(str(tmp_path / "out" / "evolution005.vtk"), "SCALARS", "Jacobian", nv)) > 1.0
+    # the result file (base/surfaceMatching.py:1506-1555 layout, npz archive)
+    res = np.load(str(tmp_path / "out" / "output.h5.npz"))
+    r = 'LDDMM Results/'
+    assert int(res[r + 'parameters/Time steps']) == 5
+    assert str(res[r + 'parameters/Deformation Kernel type']) == 'gauss'
+    assert float(res[r + 'parameters/Deformation Kernel width'][0]) == 2.5
+    assert float(res[r + 'parameters/Spatial Varifold width'][0]) == 1.0
+    np.testing.assert_array_equal(res[r + 'template/vertices'], v)
+    np.testing.assert_array_equal(res[r + 'template/faces'], f)
+    np.testing.assert_array_equal(res[r + 'target/vertices'], v * 1.15)
+    np.testing.assert_array_equal(res[r + 'variables/alpha'], at)
+    assert str(res[r + 'variables/affine']) == 'None'
+    assert rel(res[r + 'deformedTemplate/vertices'], xr[-1]) < 1e-12
+    assert rel(res[r + 'descriptors/Jacobian'], Jr[-1]) < 1e-12
+    assert rel(res[r + 'descriptors/Surface Jacobian'],
+               Surface((f, xr[-1])).computeVertexArea()[0] / Surface((f, v)).computeVertexArea()[0]) < 1e-10
+    assert rel(res[r + 'descriptors/Displacement'], xr[-1] - xr[0]) < 1e-12
