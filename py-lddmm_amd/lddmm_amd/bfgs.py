@@ -243,7 +243,9 @@ def bfgs(opt, verb=True, maxIter=1000, TestGradient=False, epsInit=0.01, memory=
             run.f_old = None
             run.grad_next = None
 
-        grad = opt.getGradient(coeff) if run.grad_next is None else ops.copy(run.grad_next)
+        # directions and gradients are never modified in place (every update
+        # makes a new object), so they are shared rather than copied
+        grad = opt.getGradient(coeff) if run.grad_next is None else run.grad_next
         if TestGradient:
             _fd_gradient_check(opt, ops, grad, run.f, coeff)
 
@@ -251,7 +253,7 @@ def bfgs(opt, verb=True, maxIter=1000, TestGradient=False, epsInit=0.01, memory=
         # the first iteration and after a restart
         if restarted or run.it == 0:
             run.memory.clear()
-            q = ops.copy(grad)
+            q = grad
         else:
             run.memory.push(run.step_taken, ops.axpy(grad, run.grad_old, -1))
             if hasattr(opt, 'twoLoop'):
@@ -262,8 +264,8 @@ def bfgs(opt, verb=True, maxIter=1000, TestGradient=False, epsInit=0.01, memory=
         # together; read back when first needed (on most iterations after the
         # line search's first trial, whose own read-back waits for them)
         dots = ops.table([grad, q], [grad, q])
-        direction = ops.copy(q)
-        run.grad_old = ops.copy(grad)
+        direction = q
+        run.grad_old = grad
         if run.it == 0 or run.it == burn_in:
             run.grad_eps = gradient_threshold(run.grad_eps, np.sqrt(dots()[0][0]))
             logging.info(f'Gradient threshold: {run.grad_eps:.6f}')

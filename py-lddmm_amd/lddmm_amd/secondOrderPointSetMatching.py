@@ -89,9 +89,10 @@ class GeodesicControl:
         obj = 0.5 * (a0 * self.options['KparDiff'].applyK(self.x0_dev, a0)).sum()
         return (obj, st) if (withTrajectory or withJacobian) else obj
 
-    def getGradient(self, coeff=1.0, update=None):
+    def getGradient(self, coeff=1.0, update=None, endpoint_gradient=None):
         """grad['a0'] = (K(x0) a0 - pa_0) / coeff with px1 = -dataTerm'(x_T),
-        pa1 = 0 (base/secondOrderPointSetMatching.py:379-460, no affine)."""
+        pa1 = 0 (base/secondOrderPointSetMatching.py:379-460, no affine);
+        endpoint_gradient: dataTerm'(x_T) when already computed."""
         if update is None:
             control, st = self.control, self.state
         else:
@@ -104,7 +105,7 @@ class GeodesicControl:
                     if update[0][k] is not None:
                         control[k] = self.control[k] - update[1] * update[0][k]
                 st = self.solveStateEquation(control=control)
-        px1 = -self.endPointGradient_device(st['xt'][-1])
+        px1 = -(self.endPointGradient_device(st['xt'][-1]) if endpoint_gradient is None else endpoint_gradient)
         pa1 = torch.zeros_like(px1)
         _, _, da0 = evol._epdiff_adjoint(st['xt'], st['at'], px1, pa1, self.options['KparDiff'], self.Tsize)
         grd = Control()

@@ -297,6 +297,25 @@ class SurfaceMatching:
         s2 = self.options['sigmaError'] ** 2
         return g if s2 == 1 else g / s2
 
+    def _fused_data_gradient(self):
+        """True when one varifold pass can give both the data term and its
+        gradient: the surface varifold term of this class (the gradient pass
+        accumulates the energy rows as well)."""
+        return (self.options['errorType'] == 'varifold'
+                and type(self).dataTerm_device is SurfaceMatching.dataTerm_device
+                and type(self).endPointGradient_device is SurfaceMatching.endPointGradient_device)
+
+    def dataTermAndGradient_device(self, xT):
+        """(data term as a device scalar, its gradient [N, 3]) at xT, both / sigmaError^2:
+        for the surface varifold one pass of the gradient kernel, whose epilogue
+        also forms the energy rows (equal to dataTerm_device's value to
+        rounding); otherwise the two evaluations."""
+        if not self._fused_data_gradient():
+            return self.dataTerm_device(xT), self.endPointGradient_device(xT)
+        e, g = sd.varifold_device(self._dist_kd(), self.fv0, self.fv1, energy=True, grad=True, v1=xT)
+        s2 = self.options['sigmaError'] ** 2
+        return (e[0], g) if s2 == 1 else (e[0] / s2, g / s2)
+
     def endPointGradient(self, endPoint=None):
         xT = self.state['xt'][-1] if endPoint is None else B.to_dev(endPoint.vertices)
         return self.endPointGradient_device(xT).cpu().numpy()
@@ -338,15 +357,20 @@ class SurfaceMatching:
     def getVariable(self):
         return self.control
 
-    def _trial(self, dr, eps):
+    def _trial(self, dr, eps, with_gradient=False):
         """Control - eps * dr, its trajectory and its (deformation, data) terms
-        as device scalars; nothing is read back."""
+        as device scalars, and with_gradient the data term's gradient at the
+        trajectory's end (else None); nothing is read back."""
         controlTry = Control()
         for k in dr.keys():
             if dr[k] is not None:
-                controlTry[k] = self.control[k] - eps * dr[k]
+                controlTry[k] = torch.add(self.control[k], dr[k], alpha=-eps)
         obj_, stateTry = self.objectiveFunDef(controlTry, withTrajectory=True)
-        return controlTry, stateTry, obj_, self.dataTerm_device(stateTry['xt'][-1])
+        xT = stateTry['xt'][-1]
+        if with_gradient:
+            data, g = self.dataTermAndGradient_device(xT)
+            return controlTry, stateTry, obj_, data, g
+        return controlTry, stateTry, obj_, self.dataTerm_device(xT), None
 
     def _record_try(self, dr, eps, controlTry, stateTry, terms, objRef):
         objTry = self.obj0 + float(terms[0] + terms[1])
@@ -363,7 +387,7 @@ class SurfaceMatching:
 
     def updateTry(self, dr, eps, objRef=None):
         """Objective at control - eps * dr (base/surfaceMatching.py:638-675)."""
-        controlTry, stateTry, obj_, data = self._trial(dr, eps)
+        controlTry, stateTry, obj_, data, _ = self._trial(dr, eps)
         return self._record_try(dr, eps, controlTry, stateTry, self._host_terms(obj_, data), objRef)
 
     def updateTryWithSlope(self, dr, eps, coeff=1.0):
@@ -374,10 +398,13 @@ class SurfaceMatching:
         (objective, slope, gradient) with the values of updateTry(dr, eps),
         -dotProduct_euclidean(g, [dr])[0] and g = getGradient(coeff,
         update=[dr, eps]).  The line search uses it to evaluate the slope of
-        a trial without a second round trip."""
-        controlTry, stateTry, obj_, data = self._trial(dr, eps)
+        a trial without a second round trip.  For the surface varifold the
+        data term and its gradient come from one varifold pass
+        (dataTermAndGradient_device): the objective equals updateTry's to
+        rounding."""
+        controlTry, stateTry, obj_, data, egrad = self._trial(dr, eps, with_gradient=True)
         self._last_try = (dr, eps, controlTry, stateTry)
-        grad = self.getGradient(coeff, update=[dr, eps])
+        grad = self.getGradient(coeff, update=[dr, eps], endpoint_gradient=egrad)
         dot = self._dot_table_device([grad], [dr])
         parts = [obj_.reshape(1), data.reshape(1), dot.reshape(1)]
         pending = getattr(self, '_pending_table', None)
@@ -402,8 +429,10 @@ class SurfaceMatching:
         self._data_cur = getattr(self, '_data_try', None)
 
     # ------------------------------------------------------------ gradient
-    def getGradient(self, coeff=1.0, update=None):
-        """Euclidean gradient wrt at (base/surfaceMatching.py:1040-1105)."""
+    def getGradient(self, coeff=1.0, update=None, endpoint_gradient=None):
+        """Euclidean gradient wrt at (base/surfaceMatching.py:1040-1105).
+        endpoint_gradient: the data term's gradient at the trajectory's end
+        when already computed."""
         if update is None:
             control = self.control
             xt = self.state['xt']
@@ -420,7 +449,7 @@ class SurfaceMatching:
                     xt, _ = self._unreduced_forward(control, energy=False)
                 else:
                     xt, _ = evol.shoot(self.x0_dev, control['at'], self.options['KparDiff'])
-        px1 = -self.endPointGradient_device(xt[-1])
+        px1 = -(self.endPointGradient_device(xt[-1]) if endpoint_gradient is None else endpoint_gradient)
         if self.unreduced:
             dct, dat = self._unreduced_gradient(control, xt, px1)
             grd = Control()
@@ -573,7 +602,7 @@ class SurfaceMatching:
         dr = Control()
         for k in dir1.keys():
             if dir1[k] is not None and dir2[k] is not None:
-                dr[k] = dir1[k] + beta * dir2[k]
+                dr[k] = torch.add(dir1[k], dir2[k], alpha=beta)
         return dr
 
     def prod(self, dir1, beta):

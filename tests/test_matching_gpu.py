@@ -69,10 +69,14 @@ def test_short_registration_decreases(env):
 @pytest.mark.parametrize("pk_dtype", ["float64", "float32"])
 def test_fused_trial_slope_keeps_the_iterates(env, pk_dtype, monkeypatch):
     """The line search's fused trial (objective, gradient and slope in one
-    read-back: updateTryWithSlope) and BFGS's batched dot table (dotTable)
-    change the number of host synchronisations only: the same registration
-    without them takes bit-identical iterates (objective history, line-search
-    decisions and final control)."""
+    read-back, the varifold data term and its gradient from one gradient pass:
+    updateTryWithSlope) and BFGS's batched dot table (dotTable) against the
+    same registration without them.  The gradients are the same kernels'
+    values; the trial's data term comes from the gradient pass's energy rows
+    instead of the energy pass, equal to rounding (float64: 1e-15 relative;
+    the compensated float32 form: about 1e-7).  Float64: the same decisions,
+    objective histories within 1e-12 and final controls within 1e-9.
+    Float32: the same final objective within 1e-5."""
     synth, SurfaceMatching, Surface, Kernel, torch = env
     (v, f), (v2, _) = synth.plain_matching_pair(seed=3, subdiv=3, radius=10.0, nbumps=20, width=3.0)
 
@@ -87,7 +91,10 @@ def test_fused_trial_slope_keeps_the_iterates(env, pk_dtype, monkeypatch):
     monkeypatch.setattr(SurfaceMatching, "updateTryWithSlope", None)
     monkeypatch.setattr(SurfaceMatching, "dotTable", None)
     h2, d2, a2 = run()
-    assert len(h1) >= 10
-    assert h1 == h2
-    assert d1 == d2
-    np.testing.assert_array_equal(a1, a2)
+    assert len(h1) >= 10 and len(h2) >= 10
+    if pk_dtype == "float64":
+        assert d1 == d2
+        np.testing.assert_allclose(h1, h2, rtol=1e-12)
+        assert rel(a1, a2) < 1e-9
+    else:
+        assert abs(h1[-1] - h2[-1]) <= 1e-5 * abs(h2[-1])
