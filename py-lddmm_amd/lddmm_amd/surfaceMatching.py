@@ -148,8 +148,10 @@ class SurfaceMatching:
             v1 = self.fv1.surfVolume()
             if v0 * v1 < 0:
                 self.fv1.flipFaces()
-            self.fv0ori = -1 if self.fvInit.surfVolume() < 0 else 1
-            self.fv1ori = -1 if self.fv1.surfVolume() < 0 else 1
+                v1 = -v1                 # flipping every face negates the volume
+            # fvInit is the template's copy: the same volume
+            self.fv0ori = -1 if v0 < 0 else 1
+            self.fv1ori = -1 if v1 < 0 else 1
         else:
             self.fv0ori = 1
             self.fv1ori = 1
@@ -340,7 +342,7 @@ class SurfaceMatching:
         st['xt'] = xt
         if withJacobian:
             st['Jt'] = evol.jacobian(xt, B.to_dev(control['at']), self.options['KparDiff'])
-        obj = e[0] * self.ds ** 2
+        obj = e[0] if self.ds == 1 else e[0] * self.ds ** 2
         return (obj, st) if (withTrajectory or withJacobian) else obj
 
     def objectiveFun(self):

@@ -258,10 +258,11 @@ class Surface:
         face order) and its multiplicity."""
         f = self.faces
         nf = f.shape[0]
-        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
-        es = np.sort(e, axis=1).astype(np.int64)
-        nv = max(int(self.vertices.shape[0]), int(es.max()) + 1 if es.size else 1)
-        keys = es[:, 0] * nv + es[:, 1]
+        a = np.concatenate([f[:, 0], f[:, 1], f[:, 2]]).astype(np.int64)
+        b = np.concatenate([f[:, 1], f[:, 2], f[:, 0]]).astype(np.int64)
+        lo, hi = np.minimum(a, b), np.maximum(a, b)
+        nv = max(int(self.vertices.shape[0]), int(hi.max()) + 1 if hi.size else 1)
+        keys = lo * nv + hi
         order = np.argsort(keys, kind='stable')
         ks = keys[order]
         first = np.ones(len(ks), dtype=bool)
