@@ -19,10 +19,8 @@ requests are gathered into one batched evaluation per round.  The reference
 has no population driver; per pair, results equal SurfaceMatching's up to
 floating-point summation order.
 """
-import concurrent.futures as cf
 import ctypes
 import logging
-import os
 import time
 
 import numpy as np
@@ -33,6 +31,47 @@ from . import surfaceDistances as sd
 from .kernelFunctions import Kernel
 from .surfaces import Surface
 
+
+
+def closed_and_volumes(surfs):
+    """Per surface: closed (no edge belongs to exactly one face, the test of
+    Surface.getEdges' boundary flags) and the signed volume sum_f det(v[f]) / 6
+    (Surface.surfVolume), for a list of surfaces in one device pass: the
+    edges of all surfaces, keyed by global vertex ids, are sorted together.
+    Returns (closed [S] bool, volumes [S] float64) on the host."""
+    S = len(surfs)
+    nv = np.array([s.vertices.shape[0] for s in surfs], dtype=np.int64)
+    nf = np.array([s.faces.shape[0] for s in surfs], dtype=np.int64)
+    voff = np.concatenate([[0], np.cumsum(nv)])
+    foff = np.concatenate([[0], np.cumsum(nf)])
+    dev = B.device()
+    F = torch.from_numpy(np.concatenate([np.asarray(s.faces, dtype=np.int64) + o
+                                         for s, o in zip(surfs, voff[:-1])])).to(dev)
+    X = B.to_dev(np.concatenate([np.asarray(s.vertices, dtype=np.float64) for s in surfs]))
+    a = torch.cat([F[:, 0], F[:, 1], F[:, 2]])
+    b = torch.cat([F[:, 1], F[:, 2], F[:, 0]])
+    keys, _ = torch.sort(torch.minimum(a, b) * int(voff[-1]) + torch.maximum(a, b))
+    new = torch.ones_like(keys, dtype=torch.bool)
+    new[1:] = keys[1:] != keys[:-1]
+    start = torch.nonzero(new).flatten()
+    cnt = torch.diff(torch.cat([start, torch.tensor([keys.numel()], device=dev)]))
+    lone = keys[start[cnt == 1]] // int(voff[-1])       # smaller vertex of every one-face edge
+    owner = torch.searchsorted(torch.from_numpy(voff[1:]).to(dev), lone, right=True)
+    open_ = torch.zeros(S, dtype=torch.bool, device=dev)
+    open_[owner] = True
+    det = (X[F[:, 0]] * torch.cross(X[F[:, 1]], X[F[:, 2]], dim=1)).sum(1)
+    det = det.cpu().numpy()
+    vol = np.array([det[foff[i]:foff[i + 1]].sum() / 6 for i in range(S)])
+    return ~open_.cpu().numpy(), vol
+
+
+def to_device_async(a, device):
+    """Host numpy array -> device tensor without waiting for the GPU: the
+    values go through page-locked memory from torch's caching host allocator
+    and are copied on the current stream (a pageable copy would block until
+    every queued kernel has finished)."""
+    t = torch.from_numpy(np.ascontiguousarray(a)).pin_memory()
+    return t.to(device, non_blocking=True)
 
 class Population:
     """Device layout of P template-target surface pairs (vertices [V, 3] of all
@@ -147,8 +186,7 @@ class Population:
 
     def rows(self, mask):
         """Boolean device mask [V] of the rows of the pairs selected by mask [P]."""
-        m = torch.as_tensor(np.asarray(mask, dtype=bool), device=self.x0.device)
-        return m[self.vpair]
+        return to_device_async(np.asarray(mask, dtype=bool), self.x0.device)[self.vpair]
 
 
 # ---------------------------------------------------------------------------
@@ -221,18 +259,14 @@ class PopulationMatching:
         self.fv0 = [Surface(surf=s) for s in templates]
         self.fv1 = [Surface(surf=s) for s in targets]
 
-        # orientation as SurfaceMatching.fix_orientation (base/surfaceMatching.py:284-318);
-        # the pairs are independent and numpy's sorts release the GIL, so a
-        # thread pool checks them side by side
-        def orient(ab):
-            a, b = ab
-            a.getEdges()
-            b.getEdges()
-            if a.bdry.max() == 0 and b.bdry.max() == 0 and a.surfVolume() * b.surfVolume() < 0:
-                b.flipFaces()
-        workers = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else 4))
-        with cf.ThreadPoolExecutor(workers) as ex:
-            list(ex.map(orient, zip(self.fv0, self.fv1)))
+        # orientation as SurfaceMatching.fix_orientation (base/surfaceMatching.py:284-318):
+        # when template and target are both closed and their signed volumes
+        # disagree, the target's faces are flipped; all pairs in one device pass
+        closed, vol = closed_and_volumes(self.fv0 + self.fv1)
+        P_ = len(self.fv0)
+        for p in range(P_):
+            if closed[p] and closed[P_ + p] and vol[p] * vol[P_ + p] < 0:
+                self.fv1[p].flipFaces()
         self.pop = Population(self.fv0, self.fv1)
         self.pop.set_graphs(o['graphs'])
         self.P = self.pop.P
@@ -267,7 +301,10 @@ class PopulationMatching:
             k = Kernel(name=kv_.name, sigma=kv_.sigma, order=kv_.order)
             k.ms_exponent = kv_.ms_exponent
             k.pk_dtype = dt
-            self.obj0_by[dt] = np.array([sd.varifoldNorm0(t, k) / s2 for t in self.fv1])
+            kd = sd._kd(k)
+            # varifoldNorm0 of every target, read back in one transfer
+            e = torch.cat([sd.varifold_device(kd, t)[0] for t in self.fv1])
+            self.obj0_by[dt] = e.cpu().numpy() / s2
         self.obj0 = self.obj0_by[self.dtype0].copy()
         self.obj = np.full(self.P, np.nan)
         self.stats = {"evaluations": 0, "gradients": 0, "rounds": 0}
@@ -306,6 +343,32 @@ class PopulationMatching:
         out[np.isnan(out) & np.asarray(mask, dtype=bool)] = 1e500
         return out
 
+    def _objective_and_gradient(self, mask, control, state, grad):
+        """_objective (updateTry form) of the pairs in mask, with their gradients
+        written to `grad` as _gradient would: the varifold gradient pass also
+        yields the energies, so a trial costs one varifold pass (the line
+        search asks for the slope at the same step whenever the trial passes
+        the sufficient-decrease test; SurfaceMatching.updateTryWithSlope does
+        the same per pair).  -> [P] host objectives (nan elsewhere)."""
+        out = np.full(self.P, np.nan)
+        s2 = self.options['sigmaError'] ** 2
+        for dt, m in self._groups(mask):
+            self.pop.set_active(m)
+            kdd = self._kd(self.Kdiff, dt)
+            self.pop.shoot(kdd, control, state, self.options['regWeight'], self.EN)
+            self.pop.varifold(self._kd(self.Kdist, dt), state[-1], energies=self.ED, grad=self.PX1)
+            rows = self.pop.rows(m)
+            torch.where(rows[:, None], self.PX1 / -s2, self.PX1, out=self.PX1)
+            self.pop.adjoint(kdd, control, state, self.PX1, grad, self.options['regWeight'], euclidean=True)
+            en = self.EN.cpu().numpy()
+            ed = self.ED.cpu().numpy()
+            v = self.obj0 + en + ed / s2
+            out[m] = v[m]
+            self.stats["evaluations"] += int(m.sum())
+            self.stats["gradients"] += int(m.sum())
+        out[np.isnan(out) & np.asarray(mask, dtype=bool)] = 1e500
+        return out
+
     def _gradient(self, mask, control, state, grad):
         """Euclidean gradient wrt the control (SurfaceMatching.getGradient) of the pairs in mask."""
         s2 = self.options['sigmaError'] ** 2
@@ -313,7 +376,7 @@ class PopulationMatching:
             self.pop.set_active(m)
             self.pop.varifold(self._kd(self.Kdist, dt), state[-1], grad=self.PX1)
             rows = self.pop.rows(m)
-            self.PX1[rows] = -self.PX1[rows] / s2
+            torch.where(rows[:, None], self.PX1 / -s2, self.PX1, out=self.PX1)
             self.pop.adjoint(self._kd(self.Kdiff, dt), control, state, self.PX1, grad,
                              self.options['regWeight'], euclidean=True)
             self.stats["gradients"] += int(m.sum())
@@ -390,7 +453,7 @@ class PopulationMatching:
             use = live & gval
             if use.any():
                 rows = self.pop.rows(use)
-                self.G[:, rows] = self.GT[:, rows]
+                torch.where(rows[None, :, None], self.GT, self.G, out=self.G)
             # L-BFGS two-loop for pairs with memory
             app = live & ~self.reset & (it > 0)
             clear = live & ~app
@@ -400,7 +463,7 @@ class PopulationMatching:
                 Y = self.G - self.GOLD
                 S = self.DIFF.clone()
                 sy = self._dot(Y, S, app)
-                rho = torch.where(torch.as_tensor(app, device=sy.device), 1.0 / sy, torch.zeros_like(sy))
+                rho = torch.where(to_device_async(app, sy.device), 1.0 / sy, torch.zeros_like(sy))
                 slots.append({"S": S, "Y": Y, "rho": rho, "valid": app.copy()})
                 # at most `memory` entries per pair
                 cnt = np.zeros(P, dtype=int)
@@ -414,7 +477,7 @@ class PopulationMatching:
                 for s in reversed(slots):
                     m = s["valid"] & app
                     dq = self._dot(s["S"], q, m)
-                    alpha = torch.where(torch.as_tensor(m, device=dq.device), s["rho"] * dq, torch.zeros_like(dq))
+                    alpha = torch.where(to_device_async(m, dq.device), s["rho"] * dq, torch.zeros_like(dq))
                     alphas.append(alpha)
                     self.pop.set_active(m)
                     self.pop.seg_axpy(q, -alpha, s["Y"])
@@ -424,21 +487,21 @@ class PopulationMatching:
                 yy = self._dot(last["Y"], last["Y"], app)
                 c = (ys / yy).cpu().numpy()
                 c = np.where(c < 1e-10, 1.0, c)
-                cv = torch.as_tensor(np.where(app, c, 1.0), device=q.device)
+                cv = to_device_async(np.where(app, c, 1.0), q.device)
                 q *= cv[self.pop.vpair][None, :, None]
                 for s, alpha in zip(slots, alphas):
                     m = s["valid"] & app
                     dy = self._dot(s["Y"], q, m)
                     beta = s["rho"] * dy
-                    coef = torch.where(torch.as_tensor(m, device=dy.device), alpha - beta, torch.zeros_like(dy))
+                    coef = torch.where(to_device_async(m, dy.device), alpha - beta, torch.zeros_like(dy))
                     self.pop.set_active(m)
                     self.pop.seg_axpy(q, coef, s["S"])
             grd2 = self._dot(self.G, self.G, live).cpu().numpy()
             qq = self._dot(q, q, live).cpu().numpy()
             grdTry = np.sqrt(np.maximum(1e-20, qq))
             lrows = self.pop.rows(live)
-            self.DIR[:, lrows] = q[:, lrows]
-            self.GOLD[:, lrows] = self.G[:, lrows]
+            torch.where(lrows[None, :, None], q, self.DIR, out=self.DIR)
+            torch.where(lrows[None, :, None], self.G, self.GOLD, out=self.GOLD)
             t_init = np.where((it == 0) | self.reset, 1.0 / grdTry, 1.0)
             upd = live & ((it == 0) | (it == burnIn))
             gradEps[upd] = np.minimum(gradEps[upd], 0.0001 * np.sqrt(grd2[upd]))
@@ -446,6 +509,7 @@ class PopulationMatching:
             derphi0 = -self._dot(self.G, self.DIR, live).cpu().numpy()
             gens, reqs = {}, {}
             res = {}
+            gt_step = np.full(P, np.nan)      # step at which GT holds each pair's trial gradient
             for p in np.nonzero(live)[0]:
                 g = _line_search(t_init[p], derphi0[p], self.obj[p], c1=1e-4, c2=0.9, maxiter=10)
                 gens[p] = g
@@ -460,13 +524,17 @@ class PopulationMatching:
                 vals = {}
                 if phi.any():
                     rows = self.pop.rows(phi)
-                    trow = torch.as_tensor(tv, device=self.A.device)[self.pop.vpair]
-                    self.CT[:, rows] = (self.A - trow[None, :, None] * self.DIR)[:, rows]
-                    f = self._objective(phi, self.CT, self.XTT)
+                    trow = to_device_async(tv, self.A.device)[self.pop.vpair]
+                    torch.where(rows[None, :, None], self.A - trow[None, :, None] * self.DIR, self.CT, out=self.CT)
+                    f = self._objective_and_gradient(phi, self.CT, self.XTT, self.GT)
+                    gt_step[phi] = tv[phi]
                     for p in np.nonzero(phi)[0]:
                         vals[p] = f[p]
                 if der.any():
-                    self._gradient(der, self.CT, self.XTT, self.GT)
+                    stale = der & ~(gt_step == tv)
+                    if stale.any():
+                        self._gradient(stale, self.CT, self.XTT, self.GT)
+                        gt_step[stale] = tv[stale]
                     d = -self._dot(self.GT, self.DIR, der).cpu().numpy()
                     for p in np.nonzero(der)[0]:
                         vals[p] = d[p]
@@ -497,10 +565,11 @@ class PopulationMatching:
                     newreset[p] = True
             if acc.any():
                 rows = self.pop.rows(acc)
-                erow = torch.as_tensor(eps, device=self.A.device)[self.pop.vpair]
-                self.DIFF[:, rows] = (-erow[None, :, None] * self.DIR)[:, rows]
-                self.A[:, rows] = self.CT[:, rows]
-                self.XT[:, rows] = self.XTT[:, rows]
+                erow = to_device_async(eps, self.A.device)[self.pop.vpair]
+                rows3 = rows[None, :, None]
+                torch.where(rows3, -erow[None, :, None] * self.DIR, self.DIFF, out=self.DIFF)
+                torch.where(rows3, self.CT, self.A, out=self.A)
+                torch.where(rows3, self.XTT, self.XT, out=self.XT)
             for p in np.nonzero(live & ~done)[0]:
                 if not np.isnan(obj_old[p]) and abs(self.obj[p] - obj_old[p]) < 1e-7:
                     if it[p] > burnIn or self.reset[p]:

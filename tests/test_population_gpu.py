@@ -204,3 +204,26 @@ def test_lockstep_registration_follows_surface_matching(env, dtype):
         assert len(h1) == len(h2), (p, len(h1), len(h2))
         assert np.max(np.abs(h1 - h2) / np.abs(h2)) < tol, (p, h1, h2)
         assert int(pm.iterations[p]) == int(sm.bfgs_iterations)
+
+
+def test_batched_orientation_check_matches_surface_methods(env):
+    """closed_and_volumes (all surfaces' edges sorted together on the device)
+    against Surface.getEdges' boundary flags and Surface.surfVolume: closed
+    spheres, flipped ones (negative volume), a mesh with a face removed (open)
+    and an ellipsoid hull."""
+    from lddmm_amd import synth
+    from lddmm_amd.population import closed_and_volumes
+    Surface = env["Surface"]
+    v1, f1 = synth.icosphere(2, 3.0)
+    v2, f2 = synth.icosphere(3, 1.5)
+    surfs = [Surface((f1, v1)), Surface((f2[:, [0, 2, 1]], v2 + 4.0)), Surface((f1[1:], v1)),
+             Surface((f2, v2 * np.array([2.0, 1.0, 0.5])))]
+    (vh, fh), _ = synth.population_pairs(1, seed=3)[0]
+    surfs.append(Surface((fh, vh)))
+    closed, vol = closed_and_volumes(surfs)
+    for s, c, v in zip(surfs, closed, vol):
+        s.getEdges()
+        assert bool(c) == (s.bdry.max() == 0)
+        assert abs(v - s.surfVolume()) <= 1e-9 * abs(s.surfVolume())
+    assert list(closed) == [True, True, False, True, True]
+    assert vol[1] < 0 < vol[0]
