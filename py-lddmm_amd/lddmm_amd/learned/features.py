@@ -60,6 +60,19 @@ def smooth(x0, values, sigma):
     return s[:, :-1] / s[:, -1:]
 
 
+def device_vertex_normals(surface, x):
+    """Unit vertex normals [N, 3] of `surface` with its vertices replaced by x
+    [N, 3] (device tensor), as synth.vertex_normals computes them on the
+    host: per vertex the sum of its faces' cross products (x2 - x1) x (x3 - x1),
+    here added in a fixed order by the CSR face gather, then normalised."""
+    from .. import surfaceDistances as sd
+    faces = surface.topology()[0].long()
+    x1, x2, x3 = x[faces[:, 0]], x[faces[:, 1]], x[faces[:, 2]]
+    nf = torch.cross(x2 - x1, x3 - x1, dim=1)
+    s = sd._gather(surface.device(vertices=x), nf, torch.zeros_like(nf))
+    return s / s.norm(dim=1, keepdim=True).clamp_min(1e-300)
+
+
 def normal_heights(template_vertices, template_faces, target_vertices, target_faces):
     """Channels [N, 3] (float64 device): the ray-cast normal height h and
     h smoothed at SMOOTH_SCALES; plus the local frames [N, 3, 3] (device)."""

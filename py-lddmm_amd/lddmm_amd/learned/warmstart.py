@@ -166,13 +166,12 @@ def refine_velocity(sm, v, steps):
     vertex normals n_T of the surface it reaches (a missed ray has height 0)
     and adds that height: v <- v + h_T n_T."""
     K = sm.options['KparDiff']
-    faces = sm.fv0.faces
     for _ in range(steps):
         xt, _ = evol.shoot(sm.x0_dev, time_constant(sm, solve_control(sm, v)), K)
-        xT = xt[-1].cpu().numpy()
-        nT = synth.vertex_normals(xT, faces)
+        xT = xt[-1].contiguous()
+        nT = NF.device_vertex_normals(sm.fv0, xT)
         hT, _ = NF.normal_raycast(xT, nT, sm.fv1.vertices, sm.fv1.faces)
-        v = v + hT[:, None] * B.to_dev(nT)
+        v = v + hT[:, None] * nT
     return v
 
 
@@ -391,7 +390,8 @@ def register(seed, net=None, target_data=None, cfg=DEFAULT, max_iter=None, a0=No
             at, _ = warm_control(sm, net, cg_iters=cg_iters, refine=refine)
         else:
             _, at = evol.landmarkEPDiff(sm.Tsize, sm.x0_dev, B.to_dev(a0), sm.options['KparDiff'])
-        obj_zero = sm.objectiveFun()
+        # the zero control leaves x_t = x_0 with no deformation energy
+        obj_zero = sm.fun_obj0(sm.fv1) / sm.options['sigmaError'] ** 2 + float(sm.dataTerm_device(sm.x0_dev))
         sm.setInitialControl(at)
         obj_warm = sm.objectiveFun()
         if obj_warm < obj_zero:

@@ -911,10 +911,7 @@ class SurfaceMatching:
             raise ValueError(f"control shape {tuple(at_.shape)} != {tuple(self.control['at'].shape)}")
         self.control['at'] = at_.clone()
         self.obj = None
-        if self.unreduced:
-            self.state['xt'], _ = self._unreduced_forward(self.control, energy=False)
-        else:
-            self.state['xt'], _ = evol.shoot(self.x0_dev, self.control['at'], self.options['KparDiff'])
+        self.objectiveFun()               # the trajectory and the objective of the new control
 
     def setTargetDataEnergy(self, value, rtol=1e-4):
         """Record the data term (varifold / current distance / sigmaError^2)

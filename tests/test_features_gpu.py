@@ -83,3 +83,18 @@ def test_raycast_out_of_range_face_is_never_hit(F):
                                        B.stream_handle()), "normal_raycast")
     assert hit.cpu().tolist() == [1, 0]
     assert abs(float(t[0]) - 1.0) < 1e-15 and float(t[1]) == 0.0
+
+
+def test_device_vertex_normals_match_host(F):
+    """device_vertex_normals (face cross products summed per vertex by the
+    CSR face gather) against synth.vertex_normals on a deformed icosphere and
+    an ellipsoid hull, with the vertices replaced by a device tensor."""
+    import torch
+    from lddmm_amd import synth as S
+    from lddmm_amd.surfaces import Surface
+    (v, f), (v2, _) = S.plain_matching_pair(seed=5, subdiv=3)
+    tv, tf = S.sphere_hull(900, 4.0, seed=3)
+    for verts, faces, moved in ((v, f, v2), (tv, tf, tv * np.array([1.5, 1.0, 0.7]))):
+        surf = Surface((faces, verts))
+        got = F.device_vertex_normals(surf, torch.tensor(moved, device='cuda')).cpu().numpy()
+        np.testing.assert_allclose(got, S.vertex_normals(moved, faces), rtol=0, atol=1e-13)
