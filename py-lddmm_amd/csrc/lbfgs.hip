@@ -58,13 +58,24 @@ int lb_state(LbState* out) {
     return LDDMM_OK;
 }
 
-// Fixed-order tree sum of red[0 .. kLbBlock) into red[0] (all lanes call it).
+// Fixed-order sum of red[0 .. kLbBlock) into red[0] (all lanes call it):
+// each wave adds its 64 values by a butterfly, then the wave sums are added
+// in wave order.
 __device__ __forceinline__ void lb_tree(double* red) {
     __syncthreads();
-    for (int k = kLbBlock / 2; k > 0; k >>= 1) {
-        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-        __syncthreads();
+    double v = red[threadIdx.x];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kLbBlock / 64; ++w) s += red[w];
+        red[0] = s;
     }
+    __syncthreads();
 }
 
 // Scalars the step uses: the previous launch's dot (prev = *prev_mul * sum of
@@ -150,7 +161,8 @@ struct LbcPtrs {
 // the vectors; lanes are combined by a fixed tree.  part[(slice * np + p) * nt + t].
 __global__ void __launch_bounds__(kLbBlock) k_lbc_dots(LbcPtrs probes, int np, LbcPtrs targets, int nt, int64_t n,
                                                        int64_t slice, double* __restrict__ part) {
-    __shared__ double red[kLbcMaxProbes * kLbcGroup][kLbBlock];
+    constexpr int kWaves = kLbBlock / 64;
+    __shared__ double red[kWaves][kLbcMaxProbes * kLbcGroup];
     const int t0 = blockIdx.y * kLbcGroup;
     const int64_t e0 = int64_t(blockIdx.x) * slice, e1 = e0 + slice < n ? e0 + slice : n;
     double acc[kLbcMaxProbes][kLbcGroup];
@@ -170,20 +182,24 @@ __global__ void __launch_bounds__(kLbBlock) k_lbc_dots(LbcPtrs probes, int np, L
             }
         }
     }
+    // each wave adds its lanes by a butterfly, then the waves' sums are
+    // added in wave order (a fixed order)
 #pragma unroll
     for (int p = 0; p < kLbcMaxProbes; ++p)
 #pragma unroll
-        for (int k = 0; k < kLbcGroup; ++k) red[p * kLbcGroup + k][threadIdx.x] = acc[p][k];
-    __syncthreads();
-    for (int h = kLbBlock / 2; h > 0; h >>= 1) {
-        if (threadIdx.x < h)
+        for (int k = 0; k < kLbcGroup; ++k) {
+            double v = acc[p][k];
 #pragma unroll
-            for (int q = 0; q < kLbcMaxProbes * kLbcGroup; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + h];
-        __syncthreads();
-    }
+            for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][p * kLbcGroup + k] = v;
+        }
+    __syncthreads();
     if (threadIdx.x < kLbcMaxProbes * kLbcGroup) {
         const int p = threadIdx.x / kLbcGroup, k = threadIdx.x % kLbcGroup;
-        if (p < np && t0 + k < nt) part[(size_t(blockIdx.x) * np + p) * nt + t0 + k] = red[threadIdx.x][0];
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) v += red[w][threadIdx.x];
+        if (p < np && t0 + k < nt) part[(size_t(blockIdx.x) * np + p) * nt + t0 + k] = v;
     }
 }
 
