@@ -189,6 +189,7 @@ template <int KW, bool MS>
 struct OpAdjoint {
     using Base = OpDiffKT<KW, true, MS>;
     static constexpr int NCH = 3 + KW;
+    static constexpr bool kMultiScale = MS;     // general (multi-scale / Laplacian) radial factors
     static constexpr int REC = Base::REC;
     using Chunks = ContiguousChunks;
     template <typename T>

@@ -27,6 +27,8 @@
 
 #include "ops.hpp"
 
+#include <type_traits>
+
 namespace lddmm {
 
 constexpr int kStepTgtMax = 16;                      // largest targets per workgroup
@@ -40,6 +42,15 @@ inline int step_targets(int n, int num_cu) {
     int t = num_cu > 0 ? (n + num_cu - 1) / num_cu : kStepTgtMax;
     return t < 1 ? 1 : (t > kStepTgtMax ? kStepTgtMax : t);
 }
+
+// The float64 multi-scale adjoint runs its pair loop without unrolling: the
+// unrolled loop needs more than the 128 registers a 1024-lane workgroup has.
+template <class OP, class = void>
+struct StepMultiScale { static constexpr bool value = false; };
+template <class OP>
+struct StepMultiScale<OP, std::void_t<decltype(OP::kMultiScale)>> { static constexpr bool value = OP::kMultiScale; };
+template <typename T, class OP>
+constexpr bool kLean() { return sizeof(T) == 8 && StepMultiScale<OP>::value; }
 
 template <typename T, class OP>
 constexpr size_t step_smem_bytes() {
@@ -86,8 +97,13 @@ __device__ __forceinline__ void step_sums(const typename OP::template Args<T>& a
 #pragma unroll
         for (int c = 0; c < NCH; ++c) acc[c] = T(0);
         if (active) {
+            if constexpr (kLean<T, OP>()) {
+#pragma unroll 1
+                for (int r = slice; r < cnt; r += nsl) OP::pair(args, tgt, &lds[r * REC], acc);
+            } else {
 #pragma unroll 2
-            for (int r = slice; r < cnt; r += nsl) OP::pair(args, tgt, &lds[r * REC], acc);
+                for (int r = slice; r < cnt; r += nsl) OP::pair(args, tgt, &lds[r * REC], acc);
+            }
         }
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
@@ -151,18 +167,26 @@ k_step_shoot(const typename OpApplyK<KW, MS>::template Args<T> args, int n, int 
              double* __restrict__ dots, double dscale) {
     using OP = OpApplyK<KW, MS>;
     __shared__ __attribute__((aligned(16))) char smem[step_smem_bytes<T, OP>()];
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * tpw + tid;
+    const bool own = tid < tpw && i < n;
+    // the epilogue's inputs, loaded before the sums so that their latency
+    // overlaps the pair work
+    double xi[KW], ai[KW];
+#pragma unroll
+    for (int c = 0; c < KW; ++c) {
+        xi[c] = own ? x[size_t(i) * KW + c] : 0.0;
+        ai[c] = own ? a[size_t(i) * KW + c] : 0.0;
+    }
     double v[OP::NCH];
     step_sums<T, OP>(args, n, n, tpw, smem, v);
-    const int tid = threadIdx.x;
     if (tid < 64) {
-        const int i = blockIdx.x * tpw + tid;
         double d = 0.0;
-        if (tid < tpw && i < n) {
+        if (own) {
 #pragma unroll
             for (int c = 0; c < KW; ++c) {
-                const size_t o = size_t(i) * KW + c;
-                xn[o] = x[o] + dt * v[c];
-                d += a[o] * v[c];
+                xn[size_t(i) * KW + c] = xi[c] + dt * v[c];
+                d += ai[c] * v[c];
             }
         }
         d = step_lane_sum(d);
@@ -178,15 +202,22 @@ k_step_adjoint(const typename OpAdjoint<KW, MS>::template Args<T> args, int n, i
                double fdiff, const double* __restrict__ pn, double* __restrict__ pc, double* __restrict__ g) {
     using OP = OpAdjoint<KW, MS>;
     __shared__ __attribute__((aligned(16))) char smem[step_smem_bytes<T, OP>()];
-    double v[OP::NCH];
-    step_sums<T, OP>(args, n, n, tpw, smem, v);
     const int tid = threadIdx.x;
     const int i = blockIdx.x * tpw + tid;
-    if (tid < tpw && i < n) {
+    const bool own = tid < tpw && i < n;
+    // p_{t+1} loaded before the sums (latency overlap), except in the
+    // multi-scale float64 form, which has no registers to spare for it
+    constexpr bool kEarly = !(sizeof(T) == 8 && MS);
+    double pi[KW];
+#pragma unroll
+    for (int c = 0; c < KW; ++c) pi[c] = kEarly && own ? pn[size_t(i) * KW + c] : 0.0;
+    double v[OP::NCH];
+    step_sums<T, OP>(args, n, n, tpw, smem, v);
+    if (own) {
 #pragma unroll
         for (int c = 0; c < KW; ++c) {
             const size_t o = size_t(i) * KW + c;
-            pc[o] = pn[o] + dt * v[c] * fdiff;
+            pc[o] = (kEarly ? pi[c] : pn[o]) + dt * v[c] * fdiff;
             g[o] = -dt * v[3 + c];
         }
     }
