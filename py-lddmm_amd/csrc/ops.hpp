@@ -19,6 +19,7 @@ template <int KW, bool MS>
 struct OpApplyK {
     static constexpr int NCH = KW;
     static constexpr int REC = 8;
+    static constexpr bool kMultiScale = MS;
     using Chunks = ContiguousChunks;
     template <typename T>
     struct Args {
@@ -84,6 +85,7 @@ template <int KW, bool SYM, bool MS>
 struct OpDiffKT {
     static constexpr int NCH = 3;
     static constexpr int REC = SYM ? 12 : 8;
+    static constexpr bool kMultiScale = MS;
     using Chunks = ContiguousChunks;
     template <typename T>
     struct Args {

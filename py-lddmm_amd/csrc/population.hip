@@ -42,7 +42,19 @@ __global__ void __launch_bounds__(kBlock)
 k_pair_pop(const typename OP::template Args<T> args, const PopItem* __restrict__ items,
            const int32_t* __restrict__ bstart, const PopPair* __restrict__ pairs, const double* __restrict__ c0s,
            T* __restrict__ phi, T* __restrict__ plo, const uint8_t* __restrict__ active) {
+    using Args = typename OP::template Args<T>;
     __shared__ __attribute__((aligned(16))) T lds[kTile * OP::REC];
+    // The multi-scale radial factors index the scale arrays with a loop
+    // counter; a private copy of the arguments indexed that way lives in
+    // scratch memory, so those operators read a workgroup copy in LDS (the
+    // single-scale copy below stays in registers).
+    __shared__ __attribute__((aligned(16))) Args sargs;
+    if constexpr (OP::kMultiScale) {
+        static_assert(sizeof(Args) % 4 == 0, "argument block is copied by 32-bit words");
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(&args);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&sargs);
+        for (int k = threadIdx.x; k < int(sizeof(Args) / 4); k += kBlock) dst[k] = src[k];
+    }
     const int e = bstart[blockIdx.x + 1];
     for (int it = bstart[blockIdx.x]; it < e; ++it) {
         const PopItem w = items[it];
@@ -50,11 +62,19 @@ k_pair_pop(const typename OP::template Args<T> args, const PopItem* __restrict__
         // same for every lane of the workgroup, so the branch is uniform)
         if (active && !active[w.pair]) continue;
         const PopPair pp = pairs[w.pair];
-        typename OP::template Args<T> a = args;
-        a.c0 = c0s + 4 * w.pair;
         const size_t off = (size_t(pp.pbase) + size_t(w.slot) * size_t(pp.n)) * OP::NCH;
-        pair_span<T, OP, R>(a, w.r0, w.r1, w.s0, w.s1, phi + off, kCompensated<T> ? plo + off : nullptr,
-                            size_t(pp.n), pp.roff, lds);
+        if constexpr (OP::kMultiScale) {
+            __syncthreads();                   // previous item's reads of sargs done
+            if (threadIdx.x == 0) sargs.c0 = c0s + 4 * w.pair;
+            __syncthreads();
+            pair_span<T, OP, R>(sargs, w.r0, w.r1, w.s0, w.s1, phi + off, kCompensated<T> ? plo + off : nullptr,
+                                size_t(pp.n), pp.roff, lds);
+        } else {
+            Args a = args;
+            a.c0 = c0s + 4 * w.pair;
+            pair_span<T, OP, R>(a, w.r0, w.r1, w.s0, w.s1, phi + off, kCompensated<T> ? plo + off : nullptr,
+                                size_t(pp.n), pp.roff, lds);
+        }
     }
 }
 

@@ -43,15 +43,6 @@ inline int step_targets(int n, int num_cu) {
     return t < 1 ? 1 : (t > kStepTgtMax ? kStepTgtMax : t);
 }
 
-// The float64 multi-scale adjoint runs its pair loop without unrolling: the
-// unrolled loop needs more than the 128 registers a 1024-lane workgroup has.
-template <class OP, class = void>
-struct StepMultiScale { static constexpr bool value = false; };
-template <class OP>
-struct StepMultiScale<OP, std::void_t<decltype(OP::kMultiScale)>> { static constexpr bool value = OP::kMultiScale; };
-template <typename T, class OP>
-constexpr bool kLean() { return sizeof(T) == 8 && StepMultiScale<OP>::value; }
-
 template <typename T, class OP>
 constexpr size_t step_smem_bytes() {
     constexpr size_t tile = size_t(kStepTile) * OP::REC * sizeof(T);
@@ -61,8 +52,10 @@ constexpr size_t step_smem_bytes() {
 
 // Per-target sums over all m sources: on return, lanes tid < tpw hold the NCH
 // channel sums of target blockIdx.x * tpw + tid in v[] (float64).  smem holds
-// step_smem_bytes<T, OP>() bytes, 16-byte aligned.
-template <typename T, class OP>
+// step_smem_bytes<T, OP>() bytes, 16-byte aligned.  LEAN runs the pair loop
+// without unrolling (the float64 multi-scale adjoint: its unrolled loop needs
+// more than the 128 registers a 1024-lane workgroup has).
+template <typename T, class OP, bool LEAN = false>
 __device__ __forceinline__ void step_sums(const typename OP::template Args<T>& args, int n, int m, int tpw,
                                           char* smem, double* v) {
     constexpr int NCH = OP::NCH;
@@ -97,7 +90,7 @@ __device__ __forceinline__ void step_sums(const typename OP::template Args<T>& a
 #pragma unroll
         for (int c = 0; c < NCH; ++c) acc[c] = T(0);
         if (active) {
-            if constexpr (kLean<T, OP>()) {
+            if constexpr (LEAN) {
 #pragma unroll 1
                 for (int r = slice; r < cnt; r += nsl) OP::pair(args, tgt, &lds[r * REC], acc);
             } else {
@@ -207,17 +200,17 @@ k_step_adjoint(const typename OpAdjoint<KW, MS>::template Args<T> args, int n, i
     const bool own = tid < tpw && i < n;
     // p_{t+1} loaded before the sums (latency overlap), except in the
     // multi-scale float64 form, which has no registers to spare for it
-    constexpr bool kEarly = !(sizeof(T) == 8 && MS);
+    constexpr bool kLean = sizeof(T) == 8 && MS;
     double pi[KW];
 #pragma unroll
-    for (int c = 0; c < KW; ++c) pi[c] = kEarly && own ? pn[size_t(i) * KW + c] : 0.0;
+    for (int c = 0; c < KW; ++c) pi[c] = !kLean && own ? pn[size_t(i) * KW + c] : 0.0;
     double v[OP::NCH];
-    step_sums<T, OP>(args, n, n, tpw, smem, v);
+    step_sums<T, OP, kLean>(args, n, n, tpw, smem, v);
     if (own) {
 #pragma unroll
         for (int c = 0; c < KW; ++c) {
             const size_t o = size_t(i) * KW + c;
-            pc[o] = (kEarly ? pi[c] : pn[o]) + dt * v[c] * fdiff;
+            pc[o] = (!kLean ? pi[c] : pn[o]) + dt * v[c] * fdiff;
             g[o] = -dt * v[3 + c];
         }
     }

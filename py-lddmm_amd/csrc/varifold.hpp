@@ -85,6 +85,7 @@ template <bool MS>
 struct OpVarifoldE {
     static constexpr int NCH = 1;
     static constexpr int REC = 8;
+    static constexpr bool kMultiScale = MS;
     using Chunks = ContiguousChunks;
     template <typename T>
     struct Args : VarifoldArgsBase { ScaleSet<T> sc; };
@@ -114,6 +115,7 @@ template <bool MS>
 struct OpVarifoldG {
     static constexpr int NCH = 8;
     static constexpr int REC = 8;
+    static constexpr bool kMultiScale = MS;
     using Chunks = TwoSurfaceChunks;
     template <typename T>
     struct Args : VarifoldArgsBase { ScaleSet<T> sc; };
