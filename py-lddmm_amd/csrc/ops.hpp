@@ -629,8 +629,10 @@ struct OpEPDiffFwd {
 // ---------------------------------------------------------------------------
 template <int KW>
 struct OpEPDiffAdj {
+    static_assert(KW >= 1 && KW <= 3, "records pack three 3-vectors after the point");
     static constexpr int NCH = 6 + 2 * KW;
-    static constexpr int REC = 16;
+    // record: u (3), a_j (3), px_j (3), pa_j (3), read as three 4-vectors
+    static constexpr int REC = 12;
     using Chunks = ContiguousChunks;
     template <typename T>
     struct Args {
@@ -662,7 +664,7 @@ struct OpEPDiffAdj {
         if (j < 0) {
             r.x0 = A.c0[0]; r.x1 = A.c0[1]; r.x2 = A.c0[2];
 #pragma unroll
-            for (int c = 0; c < KW; ++c) r.a[c] = r.px[c] = r.pa[c] = 0.0;
+            for (int c = 0; c < KW; ++c) { r.a[c] = 0.0; r.px[c] = 0.0; r.pa[c] = 0.0; }
             return;
         }
         const double* xp = A.x + size_t(j) * KW;
@@ -681,24 +683,23 @@ struct OpEPDiffAdj {
         rec[0] = T((r.x0 - A.c0[0]) * A.sc.cs);
         rec[1] = T((r.x1 - A.c0[1]) * A.sc.cs);
         rec[2] = T((r.x2 - A.c0[2]) * A.sc.cs);
-        rec[3] = T(0);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < 3; ++c) {
             const int cc = c < KW ? c : 0;
-            rec[4 + c] = c < KW ? T(r.a[cc]) : T(0);
-            rec[8 + c] = c < KW ? T(r.px[cc]) : T(0);
-            rec[12 + c] = c < KW ? T(r.pa[cc]) : T(0);
+            rec[3 + c] = c < KW ? T(r.a[cc]) : T(0);
+            rec[6 + c] = c < KW ? T(r.px[cc]) : T(0);
+            rec[9 + c] = c < KW ? T(r.pa[cc]) : T(0);
         }
     }
     template <typename T>
     __device__ static void pair(const Args<T>& A, const Tgt<T>& t, const T* rec, T* acc) {
-        Vec4<T> p = lds4(rec);
-        T dx = t.u0 - p.x, dy = t.u1 - p.y, dz = t.u2 - p.z;
+        const Vec4<T> r0 = lds4(rec), r1 = lds4(rec + 4), r2v = lds4(rec + 8);
+        T dx = t.u0 - r0.x, dy = t.u1 - r0.y, dz = t.u2 - r0.z;
         T r2 = fmaT(dz, dz, fmaT(dy, dy, dx * dx));
         T g = fast_exp2(-r2);
-        Vec4<T> a = lds4(rec + 4);
-        Vec4<T> qx = lds4(rec + 8);          // px_j
-        Vec4<T> pl = lds4(rec + 12);         // pa_j
+        const Vec4<T> a{r0.w, r1.x, r1.y, T(0)};      // a_j
+        const Vec4<T> qx{r1.z, r1.w, r2v.x, T(0)};    // px_j
+        const Vec4<T> pl{r2v.y, r2v.z, r2v.w, T(0)};  // pa_j
         // DiffKT sym: c = px_i.a_j + a_i.px_j
         T c = t.px[0] * a.x;
         if constexpr (KW > 1) c = fmaT(t.px[1], a.y, c);

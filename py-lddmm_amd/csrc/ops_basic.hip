@@ -32,46 +32,63 @@ int launch_fill(double* p, size_t count, double v, hipStream_t st) {
 // magnitudes equally well, so a sample keeps this pass at a few microseconds.
 constexpr int kCenterSample = 2048;
 
-__global__ void __launch_bounds__(1024) k_center(const double* y, int n, const double* x, int m, int dim,
-                                                 double* c0) {
-    __shared__ double lo[3][1024], hi[3][1024];
-    int t = threadIdx.x;
-    double l[3] = {1e300, 1e300, 1e300}, h[3] = {-1e300, -1e300, -1e300};
-    const int sy = n > kCenterSample ? (n + kCenterSample - 1) / kCenterSample : 1;
-    const int sx = m > kCenterSample ? (m + kCenterSample - 1) / kCenterSample : 1;
-    for (int i = t * sy; i < n; i += 1024 * sy)
-        for (int d = 0; d < dim; ++d) {
-            double v = y[size_t(i) * dim + d];
-            l[d] = fmin(l[d], v);
-            h[d] = fmax(h[d], v);
-        }
-    for (int i = t * sx; i < m; i += 1024 * sx)
-        for (int d = 0; d < dim; ++d) {
-            double v = x[size_t(i) * dim + d];
-            l[d] = fmin(l[d], v);
-            h[d] = fmax(h[d], v);
-        }
-    for (int d = 0; d < 3; ++d) { lo[d][t] = l[d]; hi[d][t] = h[d]; }
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if (t < s)
-            for (int d = 0; d < 3; ++d) {
-                lo[d][t] = fmin(lo[d][t], lo[d][t + s]);
-                hi[d][t] = fmax(hi[d][t], hi[d][t + s]);
+constexpr int kCenterThreads = 256;
+
+// Bounding box of the sampled points of one set into (l, h), one coordinate
+// per register (dim <= 3).
+__device__ __forceinline__ void center_scan(const double* p, int cnt, int dim, double (&l)[3], double (&h)[3]) {
+    const int64_t s = cnt > kCenterSample ? (cnt + kCenterSample - 1) / kCenterSample : 1;
+    for (int64_t i = int64_t(threadIdx.x) * s; i < cnt; i += kCenterThreads * s) {
+        const double* q = p + size_t(i) * dim;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (d < dim) {
+                const double v = q[d];
+                l[d] = fmin(l[d], v);
+                h[d] = fmax(h[d], v);
             }
-        __syncthreads();
+        }
     }
+}
+
+__global__ void __launch_bounds__(kCenterThreads) k_center(const double* y, int n, const double* x, int m, int dim,
+                                                           double* c0) {
+    constexpr int W = kCenterThreads / 64;
+    __shared__ double lo[3][W], hi[3][W];
+    const int t = threadIdx.x;
+    double l[3] = {1e300, 1e300, 1e300}, h[3] = {-1e300, -1e300, -1e300};
+    center_scan(y, n, dim, l, h);
+    center_scan(x, m, dim, l, h);
+    // min / max are exact in any order: wave butterflies, then the W wave values
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            l[d] = fmin(l[d], __shfl_xor(l[d], o, 64));
+            h[d] = fmax(h[d], __shfl_xor(h[d], o, 64));
+        }
+    }
+    if ((t & 63) == 0) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { lo[d][t >> 6] = l[d]; hi[d][t >> 6] = h[d]; }
+    }
+    __syncthreads();
     if (t < 4) {
         double c = 0.0;
-        // an empty or non-finite box centres at 0
-        if (t < dim && lo[t][0] <= hi[t][0]) c = 0.5 * (lo[t][0] + hi[t][0]);
+        if (t < dim && t < 3) {
+            double a = lo[t][0], b = hi[t][0];
+#pragma unroll
+            for (int w = 1; w < W; ++w) { a = fmin(a, lo[t][w]); b = fmax(b, hi[t][w]); }
+            // an empty or non-finite box centres at 0
+            if (a <= b) c = 0.5 * (a + b);
+        }
         if (!isfinite(c)) c = 0.0;
         c0[t] = c;
     }
 }
 
 int launch_center(const double* y, int n, const double* x, int m, int dim, double* c0, hipStream_t st) {
-    hipLaunchKernelGGL(k_center, dim3(1), dim3(1024), 0, st, y, n, x, m, dim, c0);
+    hipLaunchKernelGGL(k_center, dim3(1), dim3(kCenterThreads), 0, st, y, n, x, m, dim, c0);
     LDDMM_CHECK_LAUNCH();
     return LDDMM_OK;
 }

@@ -730,6 +730,12 @@ int epdiff_forward_t(const lddmm_kernel* kd, int T_, int n, double* xt, double* 
     size_t poff = cv.off;
     const size_t stride = size_t(n) * KW;
     const double dt = 1.0 / T_;
+    int tpw = 0;                                              // > 0: one-launch steps
+    if (!MS && int64_t(n) <= kStepMaxN) {
+        DeviceInfo di;
+        LDDMM_TRY(device_info(&di));
+        tpw = step_targets(n, di.num_cu);
+    }
     LDDMM_TRY(launch_center(xt, n, xt, 0, KW, c0, st));       // one centre (x_0) for all steps
     for (int k = 0; k < T_; ++k) {
         const double* x = xt + size_t(k) * stride;
@@ -737,10 +743,18 @@ int epdiff_forward_t(const lddmm_kernel* kd, int T_, int n, double* xt, double* 
         double* xn = xt + size_t(k + 1) * stride;
         double* an = at + size_t(k + 1) * stride;
         if constexpr (!MS) {
-            // one Gaussian scale: both sums of a step in one pass (OpEPDiffFwd) + one reduce
+            // one Gaussian scale: both sums of a step in one pass (OpEPDiffFwd);
+            // small sets in one launch per step, larger ones chunked + one reduce
+            jf.args = {x, a, c0, sc};
+            if (tpw > 0) {
+                hipLaunchKernelGGL((k_step_epdiff_fwd<T, KW>), dim3(unsigned((n + tpw - 1) / tpw)),
+                                   dim3(kStepThreads), 0, st, jf.args, n, tpw, dt, fdiff, x, a, xn, an);
+                LDDMM_CHECK_LAUNCH();
+                if (aff) LDDMM_TRY(epd_aff_step(AA, n, KW, 0, dt, x, a, nullptr, nullptr, xn, an, aws, st));
+                continue;
+            }
             Carve c1; c1.off = poff;
             Partials<T> pf = carve_partials<T>(base, c1, jf.plan, KW + 3, n);
-            jf.args = {x, a, c0, sc};
             LDDMM_TRY(run_job(jf, n, n, pf, st));
             hipLaunchKernelGGL((k_reduce_epdiff_fwd<T, KW>), dim3(unsigned((n + kEpiRows - 1) / kEpiRows), 2 * KW),
                                dim3(256), 0, st, pf, jf.plan.nchunk, n, dt, fdiff, x, a, xn, an);
@@ -817,7 +831,10 @@ int epdiff_adjoint_t(const lddmm_kernel* kd, int T_, int n, const double* xt, co
         double* pao = pat + size_t(t) * stride;
         Carve c; c.off = poff;
         if constexpr (!MS) {
-            // one Gaussian scale: the four sums in one pass (OpEPDiffAdj) + one reduce
+            // one Gaussian scale: the four sums in one pass (OpEPDiffAdj) + one
+            // reduce.  At registration sizes this is as fast as a one-launch
+            // step: the op's 12-value source records make the step kernel's
+            // per-lane LDS reads its limit.
             Partials<T> pa4 = carve_partials<T>(base, c, ja.plan, 6 + 2 * KW, n);
             ja.args = {x, a, px, pa, c0, T(ss.gamma), sc};
             LDDMM_TRY(run_job(ja, n, n, pa4, st));

@@ -86,6 +86,14 @@ __device__ __forceinline__ void step_sums(const typename OP::template Args<T>& a
             OP::load_raw(args, jn < m ? jn : -1, raw);
         }
         const int cnt = m - jt < kStepTile ? m - jt : kStepTile;
+        if constexpr (LEAN && !kCompensated<T>) {
+            // float64, lean: the pairs add straight into the running sums
+            if (active) {
+#pragma unroll 1
+                for (int r = slice; r < cnt; r += nsl) OP::pair(args, tgt, &lds[r * REC], hi);
+            }
+            continue;
+        }
         T acc[NCH];
 #pragma unroll
         for (int c = 0; c < NCH; ++c) acc[c] = T(0);
@@ -212,6 +220,36 @@ k_step_adjoint(const typename OpAdjoint<KW, MS>::template Args<T> args, int n, i
             const size_t o = size_t(i) * KW + c;
             pc[o] = (!kLean ? pi[c] : pn[o]) + dt * v[c] * fdiff;
             g[o] = -dt * v[3 + c];
+        }
+    }
+}
+
+// Geodesic (EPDiff) forward step, one Gaussian scale (OpEPDiffFwd's two sums):
+// x_{k+1} = x_k + dt K a_k, a_{k+1} = a_k - dt fdiff DiffKT(x_k, a_k, a_k).
+template <typename T, int KW>
+__global__ void __launch_bounds__(kStepThreads)
+k_step_epdiff_fwd(const typename OpEPDiffFwd<KW>::template Args<T> args, int n, int tpw, double dt, double fdiff,
+                  const double* __restrict__ x, const double* __restrict__ a, double* __restrict__ xn,
+                  double* __restrict__ an) {
+    using OP = OpEPDiffFwd<KW>;
+    __shared__ __attribute__((aligned(16))) char smem[step_smem_bytes<T, OP>()];
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * tpw + tid;
+    const bool own = tid < tpw && i < n;
+    double xi[KW], ai[KW];
+#pragma unroll
+    for (int c = 0; c < KW; ++c) {
+        xi[c] = own ? x[size_t(i) * KW + c] : 0.0;
+        ai[c] = own ? a[size_t(i) * KW + c] : 0.0;
+    }
+    double v[OP::NCH];
+    step_sums<T, OP>(args, n, n, tpw, smem, v);
+    if (own) {
+#pragma unroll
+        for (int c = 0; c < KW; ++c) {
+            const size_t o = size_t(i) * KW + c;
+            xn[o] = xi[c] + dt * v[c] * 1.0;
+            an[o] = ai[c] + (-dt) * v[KW + c] * fdiff;
         }
     }
 }

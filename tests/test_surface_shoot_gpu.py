@@ -146,6 +146,44 @@ def test_epdiff_forward_adjoint(mods, dtype, n, T, sig):
     assert abs(H - Hr) / abs(Hr) < (1e-11 if dtype == 'float64' else 1e-5)
 
 
+_LARGE_EPD = {}
+
+
+def _large_epdiff_case():
+    """n = 4,200 points, T = 2: above the one-launch step limit (4,096), so
+    the chunked pair kernels and their reduce kernels run; the oracle's
+    forward and adjoint are computed once for both precisions."""
+    if not _LARGE_EPD:
+        n, T, sig = 4200, 2, [1.3]
+        rng = np.random.default_rng(4200)
+        x0 = rng.normal(size=(n, 3)) * 5
+        a0 = rng.normal(size=(n, 3)) * 0.3
+        px1, pa1 = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+        xr, ar = O.secondOrderEvolution(x0, a0, sig, 1.0 / T)
+        da0, _, _, pxr, par = O.secondOrderGradient(x0, a0, px1, pa1, sig, 1.0 / T)
+        _LARGE_EPD.update(T=T, sig=sig, inputs=(x0, a0, px1, pa1), fwd=(xr, ar), adj=(da0, pxr, par))
+    return _LARGE_EPD
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_epdiff_chunked_path_above_step_limit(mods, dtype):
+    sd, pe, synth, Surface, Kernel = mods
+    case = _large_epdiff_case()
+    x0, a0, px1, pa1 = case['inputs']
+    K = Kernel(name='gauss', sigma=case['sig'])
+    K.pk_dtype = dtype
+    dt = 1.0 / case['T']
+    st = pe.secondOrderEvolution(x0, a0, K, dt)
+    xr, ar = case['fwd']
+    assert rel(st['xt'], xr) < TOL[dtype]
+    assert rel(st['at'], ar) < TOL[dtype]
+    res = pe.secondOrderGradient(x0, a0, px1, pa1, K, dt)
+    da0, pxr, par = case['adj']
+    assert rel(res['pxt'], pxr) < TOL[dtype] * 5
+    assert rel(res['pat'], par) < TOL[dtype] * 5
+    assert rel(res['da0'], da0) < TOL[dtype] * 5
+
+
 @pytest.mark.parametrize("dtype", ["float64", "float32"])
 def test_epdiff_forward_adjoint_2d(mods, dtype):
     """Planar points (dim 2) through the geodesic forward and adjoint."""
