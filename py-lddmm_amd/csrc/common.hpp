@@ -232,4 +232,27 @@ __device__ __forceinline__ Vec4<double> lds4<double>(const double* p) {
     return {a.x, a.y, b.x, b.y};
 }
 
+// N consecutive values from p (16-byte aligned) in 16-byte reads, then one
+// 8-byte read for a remaining pair (float) and a scalar read for a last
+// value: a packed record costs only the LDS bytes it holds.
+template <int N, typename T>
+__device__ __forceinline__ void lds_n(const T* p, T* v) {
+#pragma unroll
+    for (int k = 0; k + 4 <= N; k += 4) {
+        const Vec4<T> q = lds4(p + k);
+        v[k] = q.x; v[k + 1] = q.y; v[k + 2] = q.z; v[k + 3] = q.w;
+    }
+    constexpr int k = N / 4 * 4;
+    if constexpr (N - k >= 2) {
+        if constexpr (sizeof(T) == 4) {
+            const float2 q = *reinterpret_cast<const float2*>(p + k);
+            v[k] = q.x; v[k + 1] = q.y;
+        } else {
+            const double2 q = *reinterpret_cast<const double2*>(p + k);
+            v[k] = q.x; v[k + 1] = q.y;
+        }
+    }
+    if constexpr ((N - k) % 2 == 1) v[N - 1] = p[N - 1];
+}
+
 }  // namespace lddmm

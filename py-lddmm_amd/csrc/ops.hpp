@@ -58,21 +58,19 @@ struct OpApplyK {
         rec[0] = T((r.x0 - A.c0[0]) * A.sc.cs);
         rec[1] = T((r.x1 - A.c0[1]) * A.sc.cs);
         rec[2] = T((r.x2 - A.c0[2]) * A.sc.cs);
-        rec[3] = T(0);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) rec[4 + c] = c < KW ? T(r.a[c < KW ? c : 0]) : T(0);
+        for (int c = 0; c < KW; ++c) rec[3 + c] = T(r.a[c]);
     }
+    // record: u (3) then a_j (KW), packed; the pair reads only those values
     template <typename T>
     __device__ static void pair(const Args<T>& A, const Tgt<T>& t, const T* rec, T* acc) {
-        Vec4<T> p = lds4(rec);
-        T dx = t.u0 - p.x, dy = t.u1 - p.y, dz = t.u2 - p.z;
+        T v[3 + KW];
+        lds_n<3 + KW>(rec, v);
+        T dx = t.u0 - v[0], dy = t.u1 - v[1], dz = t.u2 - v[2];
         T r2 = fmaT(dz, dz, fmaT(dy, dy, dx * dx));
         T K = gK<T, MS>(A.sc, r2);
-        Vec4<T> w = lds4(rec + 4);
-        acc[0] = fmaT(K, w.x, acc[0]);
-        if constexpr (KW > 1) acc[1] = fmaT(K, w.y, acc[1]);
-        if constexpr (KW > 2) acc[2] = fmaT(K, w.z, acc[2]);
-        if constexpr (KW > 3) acc[3] = fmaT(K, w.w, acc[3]);
+#pragma unroll
+        for (int c = 0; c < KW; ++c) acc[c] = fmaT(K, v[3 + c], acc[c]);
     }
 };
 
@@ -142,34 +140,28 @@ struct OpDiffKT {
         rec[0] = T((r.x0 - A.c0[0]) * A.sc.cs);
         rec[1] = T((r.x1 - A.c0[1]) * A.sc.cs);
         rec[2] = T((r.x2 - A.c0[2]) * A.sc.cs);
-        rec[3] = T(0);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) rec[4 + c] = c < KW ? T(r.a[c < KW ? c : 0]) : T(0);
+        for (int c = 0; c < KW; ++c) rec[3 + c] = T(r.a[c]);
         if constexpr (SYM) {
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                int cc = c < KW ? c : 0;
-                rec[8 + c] = c < KW ? T(r.p[cc] - A.r2w * r.a[cc]) : T(0);
-            }
+            for (int c = 0; c < KW; ++c) rec[3 + KW + c] = T(r.p[c] - A.r2w * r.a[c]);
         }
     }
+    // record: u (3), a_j (KW) and for sym / lddmm q_j (KW), packed
+    static constexpr int NV = 3 + (SYM ? 2 : 1) * KW;
     template <typename T>
     __device__ static void pair(const Args<T>& A, const Tgt<T>& t, const T* rec, T* acc) {
-        Vec4<T> p = lds4(rec);
-        T dx = t.u0 - p.x, dy = t.u1 - p.y, dz = t.u2 - p.z;
+        T v[NV];
+        lds_n<NV>(rec, v);
+        T dx = t.u0 - v[0], dy = t.u1 - v[1], dz = t.u2 - v[2];
         T r2 = fmaT(dz, dz, fmaT(dy, dy, dx * dx));
         T g = gD<T, MS>(A.sc, r2);
-        Vec4<T> a = lds4(rec + 4);
-        T c = t.p[0] * a.x;
-        if constexpr (KW > 1) c = fmaT(t.p[1], a.y, c);
-        if constexpr (KW > 2) c = fmaT(t.p[2], a.z, c);
-        if constexpr (KW > 3) c = fmaT(t.p[3], a.w, c);
+        T c = t.p[0] * v[3];
+#pragma unroll
+        for (int k = 1; k < KW; ++k) c = fmaT(t.p[k], v[3 + k], c);
         if constexpr (SYM) {
-            Vec4<T> q = lds4(rec + 8);
-            c = fmaT(t.a[0], q.x, c);
-            if constexpr (KW > 1) c = fmaT(t.a[1], q.y, c);
-            if constexpr (KW > 2) c = fmaT(t.a[2], q.z, c);
-            if constexpr (KW > 3) c = fmaT(t.a[3], q.w, c);
+#pragma unroll
+            for (int k = 0; k < KW; ++k) c = fmaT(t.a[k], v[3 + KW + k], c);
         }
         T w = g * c;
         acc[0] = fmaT(w, dx, acc[0]);
@@ -208,28 +200,25 @@ struct OpAdjoint {
     __device__ static void store_rec(const Args<T>& A, const Raw& r, T* rec) { Base::store_rec(A, r, rec); }
     template <typename T>
     __device__ static void pair(const Args<T>& A, const Tgt<T>& t, const T* rec, T* acc) {
-        Vec4<T> p = lds4(rec);
-        T dx = t.u0 - p.x, dy = t.u1 - p.y, dz = t.u2 - p.z;
+        T v[Base::NV];                          // u, a_j, q_j (the base op's packed record)
+        lds_n<Base::NV>(rec, v);
+        T dx = t.u0 - v[0], dy = t.u1 - v[1], dz = t.u2 - v[2];
         T r2 = fmaT(dz, dz, fmaT(dy, dy, dx * dx));
         T g = gD<T, MS>(A.sc, r2);
         T K;
         if constexpr (MS) K = gK<T, MS>(A.sc, r2);
         else K = g;
-        Vec4<T> a = lds4(rec + 4);
-        Vec4<T> q = lds4(rec + 8);
-        T c = t.p[0] * a.x;
-        if constexpr (KW > 1) c = fmaT(t.p[1], a.y, c);
-        if constexpr (KW > 2) c = fmaT(t.p[2], a.z, c);
-        c = fmaT(t.a[0], q.x, c);
-        if constexpr (KW > 1) c = fmaT(t.a[1], q.y, c);
-        if constexpr (KW > 2) c = fmaT(t.a[2], q.z, c);
+        T c = t.p[0] * v[3];
+#pragma unroll
+        for (int k = 1; k < KW; ++k) c = fmaT(t.p[k], v[3 + k], c);
+#pragma unroll
+        for (int k = 0; k < KW; ++k) c = fmaT(t.a[k], v[3 + KW + k], c);
         T w = g * c;
         acc[0] = fmaT(w, dx, acc[0]);
         acc[1] = fmaT(w, dy, acc[1]);
         acc[2] = fmaT(w, dz, acc[2]);
-        acc[3] = fmaT(K, q.x, acc[3]);
-        if constexpr (KW > 1) acc[4] = fmaT(K, q.y, acc[4]);
-        if constexpr (KW > 2) acc[5] = fmaT(K, q.z, acc[5]);
+#pragma unroll
+        for (int k = 0; k < KW; ++k) acc[3 + k] = fmaT(K, v[3 + KW + k], acc[3 + k]);
     }
 };
 
