@@ -480,6 +480,9 @@ def main():
     ap.add_argument("--registration", action="store_true",
                     help="only run the plain registration (icosphere 4, float64 and float32) three times, "
                          "for rocprofv3 kernel traces of the registration loop")
+    ap.add_argument("--second-order", action="store_true",
+                    help="only run the geodesic point-set registration (2,562 points, float64 and float32: one "
+                         "untimed and one timed run each), for rocprofv3 kernel traces")
     ap.add_argument("--population", action="store_true",
                     help="only run the batched population workload (256 pairs: forward, adjoint, varifold "
                          "gradient, and the forward as 256 single-pair calls), for rocprofv3 kernel traces")
@@ -504,6 +507,9 @@ def main():
     if args.registration:
         for rep in range(3):
             registration_cfg1(torch)
+        return
+    if args.second_order:
+        print(json.dumps(second_order_registration(torch)), flush=True)
         return
     if args.population:
         print(json.dumps(population_cfg4(torch)), flush=True)

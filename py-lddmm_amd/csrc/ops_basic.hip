@@ -27,25 +27,40 @@ int launch_fill(double* p, size_t count, double v, hipStream_t st) {
 }
 
 // Centre used to shift coordinates before the float32 conversion: the midpoint
-// of the bounding box of a strided sample (at most kCenterSample points of
-// each set).  Any point inside the clouds' hull bounds the coordinate
-// magnitudes equally well, so a sample keeps this pass at a few microseconds.
+// of the bounding box of a sample of each set (every point of a set of at most
+// kCenterSample points; otherwise kCenterRuns evenly spaced runs of
+// kCenterRun consecutive points, which a wave reads as contiguous memory).
+// Any point inside the clouds' hull bounds the coordinate magnitudes equally
+// well, so a sample keeps this pass at a few microseconds.
 constexpr int kCenterSample = 2048;
-
+constexpr int kCenterRun = 32;
+constexpr int kCenterRuns = kCenterSample / kCenterRun;
 constexpr int kCenterThreads = 256;
+constexpr int kCenterPer = kCenterSample / kCenterThreads;   // samples per lane and set
 
 // Bounding box of the sampled points of one set into (l, h), one coordinate
-// per register (dim <= 3).
+// per register (dim <= 3); a lane's kCenterPer loads are all issued before
+// any is used.
 __device__ __forceinline__ void center_scan(const double* p, int cnt, int dim, double (&l)[3], double (&h)[3]) {
-    const int64_t s = cnt > kCenterSample ? (cnt + kCenterSample - 1) / kCenterSample : 1;
-    for (int64_t i = int64_t(threadIdx.x) * s; i < cnt; i += kCenterThreads * s) {
-        const double* q = p + size_t(i) * dim;
+    double v[kCenterPer][3];
+    bool ok[kCenterPer];
+#pragma unroll
+    for (int k = 0; k < kCenterPer; ++k) {
+        const int j = threadIdx.x + k * kCenterThreads;         // sample number
+        int64_t i = j;
+        if (cnt > kCenterSample)
+            i = int64_t(j / kCenterRun) * (cnt - kCenterRun) / (kCenterRuns - 1) + j % kCenterRun;
+        ok[k] = i < cnt;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v[k][d] = ok[k] && d < dim ? p[size_t(i) * dim + d] : l[d];
+    }
+#pragma unroll
+    for (int k = 0; k < kCenterPer; ++k) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            if (d < dim) {
-                const double v = q[d];
-                l[d] = fmin(l[d], v);
-                h[d] = fmax(h[d], v);
+            if (ok[k] && d < dim) {
+                l[d] = fmin(l[d], v[k][d]);
+                h[d] = fmax(h[d], v[k][d]);
             }
         }
     }

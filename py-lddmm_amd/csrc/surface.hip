@@ -163,28 +163,53 @@ int launch_features(const lddmm_surface* s, FaceRec* rec, double* c, double* ws,
 
 // Centre of the face centres' bounding box: the FaceRec layout starts with c (3) of 8 doubles,
 // so the union box is taken over a strided view of both record arrays.
-__global__ void __launch_bounds__(1024) k_center_faces(const FaceRec* a, int na, const FaceRec* b, int nb,
-                                                       double* c0) {
-    __shared__ double lo[3][1024], hi[3][1024];
-    int t = threadIdx.x;
+constexpr int kCenterFaceThreads = 256;
+constexpr int kCenterFacePer = 2048 / kCenterFaceThreads;   // sampled records per lane
+
+__global__ void __launch_bounds__(kCenterFaceThreads) k_center_faces(const FaceRec* a, int na, const FaceRec* b,
+                                                                    int nb, double* c0) {
+    constexpr int W = kCenterFaceThreads / 64;
+    __shared__ double lo[3][W], hi[3][W];
+    const int t = threadIdx.x;
     double l[3] = {1e300, 1e300, 1e300}, h[3] = {-1e300, -1e300, -1e300};
-    // strided sample of at most 2048 records (any central point serves)
+    // strided sample of at most 2048 records (any central point serves); a
+    // lane's loads are all issued before any is used
     const int tot = na + nb;
     const int stp = tot > 2048 ? (tot + 2047) / 2048 : 1;
-    for (int i = t * stp; i < tot; i += 1024 * stp) {
-        const FaceRec& f = i < na ? a[i] : b[i - na];
-        double c[3] = {f.c0, f.c1, f.c2};
-        for (int d = 0; d < 3; ++d) { l[d] = fmin(l[d], c[d]); h[d] = fmax(h[d], c[d]); }
+    double c[kCenterFacePer][3];
+#pragma unroll
+    for (int k = 0; k < kCenterFacePer; ++k) {
+        const int64_t i = (int64_t(t) + int64_t(k) * kCenterFaceThreads) * stp;
+        const FaceRec* f = i < na ? a + i : (i < tot ? b + (i - na) : nullptr);
+        c[k][0] = f ? f->c0 : 0.0;             // unused past the sample
+        c[k][1] = f ? f->c1 : 0.0;
+        c[k][2] = f ? f->c2 : 0.0;
     }
-    for (int d = 0; d < 3; ++d) { lo[d][t] = l[d]; hi[d][t] = h[d]; }
+#pragma unroll
+    for (int k = 0; k < kCenterFacePer; ++k) {
+        const int64_t i = (int64_t(t) + int64_t(k) * kCenterFaceThreads) * stp;
+        if (i < tot) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) { l[d] = fmin(l[d], c[k][d]); h[d] = fmax(h[d], c[k][d]); }
+        }
+    }
+    // min / max are exact in any order: wave butterflies, then the W wave values
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            l[d] = fmin(l[d], __shfl_xor(l[d], o, 64));
+            h[d] = fmax(h[d], __shfl_xor(h[d], o, 64));
+        }
+    }
+    if ((t & 63) == 0) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { lo[d][t >> 6] = l[d]; hi[d][t >> 6] = h[d]; }
+    }
     __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if (t < s)
-            for (int d = 0; d < 3; ++d) {
-                lo[d][t] = fmin(lo[d][t], lo[d][t + s]);
-                hi[d][t] = fmax(hi[d][t], hi[d][t + s]);
-            }
-        __syncthreads();
+    if (t < 3) {
+#pragma unroll
+        for (int w = 1; w < W; ++w) { lo[t][0] = fmin(lo[t][0], lo[t][w]); hi[t][0] = fmax(hi[t][0], hi[t][w]); }
     }
     if (t < 4) {
         double c = 0.0;
@@ -248,7 +273,7 @@ int varifold_core(const lddmm_kernel* kd, int n1, int n2, bool has_tgt, VfShape 
     FaceRec* r1 = reinterpret_cast<FaceRec*>(base + cv.take(size_t(n1) * sizeof(FaceRec)));
     FaceRec* r2 = reinterpret_cast<FaceRec*>(base + cv.take(size_t(std::max(n2, 1)) * sizeof(FaceRec)));
     LDDMM_TRY(feat(r1, has_tgt ? r2 : nullptr));
-    hipLaunchKernelGGL(k_center_faces, dim3(1), dim3(1024), 0, st, r1, n1, r2, n2, c0);
+    hipLaunchKernelGGL(k_center_faces, dim3(1), dim3(kCenterFaceThreads), 0, st, r1, n1, r2, n2, c0);
     LDDMM_CHECK_LAUNCH();
 
     if (!want_grad) {
