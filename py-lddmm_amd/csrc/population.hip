@@ -357,15 +357,16 @@ __global__ void k_pop_vf_grad_epi(const T* __restrict__ phi, const T* __restrict
         D[c] = pop_sum(phi, plo, pp, 8, i, 5 + c, 0, pp.nchunk);
     }
     double A1 = pop_sum(phi, plo, pp, 8, i, 3, 0, pp.c1), A2 = pop_sum(phi, plo, pp, 8, i, 3, pp.c1, pp.nchunk);
-    double B1 = pop_sum(phi, plo, pp, 8, i, 4, 0, pp.c1), B2 = pop_sum(phi, plo, pp, 8, i, 4, pp.c1, pp.nchunk);
+    // channel 4 holds W = A + B
+    double W1 = pop_sum(phi, plo, pp, 8, i, 4, 0, pp.c1), W2 = pop_sum(phi, plo, pp, 8, i, 4, pp.c1, pp.nchunk);
     const FaceRec f = r1[i];
-    double AmB = (A1 + A2) - (B1 + B2);
+    double AmB = 2.0 * (A1 + A2) - (W1 + W2);
     double n[3] = {f.n0, f.n1, f.n2};
     for (int c = 0; c < 3; ++c) {
         z1[3 * size_t(i) + c] = f.fw * (2.0 * S[c] + AmB * n[c]);
         dz1[3 * size_t(i) + c] = (2.0 / 3.0) * fD * f.aw * D[c];
     }
-    erow[i] = f.aw * ((A1 + B1) + 2.0 * (A2 + B2));
+    erow[i] = f.aw * (W1 + 2.0 * W2);
 }
 
 // ------------------------------------------------------------------- host

@@ -67,7 +67,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_vf_grad_epi(Partials<T> part, int c1, int nchunk, const FaceRec* f1, int n1,
                                                      double fK, double fD, double zf, double sd, double dzf,
                                                      double* z1, double* dz1, double* erow) {
-    constexpr int NV = 10;       // S[3], D[3], A1, A2, B1, B2
+    constexpr int NV = 10;       // S[3], D[3], A1, A2, W1, W2 (W = A + B)
     __shared__ double red[NV][kEpiLanes][kEpiRows];
     const int fl = threadIdx.x % kEpiRows, lane = threadIdx.x / kEpiRows;
     const int i = blockIdx.x * kEpiRows + fl;
@@ -89,16 +89,16 @@ __global__ void __launch_bounds__(256) k_vf_grad_epi(Partials<T> part, int c1, i
     double t[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) t[q] = ((red[q][0][fl] + red[q][1][fl]) + red[q][2][fl]) + red[q][3][fl];
-    const double A1 = t[6], A2 = t[7], B1 = t[8], B2 = t[9];
+    const double A1 = t[6], A2 = t[7], W1 = t[8], W2 = t[9];
     const FaceRec f = f1[i];
-    const double AmB = (A1 + A2) - (B1 + B2);
+    const double AmB = 2.0 * (A1 + A2) - (W1 + W2);
     const double n[3] = {f.n0, f.n1, f.n2};
     for (int c = 0; c < 3; ++c) {
         z1[3 * size_t(i) + c] = f.fw * fK * zf * (2.0 * sd * t[c] + AmB * n[c]);
         dz1[3 * size_t(i) + c] = dzf * fD * f.aw * t[3 + c];
     }
     // surface-2 sources carry weight -a_j here; the cross term of the energy is -2<S1,S2>
-    erow[i] = f.aw * fK * ((A1 + B1) + 2.0 * (A2 + B2));
+    erow[i] = f.aw * fK * (W1 + 2.0 * W2);
 }
 
 __global__ void __launch_bounds__(1024) k_sum_rows(const double* rows, int n, double* out) {

@@ -110,7 +110,9 @@ struct OpVarifoldE {
 };
 
 // Gradient sums with c = sqrt(d) n_i.n_j: S1 = sum g a_j c n_j, A = sum g a_j,
-// B = sum g a_j c^2, D = sum g' a_j (1 + c^2) (u_i - u_j)
+// W = sum g a_j (1 + c^2) (= A + B with B = sum g a_j c^2, which the
+// epilogues recover as W - A), D = sum g' a_j (1 + c^2) (u_i - u_j).
+// Channels: [0, 3) S1, 3 A, 4 W, [5, 8) D.
 template <bool MS>
 struct OpVarifoldG {
     static constexpr int NCH = 8;
@@ -141,11 +143,11 @@ struct OpVarifoldG {
         acc[0] = fma_(Kac, nj.x, acc[0]);
         acc[1] = fma_(Kac, nj.y, acc[1]);
         acc[2] = fma_(Kac, nj.z, acc[2]);
-        T Kac2 = Kac * c;
+        const T wk = fma_(Kac, c, Ka);         // Ka (1 + c^2)
         acc[3] += Ka;
-        acc[4] += Kac2;
+        acc[4] += wk;
         T w;
-        if constexpr (!MS) w = Ka + Kac2;
+        if constexpr (!MS) w = wk;
         else { T Kd = gd * p.w; w = fma_(Kd * c, c, Kd); }
         acc[5] = fma_(w, dx, acc[5]);
         acc[6] = fma_(w, dy, acc[6]);
