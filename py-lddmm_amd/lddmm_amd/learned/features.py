@@ -31,25 +31,54 @@ def local_frame(normals):
 
 def normal_raycast(orig, dirs, tv, tf):
     """Device call of lddmm_normal_raycast.  orig, dirs [n, 3], tv [nv, 3]
-    (float64, numpy or device), tf [nf, 3] integer faces (host array, checked
-    here: every index must lie in [0, nv)).  Returns (t [n] float64, hit [n]
-    int32) device tensors."""
-    tf = np.asarray(tf)
+    (float64, numpy or device), tf [nf, 3] integer faces: a host array
+    (checked here: every index must lie in [0, nv)) or the int32 device faces
+    of a Surface (surface_on_device: checked when the Surface built its
+    adjacency).  Returns (t [n] float64, hit [n] int32) device tensors."""
     nv = int(np.asarray(tv).shape[0]) if not isinstance(tv, torch.Tensor) else int(tv.shape[0])
-    if tf.size and (tf.min() < 0 or tf.max() >= nv):
-        raise ValueError("normal_raycast: face index out of range")
+    on_device = isinstance(tf, torch.Tensor)
+    if on_device:
+        if tf.dtype != torch.int32 or tf.ndim != 2 or tf.shape[1] != 3:
+            raise ValueError("normal_raycast: device faces must be int32 [nf, 3]")
+    else:
+        tf = np.asarray(tf)
+        if tf.size and (tf.min() < 0 or tf.max() >= nv):
+            raise ValueError("normal_raycast: face index out of range")
     o = B.to_dev(orig).contiguous()
     d = B.to_dev(dirs).contiguous()
     if o.shape != d.shape or o.ndim != 2 or o.shape[1] != 3:
         raise ValueError("normal_raycast: origins and directions must be [n, 3]")
     v = B.to_dev(tv).contiguous()
-    f = torch.from_numpy(np.ascontiguousarray(tf, dtype=np.int32)).to(o.device)
+    if on_device:
+        if tf.device != o.device:
+            raise ValueError("normal_raycast: device faces must be on the rays' device")
+        f = tf.contiguous()
+    else:
+        f = torch.from_numpy(np.ascontiguousarray(tf, dtype=np.int32)).to(o.device)
     n = o.shape[0]
     t = torch.empty(n, dtype=torch.float64, device=o.device)
     hit = torch.empty(n, dtype=torch.int32, device=o.device)
     B.check(B.LIB.lddmm_normal_raycast(B.ptr(o), B.ptr(d), n, B.ptr(v), v.shape[0], B.ptr(f), f.shape[0],
                                        B.ptr(t), B.ptr(hit), B.stream_handle()), "normal_raycast")
     return t, hit
+
+
+def local_frame_device(n):
+    """local_frame on the device: [N, 3, 3] with rows n, t1, t2 for unit
+    normals n [N, 3] (float64 device tensor)."""
+    e1 = torch.tensor([1.0, 0.0, 0.0], dtype=n.dtype, device=n.device)
+    e2 = torch.tensor([0.0, 1.0, 0.0], dtype=n.dtype, device=n.device)
+    ax = torch.where(n[:, :1].abs() < 0.9, e1, e2)
+    t1 = ax - (ax * n).sum(1, keepdim=True) * n
+    t1 = t1 / t1.norm(dim=1, keepdim=True)
+    return torch.stack([n, t1, torch.cross(n, t1, dim=1)], 1)
+
+
+def surface_on_device(surface):
+    """(vertices [nv, 3] float64, faces [nf, 3] int32) device tensors of a
+    Surface, cached by the Surface until its vertices or faces change."""
+    _, keep = surface.device()
+    return keep[0], keep[2]
 
 
 def smooth(x0, values, sigma):
@@ -71,6 +100,22 @@ def device_vertex_normals(surface, x):
     nf = torch.cross(x2 - x1, x3 - x1, dim=1)
     s = sd._gather(surface.device(vertices=x), nf, torch.zeros_like(nf))
     return s / s.norm(dim=1, keepdim=True).clamp_min(1e-300)
+
+
+def surface_normal_heights(template, target):
+    """normal_heights of two Surfaces with every step on the device: the
+    template's unit vertex normals from its faces (device_vertex_normals),
+    the ray cast against the target's cached device vertices and faces, the
+    smoothed channels and the local frames (local_frame_device).  Equal to
+    normal_heights of the surfaces' arrays up to the rounding of the normals'
+    face sums (the host adds them in another order)."""
+    x0, _ = surface_on_device(template)
+    nrm = device_vertex_normals(template, x0)
+    tv, tf = surface_on_device(target)
+    h, _ = normal_raycast(x0, nrm, tv, tf)
+    h = h[:, None]
+    ch = [h] + [smooth(x0, h, s) for s in SMOOTH_SCALES]
+    return torch.cat(ch, 1), local_frame_device(nrm)
 
 
 def normal_heights(template_vertices, template_faces, target_vertices, target_faces):

@@ -121,7 +121,7 @@ def solve_control(sm, v, iters=SOLVE_ITERS):
 
 def ray_cast_heights(sm):
     """Normal-height channels [N, 3] and local frames [N, 3, 3] of the pair (device)."""
-    return NF.normal_heights(sm.fv0.vertices, sm.fv0.faces, sm.fv1.vertices, sm.fv1.faces)
+    return NF.surface_normal_heights(sm.fv0, sm.fv1)
 
 
 def linear_miss(sm, ch, fr, a=None):
@@ -170,7 +170,7 @@ def refine_velocity(sm, v, steps):
         xt, _ = evol.shoot(sm.x0_dev, time_constant(sm, solve_control(sm, v)), K)
         xT = xt[-1].contiguous()
         nT = NF.device_vertex_normals(sm.fv0, xT)
-        hT, _ = NF.normal_raycast(xT, nT, sm.fv1.vertices, sm.fv1.faces)
+        hT, _ = NF.normal_raycast(xT, nT, *NF.surface_on_device(sm.fv1))
         v = v + hT[:, None] * nT
     return v
 

@@ -140,9 +140,8 @@ class SurfaceMatching:
 
     def fix_orientation(self):
         """Flip the target's faces when the signed volumes disagree (base/surfaceMatching.py:284-318)."""
-        self.fv0.getEdges()
-        self.fv1.getEdges()
-        self.closed = self.fv0.bdry.max() == 0 and self.fv1.bdry.max() == 0
+        # getEdges' boundary test; the edge tables themselves are built on access
+        self.closed = self.fv0.is_closed() and self.fv1.is_closed()
         if self.closed:
             v0 = self.fv0.surfVolume()
             v1 = self.fv1.surfVolume()

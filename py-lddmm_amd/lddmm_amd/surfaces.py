@@ -58,9 +58,19 @@ class Surface:
             self.component = np.empty(0)
             self.weights = np.empty(0)
             self.face_weights = np.empty(0)
-        self.edges = None
-        self.edgeFaces = None
-        self.faceEdges = None
+
+    # The edge tables (edges, edgeFaces, faceEdges, bdry, vertices_bdry) are
+    # built by getEdges on first access, so a matching that only needs to know
+    # whether a surface is closed (is_closed) does not pay for them.
+    _EDGE_TABLES = ('edges', 'edgeFaces', 'faceEdges', 'bdry', 'vertices_bdry')
+
+    def __getattr__(self, name):
+        if name in Surface._EDGE_TABLES and 'faces' in self.__dict__:
+            if self.faces.ndim != 2:               # an empty Surface() has no tables
+                return None
+            self.getEdges()
+            return self.__dict__[name]
+        raise AttributeError(name)
 
     # ------------------------------------------------------------------ I/O
     def read(self, filename):
@@ -249,6 +259,24 @@ class Surface:
 
     def surfArea(self):
         return float(np.sqrt((self.surfel ** 2).sum(axis=1)).sum())
+
+    def is_closed(self):
+        """True when no edge belongs to exactly one face (getEdges' boundary
+        test, bdry.max() == 0), from one sort of the undirected edge keys."""
+        if 'bdry' in self.__dict__:
+            return bool(self.bdry.size == 0 or self.bdry.max() == 0)
+        f = self.faces
+        if f.shape[0] == 0:
+            return True
+        a = f.ravel()
+        b = f[:, [1, 2, 0]].ravel()
+        nv = max(int(self.vertices.shape[0]), int(f.max()) + 1)
+        keys = np.sort(np.minimum(a, b).astype(np.int64) * nv + np.maximum(a, b))
+        lone = np.ones(keys.shape[0], dtype=bool)       # key equal to neither neighbour
+        eq = keys[1:] == keys[:-1]
+        lone[1:] &= ~eq
+        lone[:-1] &= ~eq
+        return not bool(lone.any())
 
     def getEdges(self):
         """Edges, edge -> faces and boundary flags (sparse; the reference builds

@@ -98,3 +98,21 @@ def test_device_vertex_normals_match_host(F):
         surf = Surface((faces, verts))
         got = F.device_vertex_normals(surf, torch.tensor(moved, device='cuda')).cpu().numpy()
         np.testing.assert_allclose(got, S.vertex_normals(moved, faces), rtol=0, atol=1e-13)
+
+
+def test_surface_normal_heights_match_array_path(F):
+    """The device-resident feature path (surface_normal_heights: device
+    normals, cached device target, device frames) against normal_heights on
+    the arrays: equal up to the rounding of the normals' face sums."""
+    from lddmm_amd import synth as S
+    from lddmm_amd.surfaces import Surface
+    (v, f), (v2, _) = S.plain_matching_pair(seed=7, subdiv=4)
+    print(f"[features] device path: {len(v)} template vertices", flush=True)
+    ch, fr = F.normal_heights(v, f, v2, f)
+    ch_d, fr_d = F.surface_normal_heights(Surface((f, v)), Surface((f, v2)))
+    np.testing.assert_allclose(ch_d.cpu().numpy(), ch.cpu().numpy(), rtol=0, atol=1e-11)
+    np.testing.assert_allclose(fr_d.cpu().numpy(), fr.cpu().numpy(), rtol=0, atol=1e-12)
+    # the host path's frames are local_frame of the host normals
+    np.testing.assert_allclose(fr.cpu().numpy(), F.local_frame(S.vertex_normals(v, f)), rtol=0, atol=0)
+    with pytest.raises(ValueError):
+        F.normal_raycast(v, S.vertex_normals(v, f), v2, F.surface_on_device(Surface((f, v2)))[1].long())

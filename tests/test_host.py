@@ -63,3 +63,26 @@ def test_reads_reference_test_surface():
     assert S.vertices.shape == (1001, 3) and S.faces.shape == (1998, 3)
     S.getEdges()
     assert S.bdry.max() == 0
+
+
+def test_closed_test_and_lazy_edge_tables():
+    """Surface.is_closed equals getEdges' boundary test (bdry.max() == 0) on
+    closed, open and doubled meshes; the edge tables appear on first access
+    with getEdges' values."""
+    v, f = synth.icosphere(2, 3.0)
+    rng = np.random.default_rng(0)
+    cases = [f, f[1:], f[rng.permutation(f.shape[0])[:-7]], np.concatenate([f, f])]
+    expect = [True, False, False, True]       # the doubled mesh: every edge has four faces
+    for faces, want in zip(cases, expect):
+        S = Surface((faces, v))
+        closed = S.is_closed()
+        T = Surface((faces, v))
+        T.getEdges()
+        assert closed == bool(T.bdry.max() == 0) == want
+        assert np.array_equal(S.edges, T.edges)         # built on first access
+        assert np.array_equal(S.edgeFaces, T.edgeFaces) and np.array_equal(S.faceEdges, T.faceEdges)
+        assert np.array_equal(S.bdry, T.bdry) and np.array_equal(S.vertices_bdry, T.vertices_bdry)
+        assert S.is_closed() == closed                  # from the tables once they exist
+    assert Surface().edges is None
+    with pytest.raises(AttributeError):
+        Surface().no_such_table
