@@ -390,10 +390,11 @@ def register(seed, net=None, target_data=None, cfg=DEFAULT, max_iter=None, a0=No
             at, _ = warm_control(sm, net, cg_iters=cg_iters, refine=refine)
         else:
             _, at = evol.landmarkEPDiff(sm.Tsize, sm.x0_dev, B.to_dev(a0), sm.options['KparDiff'])
-        # the zero control leaves x_t = x_0 with no deformation energy
-        obj_zero = sm.fun_obj0(sm.fv1) / sm.options['sigmaError'] ** 2 + float(sm.dataTerm_device(sm.x0_dev))
         sm.setInitialControl(at)
         obj_warm = sm.objectiveFun()
+        # the zero control leaves x_t = x_0 with no deformation energy; the
+        # target's self term obj0 is the one objectiveFun just computed
+        obj_zero = sm.obj0 + float(sm.dataTerm_device(sm.x0_dev))
         if obj_warm < obj_zero:
             used_warm = True
         else:                                   # fallback: zero momentum
