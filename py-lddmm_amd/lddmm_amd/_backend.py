@@ -218,6 +218,17 @@ def ptr(t):
 _NP_DTYPE = {torch.float64: np.float64, torch.float32: np.float32, torch.int32: np.int32, torch.int64: np.int64}
 
 
+def upload(arr):
+    """Device copy of a contiguous numpy array on the current stream, without
+    waiting for the GPU: the values are first copied into page-locked memory
+    from torch's caching host allocator (a pageable copy would wait until every
+    queued kernel has finished).  The caller may reuse `arr` at once."""
+    t = torch.from_numpy(np.ascontiguousarray(arr))
+    if t.numel() == 0:
+        return t.to(device())
+    return t.pin_memory().to(device(), non_blocking=True)
+
+
 def to_dev(a, dtype=torch.float64):
     """Contiguous device tensor of `dtype` (numpy arrays are copied host->device)."""
     if isinstance(a, torch.Tensor):
@@ -225,10 +236,8 @@ def to_dev(a, dtype=torch.float64):
                 and a.device.index == torch.cuda.current_device()):
             return a                     # what .to(...).contiguous() would return
         return a.to(device=device(), dtype=dtype).contiguous()
-    dev = device()
     npt = _NP_DTYPE.get(dtype) or torch.empty((), dtype=dtype).numpy().dtype
-    arr = np.ascontiguousarray(np.asarray(a), dtype=npt)
-    return torch.from_numpy(arr).to(dev)
+    return upload(np.ascontiguousarray(np.asarray(a), dtype=npt))
 
 
 ENGINE_VALU, ENGINE_MFMA = 0, 1

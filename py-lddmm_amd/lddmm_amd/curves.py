@@ -104,9 +104,7 @@ class Curve:
             idx = np.zeros(max(2 * s32.shape[0], 1), dtype=np.int32)
             B.check(B.LIB.lddmm_curve_adjacency(s32.ctypes.data, s32.shape[0], nv, ptr.ctypes.data,
                                                 idx.ctypes.data), "curve_adjacency")
-            dev = B.device()
-            self._topo = (torch.from_numpy(s32).to(dev), torch.from_numpy(ptr).to(dev),
-                          torch.from_numpy(idx).to(dev))
+            self._topo = (B.upload(s32), B.upload(ptr), B.upload(idx))
         return self._topo
 
     def device(self, vertices=None):

@@ -63,9 +63,9 @@ class Segments:
         off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
         self.n = len(counts)
         self.P = int(off[-1])
-        self.off = torch.from_numpy(off).to(device)
-        self.seg = torch.from_numpy(np.repeat(np.arange(self.n, dtype=np.int32), counts)).to(device)
-        self.all = torch.tensor([0, self.P], dtype=torch.int32, device=device)
+        self.off = B.upload(off).to(device)
+        self.seg = B.upload(np.repeat(np.arange(self.n, dtype=np.int32), counts)).to(device)
+        self.all = B.upload(np.array([0, self.P], dtype=np.int32)).to(device)
 
 
 class MomentumNet:

@@ -45,8 +45,7 @@ def closed_and_volumes(surfs):
     voff = np.concatenate([[0], np.cumsum(nv)])
     foff = np.concatenate([[0], np.cumsum(nf)])
     dev = B.device()
-    F = torch.from_numpy(np.concatenate([np.asarray(s.faces, dtype=np.int64) + o
-                                         for s, o in zip(surfs, voff[:-1])])).to(dev)
+    F = B.upload(np.concatenate([np.asarray(s.faces, dtype=np.int64) + o for s, o in zip(surfs, voff[:-1])]))
     X = B.to_dev(np.concatenate([np.asarray(s.vertices, dtype=np.float64) for s in surfs]))
     a = torch.cat([F[:, 0], F[:, 1], F[:, 2]])
     b = torch.cat([F[:, 1], F[:, 2], F[:, 0]])
@@ -56,7 +55,7 @@ def closed_and_volumes(surfs):
     start = torch.nonzero(new).flatten()
     cnt = torch.diff(torch.cat([start, torch.tensor([keys.numel()], device=dev)]))
     lone = keys[start[cnt == 1]] // int(voff[-1])       # smaller vertex of every one-face edge
-    owner = torch.searchsorted(torch.from_numpy(voff[1:]).to(dev), lone, right=True)
+    owner = torch.searchsorted(B.upload(voff[1:]), lone, right=True)
     open_ = torch.zeros(S, dtype=torch.bool, device=dev)
     open_[owner] = True
     det = (X[F[:, 0]] * torch.cross(X[F[:, 1]], X[F[:, 2]], dim=1)).sum(1)
@@ -100,13 +99,13 @@ class Population:
         w = np.concatenate([np.asarray(s.weights, dtype=np.float64) for s in templates])
         tw = np.concatenate([np.asarray(s.weights, dtype=np.float64) for s in targets])
         self._keep = {
-            "faces": torch.from_numpy(faces).to(dev), "ptr": torch.from_numpy(ptr).to(dev),
-            "idx": torch.from_numpy(idx).to(dev), "tvert": B.to_dev(np.concatenate(gv)),
-            "tfaces": torch.from_numpy(tfaces).to(dev),
+            "faces": B.upload(faces), "ptr": B.upload(ptr),
+            "idx": B.upload(idx), "tvert": B.to_dev(np.concatenate(gv)),
+            "tfaces": B.upload(tfaces),
             "w": None if np.all(w == 1) else B.to_dev(w), "tw": None if np.all(tw == 1) else B.to_dev(tw),
         }
         self.x0 = B.to_dev(np.concatenate(tv))
-        self.vpair = torch.from_numpy(np.repeat(np.arange(self.P), np.diff(self.voff))).to(dev)
+        self.vpair = B.upload(np.repeat(np.arange(self.P), np.diff(self.voff)))
         k = self._keep
         d = B.LddmmPopulationDesc()
         d.npairs = self.P

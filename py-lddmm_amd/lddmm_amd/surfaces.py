@@ -325,9 +325,7 @@ class Surface:
             idx = np.zeros(3 * f32.shape[0], dtype=np.int32)
             B.check(B.LIB.lddmm_surface_adjacency(f32.ctypes.data, f32.shape[0], nv, ptr.ctypes.data,
                                                   idx.ctypes.data), "surface_adjacency")
-            dev = B.device()
-            self._topo = (torch.from_numpy(f32).to(dev), torch.from_numpy(ptr).to(dev),
-                          torch.from_numpy(idx).to(dev))
+            self._topo = (B.upload(f32), B.upload(ptr), B.upload(idx))
         return self._topo
 
     def device(self, vertices=None):
