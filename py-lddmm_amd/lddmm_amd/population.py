@@ -292,21 +292,27 @@ class PopulationMatching:
         self.PX1 = torch.zeros((V, 3), dtype=torch.float64, device=dev)
         self.EN = torch.zeros(self.P, dtype=torch.float64, device=dev)
         self.ED = torch.zeros(self.P, dtype=torch.float64, device=dev)
-        # |target|^2 / sigmaError^2 at both precisions; like SurfaceMatching, a pair
-        # uses the value of its precision at the last full objective evaluation
-        s2 = o['sigmaError'] ** 2
-        self.obj0_by = {}
-        for dt in {self.dtype0, 'float64'}:
+        # |target|^2 / sigmaError^2 per precision; like SurfaceMatching, a pair
+        # uses the value of its precision at the last full objective evaluation.
+        # The float64 values of a float32 run are computed when a pair first
+        # switches precision (obj0_of).
+        self._obj0_by = {}
+        self.obj0 = self.obj0_of(self.dtype0).copy()
+        self.obj = np.full(self.P, np.nan)
+        self.stats = {"evaluations": 0, "gradients": 0, "rounds": 0}
+
+    def obj0_of(self, dt):
+        """varifoldNorm0 / sigmaError^2 of every target at precision dt (cached)."""
+        if dt not in self._obj0_by:
+            kv_ = self.Kdist
             k = Kernel(name=kv_.name, sigma=kv_.sigma, order=kv_.order)
             k.ms_exponent = kv_.ms_exponent
             k.pk_dtype = dt
             kd = sd._kd(k)
             # varifoldNorm0 of every target, read back in one transfer
             e = torch.cat([sd.varifold_device(kd, t)[0] for t in self.fv1])
-            self.obj0_by[dt] = e.cpu().numpy() / s2
-        self.obj0 = self.obj0_by[self.dtype0].copy()
-        self.obj = np.full(self.P, np.nan)
-        self.stats = {"evaluations": 0, "gradients": 0, "rounds": 0}
+            self._obj0_by[dt] = e.cpu().numpy() / self.options['sigmaError'] ** 2
+        return self._obj0_by[dt]
 
     # ---------------------------------------------------------- evaluations
     def _kd(self, K, dtype):
@@ -328,7 +334,7 @@ class PopulationMatching:
         out = np.full(self.P, np.nan)
         if full:
             for p in np.nonzero(mask)[0]:
-                self.obj0[p] = self.obj0_by[self.dtype[p]][p]
+                self.obj0[p] = self.obj0_of(self.dtype[p])[p]
         s2 = self.options['sigmaError'] ** 2
         for dt, m in self._groups(mask):
             self.pop.set_active(m)
