@@ -549,33 +549,47 @@ class SurfaceMatching:
         PP = lambda ts: (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
         for entry in storedGrad:
             if len(entry) == 2:
-                entry.append({'s': flat(entry[0]), 'y': flat(entry[1])})
+                c = {'s': flat(entry[0]), 'y': flat(entry[1])}
+                c['sp'], c['yp'] = c['s'].data_ptr(), c['y'].data_ptr()     # raw pointers, read once
+                entry.append(c)
         cache = [e[2] for e in storedGrad]
+        VP = lambda ptrs: (ctypes.c_void_p * len(ptrs))(*ptrs)
         if m == 0:
             q.copy_(g)
         elif m <= LBFGS_COMPACT_MAX:
-            S = [c['s'] for c in cache]
-            Y = [c['y'] for c in cache]
+            sp = [c['sp'] for c in cache]
+            yp = [c['yp'] for c in cache]
             new = [j for j, c in enumerate(cache) if 'col' not in c]
-            gd = torch.empty(2 * m, dtype=torch.float64, device=g.device)
+            st = B.stream_handle()
             if new == [m - 1]:
+                # the usual case, one new pair: its Gram column and S^T g, Y^T g
+                # in one dot launch, rows 0 and 1 of `out`; every argument of
+                # both calls is prepared first, so the two launches follow
+                # each other without host work in between
                 out = torch.empty((2, 2 * m), dtype=torch.float64, device=g.device)
-                B.check(B.LIB.lddmm_lbfgs_multidot(PP([Y[m - 1], g]), 2, PP(S + Y), 2 * m, n, B.ptr(out),
-                                                   B.stream_handle()), "lbfgs_multidot")
-                cache[m - 1]['col'] = out[0]
-                gd = out[1]
+                optr = out.data_ptr()
+                c = cache[m - 1]
+                c['col'], c['cp'], c['cl'] = out[0], optr, m
+                probes, targets = VP([yp[m - 1], g.data_ptr()]), VP(sp + yp)
+                sarr, yarr, carr = VP(sp), VP(yp), VP([c['cp'] for c in cache])
+                clen = (ctypes.c_int32 * m)(*[c['cl'] for c in cache])
+                B.check(B.LIB.lddmm_lbfgs_multidot(probes, 2, targets, 2 * m, n, ctypes.c_void_p(optr), st),
+                        "lbfgs_multidot")
+                B.check(B.LIB.lddmm_lbfgs_compact(B.ptr(g), n, m, sarr, yarr, carr, clen,
+                                                  ctypes.c_void_p(optr + 2 * m * 8), B.ptr(q), st), "lbfgs_compact")
             else:
+                gd = torch.empty(2 * m, dtype=torch.float64, device=g.device)
                 for j in new:
                     col = torch.empty(2 * (j + 1), dtype=torch.float64, device=g.device)
-                    B.check(B.LIB.lddmm_lbfgs_multidot(PP([Y[j]]), 1, PP(S[:j + 1] + Y[:j + 1]), 2 * (j + 1), n,
-                                                       B.ptr(col), B.stream_handle()), "lbfgs_multidot")
+                    B.check(B.LIB.lddmm_lbfgs_multidot(VP([yp[j]]), 1, VP(sp[:j + 1] + yp[:j + 1]), 2 * (j + 1), n,
+                                                       B.ptr(col), st), "lbfgs_multidot")
                     cache[j]['col'] = col
-                B.check(B.LIB.lddmm_lbfgs_multidot(PP([g]), 1, PP(S + Y), 2 * m, n, B.ptr(gd), B.stream_handle()),
+                    cache[j]['cp'], cache[j]['cl'] = col.data_ptr(), j + 1
+                B.check(B.LIB.lddmm_lbfgs_multidot(VP([g.data_ptr()]), 1, VP(sp + yp), 2 * m, n, B.ptr(gd), st),
                         "lbfgs_multidot")
-            cols = [c['col'] for c in cache]
-            clen = (ctypes.c_int32 * m)(*[c.numel() // 2 for c in cols])
-            B.check(B.LIB.lddmm_lbfgs_compact(B.ptr(g), n, m, PP(S), PP(Y), PP(cols), clen, B.ptr(gd), B.ptr(q),
-                                              B.stream_handle()), "lbfgs_compact")
+                clen = (ctypes.c_int32 * m)(*[c['cl'] for c in cache])
+                B.check(B.LIB.lddmm_lbfgs_compact(B.ptr(g), n, m, VP(sp), VP(yp), VP([c['cp'] for c in cache]),
+                                                  clen, B.ptr(gd), B.ptr(q), st), "lbfgs_compact")
         else:
             for c in cache:
                 if 'sc' not in c:
