@@ -411,17 +411,17 @@ class SurfaceMatching:
         pending = getattr(self, '_pending_table', None)
         if pending is not None and pending[1] is None:
             parts.append(pending[0].reshape(-1))        # BFGS's dot table, launched before the trial
-        host = torch.cat(parts).cpu()
+        host = torch.cat(parts).cpu().numpy()          # numpy from here: no per-element torch ops
         if len(parts) == 4:
-            pending[1] = host[3:].numpy().reshape(tuple(pending[0].shape))
+            pending[1] = host[3:].reshape(tuple(pending[0].shape))
         objTry = self._record_try(dr, eps, controlTry, stateTry, host[:2], None)
         return objTry, -float(host[2]), grad
 
     @staticmethod
     def _host_terms(obj, data):
-        """(deformation term, data term) as one float64 host tensor: a single
+        """(deformation term, data term) as one float64 host array: a single
         device-to-host transfer for both."""
-        return torch.stack([obj.reshape(()), data.reshape(())]).cpu()
+        return torch.stack([obj.reshape(()), data.reshape(())]).cpu().numpy()
 
     def acceptVarTry(self):
         self.obj = self.objTry
