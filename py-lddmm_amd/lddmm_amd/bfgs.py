@@ -127,8 +127,17 @@ class _Memory:
         self.pairs = []
 
     def push(self, s, y):
+        """Appends (s, y); returns the entries dropped from the front."""
         self.pairs.append([s, y])
-        del self.pairs[:max(0, len(self.pairs) - self.size)]
+        n = max(0, len(self.pairs) - self.size)
+        dropped = self.pairs[:n]
+        del self.pairs[:n]
+        return dropped
+
+    def undo_push(self, dropped):
+        """Reverts the last push, which dropped `dropped`."""
+        self.pairs.pop()
+        self.pairs[:0] = dropped
 
     def clear(self):
         self.pairs = []
@@ -182,6 +191,41 @@ def _fd_gradient_check(opt, ops, grad, f, coeff):
     logging.info('Test Gradient: %.4f %.4f' % ((fp - fm) / (2 * h), -ops.dot(grad, d) * coeff))
 
 
+class _Ahead:
+    """The next iteration's quasi-Newton work, queued while the line search
+    waits for a trial's values: for trial step t with gradient g, the pair
+    (s, y) = (-t d, g - g_old) is pushed, the direction q = twoLoop(g) and
+    the dot table of (g, q) launched.  It is used when t is the accepted step
+    and g the gradient the search returns; otherwise the push is reverted.
+    The values are those the next iteration would compute (the same calls on
+    the same inputs), so no decision changes."""
+
+    def __init__(self):
+        self.entry = None          # (t, g, s, q, dots, dropped)
+
+    def launch(self, run, opt, ops, direction, t, g):
+        self.revert(run)
+        s = ops.scale(direction, -t)
+        dropped = run.memory.push(s, ops.axpy(g, run.grad_old, -1))
+        q = opt.twoLoop(g, run.memory.pairs)
+        self.entry = (t, g, s, q, ops.table([g, q], [g, q]), dropped)
+
+    def revert(self, run):
+        if self.entry is not None:
+            run.memory.undo_push(self.entry[5])
+            self.entry = None
+
+    def take(self, run, t, g):
+        """(s, q, dots) when the queued work is for step t and gradient g;
+        otherwise the queued push is reverted and None returned."""
+        e = self.entry
+        if e is not None and e[0] == t and e[1] is g:
+            self.entry = None
+            return e[2], e[3], e[4]
+        self.revert(run)
+        return None
+
+
 class _Run:
     """State of one optimisation."""
 
@@ -192,6 +236,7 @@ class _Run:
         self.step_taken = None     # s of the last accepted step (-t q)
         self.grad_old = None
         self.grad_next = None      # gradient the line search left at the accepted point
+        self.ahead_taken = None    # (q, dots) of the next iteration, queued during the line search
         self.memory = _Memory(memory)
         self.grad_eps = None
         self.history = []
@@ -217,6 +262,10 @@ def bfgs(opt, verb=True, maxIter=1000, TestGradient=False, epsInit=0.01, memory=
         opt.startOptim()
 
     run = _Run(opt, memory)
+    # the next iteration's direction is queued during the line search when the
+    # object computes it itself and its fused trial can run work before the
+    # read-back
+    ahead = _Ahead() if (hasattr(opt, 'twoLoop') and getattr(opt, 'queue_ahead', False)) else None
     if hasattr(opt, 'gradEps'):
         run.grad_eps = opt.gradEps
     elif hasattr(opt, 'options') and 'gradEps' in opt.options:
@@ -254,16 +303,22 @@ def bfgs(opt, verb=True, maxIter=1000, TestGradient=False, epsInit=0.01, memory=
         if restarted or run.it == 0:
             run.memory.clear()
             q = grad
+            dots = None
+        elif run.ahead_taken is not None:
+            q, dots = run.ahead_taken          # queued during the last line search
         else:
             run.memory.push(run.step_taken, ops.axpy(grad, run.grad_old, -1))
             if hasattr(opt, 'twoLoop'):
                 q = opt.twoLoop(grad, run.memory.pairs)
             else:
                 q = quasi_newton_direction(grad, run.memory.pairs, ops)
+            dots = None
+        run.ahead_taken = None
         # |g|^2, |q|^2 and the slope <g, q> along the new direction, launched
         # together; read back when first needed (on most iterations after the
         # line search's first trial, whose own read-back waits for them)
-        dots = ops.table([grad, q], [grad, q])
+        if dots is None:
+            dots = ops.table([grad, q], [grad, q])
         direction = q
         run.grad_old = grad
         if run.it == 0 or run.it == burn_in:
@@ -271,18 +326,29 @@ def bfgs(opt, verb=True, maxIter=1000, TestGradient=False, epsInit=0.01, memory=
             logging.info(f'Gradient threshold: {run.grad_eps:.6f}')
         t_init = (1. / np.sqrt(np.maximum(1e-20, dots()[1][1]))) if (run.it == 0 or restarted) else 1.
 
+        spec = None
+        if ahead is not None:
+            spec = lambda t, g, d=direction: ahead.launch(run, opt, ops, d, t, g)
         step, _, _, f_new, _, run.grad_next = line_search_weak_wolfe(
             opt, direction, gfk=grad, old_fval=run.f, old_old_fval=run.f_old, c1=WOLFE_C1, c2=WOLFE_C2,
-            amax=None, t_init=t_init, maxiter=WOLFE_MAX_GRADIENTS, slope0=lambda: -dots()[0][1])
+            amax=None, t_init=t_init, maxiter=WOLFE_MAX_GRADIENTS, slope0=lambda: -dots()[0][1],
+            speculate=spec)
         grad_norm = np.sqrt(dots()[0][0])
         if step is not None:
-            run.step_taken = ops.scale(direction, -step)
+            taken = ahead.take(run, step, run.grad_next) if ahead is not None else None
+            if taken is not None:
+                run.step_taken, q_next, dots_next = taken
+                run.ahead_taken = (q_next, dots_next)
+            else:
+                run.step_taken = ops.scale(direction, -step)
             run.f_old = run.f
             opt.acceptVarTry()
             run.f = f_new
         else:
             logging.info('Wolfe search unsuccessful')
             run.grad_next = None
+            if ahead is not None:
+                ahead.revert(run)
 
         outcome = decide(step is not None, restarted, run.it, burn_in, run.f, run.f_old, grad_norm,
                          run.grad_eps, may_stop)

@@ -29,12 +29,13 @@ class _Probe:
     the slope is then counted, and its gradient kept, only if the search asks
     for it, so the counts and every decision are those of separate calls."""
 
-    def __init__(self, opt, direction, euclidean):
+    def __init__(self, opt, direction, euclidean, speculate=None):
         self.opt = opt
         self.direction = direction
         self.coeff = getattr(opt, "gradCoeff", 1)
         self._dots = opt.dotProduct_euclidean if euclidean else opt.dotProduct
         self._with_slope = getattr(opt, "updateTryWithSlope", None) if euclidean else None
+        self._speculate = speculate if self._with_slope is not None else None
         self._pending = None       # (t, slope, gradient) of the last trial
         self.n_values = 0
         self.n_slopes = 0
@@ -47,7 +48,12 @@ class _Probe:
         self.n_values += 1
         self._pending = None
         if self._with_slope is not None and t != 0:
-            f, slope, grad = self._with_slope(self.direction, t, self.coeff)
+            if self._speculate is not None:
+                # the caller's work for "t accepted" is queued before the read-back
+                f, slope, grad = self._with_slope(self.direction, t, self.coeff,
+                                                  before_readback=lambda g: self._speculate(t, g))
+            else:
+                f, slope, grad = self._with_slope(self.direction, t, self.coeff)
             self._pending = (t, slope, grad)
             return f
         return self.opt.updateTry(self.direction, t)
@@ -72,14 +78,18 @@ def _next_trial(t, f_t, f0, slope0, lo, hi, decreased):
 
 
 def line_search_weak_wolfe(opt, pk, gfk=None, old_fval=None, old_old_fval=None, c1=0.001, c2=0.9,
-                           amax=None, maxiter=100, euclidean=True, t_init=1., slope0=None):
+                           amax=None, maxiter=100, euclidean=True, t_init=1., slope0=None, speculate=None):
     """Returns (t or None, objective evaluations, slope evaluations, f(t),
     f(0), gradient at t or None) for the direction pk (amax and old_old_fval
     are part of the reference's signature and unused).  slope0, when given,
     is the slope at t = 0, -<gfk, pk>, computed by the caller, or a function
     returning it; the function is called after the first trial's objective,
-    where the slope is first needed."""
-    probe = _Probe(opt, pk, euclidean)
+    where the slope is first needed.  speculate(t, g), when given and the
+    object has updateTryWithSlope, is called for every trial t with its
+    gradient g after the trial's work is queued and before its values are
+    read back (the caller may queue what it would do if t were accepted);
+    it changes no decision of the search."""
+    probe = _Probe(opt, pk, euclidean, speculate)
     if slope0 is None:
         if gfk is None:
             gfk = opt.getGradient(probe.coeff)

@@ -58,6 +58,9 @@ class State(dict):
 class SurfaceMatching:
     # the reference builds the unreduced (control-point) form for surfaces only
     unreduced_supported = True
+    # bfgs may queue the next iteration's direction inside updateTryWithSlope
+    # (before_readback) while the trial's values travel to the host
+    queue_ahead = True
 
     def __init__(self, Template=None, Target=None, options=None):
         self.setInitialOptions(options)
@@ -391,7 +394,7 @@ class SurfaceMatching:
         controlTry, stateTry, obj_, data, _ = self._trial(dr, eps)
         return self._record_try(dr, eps, controlTry, stateTry, self._host_terms(obj_, data), objRef)
 
-    def updateTryWithSlope(self, dr, eps, coeff=1.0):
+    def updateTryWithSlope(self, dr, eps, coeff=1.0, before_readback=None):
         """updateTry(dr, eps) with the gradient at the trial control and the
         slope along dr, in one read-back.  The gradient's kernels are queued
         behind the trial's before anything comes back to the host, and the
@@ -402,7 +405,9 @@ class SurfaceMatching:
         a trial without a second round trip.  For the surface varifold the
         data term and its gradient come from one varifold pass
         (dataTermAndGradient_device): the objective equals updateTry's to
-        rounding."""
+        rounding.  before_readback(g), when given, is called with the
+        gradient once everything of the trial is queued, before the host
+        waits for the values."""
         controlTry, stateTry, obj_, data, egrad = self._trial(dr, eps, with_gradient=True)
         self._last_try = (dr, eps, controlTry, stateTry)
         grad = self.getGradient(coeff, update=[dr, eps], endpoint_gradient=egrad)
@@ -411,7 +416,19 @@ class SurfaceMatching:
         pending = getattr(self, '_pending_table', None)
         if pending is not None and pending[1] is None:
             parts.append(pending[0].reshape(-1))        # BFGS's dot table, launched before the trial
-        host = torch.cat(parts).cpu().numpy()          # numpy from here: no per-element torch ops
+        cat = torch.cat(parts)
+        if before_readback is None:
+            host = cat.cpu().numpy()                    # numpy from here: no per-element torch ops
+        else:
+            # the values go to page-locked memory ahead of the caller's work on
+            # the stream, and the host waits for that copy only
+            pinned = torch.empty(cat.shape, dtype=cat.dtype, pin_memory=True)
+            pinned.copy_(cat, non_blocking=True)
+            done = torch.cuda.Event()
+            done.record()
+            before_readback(grad)
+            done.synchronize()
+            host = pinned.numpy()
         if len(parts) == 4:
             pending[1] = host[3:].reshape(tuple(pending[0].shape))
         objTry = self._record_try(dr, eps, controlTry, stateTry, host[:2], None)

@@ -257,3 +257,77 @@ def test_stop_condition_veto():
     # the run ends at maxIter or when even steepest descent finds no step
     assert P.bfgs_iterations == 40 or outcomes[-2:] == [O.RESTART, O.STUCK]
     assert P.procedure_ends == 1
+
+
+# ------------------------------------------------------- queued-ahead direction
+class _Rosenbrock:
+    """Chained Rosenbrock objective with the fused-trial protocol of the
+    matching drivers (updateTryWithSlope with before_readback, twoLoop):
+    enough curvature that the weak-Wolfe search rejects and extends trials."""
+
+    def __init__(self, n=12, queue_ahead=True):
+        self.x = np.linspace(-1.2, 1.0, n)
+        self.queue_ahead = queue_ahead
+        self.trial = None
+        self.calls = {"ahead": 0}
+
+    @staticmethod
+    def f(x):
+        return float(np.sum(100.0 * (x[1:] - x[:-1] ** 2) ** 2 + (1.0 - x[:-1]) ** 2))
+
+    @staticmethod
+    def g(x):
+        d = np.zeros_like(x)
+        r = x[1:] - x[:-1] ** 2
+        d[:-1] += -400.0 * x[:-1] * r - 2.0 * (1.0 - x[:-1])
+        d[1:] += 200.0 * r
+        return d
+
+    def getVariable(self):
+        return self.x
+
+    def objectiveFun(self):
+        return self.f(self.x)
+
+    def updateTry(self, d, t, objRef=None):
+        self.trial = self.x - t * d
+        return self.f(self.trial)
+
+    def acceptVarTry(self):
+        self.x = self.trial
+
+    def getGradient(self, coeff=1.0, update=None):
+        x = self.x if update is None else self.x - update[1] * update[0]
+        return self.g(x) / coeff
+
+    def dotProduct_euclidean(self, u, vs):
+        return [float(np.dot(u, v)) for v in vs]
+
+    def updateTryWithSlope(self, d, t, coeff=1.0, before_readback=None):
+        f = self.updateTry(d, t)
+        g = self.getGradient(coeff, update=[d, t])
+        if before_readback is not None:
+            self.calls["ahead"] += 1
+            before_readback(g)
+        return f, -float(np.dot(g, d)), g
+
+    def twoLoop(self, g, pairs):
+        return B.quasi_newton_direction(g, pairs, B._VectorOps(self))
+
+
+def test_queued_ahead_direction_keeps_every_decision():
+    """bfgs with the next iteration's direction queued during the line search
+    (the object's queue_ahead) against the same run without it: the same
+    objective history, decisions and final variable, with rejected trials
+    and a memory window that slides (memory 3)."""
+    runs = []
+    for ahead in (False, True):
+        obj = _Rosenbrock(queue_ahead=ahead)
+        x = B.bfgs(obj, verb=False, maxIter=60, memory=3)
+        runs.append((np.array(obj.bfgs_history), obj.bfgs_decisions, np.array(x), obj.calls["ahead"]))
+    (h0, d0, x0, a0), (h1, d1, x1, a1) = runs
+    assert a0 == 0 and a1 > len(h1)              # queued on every trial, rejected ones included
+    np.testing.assert_array_equal(h1, h0)
+    assert d1 == d0
+    np.testing.assert_array_equal(x1, x0)
+    assert any(o is B.Outcome.ACCEPT for _, o in d0) and len(h0) > 20
